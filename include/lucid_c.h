@@ -1,0 +1,123 @@
+/*
+ * lucid_c.h — C ABI of the MI355X exact order-independent-transparency rasterizer.
+ *
+ * The entry points replace the reference's LucidRenderer (src/lucid_renderer.h:21-96):
+ *
+ *   lucid_create       <- LucidRenderer::exConstruct   (src/lucid_renderer.cpp:186-317)
+ *   lucid_destroy      <- ~LucidRenderer
+ *   lucid_render       <- LucidRenderer::render        (src/lucid_renderer.cpp:319-350),
+ *                         with setupInputData/uploadInstances (:352-451) folded in
+ *   lucid_read_info    <- the async cmds.download(m_info) (:343-346) consumed by
+ *                         verifyInfo/getStats (:629-836), done synchronously here
+ *   lucid_verify_info  <- LucidRenderer::verifyInfo    (:629-707)
+ *   lucid_read_bin_lists  exposes g_bin_quads/g_bin_tris (bin_dispatcher.glsl:39-41)
+ *   lucid_bin_size / lucid_block_size / lucid_max_visible_quads / lucid_max_scene_quads
+ *                      <- binSize()/blockSize()/maxVisibleQuads()/maxSceneQuads() (:43-49)
+ *
+ * Conventions: plain C types only; every call returns an int status (LUCID_OK == 0,
+ * negative on error); no exception crosses the boundary. Device pointers are HIP device
+ * pointers on the renderer's device; `stream` is a hipStream_t passed as void* (NULL =
+ * the default stream). Rendering is asynchronous on `stream`; lucid_finish waits for it.
+ */
+#ifndef LUCID_C_H
+#define LUCID_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "lucid_structs.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LUCID_OK 0
+#define LUCID_ERR_INVALID_ARG (-1)
+#define LUCID_ERR_OUT_OF_MEMORY (-2)
+#define LUCID_ERR_HIP (-3)
+#define LUCID_ERR_NO_DEVICE (-4)
+#define LUCID_ERR_CAPACITY (-5)  /* a frame exceeded an internal buffer; buffers were grown */
+#define LUCID_ERR_UNSUPPORTED (-6)
+
+/* LucidRenderOpt bits (src/lucid_renderer.h:10-11). Debug/timer options are accepted
+ * and ignored; rocprof replaces them. */
+#define LUCID_OPT_DEBUG_QUAD_SETUP (1u << 0)
+#define LUCID_OPT_DEBUG_BIN_COUNTER (1u << 1)
+#define LUCID_OPT_DEBUG_BIN_DISPATCHER (1u << 2)
+#define LUCID_OPT_DEBUG_RASTER (1u << 3)
+#define LUCID_OPT_TIMERS (1u << 4)
+#define LUCID_OPT_ADDITIVE_BLENDING (1u << 5)
+#define LUCID_OPT_VISUALIZE_ERRORS (1u << 6)
+#define LUCID_OPT_ALPHA_THRESHOLD (1u << 7)
+/* Extension: shaded samples stay fp32 instead of the reference's RGBA8 truncation
+ * (encodeRGBA8, shared/funcs.glsl:118-121). Used by the differentiable path. */
+#define LUCID_OPT_FP32_SAMPLES (1u << 16)
+
+typedef struct lucid_renderer lucid_renderer;
+
+typedef struct lucid_create_info {
+	int view_width, view_height; /* at most 2560 x 2048 (src/lucid_renderer.h:27) */
+	uint32_t opts;               /* LUCID_OPT_* */
+	int device;                  /* HIP device ordinal */
+	int64_t max_quads;           /* initial quad capacity (0: 1M); grows on demand */
+} lucid_create_info;
+
+typedef struct lucid_frame_inputs {
+	/* host arrays (uploaded when they change) */
+	const lucid_instance_data *instances; /* InstanceData[num_instances] */
+	int num_instances;
+	const uint32_t *instance_colors; /* RGBA8 per instance, or NULL (white) */
+	const lucid_config *config;      /* LucidConfig (std140 bytes) */
+	/* device arrays */
+	const uint32_t *quad_indices; /* u32 indices, addressed through InstanceData */
+	const float *positions;       /* float3 per vertex */
+	int64_t num_verts;
+	const uint32_t *colors_rgba8; /* RGBA8 per vertex (reference format), or NULL */
+	const float *colors_rgb;      /* float3 per vertex, or NULL (used when colors_rgba8 is NULL) */
+	const float *tri_alpha;       /* alpha per triangle id (2 per quad), with colors_rgb */
+	const uint32_t *normals;      /* packed 10:10:10 normal per vertex, or NULL */
+} lucid_frame_inputs;
+
+typedef struct lucid_frame_outputs {
+	uint8_t *image_rgba8;   /* device, width*height*4, row-major, row 0 = top; may be NULL */
+	float *image_rgba32f;   /* device, width*height*4 float; may be NULL */
+	uint32_t *frag_counts;  /* device, width*height fragments per pixel; may be NULL */
+} lucid_frame_outputs;
+
+const char *lucid_error_string(int status);
+
+int lucid_create(const lucid_create_info *info, lucid_renderer **out);
+void lucid_destroy(lucid_renderer *r);
+
+int lucid_render(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_frame_outputs *out,
+				 void *stream);
+/* Waits for the last frame. Returns LUCID_ERR_CAPACITY if it overflowed an internal buffer
+ * (the buffers have then been grown and the frame must be rendered again). */
+int lucid_finish(lucid_renderer *r);
+
+/* Synchronous copy of LucidInfo followed by the 10 per-bin arrays (g_counts);
+ * num_u32 must be at least LUCID_INFO_U32_SIZE + 10 * bin_count. */
+int lucid_read_info(lucid_renderer *r, uint32_t *dst, size_t num_u32);
+/* Checks the prefix-sum invariants of the last frame; returns the number of violations. */
+int lucid_verify_info(lucid_renderer *r);
+/* Copies the per-bin quad and tri lists of the last frame (host buffers, capacity in entries);
+ * num_quads and num_tris receive the totals. */
+int lucid_read_bin_lists(lucid_renderer *r, uint32_t *quads, size_t quad_cap, uint32_t *tris,
+						 size_t tri_cap, size_t *num_quads, size_t *num_tris);
+/* Copies per-triangle setup records of the last frame in the reference's storage order:
+ * per triangle 20 u32 = bary0, bary1, scan0, scan1, depth (quad_setup.glsl:305-339),
+ * and the per-quad AABB words (0xffffffff = rejected). */
+int lucid_read_setup(lucid_renderer *r, uint32_t *tri_records, size_t num_tris,
+					 uint32_t *quad_aabbs, size_t num_quads);
+
+int lucid_bin_count(const lucid_renderer *r);
+int lucid_bin_size(const lucid_renderer *r);
+int lucid_block_size(const lucid_renderer *r);
+int64_t lucid_max_visible_quads(const lucid_renderer *r);
+int64_t lucid_max_scene_quads(const lucid_renderer *r);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

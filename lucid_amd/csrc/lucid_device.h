@@ -1,0 +1,323 @@
+// Device-side definitions shared by the Lucid HIP kernels (gfx950, wave64).
+//
+// Arithmetic conventions (they make bin counts, bin lists and per-pixel
+// fragment counts bit-reproducible on the CPU):
+//   * build with -ffp-contract=off: no fused multiply-add anywhere;
+//   * expressions keep the GLSL source's left-to-right order;
+//   * division and sqrt are IEEE correctly rounded (HIP's default for fp32);
+//   * min/max are IEEE minNum/maxNum (v_min_f32/v_max_f32 in IEEE mode);
+//   * float->int/uint conversions truncate, saturate and map NaN to 0.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/lucid_structs.h"
+
+namespace lucid {
+
+constexpr int kBinSize = 32;
+constexpr int kBinShift = 5;
+constexpr int kBinMask = 31;
+constexpr int kBlockSize = 8;
+constexpr uint32_t kInvalidAabb = 0xffffffffu;
+constexpr int kLowLevelLimit = 1024; // bin_categorizer.glsl:352
+constexpr int kMaxBlockTris = 256;   // raster_low.glsl:17 (promotion threshold)
+constexpr int kLowRowCap = 1024;     // records per block row of a low bin (< 1024 tris)
+
+// Internal control words (separate from LucidInfo, whose layout is the reference's).
+enum CtrlWord : int {
+	kCtrlNumLarge = 0,      // entries in the large-quad list
+	kCtrlRecordCursor = 1,  // bump allocator of the raster_high record pool (uint4 units)
+	kCtrlFallbackCount = 2, // entries in the exact-fallback queue
+	kCtrlOverflow = 3,      // bit 0 bin lists, 1 record pool, 2 fallback queue, 3 spill pool
+	kCtrlSpillCursor = 4,   // bump allocator of the deep-pixel spill pool (u64 units)
+	kCtrlTotalBinQuads = 5,
+	kCtrlTotalBinTris = 6,
+	kCtrlFallbackPixels = 7,
+	kCtrlWords = 16,
+};
+
+enum ColorMode : int { kColorNone = 0, kColorRGBA8 = 1, kColorF32 = 2 };
+
+// Per-frame constants (passed by value to every kernel).
+struct FrameConst {
+	int width, height;
+	int bin_count_x, bin_count_y, bin_count;
+	int num_quads, num_instances;
+	int num_bin_wgs; // work-groups of the bin counter/dispatcher
+	uint32_t opts;
+	uint32_t backface;
+	int color_mode;
+	int has_normals;
+	float vp[16]; // column-major view-projection matrix
+	float origin[3], dir0[3], dirx[3], diry[3], ray_dir0[3];
+	float ambient[3], sun_color[3], sun_dir[3];
+	float sun_power, ambient_power;
+	float background[4];
+};
+
+// Device buffers of one frame (passed by value).
+struct FrameBufs {
+	// inputs
+	const lucid_instance_data *instances;
+	const int *inst_prefix; // num_instances + 1
+	const uint32_t *inst_colors;
+	const uint32_t *quad_indices;
+	const float *positions;
+	const uint32_t *colors_rgba8;
+	const float *colors_rgb;
+	const float *tri_alpha;
+	const uint32_t *normals;
+	// info + control
+	uint32_t *info;   // LucidInfo followed by 10 * bin_count counters
+	uint32_t *ctrl;   // kCtrlWords
+	// setup records
+	uint32_t *quad_aabb;  // per quad
+	uint4 *tri_scan;      // 2 per tri
+	uint4 *tri_depth;     // 1 per tri
+	uint4 *tri_bary;      // 2 per tri
+	uint32_t *tri_normal; // 1 per tri
+	uint4 *quad_color;    // 1 per quad (RGBA8 vertex colours)
+	uint4 *quad_normal;   // 1 per quad (vertex normals)
+	float4 *tri_color;    // 3 per tri (fp32 vertex colours: rgb0+alpha, rgb1, rgb2)
+	uint32_t *large_list; // quads whose bin AABB covers more than 4 bins
+	// binning
+	uint32_t *wg_hist; // num_bin_wgs * 2 * bin_count
+	uint32_t *bin_quads, *bin_tris;
+	uint64_t bin_quads_cap, bin_tris_cap;
+	// raster
+	uint4 *low_records;  // bin_count * 4 * kLowRowCap
+	uint4 *high_records; // pool
+	uint64_t high_records_cap;
+	uint4 *fallback_queue; // 2 uint4 per entry
+	uint32_t fallback_cap;
+	uint64_t *spill; // deep-pixel spill pool
+	uint64_t spill_cap;
+	// outputs
+	uint8_t *image;
+	float *image_f;
+	uint32_t *frag_counts;
+};
+
+// ------------------------------------------------------------------ conversions
+
+__device__ __forceinline__ int f2i(float x) {
+	// v_cvt_i32_f32 semantics: truncate, saturate, NaN -> 0
+	if(!(x == x))
+		return 0;
+	if(x >= 2147483648.0f)
+		return 2147483647;
+	if(x <= -2147483648.0f)
+		return (int)0x80000000u;
+	return (int)x;
+}
+
+__device__ __forceinline__ uint32_t f2u(float x) {
+	if(!(x > 0.0f))
+		return 0u;
+	if(x >= 4294967296.0f)
+		return 0xffffffffu;
+	return (uint32_t)x;
+}
+
+__device__ __forceinline__ float bits2f(uint32_t u) { return __uint_as_float(u); }
+__device__ __forceinline__ uint32_t f2bits(float f) { return __float_as_uint(f); }
+
+__device__ __forceinline__ float fmx(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ float fmn(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ float sat(float x) { return fmn(fmx(x, 0.0f), 1.0f); }
+
+// Monotone map of float order onto uint order (for sort keys).
+__device__ __forceinline__ uint32_t ord_f32(float f) {
+	uint32_t u = f2bits(f);
+	return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// ------------------------------------------------------------------ small vectors
+
+struct V3 {
+	float x, y, z;
+};
+__device__ __forceinline__ V3 mk3(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 sub3(V3 a, V3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ V3 add3(V3 a, V3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ V3 mul3(V3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ V3 neg3(V3 a) { return mk3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ float dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross3(V3 a, V3 b) {
+	return mk3(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
+}
+__device__ __forceinline__ V3 ld3(const float v[3]) { return mk3(v[0], v[1], v[2]); }
+
+// ------------------------------------------------------------------ encoders
+
+__device__ __forceinline__ uint32_t encode_aabb28(uint32_t x0, uint32_t y0, uint32_t x1,
+												  uint32_t y1) {
+	return (x0 & 0x7fu) | ((y0 & 0x7fu) << 7) | ((x1 & 0x7fu) << 14) | ((y1 & 0x7fu) << 21);
+}
+
+__device__ __forceinline__ uint32_t encode_rgba8(float r, float g, float b, float a) {
+	return f2u(r * 255.0f) | (f2u(g * 255.0f) << 8) | (f2u(b * 255.0f) << 16) |
+		   (f2u(a * 255.0f) << 24);
+}
+
+__device__ __forceinline__ float4 decode_rgba8(uint32_t c) {
+	return make_float4((float)(c & 0xffu) * (1.0f / 255.0f),
+					   (float)((c >> 8) & 0xffu) * (1.0f / 255.0f),
+					   (float)((c >> 16) & 0xffu) * (1.0f / 255.0f),
+					   (float)(c >> 24) * (1.0f / 255.0f));
+}
+
+__device__ __forceinline__ uint32_t encode_normal(V3 n) {
+	uint32_t x = f2u(512.0f + n.x * 511.0f) & 0x3ffu;
+	uint32_t y = f2u(512.0f + n.y * 511.0f) & 0x3ffu;
+	uint32_t z = f2u(512.0f + n.z * 511.0f) & 0x3ffu;
+	return x | (y << 10) | (z << 20);
+}
+
+__device__ __forceinline__ V3 decode_normal(uint32_t n) {
+	return mk3(((float)(n & 0x3ffu) - 512.0f) * (1.0f / 511.0f),
+			   ((float)((n >> 10) & 0x3ffu) - 512.0f) * (1.0f / 511.0f),
+			   ((float)((n >> 20) & 0x3ffu) - 512.0f) * (1.0f / 511.0f));
+}
+
+// ------------------------------------------------------------------ wave helpers
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+__device__ __forceinline__ void wave_sync_lds() {
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t wave_inclusive_add(uint32_t v) {
+	int l = lane_id();
+#pragma unroll
+	for(int d = 1; d < 64; d <<= 1) {
+		uint32_t t = __shfl_up(v, d, 64);
+		if(l >= d)
+			v += t;
+	}
+	return v;
+}
+
+__device__ __forceinline__ uint32_t wave_inclusive_max(uint32_t v) {
+	int l = lane_id();
+#pragma unroll
+	for(int d = 1; d < 64; d <<= 1) {
+		uint32_t t = __shfl_up(v, d, 64);
+		if(l >= d)
+			v = v > t ? v : t;
+	}
+	return v;
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// Index of the k-th (0-based) set bit of m (k < popcount(m)).
+__device__ __forceinline__ int select_bit64(uint64_t m, uint32_t k) {
+	int pos = 0;
+	uint32_t lo = (uint32_t)m;
+	uint32_t c = __popc(lo);
+	uint32_t w = lo;
+	if(k >= c) {
+		k -= c;
+		pos = 32;
+		w = (uint32_t)(m >> 32);
+	}
+	c = __popc(w & 0xffffu);
+	if(k >= c) {
+		k -= c, pos += 16, w >>= 16;
+	}
+	c = __popc(w & 0xffu);
+	if(k >= c) {
+		k -= c, pos += 8, w >>= 8;
+	}
+	c = __popc(w & 0xfu);
+	if(k >= c) {
+		k -= c, pos += 4, w >>= 4;
+	}
+	c = __popc(w & 0x3u);
+	if(k >= c) {
+		k -= c, pos += 2, w >>= 2;
+	}
+	c = w & 1u;
+	if(k >= c)
+		pos += 1;
+	return pos;
+}
+
+// ------------------------------------------------------------------ scanline rows
+
+// Scanline state of one triangle inside one bin (scanline.glsl:13-26).
+struct RowScan {
+	float mn[3], mx[3], step[3];
+};
+
+// Starts the scan at absolute row `start_row` for a bin whose left edge is bin_x.
+__device__ __forceinline__ void row_scan_init(RowScan &s, uint4 scan0, uint4 scan1, float startx,
+											  float starty) {
+	uint32_t signs = scan1.w;
+	float sc[3] = {bits2f(scan0.x), bits2f(scan0.y), bits2f(scan0.z)};
+	float st[3] = {bits2f(scan1.x), bits2f(scan1.y), bits2f(scan1.z)};
+#pragma unroll
+	for(int i = 0; i < 3; i++) {
+		float v = sc[i] + (st[i] * starty - startx);
+		bool xneg = (signs >> i) & 1u;
+		s.mn[i] = xneg ? -__builtin_huge_valf() : v;
+		s.mx[i] = xneg ? v : __builtin_huge_valf();
+		s.step[i] = st[i];
+	}
+}
+
+// One row of rasterBinStep (raster.glsl:116-140): returns the span packed as
+// min | max << 5 (empty rows are (31, 0)) and advances the state.
+__device__ __forceinline__ uint32_t row_scan_step(RowScan &s) {
+	int a = f2i(fmx(fmx(s.mn[0], s.mn[1]), fmx(s.mn[2], 0.0f)));
+	int b = f2i(fmn(fmn(s.mx[0], s.mx[1]), fmn(s.mx[2], (float)kBinSize))) - 1;
+	if(a > b)
+		a = kBinMask, b = 0;
+#pragma unroll
+	for(int i = 0; i < 3; i++) {
+		s.mn[i] = s.mn[i] + s.step[i];
+		s.mx[i] = s.mx[i] + s.step[i];
+	}
+	return (uint32_t)a | ((uint32_t)b << 5);
+}
+
+__device__ __forceinline__ uint32_t span_bx_mask(uint32_t span) {
+	uint32_t a = span & 31u, b = (span >> 5) & 31u;
+	return ((15u << (a >> 3)) & (15u >> (3u - (b >> 3)))) & 15u;
+}
+
+// Block-row record: 8 row spans (10 bits each) of one triangle in one 8-row band.
+//   x = triangle id, y = rows 0-2, z = rows 3-5, w = rows 6-7 | bx_mask << 20
+__device__ __forceinline__ uint32_t record_span(uint4 rec, int row) {
+	uint32_t w = row < 3 ? rec.y : row < 6 ? rec.z : rec.w;
+	int sh = (row < 3 ? row : row < 6 ? row - 3 : row - 6) * 10;
+	return (w >> sh) & 0x3ffu;
+}
+
+__device__ __forceinline__ uint32_t record_bx_mask(uint4 rec) { return (rec.w >> 20) & 15u; }
+
+// 64-bit coverage mask of block column bx (bit y*8 + x) from a record.
+__device__ __forceinline__ uint64_t record_block_mask(uint4 rec, int bx) {
+	uint64_t m = 0;
+	int x0 = bx * 8;
+#pragma unroll
+	for(int r = 0; r < 8; r++) {
+		uint32_t sp = record_span(rec, r);
+		int a = (int)(sp & 31u) - x0, b = (int)((sp >> 5) & 31u) - x0;
+		a = a < 0 ? 0 : a;
+		b = b > 7 ? 7 : b;
+		if(b >= a) {
+			uint32_t bits = ((1u << (b - a + 1)) - 1u) << a;
+			m |= (uint64_t)bits << (r * 8);
+		}
+	}
+	return m;
+}
+
+} // namespace lucid

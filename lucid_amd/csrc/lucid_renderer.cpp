@@ -1,0 +1,521 @@
+// Host driver of the MI355X Lucid rasterizer and its C ABI (include/lucid_c.h).
+//
+// Mirrors LucidRenderer (src/lucid_renderer.h:21-96, src/lucid_renderer.cpp:186-707):
+//   * construction sizes the bin grid from the view size (:186-232) and owns every internal
+//     device buffer (:238-313);
+//   * render() uploads instance data when it changes (:352-429), builds the per-frame
+//     constants from LucidConfig (:431-451) and enqueues the stage kernels on one stream
+//     (:319-350): clear, setup, binning (count, scan, categorise, dispatch), raster_low,
+//     raster_high, exact fallback;
+//   * verifyInfo/getStats read LucidInfo + per-bin counters back (:629-836).
+// No CUDA compatibility layer: plain HIP runtime calls for gfx950.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/lucid_c.h"
+#include "lucid_device.h"
+
+namespace lucid {
+hipError_t launch_setup(const FrameConst &c, const FrameBufs &b, uint32_t n_info,
+						hipStream_t stream);
+hipError_t launch_binning(const FrameConst &c, const FrameBufs &b, hipStream_t stream);
+hipError_t launch_raster(const FrameConst &c, const FrameBufs &b, hipStream_t stream);
+} // namespace lucid
+
+using namespace lucid;
+
+namespace {
+
+constexpr int kMaxWidth = 2560, kMaxHeight = 2048;   // src/lucid_renderer.h:27
+constexpr int kMaxInstances = 64 * 1024;            // src/lucid_renderer.h:28
+constexpr int64_t kMaxQuads = (int64_t)1 << 28;     // quad ids share a word with 4 flag bits
+constexpr int kNumBinWgs = 256;
+
+struct DevBuf {
+	void *ptr = nullptr;
+	size_t bytes = 0;
+	~DevBuf() { release(); }
+	void release() {
+		if(ptr)
+			(void)hipFree(ptr);
+		ptr = nullptr, bytes = 0;
+	}
+	// Grows (never shrinks); contents are not preserved.
+	hipError_t reserve(size_t need) {
+		if(need <= bytes)
+			return hipSuccess;
+		release();
+		hipError_t e = hipMalloc(&ptr, need);
+		if(e != hipSuccess) {
+			ptr = nullptr;
+			return e;
+		}
+		bytes = need;
+		return hipSuccess;
+	}
+	template <class T> T *as() const { return (T *)ptr; }
+};
+
+} // namespace
+
+struct lucid_renderer {
+	int device = 0;
+	int width = 0, height = 0;
+	uint32_t opts = 0;
+	int bin_count_x = 0, bin_count_y = 0, bin_count = 0;
+
+	DevBuf info, ctrl;
+	DevBuf instances, inst_prefix, inst_colors;
+	DevBuf quad_aabb, tri_scan, tri_depth, tri_bary, tri_normal, quad_color, quad_normal,
+		tri_color, large_list;
+	DevBuf wg_hist, bin_quads, bin_tris;
+	DevBuf low_records, high_records, fallback_queue, spill;
+
+	int64_t quad_cap = 0;
+	uint64_t bin_quads_cap = 0, bin_tris_cap = 0, high_records_cap = 0, spill_cap = 0;
+	uint32_t fallback_cap = 0;
+
+	std::vector<lucid_instance_data> host_instances;
+	std::vector<uint32_t> host_inst_colors;
+	bool inst_colors_null = true;
+
+	uint32_t *status = nullptr; // pinned copy of the control words of the last frame
+	hipEvent_t done = nullptr;
+	bool frame_pending = false;
+	int64_t last_num_quads = 0;
+	hipStream_t last_stream = nullptr;
+
+	~lucid_renderer() {
+		if(done)
+			(void)hipEventDestroy(done);
+		if(status)
+			(void)hipHostFree(status);
+	}
+};
+
+namespace {
+
+int hip_status(hipError_t e) {
+	if(e == hipSuccess)
+		return LUCID_OK;
+	if(e == hipErrorOutOfMemory)
+		return LUCID_ERR_OUT_OF_MEMORY;
+	return LUCID_ERR_HIP;
+}
+
+#define LUCID_CHECK(expr)                                                                          \
+	do {                                                                                           \
+		hipError_t _e = (expr);                                                                    \
+		if(_e != hipSuccess)                                                                       \
+			return hip_status(_e);                                                                 \
+	} while(0)
+
+int ensure_quad_capacity(lucid_renderer *r, int64_t nq) {
+	if(nq <= r->quad_cap)
+		return LUCID_OK;
+	int64_t cap = std::max<int64_t>(nq, std::max<int64_t>(r->quad_cap * 3 / 2, 1024));
+	int64_t tris = cap * 2;
+	LUCID_CHECK(r->quad_aabb.reserve(cap * 4));
+	LUCID_CHECK(r->tri_scan.reserve(tris * 32));
+	LUCID_CHECK(r->tri_depth.reserve(tris * 16));
+	LUCID_CHECK(r->tri_bary.reserve(tris * 32));
+	LUCID_CHECK(r->tri_normal.reserve(tris * 4));
+	LUCID_CHECK(r->quad_color.reserve(cap * 16));
+	LUCID_CHECK(r->quad_normal.reserve(cap * 16));
+	LUCID_CHECK(r->tri_color.reserve(tris * 48));
+	LUCID_CHECK(r->large_list.reserve(cap * 4));
+	// small quads cover at most 4 bins: 4 entries per quad is an exact bound
+	uint64_t bq = (uint64_t)cap * 4;
+	LUCID_CHECK(r->bin_quads.reserve(bq * 4));
+	r->bin_quads_cap = bq;
+	uint64_t bt = std::max<uint64_t>(r->bin_tris_cap, (uint64_t)tris * 2 + 65536);
+	LUCID_CHECK(r->bin_tris.reserve(bt * 4));
+	r->bin_tris_cap = bt;
+	r->quad_cap = cap;
+	return LUCID_OK;
+}
+
+int grow_after_overflow(lucid_renderer *r) {
+	uint32_t flags = r->status[kCtrlOverflow];
+	if(!flags)
+		return LUCID_OK;
+	if(flags & 1u) {
+		uint64_t need = std::max<uint64_t>(r->status[kCtrlTotalBinTris], r->bin_tris_cap * 2);
+		LUCID_CHECK(r->bin_tris.reserve(need * 4));
+		r->bin_tris_cap = need;
+	}
+	if(flags & 2u) {
+		uint64_t need = std::max<uint64_t>((uint64_t)r->status[kCtrlRecordCursor],
+										   r->high_records_cap * 2);
+		LUCID_CHECK(r->high_records.reserve(need * 16));
+		r->high_records_cap = need;
+	}
+	if(flags & 8u) {
+		uint64_t need = std::max<uint64_t>((uint64_t)r->status[kCtrlSpillCursor], r->spill_cap * 2);
+		LUCID_CHECK(r->spill.reserve(need * 8));
+		r->spill_cap = need;
+	}
+	return LUCID_ERR_CAPACITY;
+}
+
+void fill_const(const lucid_renderer *r, const lucid_config &cfg, int64_t nq, int ninst,
+				int color_mode, bool has_normals, FrameConst &c) {
+	memset(&c, 0, sizeof(c));
+	c.width = r->width, c.height = r->height;
+	c.bin_count_x = r->bin_count_x, c.bin_count_y = r->bin_count_y, c.bin_count = r->bin_count;
+	c.num_quads = (int)nq;
+	c.num_instances = ninst;
+	c.num_bin_wgs = kNumBinWgs;
+	c.opts = r->opts;
+	c.backface = cfg.enable_backface_culling;
+	c.color_mode = color_mode;
+	c.has_normals = has_normals ? 1 : 0;
+	for(int col = 0; col < 4; col++) {
+		const lucid_vec4 &v = cfg.view_proj_matrix.col[col];
+		c.vp[col * 4 + 0] = v.x, c.vp[col * 4 + 1] = v.y, c.vp[col * 4 + 2] = v.z,
+					   c.vp[col * 4 + 3] = v.w;
+	}
+	const lucid_frustum &f = cfg.frustum;
+	float o[3] = {f.ws_origin0.x, f.ws_origin0.y, f.ws_origin0.z};
+	float d0[3] = {f.ws_dir0.x, f.ws_dir0.y, f.ws_dir0.z};
+	float dx[3] = {f.ws_dirx.x, f.ws_dirx.y, f.ws_dirx.z};
+	float dy[3] = {f.ws_diry.x, f.ws_diry.y, f.ws_diry.z};
+	for(int i = 0; i < 3; i++) {
+		c.origin[i] = o[i], c.dir0[i] = d0[i], c.dirx[i] = dx[i], c.diry[i] = dy[i];
+		// quad_setup.glsl:397-398: ws_dir0 + (ws_dirx + ws_diry) * 0.5
+		c.ray_dir0[i] = d0[i] + (dx[i] + dy[i]) * 0.5f;
+	}
+	const lucid_lighting &l = cfg.lighting;
+	c.ambient[0] = l.ambient_color.x, c.ambient[1] = l.ambient_color.y, c.ambient[2] = l.ambient_color.z;
+	c.sun_color[0] = l.sun_color.x, c.sun_color[1] = l.sun_color.y, c.sun_color[2] = l.sun_color.z;
+	c.sun_dir[0] = l.sun_dir.x, c.sun_dir[1] = l.sun_dir.y, c.sun_dir[2] = l.sun_dir.z;
+	c.sun_power = l.sun_power, c.ambient_power = l.ambient_power;
+	c.background[0] = cfg.background_color.x, c.background[1] = cfg.background_color.y;
+	c.background[2] = cfg.background_color.z, c.background[3] = cfg.background_color.w;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *lucid_error_string(int status) {
+	switch(status) {
+	case LUCID_OK: return "ok";
+	case LUCID_ERR_INVALID_ARG: return "invalid argument";
+	case LUCID_ERR_OUT_OF_MEMORY: return "out of device memory";
+	case LUCID_ERR_HIP: return "HIP runtime error";
+	case LUCID_ERR_NO_DEVICE: return "no HIP device";
+	case LUCID_ERR_CAPACITY: return "internal capacity exceeded; buffers grown, render again";
+	case LUCID_ERR_UNSUPPORTED: return "unsupported input (albedo textures are not implemented)";
+	default: return "unknown error";
+	}
+}
+
+int lucid_create(const lucid_create_info *ci, lucid_renderer **out) {
+	if(!ci || !out || ci->view_width <= 0 || ci->view_height <= 0 || ci->view_width > kMaxWidth ||
+	   ci->view_height > kMaxHeight || ci->max_quads < 0 || ci->max_quads > kMaxQuads)
+		return LUCID_ERR_INVALID_ARG;
+	*out = nullptr;
+	int ndev = 0;
+	if(hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+		return LUCID_ERR_NO_DEVICE;
+	if(ci->device < 0 || ci->device >= ndev)
+		return LUCID_ERR_INVALID_ARG;
+	LUCID_CHECK(hipSetDevice(ci->device));
+	lucid_renderer *r = new(std::nothrow) lucid_renderer();
+	if(!r)
+		return LUCID_ERR_OUT_OF_MEMORY;
+	r->device = ci->device;
+	r->width = ci->view_width, r->height = ci->view_height;
+	r->opts = ci->opts;
+	r->bin_count_x = (r->width + kBinSize - 1) / kBinSize;
+	r->bin_count_y = (r->height + kBinSize - 1) / kBinSize;
+	r->bin_count = r->bin_count_x * r->bin_count_y;
+	const int NB = r->bin_count;
+	int rc = LUCID_OK;
+	auto chk = [&](hipError_t e) {
+		if(e != hipSuccess && rc == LUCID_OK)
+			rc = hip_status(e);
+	};
+	chk(r->info.reserve(((size_t)LUCID_INFO_U32_SIZE + 10 * (size_t)NB) * 4));
+	chk(r->ctrl.reserve(kCtrlWords * 4));
+	chk(r->wg_hist.reserve((size_t)kNumBinWgs * 2 * NB * 4));
+	chk(r->low_records.reserve((size_t)NB * 4 * kLowRowCap * 16));
+	r->fallback_cap = (uint32_t)NB * 16;
+	chk(r->fallback_queue.reserve((size_t)r->fallback_cap * 32));
+	r->high_records_cap = 1u << 22;
+	chk(r->high_records.reserve(r->high_records_cap * 16));
+	r->spill_cap = 1u << 20;
+	chk(r->spill.reserve(r->spill_cap * 8));
+	chk(hipHostMalloc((void **)&r->status, kCtrlWords * 4, hipHostMallocDefault));
+	chk(hipEventCreateWithFlags(&r->done, hipEventDisableTiming));
+	if(rc == LUCID_OK) {
+		int e = ensure_quad_capacity(r, ci->max_quads > 0 ? ci->max_quads : (1 << 20));
+		if(e != LUCID_OK)
+			rc = e;
+	}
+	if(rc != LUCID_OK) {
+		delete r;
+		return rc;
+	}
+	memset(r->status, 0, kCtrlWords * 4);
+	*out = r;
+	return LUCID_OK;
+}
+
+void lucid_destroy(lucid_renderer *r) {
+	if(!r)
+		return;
+	(void)hipSetDevice(r->device);
+	if(r->frame_pending)
+		(void)hipEventSynchronize(r->done);
+	delete r;
+}
+
+int lucid_render(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_frame_outputs *out,
+				 void *stream_ptr) {
+	if(!r || !in || !out || !in->config || in->num_instances < 0 ||
+	   in->num_instances > kMaxInstances || (in->num_instances > 0 && !in->instances))
+		return LUCID_ERR_INVALID_ARG;
+	hipStream_t stream = (hipStream_t)stream_ptr;
+	LUCID_CHECK(hipSetDevice(r->device));
+
+	// --- instances (uploadInstances, src/lucid_renderer.cpp:352-429)
+	std::vector<int> prefix(in->num_instances + 1, 0);
+	int64_t nq = 0;
+	bool has_colors_flag = false;
+	for(int i = 0; i < in->num_instances; i++) {
+		const lucid_instance_data &ins = in->instances[i];
+		if(ins.num_quads < 0 || ins.index_offset < 0)
+			return LUCID_ERR_INVALID_ARG;
+		if(ins.flags & LUCID_INST_HAS_ALBEDO_TEXTURE)
+			return LUCID_ERR_UNSUPPORTED;
+		if(ins.flags & LUCID_INST_HAS_VERTEX_COLORS)
+			has_colors_flag = true;
+		prefix[i] = (int)nq;
+		nq += ins.num_quads;
+		if(nq >= kMaxQuads)
+			return LUCID_ERR_INVALID_ARG;
+	}
+	prefix[in->num_instances] = (int)nq;
+	if(nq > 0 && (!in->quad_indices || !in->positions))
+		return LUCID_ERR_INVALID_ARG;
+	int color_mode = kColorNone;
+	if(in->colors_rgba8)
+		color_mode = kColorRGBA8;
+	else if(in->colors_rgb)
+		color_mode = kColorF32;
+	if(has_colors_flag && color_mode == kColorNone)
+		return LUCID_ERR_INVALID_ARG;
+
+	if(r->frame_pending) {
+		LUCID_CHECK(hipEventSynchronize(r->done));
+		r->frame_pending = false;
+		int g = grow_after_overflow(r);
+		if(g != LUCID_OK && g != LUCID_ERR_CAPACITY)
+			return g;
+		memset(r->status, 0, kCtrlWords * 4);
+	}
+	int e = ensure_quad_capacity(r, nq);
+	if(e != LUCID_OK)
+		return e;
+
+	bool inst_changed = r->host_instances.size() != (size_t)in->num_instances ||
+						(in->num_instances &&
+						 memcmp(r->host_instances.data(), in->instances,
+								sizeof(lucid_instance_data) * in->num_instances) != 0);
+	bool colors_null = in->instance_colors == nullptr;
+	bool colors_changed = colors_null != r->inst_colors_null ||
+						  (!colors_null &&
+						   (r->host_inst_colors.size() != (size_t)in->num_instances ||
+							memcmp(r->host_inst_colors.data(), in->instance_colors,
+								   4 * (size_t)in->num_instances) != 0));
+	if(inst_changed || colors_changed) {
+		size_t n = std::max(1, in->num_instances);
+		LUCID_CHECK(r->instances.reserve(n * sizeof(lucid_instance_data)));
+		LUCID_CHECK(r->inst_prefix.reserve((n + 1) * 4));
+		LUCID_CHECK(r->inst_colors.reserve(n * 4));
+		if(in->num_instances) {
+			LUCID_CHECK(hipMemcpy(r->instances.ptr, in->instances,
+								  sizeof(lucid_instance_data) * in->num_instances,
+								  hipMemcpyHostToDevice));
+			if(!colors_null)
+				LUCID_CHECK(hipMemcpy(r->inst_colors.ptr, in->instance_colors,
+									  4 * (size_t)in->num_instances, hipMemcpyHostToDevice));
+		}
+		LUCID_CHECK(hipMemcpy(r->inst_prefix.ptr, prefix.data(), 4 * prefix.size(),
+							  hipMemcpyHostToDevice));
+		r->host_instances.assign(in->instances, in->instances + in->num_instances);
+		r->inst_colors_null = colors_null;
+		if(!colors_null)
+			r->host_inst_colors.assign(in->instance_colors, in->instance_colors + in->num_instances);
+	}
+
+	FrameConst c;
+	fill_const(r, *in->config, nq, in->num_instances, color_mode, in->normals != nullptr, c);
+	FrameBufs b;
+	memset(&b, 0, sizeof(b));
+	b.instances = r->instances.as<lucid_instance_data>();
+	b.inst_prefix = r->inst_prefix.as<int>();
+	b.inst_colors = colors_null ? nullptr : r->inst_colors.as<uint32_t>();
+	b.quad_indices = in->quad_indices;
+	b.positions = in->positions;
+	b.colors_rgba8 = in->colors_rgba8;
+	b.colors_rgb = in->colors_rgb;
+	b.tri_alpha = in->tri_alpha;
+	b.normals = in->normals;
+	b.info = r->info.as<uint32_t>();
+	b.ctrl = r->ctrl.as<uint32_t>();
+	b.quad_aabb = r->quad_aabb.as<uint32_t>();
+	b.tri_scan = r->tri_scan.as<uint4>();
+	b.tri_depth = r->tri_depth.as<uint4>();
+	b.tri_bary = r->tri_bary.as<uint4>();
+	b.tri_normal = r->tri_normal.as<uint32_t>();
+	b.quad_color = r->quad_color.as<uint4>();
+	b.quad_normal = r->quad_normal.as<uint4>();
+	b.tri_color = r->tri_color.as<float4>();
+	b.large_list = r->large_list.as<uint32_t>();
+	b.wg_hist = r->wg_hist.as<uint32_t>();
+	b.bin_quads = r->bin_quads.as<uint32_t>();
+	b.bin_tris = r->bin_tris.as<uint32_t>();
+	b.bin_quads_cap = r->bin_quads_cap;
+	b.bin_tris_cap = r->bin_tris_cap;
+	b.low_records = r->low_records.as<uint4>();
+	b.high_records = r->high_records.as<uint4>();
+	b.high_records_cap = r->high_records_cap;
+	b.fallback_queue = r->fallback_queue.as<uint4>();
+	b.fallback_cap = r->fallback_cap;
+	b.spill = r->spill.as<uint64_t>();
+	b.spill_cap = r->spill_cap;
+	b.image = out->image_rgba8;
+	b.image_f = out->image_rgba32f;
+	b.frag_counts = out->frag_counts;
+
+	uint32_t n_info = LUCID_INFO_U32_SIZE + 10u * (uint32_t)r->bin_count;
+	LUCID_CHECK(launch_setup(c, b, n_info, stream));
+	LUCID_CHECK(launch_binning(c, b, stream));
+	LUCID_CHECK(launch_raster(c, b, stream));
+	LUCID_CHECK(hipMemcpyAsync(r->status, r->ctrl.ptr, kCtrlWords * 4, hipMemcpyDeviceToHost,
+							   stream));
+	LUCID_CHECK(hipEventRecord(r->done, stream));
+	r->frame_pending = true;
+	r->last_num_quads = nq;
+	r->last_stream = stream;
+	return LUCID_OK;
+}
+
+int lucid_finish(lucid_renderer *r) {
+	if(!r)
+		return LUCID_ERR_INVALID_ARG;
+	LUCID_CHECK(hipSetDevice(r->device));
+	if(!r->frame_pending)
+		return LUCID_OK;
+	LUCID_CHECK(hipEventSynchronize(r->done));
+	r->frame_pending = false;
+	int g = grow_after_overflow(r);
+	memset(r->status, 0, kCtrlWords * 4);
+	return g;
+}
+
+int lucid_read_info(lucid_renderer *r, uint32_t *dst, size_t num_u32) {
+	if(!r || !dst)
+		return LUCID_ERR_INVALID_ARG;
+	size_t n = (size_t)LUCID_INFO_U32_SIZE + 10 * (size_t)r->bin_count;
+	if(num_u32 < n)
+		return LUCID_ERR_INVALID_ARG;
+	LUCID_CHECK(hipSetDevice(r->device));
+	LUCID_CHECK(hipStreamSynchronize(r->last_stream));
+	LUCID_CHECK(hipMemcpy(dst, r->info.ptr, n * 4, hipMemcpyDeviceToHost));
+	return LUCID_OK;
+}
+
+int lucid_verify_info(lucid_renderer *r) {
+	if(!r)
+		return LUCID_ERR_INVALID_ARG;
+	std::vector<uint32_t> buf((size_t)LUCID_INFO_U32_SIZE + 10 * (size_t)r->bin_count);
+	int e = lucid_read_info(r, buf.data(), buf.size());
+	if(e != LUCID_OK)
+		return e;
+	const int NB = r->bin_count;
+	const uint32_t *cnt = buf.data() + LUCID_INFO_U32_SIZE;
+	int errors = 0;
+	// src/lucid_renderer.cpp:650-680
+	for(int k = 0; k < 2; k++) {
+		const uint32_t *counts = cnt + (k ? LUCID_COUNTS_TRI_COUNTS : LUCID_COUNTS_QUAD_COUNTS) * NB;
+		const uint32_t *offs = cnt + (k ? LUCID_COUNTS_TRI_OFFSETS : LUCID_COUNTS_QUAD_OFFSETS) * NB;
+		const uint32_t *temp =
+			cnt + (k ? LUCID_COUNTS_TRI_OFFSETS_TEMP : LUCID_COUNTS_QUAD_OFFSETS_TEMP) * NB;
+		for(int i = 0; i < NB; i++) {
+			if(i > 0 && offs[i] != offs[i - 1] + counts[i - 1])
+				errors++;
+			if(temp[i] != offs[i] + counts[i])
+				errors++;
+		}
+	}
+	return errors;
+}
+
+int lucid_read_bin_lists(lucid_renderer *r, uint32_t *quads, size_t quad_cap, uint32_t *tris,
+						 size_t tri_cap, size_t *num_quads, size_t *num_tris) {
+	if(!r)
+		return LUCID_ERR_INVALID_ARG;
+	LUCID_CHECK(hipSetDevice(r->device));
+	LUCID_CHECK(hipStreamSynchronize(r->last_stream));
+	uint32_t ctrl[kCtrlWords];
+	LUCID_CHECK(hipMemcpy(ctrl, r->ctrl.ptr, sizeof(ctrl), hipMemcpyDeviceToHost));
+	size_t nq = ctrl[kCtrlTotalBinQuads], nt = ctrl[kCtrlTotalBinTris];
+	if(num_quads)
+		*num_quads = nq;
+	if(num_tris)
+		*num_tris = nt;
+	if(ctrl[kCtrlOverflow] & 1u)
+		return LUCID_ERR_CAPACITY;
+	if(quads && quad_cap >= nq && nq)
+		LUCID_CHECK(hipMemcpy(quads, r->bin_quads.ptr, nq * 4, hipMemcpyDeviceToHost));
+	if(tris && tri_cap >= nt && nt)
+		LUCID_CHECK(hipMemcpy(tris, r->bin_tris.ptr, nt * 4, hipMemcpyDeviceToHost));
+	if((quads && quad_cap < nq) || (tris && tri_cap < nt))
+		return LUCID_ERR_INVALID_ARG;
+	return LUCID_OK;
+}
+
+int lucid_read_setup(lucid_renderer *r, uint32_t *tri_records, size_t num_tris,
+					 uint32_t *quad_aabbs, size_t num_quads) {
+	if(!r)
+		return LUCID_ERR_INVALID_ARG;
+	size_t q = (size_t)r->last_num_quads;
+	if((tri_records && num_tris < 2 * q) || (quad_aabbs && num_quads < q))
+		return LUCID_ERR_INVALID_ARG;
+	LUCID_CHECK(hipSetDevice(r->device));
+	LUCID_CHECK(hipStreamSynchronize(r->last_stream));
+	if(quad_aabbs && q)
+		LUCID_CHECK(hipMemcpy(quad_aabbs, r->quad_aabb.ptr, q * 4, hipMemcpyDeviceToHost));
+	if(tri_records && q) {
+		size_t t = 2 * q;
+		std::vector<uint32_t> scan(t * 8), depth(t * 4), bary(t * 8);
+		LUCID_CHECK(hipMemcpy(scan.data(), r->tri_scan.ptr, t * 32, hipMemcpyDeviceToHost));
+		LUCID_CHECK(hipMemcpy(depth.data(), r->tri_depth.ptr, t * 16, hipMemcpyDeviceToHost));
+		LUCID_CHECK(hipMemcpy(bary.data(), r->tri_bary.ptr, t * 32, hipMemcpyDeviceToHost));
+		for(size_t i = 0; i < t; i++) {
+			uint32_t *o = tri_records + i * 20;
+			memcpy(o, &bary[i * 8], 32);
+			memcpy(o + 8, &scan[i * 8], 32);
+			memcpy(o + 16, &depth[i * 4], 16);
+		}
+	}
+	return LUCID_OK;
+}
+
+int lucid_bin_count(const lucid_renderer *r) { return r ? r->bin_count : 0; }
+int lucid_bin_size(const lucid_renderer *r) { return r ? kBinSize : 0; }
+int lucid_block_size(const lucid_renderer *r) { return r ? kBlockSize : 0; }
+int64_t lucid_max_visible_quads(const lucid_renderer *r) { return r ? r->quad_cap : 0; }
+// src/lucid_renderer.h:49 allows 5/2 of the visible-quad arena; here ids index input quads.
+int64_t lucid_max_scene_quads(const lucid_renderer *r) { return r ? kMaxQuads - 1 : 0; }
+
+} // extern "C"

@@ -1,0 +1,365 @@
+"""Seeded synthetic scenes and LucidConfig bytes for the exact-OIT rasterizer.
+
+The reference builds ``LucidConfig`` from libfwk's Camera/Frustum
+(src/shading.cpp:46-74, src/lucid_renderer.cpp:431-451); libfwk is not
+available, so the camera model is fixed here and the same config bytes feed
+both the HIP path and the oracle:
+
+* world space == view space: the camera sits at the origin, looks down -Z,
+  +Y is up;
+* symmetric perspective with 90 degree vertical FOV, depth range
+  [1/16, 1024] (src/lucid_app.cpp:95), GL-style clip z in [-w, w] and a
+  flipped y row so that image row 0 is NDC y = -1 (top of the image);
+* corner rays are unnormalised with z = -1, so a plane's ``inv_ray_pos``
+  equals 1 / view depth; ``ws_dirx``/``ws_diry`` step one pixel
+  (src/shading.cpp:59-60);
+* identity lighting (ambient 1, sun power 0) so finalShading reduces to an
+  sRGB round trip (shared/funcs.glsl:261-271).
+
+Soup triangles become degenerate quads ``(a, b, c, c)`` exactly as the
+reference's quad generator emits unpaired triangles
+(src/quad_generator.cpp:192); their second triangle is culled by index
+(quad_setup.glsl:144). Draw calls are split into instances of at most 1024
+quads (src/lucid_renderer.cpp:352-429).
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import hashlib
+import math
+from typing import Optional
+
+import numpy as np
+
+# definitions.glsl:84-94
+INST_HAS_VERTEX_COLORS = 0x001
+INST_HAS_VERTEX_NORMALS = 0x004
+INST_HAS_COLOR = 0x200
+
+MAX_INSTANCE_QUADS = 1024  # src/lucid_renderer.h:29
+
+INSTANCE_DTYPE = np.dtype(
+    [("index_offset", "<i4"), ("vertex_offset", "<i4"), ("num_quads", "<i4"), ("flags", "<u4")]
+)
+
+CONFIG_BYTES = 352
+NEAR, FAR = 1.0 / 16.0, 1024.0
+
+
+@dataclasses.dataclass
+class Camera:
+    width: int
+    height: int
+    fov_y_deg: float = 90.0
+    near: float = NEAR
+    far: float = FAR
+
+    @property
+    def aspect(self) -> float:
+        return self.width / self.height
+
+    @property
+    def focal(self) -> float:
+        return 1.0 / math.tan(math.radians(self.fov_y_deg) * 0.5)
+
+    def view_proj(self) -> np.ndarray:
+        """Column-major 4x4 (cols[c][r]) projection with flipped y."""
+        f, a, n, fa = self.focal, self.aspect, self.near, self.far
+        m = np.zeros((4, 4), np.float32)  # m[col][row]
+        m[0][0] = f / a
+        m[1][1] = -f
+        m[2][2] = (fa + n) / (n - fa)
+        m[2][3] = -1.0
+        m[3][2] = 2.0 * fa * n / (n - fa)
+        return m
+
+    def ndc_to_view(self, ndc_xy: np.ndarray, depth: np.ndarray) -> np.ndarray:
+        """Points at view depth `depth` (positive) that project to NDC `ndc_xy`."""
+        f, a = self.focal, self.aspect
+        d = np.asarray(depth, np.float64)[..., None]
+        x = ndc_xy[..., 0:1] * (a / f) * d
+        y = -ndc_xy[..., 1:2] / f * d
+        return np.concatenate([x, y, -d], axis=-1)
+
+    def corner_dirs(self) -> np.ndarray:
+        """Ray directions at screen corners (x, y) = (0,0), (0,H), (W,H), (W,0)."""
+        f, a = np.float32(self.focal), np.float32(self.aspect)
+        af = np.float32(a / f)
+        inv_f = np.float32(1.0) / f
+        return np.array(
+            [[-af, inv_f, -1.0], [-af, -inv_f, -1.0], [af, -inv_f, -1.0], [af, inv_f, -1.0]],
+            np.float32,
+        )
+
+
+def make_config(
+    camera: Camera,
+    num_instances: int,
+    *,
+    ambient=(1.0, 1.0, 1.0),
+    ambient_power: float = 1.0,
+    sun_color=(1.0, 1.0, 1.0),
+    sun_dir=(0.0, 0.0, -1.0),
+    sun_power: float = 0.0,
+    background=(0.0, 0.0, 0.0, 1.0),
+    backface_culling: bool = False,
+    packet_size: int = 1,
+) -> bytes:
+    """LucidConfig in std140 layout (shared/structures.glsl:101-111)."""
+    buf = np.zeros(CONFIG_BYTES // 4, np.float32)
+    u32 = buf.view(np.uint32)
+    i32 = buf.view(np.int32)
+    dirs = camera.corner_dirs()
+    # Frustum: ws_origins[4] at 0, ws_dirs[4] at 16, origin0 at 32, dir0 at 36, dirx 40, diry 44
+    for i in range(4):
+        buf[4 * i : 4 * i + 4] = (0.0, 0.0, 0.0, 0.0)
+        buf[16 + 4 * i : 16 + 4 * i + 3] = dirs[i]
+    buf[32:36] = (0.0, 0.0, 0.0, 1.0)
+    buf[36:39] = dirs[0]
+    dirx = (dirs[3] - dirs[0]) * (np.float32(1.0) / np.float32(camera.width))
+    diry = (dirs[1] - dirs[0]) * (np.float32(1.0) / np.float32(camera.height))
+    buf[40:43] = dirx.astype(np.float32)
+    buf[44:47] = diry.astype(np.float32)
+    buf[48:64] = camera.view_proj().reshape(-1)
+    # Lighting at word 64: ambient_color, sun_color, sun_dir, sun_power, ambient_power
+    buf[64:68] = (*ambient, 1.0)
+    buf[68:72] = (*sun_color, 1.0)
+    buf[72:76] = (*sun_dir, 0.0)
+    buf[76] = sun_power
+    buf[77] = ambient_power
+    buf[80:84] = background
+    u32[84] = 1 if backface_culling else 0
+    i32[85] = num_instances
+    i32[86] = packet_size
+    return buf.tobytes()
+
+
+@dataclasses.dataclass
+class Scene:
+    """Host-side scene arrays in the layouts the C ABI takes."""
+
+    width: int
+    height: int
+    positions: np.ndarray  # (V, 3) float32
+    quad_indices: np.ndarray  # (Q, 4) uint32
+    instances: np.ndarray  # INSTANCE_DTYPE
+    config: bytes
+    colors_rgba8: Optional[np.ndarray] = None  # (V,) uint32
+    colors_rgb: Optional[np.ndarray] = None  # (V, 3) float32
+    tri_alpha: Optional[np.ndarray] = None  # (2Q,) float32, by triangle id
+    normals: Optional[np.ndarray] = None  # (V,) uint32 packed 10:10:10
+    instance_colors: Optional[np.ndarray] = None  # (N,) uint32
+    name: str = ""
+
+    @property
+    def num_quads(self) -> int:
+        return int(self.quad_indices.shape[0])
+
+    @property
+    def num_verts(self) -> int:
+        return int(self.positions.shape[0])
+
+    def input_bytes(self) -> int:
+        b = self.quad_indices.nbytes + self.positions.nbytes + self.instances.nbytes
+        for arr in (self.colors_rgba8, self.colors_rgb, self.tri_alpha, self.normals,
+                    self.instance_colors):
+            if arr is not None:
+                b += arr.nbytes
+        return int(b)
+
+    def digest(self) -> str:
+        h = hashlib.sha256()
+        for arr in (self.positions, self.quad_indices, self.instances, self.colors_rgba8,
+                    self.colors_rgb, self.tri_alpha, self.normals, self.instance_colors):
+            if arr is not None:
+                h.update(np.ascontiguousarray(arr).tobytes())
+        h.update(self.config)
+        return h.hexdigest()
+
+
+def split_instances(num_quads: int, flags: int, quad_offset: int = 0,
+                    vertex_offset: int = 0) -> np.ndarray:
+    n = (num_quads + MAX_INSTANCE_QUADS - 1) // MAX_INSTANCE_QUADS
+    inst = np.zeros(n, INSTANCE_DTYPE)
+    starts = np.arange(n, dtype=np.int64) * MAX_INSTANCE_QUADS
+    inst["index_offset"] = (starts + quad_offset) * 4
+    inst["vertex_offset"] = vertex_offset
+    inst["num_quads"] = np.minimum(MAX_INSTANCE_QUADS, num_quads - starts)
+    inst["flags"] = flags
+    return inst
+
+
+def pack_rgba8(rgb: np.ndarray, alpha: np.ndarray) -> np.ndarray:
+    c = np.concatenate([rgb, alpha[..., None]], axis=-1)
+    q = np.clip(np.floor(c.astype(np.float64) * 255.0 + 0.5), 0, 255).astype(np.uint32)
+    return q[..., 0] | (q[..., 1] << 8) | (q[..., 2] << 16) | (q[..., 3] << 24)
+
+
+def soup_scene(width: int, height: int, ndc_verts: np.ndarray, depth: np.ndarray,
+               rgb: np.ndarray, alpha: np.ndarray, *, color_format: str = "f32",
+               camera: Optional[Camera] = None, name: str = "", **config_kw) -> Scene:
+    """Triangle soup; each triangle owns its 3 vertices and becomes quad (a,b,c,c).
+
+    ndc_verts: (T,3,2) NDC positions; depth: (T,) or (T,3) view depths;
+    rgb: (T,3,3) per-vertex colour in [0,1]; alpha: (T,) per-triangle alpha.
+    """
+    cam = camera or Camera(width, height)
+    t = ndc_verts.shape[0]
+    depth = np.asarray(depth, np.float64)
+    if depth.ndim == 1:
+        depth = np.repeat(depth[:, None], 3, axis=1)
+    pos = cam.ndc_to_view(ndc_verts.astype(np.float64), depth).astype(np.float32)
+    positions = pos.reshape(-1, 3)
+    base = np.arange(t, dtype=np.uint32) * 3
+    quads = np.stack([base, base + 1, base + 2, base + 2], axis=1).astype(np.uint32)
+    flags = INST_HAS_VERTEX_COLORS
+    inst = split_instances(t, flags)
+    sc = Scene(width, height, positions, quads, inst,
+               make_config(cam, len(inst), **config_kw), name=name)
+    rgb = rgb.astype(np.float32).reshape(-1, 3)
+    if color_format == "rgba8":
+        sc.colors_rgba8 = pack_rgba8(rgb, np.repeat(alpha.astype(np.float32), 3))
+    else:
+        sc.colors_rgb = rgb
+        ta = np.zeros(2 * t, np.float32)
+        ta[0::2] = alpha
+        sc.tri_alpha = ta
+    return sc
+
+
+# ----------------------------------------------------------------------------- configs
+
+
+def config0(seed: int = 1000, num_tris: int = 10_000, width: int = 512, height: int = 512,
+            color_format: str = "f32") -> Scene:
+    """10k soup triangles: centroids uniform in NDC, vertex offsets N(0, 0.05) NDC,
+    view depth U[1,100] per triangle, RGB U[0,1] per vertex, alpha U[0.2,0.8]."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-1.0, 1.0, (num_tris, 1, 2))
+    e = rng.normal(0.0, 0.05, (num_tris, 3, 2))
+    depth = rng.uniform(1.0, 100.0, num_tris)
+    rgb = rng.uniform(0.0, 1.0, (num_tris, 3, 3))
+    alpha = rng.uniform(0.2, 0.8, num_tris)
+    return soup_scene(width, height, c + e, depth, rgb, alpha, color_format=color_format,
+                      name=f"c0_{num_tris}_{width}x{height}")
+
+
+def _lognormal_edges(rng, n, median_px, sigma, width):
+    lengths = median_px * np.exp(rng.normal(0.0, sigma, (n, 3)))
+    ang = rng.uniform(0.0, 2.0 * np.pi, (n, 3))
+    off = np.stack([np.cos(ang), np.sin(ang)], axis=-1) * lengths[..., None]
+    return off * (2.0 / width)  # pixels -> NDC (x and y use the same pixel scale below)
+
+
+def config1(seed: int = 1001, num_tris: int = 1_000_000, width: int = 1920, height: int = 1080,
+            num_blobs: int = 64, blob_sigma: float = 0.1, median_px: float = 6.0,
+            edge_sigma: float = 1.1, color_format: str = "f32") -> Scene:
+    """1M soup triangles clustered in 64 Gaussian blobs (sigma 0.1 NDC), log-normal
+    vertex offsets (median 6 px), view depth U[1,1000]."""
+    rng = np.random.default_rng(seed)
+    centers = rng.uniform(-0.8, 0.8, (num_blobs, 2))
+    which = rng.integers(0, num_blobs, num_tris)
+    c = centers[which] + rng.normal(0.0, blob_sigma, (num_tris, 2))
+    off = _lognormal_edges(rng, num_tris, median_px, edge_sigma, width)
+    off[..., 1] *= width / height  # same pixel length in y
+    ndc = c[:, None, :] + off
+    depth = rng.uniform(1.0, 1000.0, num_tris)
+    rgb = rng.uniform(0.0, 1.0, (num_tris, 3, 3))
+    alpha = rng.uniform(0.2, 0.8, num_tris)
+    return soup_scene(width, height, ndc, depth, rgb, alpha, color_format=color_format,
+                      name=f"c1_{num_tris}_{width}x{height}")
+
+
+def planes_scene(width: int, height: int, layers, color_format: str = "f32") -> Scene:
+    """Screen-covering fronto-parallel quads (the reference's #planes setup,
+    src/scene_setup.cpp:194-222, made screen-aligned). layers: list of
+    (depth, (r, g, b), alpha). Each layer is a proper 2-triangle quad."""
+    cam = Camera(width, height)
+    n = len(layers)
+    positions = []
+    quads = []
+    rgb = []
+    tri_alpha = np.zeros(2 * n, np.float32)
+    # A small rotation keeps every edge off the exact screen axes: an edge with a zero x
+    # slope divides by zero in the reference's scanline setup (quad_setup.glsl:325-327) and
+    # loses coverage; the reference's #planes setup tilts its camera for the same reason
+    # (src/scene_setup.cpp:216-218).
+    ang = 0.013
+    rot = np.array([[np.cos(ang), -np.sin(ang)], [np.sin(ang), np.cos(ang)]])
+    for i, (d, col, a) in enumerate(layers):
+        ndc = np.array([[-1.2, -1.2], [1.2, -1.2], [1.2, 1.2], [-1.2, 1.2]]) @ rot.T
+        p = cam.ndc_to_view(ndc, np.full(4, d)).astype(np.float32)
+        positions.append(p)
+        quads.append([4 * i, 4 * i + 1, 4 * i + 2, 4 * i + 3])
+        rgb.append(np.tile(np.asarray(col, np.float32), (4, 1)))
+        tri_alpha[2 * i] = tri_alpha[2 * i + 1] = a
+    positions = np.concatenate(positions).astype(np.float32)
+    inst = split_instances(n, INST_HAS_VERTEX_COLORS)
+    sc = Scene(width, height, positions, np.asarray(quads, np.uint32), inst,
+               make_config(cam, len(inst)), name=f"planes_{n}")
+    rgb = np.concatenate(rgb).astype(np.float32)
+    if color_format == "rgba8":
+        sc.colors_rgba8 = pack_rgba8(rgb, np.repeat(tri_alpha[0::2], 4))
+    else:
+        sc.colors_rgb = rgb
+        sc.tri_alpha = tri_alpha
+    return sc
+
+
+def quad_soup_scene(width: int, height: int, ndc: np.ndarray, depth: np.ndarray, rgb: np.ndarray,
+                    alpha: np.ndarray, *, color_format: str = "f32", name: str = "",
+                    **config_kw) -> Scene:
+    """Independent proper quads (2 triangles each): ndc (N,4,2), depth (N,4) view depths,
+    rgb (N,4,3), alpha (N,) per quad (both triangles)."""
+    cam = Camera(width, height)
+    n = ndc.shape[0]
+    pos = cam.ndc_to_view(ndc.astype(np.float64), depth.astype(np.float64)).astype(np.float32)
+    positions = pos.reshape(-1, 3)
+    base = np.arange(n, dtype=np.uint32) * 4
+    quads = np.stack([base, base + 1, base + 2, base + 3], axis=1).astype(np.uint32)
+    inst = split_instances(n, INST_HAS_VERTEX_COLORS)
+    sc = Scene(width, height, positions, quads, inst, make_config(cam, len(inst), **config_kw),
+               name=name)
+    rgbf = rgb.astype(np.float32).reshape(-1, 3)
+    if color_format == "rgba8":
+        sc.colors_rgba8 = pack_rgba8(rgbf, np.repeat(alpha.astype(np.float32), 4))
+    else:
+        sc.colors_rgb = rgbf
+        sc.tri_alpha = np.repeat(alpha.astype(np.float32), 2)
+    return sc
+
+
+def tilted_planes(width: int, height: int, n: int, seed: int = 7, extent: float = 1.1,
+                  color_format: str = "f32") -> Scene:
+    """n large quads with random depth slopes: per-pixel order changes across the screen,
+    so block-level pre-sorting is wrong in many places (exercises the exact fallback)."""
+    rng = np.random.default_rng(seed)
+    ang = 0.011  # keeps edges off the exact screen axes (see planes_scene)
+    rot = np.array([[np.cos(ang), -np.sin(ang)], [np.sin(ang), np.cos(ang)]])
+    corners = (np.array([[-1, -1], [1, -1], [1, 1], [-1, 1]], np.float64) * extent) @ rot.T
+    ndc = np.repeat(corners[None], n, axis=0)
+    z0 = rng.uniform(5.0, 15.0, (n, 1))
+    slope = rng.uniform(-4.0, 4.0, (n, 2))
+    depth = z0 + (corners[None, :, 0] * slope[:, 0:1] + corners[None, :, 1] * slope[:, 1:2])
+    rgb = rng.uniform(0.0, 1.0, (n, 4, 3))
+    alpha = rng.uniform(0.05, 0.3, n)
+    return quad_soup_scene(width, height, ndc, depth, rgb, alpha, color_format=color_format,
+                           name=f"tilted_{n}")
+
+
+def block_stack(width: int, height: int, n: int, center_px=(12, 12), size_px: float = 6.0,
+                seed: int = 3, color_format: str = "f32") -> Scene:
+    """n small soup triangles stacked over one 8x8 block at random depths."""
+    rng = np.random.default_rng(seed)
+    cx = (np.float64(center_px[0]) + 0.5) / width * 2 - 1
+    cy = (np.float64(center_px[1]) + 0.5) / height * 2 - 1
+    s = size_px / width * 2
+    base = np.array([[-s, -s], [s, -0.5 * s], [0.0, s]])
+    jitter = rng.uniform(-0.3, 0.3, (n, 3, 2)) * s
+    ndc = np.array([cx, cy])[None, None, :] + base[None] + jitter
+    depth = rng.uniform(2.0, 50.0, n)
+    rgb = rng.uniform(0.0, 1.0, (n, 3, 3))
+    alpha = rng.uniform(0.02, 0.2, n)
+    return soup_scene(width, height, ndc, depth, rgb, alpha, color_format=color_format,
+                      name=f"stack_{n}")

@@ -1,0 +1,161 @@
+"""ctypes front-end of the C oracle — TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, as the checker. The product (lucid_amd) never imports it.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liblucid_oracle.so")
+
+INFO_U32 = 1152
+MODE_EXACT, MODE_REF3 = 0, 1
+OPT_FP32_SAMPLES, OPT_ADDITIVE = 0x1, 0x2
+
+
+class _Inputs(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_int), ("height", ctypes.c_int),
+        ("config", ctypes.c_void_p),
+        ("instances", ctypes.c_void_p), ("num_instances", ctypes.c_int),
+        ("instance_colors", ctypes.c_void_p),
+        ("quad_indices", ctypes.c_void_p),
+        ("positions", ctypes.c_void_p), ("num_verts", ctypes.c_int),
+        ("colors_rgba8", ctypes.c_void_p), ("colors_rgb", ctypes.c_void_p),
+        ("tri_alpha", ctypes.c_void_p), ("normals", ctypes.c_void_p),
+        ("mode", ctypes.c_int), ("opts", ctypes.c_uint32),
+    ]
+
+
+class _Outputs(ctypes.Structure):
+    _fields_ = [
+        ("info", ctypes.c_void_p), ("quad_aabb", ctypes.c_void_p),
+        ("tri_records", ctypes.c_void_p), ("tri_normals", ctypes.c_void_p),
+        ("quad_colors", ctypes.c_void_p),
+        ("bin_quads", ctypes.c_void_p), ("bin_tris", ctypes.c_void_p),
+        ("bin_capacity", ctypes.c_size_t),
+        ("image", ctypes.c_void_p), ("image_f", ctypes.c_void_p),
+        ("frag_counts", ctypes.c_void_p), ("bin_levels", ctypes.c_void_p),
+        ("num_fragments", ctypes.c_uint64), ("num_ref_errors", ctypes.c_uint32),
+    ]
+
+
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.oracle_render.argtypes = [ctypes.POINTER(_Inputs), ctypes.POINTER(_Outputs)]
+        _lib.oracle_render.restype = ctypes.c_int
+        _lib.oracle_encode_aabb28.restype = ctypes.c_uint32
+        _lib.oracle_encode_aabb28.argtypes = [ctypes.c_uint32] * 4
+        _lib.oracle_encode_rgba8.restype = ctypes.c_uint32
+        _lib.oracle_encode_rgba8.argtypes = [ctypes.c_float] * 4
+        _lib.oracle_encode_normal_uint.restype = ctypes.c_uint32
+        _lib.oracle_encode_normal_uint.argtypes = [ctypes.c_float] * 3
+        _lib.oracle_f2i.restype = ctypes.c_int32
+        _lib.oracle_f2i.argtypes = [ctypes.c_float]
+        _lib.oracle_f2u.restype = ctypes.c_uint32
+        _lib.oracle_f2u.argtypes = [ctypes.c_float]
+        _lib.oracle_num_threads.restype = ctypes.c_int
+    return _lib
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data
+
+
+def bin_count(width: int, height: int) -> int:
+    return ((width + 31) // 32) * ((height + 31) // 32)
+
+
+def render(scene, mode: int = MODE_EXACT, opts: int = 0, bin_capacity: Optional[int] = None,
+           want_records: bool = True) -> dict:
+    """Runs the oracle on a lucid_amd.scenes.Scene; returns numpy outputs."""
+    L = lib()
+    w, h = scene.width, scene.height
+    nb = bin_count(w, h)
+    q = scene.num_quads
+    if bin_capacity is None:
+        bin_capacity = max(1024, 8 * q)
+    keep = []
+
+    def c(a, dtype):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=dtype)
+        keep.append(a)
+        return a
+
+    cfg = np.frombuffer(scene.config, np.uint8).copy()
+    keep.append(cfg)
+    inp = _Inputs(
+        w, h, cfg.ctypes.data,
+        _ptr(c(scene.instances, scene.instances.dtype)), len(scene.instances),
+        _ptr(c(scene.instance_colors, np.uint32)),
+        _ptr(c(scene.quad_indices, np.uint32)),
+        _ptr(c(scene.positions, np.float32)), scene.num_verts,
+        _ptr(c(scene.colors_rgba8, np.uint32)), _ptr(c(scene.colors_rgb, np.float32)),
+        _ptr(c(scene.tri_alpha, np.float32)), _ptr(c(scene.normals, np.uint32)),
+        mode, opts,
+    )
+    out_np = {
+        "info": np.zeros(INFO_U32 + 10 * nb, np.uint32),
+        "quad_aabb": np.zeros(q, np.uint32),
+        "tri_records": np.zeros((2 * q, 20) if want_records else (0, 20), np.uint32),
+        "tri_normals": np.zeros(2 * q, np.uint32),
+        "quad_colors": np.zeros((q, 4), np.uint32),
+        "bin_quads": np.zeros(bin_capacity, np.uint32),
+        "bin_tris": np.zeros(bin_capacity, np.uint32),
+        "image": np.zeros((h, w, 4), np.uint8),
+        "image_f": np.zeros((h, w, 4), np.float32),
+        "frag_counts": np.zeros((h, w), np.uint32),
+        "bin_levels": np.zeros(nb, np.uint32),
+    }
+    out = _Outputs(
+        out_np["info"].ctypes.data, out_np["quad_aabb"].ctypes.data,
+        out_np["tri_records"].ctypes.data if want_records else None,
+        out_np["tri_normals"].ctypes.data, out_np["quad_colors"].ctypes.data,
+        out_np["bin_quads"].ctypes.data, out_np["bin_tris"].ctypes.data, bin_capacity,
+        out_np["image"].ctypes.data, out_np["image_f"].ctypes.data,
+        out_np["frag_counts"].ctypes.data, out_np["bin_levels"].ctypes.data, 0, 0,
+    )
+    rc = L.oracle_render(ctypes.byref(inp), ctypes.byref(out))
+    if rc != 0:
+        raise RuntimeError(f"oracle_render failed: {rc}")
+    out_np["num_fragments"] = int(out.num_fragments)
+    out_np["num_ref_errors"] = int(out.num_ref_errors)
+    out_np["bin_count"] = nb
+    return out_np
+
+
+def counts(info: np.ndarray, nb: int, k: int) -> np.ndarray:
+    return info[INFO_U32 + k * nb: INFO_U32 + (k + 1) * nb]
+
+
+def bin_lists(res: dict) -> tuple:
+    """Per-bin sorted quad/tri lists as tuples of numpy arrays."""
+    nb = res["bin_count"]
+    info = res["info"]
+    qc, qo = counts(info, nb, 0), counts(info, nb, 1)
+    tc, to = counts(info, nb, 3), counts(info, nb, 4)
+    ql = [np.sort(res["bin_quads"][qo[b]:qo[b] + qc[b]]) for b in range(nb)]
+    tl = [np.sort(res["bin_tris"][to[b]:to[b] + tc[b]]) for b in range(nb)]
+    return ql, tl

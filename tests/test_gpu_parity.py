@@ -1,0 +1,177 @@
+"""GPU parity: the HIP path (through the C ABI) against the C oracle.
+
+Bar: bin counts, per-bin lists (as multisets) and per-pixel fragment
+counts bit-exact; blended image within 1/255 per channel (exact per-pixel order, ties by
+triangle id). Scenes cover the reference's edge cases: empty frame, culled/between-sample
+quads, low bins promoted to high (> 256 tris in an 8x8 block), high bins (>= 1024 tris),
+blocks too large for LDS, per-pixel ordering violations (exact fallback) and pixels deeper
+than the fallback's LDS capacity (global spill).
+"""
+
+import numpy as np
+import pytest
+
+from lucid_amd import scenes
+from lucid_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+INFO_U32 = 1152
+
+
+def _render_both(sc, oracle_mod, opts=0):
+    from lucid_amd.renderer import GpuScene, LucidRenderer
+    print(f"\n[case] {sc.name} {sc.width}x{sc.height} quads={sc.num_quads} opts={opts:#x}",
+          flush=True)
+    r = LucidRenderer((sc.width, sc.height), opts=opts)
+    out = r.render(GpuScene(sc), r.alloc_outputs(rgba8=True, rgba32f=True, counts=True))
+    gpu = {
+        "image": out["image"].cpu().numpy(),
+        "image_f": out["image_f"].cpu().numpy(),
+        "frag_counts": out["frag_counts"].cpu().numpy().astype(np.uint32),
+        "info": r.read_info(),
+        "verify": r.verify_info(),
+    }
+    gpu["bin_quads"], gpu["bin_tris"] = r.read_bin_lists()
+    gpu["stats"] = r.get_stats()
+    oopts = oracle_mod.OPT_FP32_SAMPLES if opts & _lib.OPT_FP32_SAMPLES else 0
+    if opts & _lib.OPT_ADDITIVE_BLENDING:
+        oopts |= oracle_mod.OPT_ADDITIVE
+    ref = oracle_mod.render(sc, opts=oopts)
+    return r, gpu, ref
+
+
+def _check(sc, gpu, ref, oracle_mod, max_diff=1):
+    nb = ref["bin_count"]
+    ginfo, rinfo = gpu["info"], ref["info"]
+    # LucidInfo counters restated by the oracle
+    assert ginfo[0] == rinfo[0], "num_input_quads"
+    assert ginfo[1] == rinfo[1] and ginfo[2] == rinfo[2], "num_visible_quads (small, large)"
+    assert np.array_equal(ginfo[32:36], rinfo[32:36]), "num_rejected_quads"
+    assert np.array_equal(ginfo[5:10], rinfo[5:10]), "bin_level_counts (incl. promotions)"
+    # per-bin counts and offsets
+    for k in range(6):
+        g = ginfo[INFO_U32 + k * nb: INFO_U32 + (k + 1) * nb]
+        o = rinfo[INFO_U32 + k * nb: INFO_U32 + (k + 1) * nb]
+        assert np.array_equal(g, o), f"g_counts array {k}"
+    assert gpu["verify"] == 0, "verifyInfo invariants"
+    # per-bin lists as multisets
+    qc, qo = ginfo[INFO_U32: INFO_U32 + nb], ginfo[INFO_U32 + nb: INFO_U32 + 2 * nb]
+    tc, to = ginfo[INFO_U32 + 3 * nb: INFO_U32 + 4 * nb], ginfo[INFO_U32 + 4 * nb: INFO_U32 + 5 * nb]
+    for b in range(nb):
+        gq = np.sort(gpu["bin_quads"][qo[b]: qo[b] + qc[b]])
+        rq = np.sort(ref["bin_quads"][qo[b]: qo[b] + qc[b]])
+        assert np.array_equal(gq, rq), f"bin {b} quad list"
+        gt = np.sort(gpu["bin_tris"][to[b]: to[b] + tc[b]])
+        rt = np.sort(ref["bin_tris"][to[b]: to[b] + tc[b]])
+        assert np.array_equal(gt, rt), f"bin {b} tri list"
+    # per-pixel fragment counts, bit-exact
+    mism = np.argwhere(gpu["frag_counts"] != ref["frag_counts"])
+    assert mism.size == 0, f"{len(mism)} pixels with different fragment counts, first {mism[:5]}"
+    assert int(ginfo[60]) == ref["num_fragments"], "stats[0] (fragments)"
+    # colour within 1/255 (tolerance in u8 steps)
+    d = np.abs(gpu["image"].astype(np.int32) - ref["image"].astype(np.int32))
+    assert d.max() <= max_diff, f"max colour diff {d.max()}/255 at {np.argwhere(d == d.max())[:3]}"
+    df = np.abs(gpu["image_f"] - ref["image_f"]).max()
+    assert df <= 1.0 / 255.0, f"float image diff {df}"
+    return d
+
+
+def test_c0_small(oracle_mod):
+    sc = scenes.config0(num_tris=2000, width=128, height=128)
+    r, gpu, ref = _render_both(sc, oracle_mod)
+    _check(sc, gpu, ref, oracle_mod)
+
+
+def test_c0_full_f32(oracle_mod):
+    sc = scenes.config0()
+    r, gpu, ref = _render_both(sc, oracle_mod)
+    d = _check(sc, gpu, ref, oracle_mod)
+    print(f"[c0] fragments {ref['num_fragments']} exact-colour pixels "
+          f"{(d.max(-1) == 0).mean():.6f}", flush=True)
+
+
+def test_c0_full_rgba8(oracle_mod):
+    sc = scenes.config0(color_format="rgba8")
+    r, gpu, ref = _render_both(sc, oracle_mod)
+    _check(sc, gpu, ref, oracle_mod)
+
+
+def test_c0_fp32_samples(oracle_mod):
+    sc = scenes.config0(num_tris=3000, width=256, height=192)
+    r, gpu, ref = _render_both(sc, oracle_mod, opts=_lib.OPT_FP32_SAMPLES)
+    _check(sc, gpu, ref, oracle_mod)
+
+
+def test_additive(oracle_mod):
+    sc = scenes.config0(num_tris=1500, width=128, height=96)
+    r, gpu, ref = _render_both(sc, oracle_mod, opts=_lib.OPT_ADDITIVE_BLENDING)
+    _check(sc, gpu, ref, oracle_mod)
+
+
+def test_empty_frame(oracle_mod):
+    sc = scenes.config0(num_tris=0, width=64, height=64)
+    r, gpu, ref = _render_both(sc, oracle_mod)
+    _check(sc, gpu, ref, oracle_mod)
+    assert (gpu["frag_counts"] == 0).all()
+
+
+def test_planes_closed_form(oracle_mod):
+    layers = [(3.0, (1.0, 0.0, 0.0), 0.5), (7.0, (0.0, 1.0, 0.0), 0.25),
+              (5.0, (0.0, 0.0, 1.0), 0.75)]
+    sc = scenes.planes_scene(96, 64, layers)
+    r, gpu, ref = _render_both(sc, oracle_mod, opts=_lib.OPT_FP32_SAMPLES)
+    _check(sc, gpu, ref, oracle_mod)
+    # nearest first: red (3), blue (5), green (7)
+    c = np.zeros(3)
+    t = 1.0
+    for _, col, a in sorted(layers):
+        c += np.array(col) * a * t
+        t *= 1 - a
+    got = gpu["image_f"][20, 70, :3]
+    assert np.allclose(got, c, atol=2e-3), (got, c)
+    assert np.median(gpu["frag_counts"]) == 3
+
+
+def test_promotion_low_to_high(oracle_mod):
+    sc = scenes.block_stack(64, 64, 300)
+    r, gpu, ref = _render_both(sc, oracle_mod)
+    _check(sc, gpu, ref, oracle_mod)
+    assert gpu["stats"]["promoted_bins"] >= 1
+
+
+def test_high_bin(oracle_mod):
+    sc = scenes.block_stack(96, 64, 700, center_px=(40, 20), size_px=10.0)
+    r, gpu, ref = _render_both(sc, oracle_mod)
+    _check(sc, gpu, ref, oracle_mod)
+    assert gpu["stats"]["high_bins"] >= 1
+
+
+def test_block_over_lds_capacity(oracle_mod):
+    sc = scenes.block_stack(64, 64, 1300, center_px=(20, 20), size_px=5.0, seed=11)
+    r, gpu, ref = _render_both(sc, oracle_mod)
+    _check(sc, gpu, ref, oracle_mod)
+
+
+def test_ordering_violations_fallback(oracle_mod):
+    sc = scenes.tilted_planes(128, 96, 24)
+    r, gpu, ref = _render_both(sc, oracle_mod)
+    _check(sc, gpu, ref, oracle_mod)
+
+
+def test_deep_pixels_spill(oracle_mod):
+    sc = scenes.tilted_planes(48, 40, 600, seed=5)
+    r, gpu, ref = _render_both(sc, oracle_mod)
+    _check(sc, gpu, ref, oracle_mod)
+    assert int(ref["frag_counts"].max()) > 512
+
+
+def test_determinism(oracle_mod):
+    from lucid_amd.renderer import GpuScene, LucidRenderer
+    sc = scenes.tilted_planes(64, 64, 40, seed=9)
+    r = LucidRenderer((sc.width, sc.height))
+    g = GpuScene(sc)
+    a = r.render(g, r.alloc_outputs(rgba32f=True))
+    fa = a["image_f"].cpu().numpy().copy()
+    b = r.render(g, r.alloc_outputs(rgba32f=True))
+    assert np.array_equal(fa, b["image_f"].cpu().numpy())
