@@ -110,6 +110,13 @@ int lucid_read_bin_lists(lucid_renderer *r, uint32_t *quads, size_t quad_cap, ui
 int lucid_read_setup(lucid_renderer *r, uint32_t *tri_records, size_t num_tris,
 					 uint32_t *quad_aabbs, size_t num_quads);
 
+/* Per-stage GPU times (ms) of the last frame, measured with HIP events on the frame's
+ * stream when the renderer was created with LUCID_OPT_TIMERS; replaces the reference's
+ * in-shader timers (LucidRenderOpt::timers, src/lucid_renderer.cpp:582-596,754-762).
+ * Stages: 0 setup, 1 binning, 2 raster_low, 3 raster_high, 4 exact fallback.
+ * Returns the number of stages written, or a negative status. */
+int lucid_stage_times(lucid_renderer *r, float *ms, int max_stages);
+
 int lucid_bin_count(const lucid_renderer *r);
 int lucid_bin_size(const lucid_renderer *r);
 int lucid_block_size(const lucid_renderer *r);

@@ -21,6 +21,7 @@ LUCID_ERR_NO_DEVICE = -4
 LUCID_ERR_CAPACITY = -5
 LUCID_ERR_UNSUPPORTED = -6
 
+OPT_TIMERS = 1 << 4
 OPT_ADDITIVE_BLENDING = 1 << 5
 OPT_VISUALIZE_ERRORS = 1 << 6
 OPT_ALPHA_THRESHOLD = 1 << 7
@@ -32,6 +33,7 @@ INFO_U32 = 1152
 EXPORTED = (
     "lucid_error_string", "lucid_create", "lucid_destroy", "lucid_render", "lucid_finish",
     "lucid_read_info", "lucid_verify_info", "lucid_read_bin_lists", "lucid_read_setup",
+    "lucid_stage_times",
     "lucid_bin_count", "lucid_bin_size", "lucid_block_size", "lucid_max_visible_quads",
     "lucid_max_scene_quads",
 )
@@ -107,6 +109,8 @@ def load(path: str = LIB_PATH):
                                          ctypes.POINTER(sz)]
     lib.lucid_read_setup.restype = i32
     lib.lucid_read_setup.argtypes = [vp, vp, sz, vp, sz]
+    lib.lucid_stage_times.restype = i32
+    lib.lucid_stage_times.argtypes = [vp, vp, i32]
     for name in ("lucid_bin_count", "lucid_bin_size", "lucid_block_size"):
         getattr(lib, name).restype = i32
         getattr(lib, name).argtypes = [vp]

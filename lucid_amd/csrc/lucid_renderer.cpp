@@ -24,7 +24,8 @@ namespace lucid {
 hipError_t launch_setup(const FrameConst &c, const FrameBufs &b, uint32_t n_info,
 						hipStream_t stream);
 hipError_t launch_binning(const FrameConst &c, const FrameBufs &b, hipStream_t stream);
-hipError_t launch_raster(const FrameConst &c, const FrameBufs &b, hipStream_t stream);
+hipError_t launch_raster(const FrameConst &c, const FrameBufs &b, hipStream_t stream,
+						 hipEvent_t *events);
 } // namespace lucid
 
 using namespace lucid;
@@ -86,6 +87,9 @@ struct lucid_renderer {
 
 	uint32_t *status = nullptr; // pinned copy of the control words of the last frame
 	hipEvent_t done = nullptr;
+	static constexpr int kStages = 5;
+	hipEvent_t stage_ev[kStages + 1] = {};
+	bool timers = false;
 	bool frame_pending = false;
 	int64_t last_num_quads = 0;
 	hipStream_t last_stream = nullptr;
@@ -93,6 +97,9 @@ struct lucid_renderer {
 	~lucid_renderer() {
 		if(done)
 			(void)hipEventDestroy(done);
+		for(hipEvent_t e : stage_ev)
+			if(e)
+				(void)hipEventDestroy(e);
 		if(status)
 			(void)hipHostFree(status);
 	}
@@ -254,6 +261,10 @@ int lucid_create(const lucid_create_info *ci, lucid_renderer **out) {
 	chk(r->spill.reserve(r->spill_cap * 8));
 	chk(hipHostMalloc((void **)&r->status, kCtrlWords * 4, hipHostMallocDefault));
 	chk(hipEventCreateWithFlags(&r->done, hipEventDisableTiming));
+	r->timers = (ci->opts & LUCID_OPT_TIMERS) != 0;
+	if(r->timers)
+		for(hipEvent_t &e : r->stage_ev)
+			chk(hipEventCreate(&e));
 	if(rc == LUCID_OK) {
 		int e = ensure_quad_capacity(r, ci->max_quads > 0 ? ci->max_quads : (1 << 20));
 		if(e != LUCID_OK)
@@ -313,8 +324,8 @@ int lucid_render(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_fr
 	if(has_colors_flag && color_mode == kColorNone)
 		return LUCID_ERR_INVALID_ARG;
 
-	if(r->frame_pending) {
-		LUCID_CHECK(hipEventSynchronize(r->done));
+	// A finished earlier frame may have reported an overflow: grow before this one.
+	if(r->frame_pending && hipEventQuery(r->done) == hipSuccess) {
 		r->frame_pending = false;
 		int g = grow_after_overflow(r);
 		if(g != LUCID_OK && g != LUCID_ERR_CAPACITY)
@@ -397,9 +408,15 @@ int lucid_render(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_fr
 	b.frag_counts = out->frag_counts;
 
 	uint32_t n_info = LUCID_INFO_U32_SIZE + 10u * (uint32_t)r->bin_count;
+	if(r->timers)
+		LUCID_CHECK(hipEventRecord(r->stage_ev[0], stream));
 	LUCID_CHECK(launch_setup(c, b, n_info, stream));
+	if(r->timers)
+		LUCID_CHECK(hipEventRecord(r->stage_ev[1], stream));
 	LUCID_CHECK(launch_binning(c, b, stream));
-	LUCID_CHECK(launch_raster(c, b, stream));
+	if(r->timers)
+		LUCID_CHECK(hipEventRecord(r->stage_ev[2], stream));
+	LUCID_CHECK(launch_raster(c, b, stream, r->timers ? r->stage_ev + 3 : nullptr));
 	LUCID_CHECK(hipMemcpyAsync(r->status, r->ctrl.ptr, kCtrlWords * 4, hipMemcpyDeviceToHost,
 							   stream));
 	LUCID_CHECK(hipEventRecord(r->done, stream));
@@ -509,6 +526,19 @@ int lucid_read_setup(lucid_renderer *r, uint32_t *tri_records, size_t num_tris,
 		}
 	}
 	return LUCID_OK;
+}
+
+int lucid_stage_times(lucid_renderer *r, float *ms, int max_stages) {
+	if(!r || !ms || max_stages < 0)
+		return LUCID_ERR_INVALID_ARG;
+	if(!r->timers)
+		return 0;
+	LUCID_CHECK(hipSetDevice(r->device));
+	LUCID_CHECK(hipEventSynchronize(r->stage_ev[lucid_renderer::kStages]));
+	int n = std::min(max_stages, (int)lucid_renderer::kStages);
+	for(int i = 0; i < n; i++)
+		LUCID_CHECK(hipEventElapsedTime(&ms[i], r->stage_ev[i], r->stage_ev[i + 1]));
+	return n;
 }
 
 int lucid_bin_count(const lucid_renderer *r) { return r ? r->bin_count : 0; }

@@ -923,29 +923,45 @@ __global__ __launch_bounds__(kRasterThreads) void fallback_kernel(FrameConst c, 
 		frags += n;
 	}
 	if(lane == 0 && pixels) {
+		lucid_info *info = (lucid_info *)b.info;
 		atomicAdd(&b.ctrl[kCtrlFallbackPixels], pixels);
-		if(whole)
-			atomicAdd(&((lucid_info *)b.info)->stats[0], frags);
+		// stats[2]: pixels resolved by the exact per-pixel sort (the reference counts the
+		// pixels its 3-deep window gets wrong there, shading.glsl:258-265)
+		atomicAdd(&info->stats[2], pixels);
+		if(whole) {
+			atomicAdd(&info->stats[0], frags);
+			if(w == 0)
+				atomicAdd(&info->stats[3], 1u); // stats[3]: blocks too large for LDS
+		}
 	}
 }
 
 // ------------------------------------------------------------------ launch
 
 template <bool FP32>
-static void launch_raster_t(const FrameConst &c, const FrameBufs &b, hipStream_t stream) {
+static void launch_raster_t(const FrameConst &c, const FrameBufs &b, hipStream_t stream,
+							hipEvent_t *ev) {
 	hipLaunchKernelGGL(raster_low_kernel<FP32>, dim3(c.bin_count), dim3(kRasterThreads), 0, stream,
 					   c, b);
+	if(ev)
+		(void)hipEventRecord(ev[0], stream);
 	hipLaunchKernelGGL(raster_high_kernel<FP32>, dim3(c.bin_count * 4), dim3(kRasterThreads), 0,
 					   stream, c, b);
+	if(ev)
+		(void)hipEventRecord(ev[1], stream);
 	hipLaunchKernelGGL(fallback_kernel<FP32>, dim3(b.fallback_cap), dim3(kRasterThreads), 0, stream,
 					   c, b);
+	if(ev)
+		(void)hipEventRecord(ev[2], stream);
 }
 
-hipError_t launch_raster(const FrameConst &c, const FrameBufs &b, hipStream_t stream) {
+// events (optional): recorded after raster_low, raster_high and the fallback kernel.
+hipError_t launch_raster(const FrameConst &c, const FrameBufs &b, hipStream_t stream,
+						 hipEvent_t *events) {
 	if(c.opts & (1u << 16))
-		launch_raster_t<true>(c, b, stream);
+		launch_raster_t<true>(c, b, stream, events);
 	else
-		launch_raster_t<false>(c, b, stream);
+		launch_raster_t<false>(c, b, stream, events);
 	return hipGetLastError();
 }
 

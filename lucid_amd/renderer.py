@@ -171,6 +171,16 @@ class LucidRenderer:
                                              aabb.ctypes.data, num_quads), "lucid_read_setup")
         return recs, aabb
 
+    STAGES = ("setup", "binning", "raster_low", "raster_high", "fallback")
+
+    def stage_times(self) -> dict:
+        """Per-stage GPU ms of the last frame (needs opts & OPT_TIMERS)."""
+        ms = np.zeros(len(self.STAGES), np.float32)
+        n = self.lib.lucid_stage_times(self.handle, ms.ctypes.data, len(ms))
+        if n < 0:
+            _lib.check(n, "lucid_stage_times")
+        return {name: float(ms[i]) for i, name in enumerate(self.STAGES[:n])}
+
     def get_stats(self) -> dict:
         """Counters of the last frame (src/lucid_renderer.cpp:709-836)."""
         info = self.read_info()
@@ -191,4 +201,5 @@ class LucidRenderer:
             "high_bins": int(levels[BIN_LEVEL_HIGH]),
             "promoted_bins": int(levels.sum() - nb),
             "fragments": int(info[60]), "block_tris": int(info[61]),
+            "fallback_pixels": int(info[62]), "oversized_blocks": int(info[63]),
         }

@@ -232,12 +232,13 @@ def soup_scene(width: int, height: int, ndc_verts: np.ndarray, depth: np.ndarray
 
 
 def config0(seed: int = 1000, num_tris: int = 10_000, width: int = 512, height: int = 512,
-            color_format: str = "f32") -> Scene:
-    """10k soup triangles: centroids uniform in NDC, vertex offsets N(0, 0.05) NDC,
+            color_format: str = "f32", vertex_sigma: float = 0.0615) -> Scene:
+    """10k soup triangles: centroids uniform in NDC, Gaussian vertex offsets in NDC (sigma
+    calibrated to a mean depth complexity of ~8 fragments/pixel at 10k tris / 512x512),
     view depth U[1,100] per triangle, RGB U[0,1] per vertex, alpha U[0.2,0.8]."""
     rng = np.random.default_rng(seed)
     c = rng.uniform(-1.0, 1.0, (num_tris, 1, 2))
-    e = rng.normal(0.0, 0.05, (num_tris, 3, 2))
+    e = rng.normal(0.0, vertex_sigma, (num_tris, 3, 2))
     depth = rng.uniform(1.0, 100.0, num_tris)
     rgb = rng.uniform(0.0, 1.0, (num_tris, 3, 3))
     alpha = rng.uniform(0.2, 0.8, num_tris)
@@ -254,12 +255,16 @@ def _lognormal_edges(rng, n, median_px, sigma, width):
 
 def config1(seed: int = 1001, num_tris: int = 1_000_000, width: int = 1920, height: int = 1080,
             num_blobs: int = 64, blob_sigma: float = 0.1, median_px: float = 6.0,
-            edge_sigma: float = 1.1, color_format: str = "f32") -> Scene:
-    """1M soup triangles clustered in 64 Gaussian blobs (sigma 0.1 NDC), log-normal
-    vertex offsets (median 6 px), view depth U[1,1000]."""
+            edge_sigma: float = 1.03, blob_weight_sigma: float = 2.3,
+            color_format: str = "f32") -> Scene:
+    """1M soup triangles clustered in 64 Gaussian blobs (sigma 0.1 NDC) with log-normal
+    populations, log-normal vertex offsets (median 6 px), view depth U[1,1000]. The two
+    log-normal sigmas are calibrated to a mean depth complexity of ~32 and a maximum of
+    ~500 fragments/pixel."""
     rng = np.random.default_rng(seed)
     centers = rng.uniform(-0.8, 0.8, (num_blobs, 2))
-    which = rng.integers(0, num_blobs, num_tris)
+    weights = np.exp(rng.normal(0.0, blob_weight_sigma, num_blobs))
+    which = rng.choice(num_blobs, num_tris, p=weights / weights.sum())
     c = centers[which] + rng.normal(0.0, blob_sigma, (num_tris, 2))
     off = _lognormal_edges(rng, num_tris, median_px, edge_sigma, width)
     off[..., 1] *= width / height  # same pixel length in y

@@ -1048,6 +1048,8 @@ static int raster_bin(const frame *f, oracle_outputs *out, int bin, int level, b
 	}
 	res->frags = total;
 
+	if(f->in->mode == ORACLE_MODE_COUNTS)
+		return 0;
 	if(f->in->mode == ORACLE_MODE_EXACT) {
 		/* bucket by pixel, then sort each pixel's list by (near_key, tri) */
 		uint32_t start[32 * 32 + 1];

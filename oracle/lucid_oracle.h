@@ -23,6 +23,7 @@ extern "C" {
 /* blend modes */
 #define ORACLE_MODE_EXACT 0 /* full per-pixel sort, ties broken by triangle index */
 #define ORACLE_MODE_REF3 1  /* the reference's block sort + 3-deep reduction window */
+#define ORACLE_MODE_COUNTS 2 /* coverage and bins only: no shading or blending */
 
 /* option bits */
 #define ORACLE_OPT_FP32_SAMPLES 0x1u /* no RGBA8 quantisation of shaded samples */

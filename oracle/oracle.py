@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "liblucid_oracle.so")
 
 INFO_U32 = 1152
-MODE_EXACT, MODE_REF3 = 0, 1
+MODE_EXACT, MODE_REF3, MODE_COUNTS = 0, 1, 2
 OPT_FP32_SAMPLES, OPT_ADDITIVE = 0x1, 0x2
 
 
