@@ -159,7 +159,7 @@ def main():
         frame_ms.append(sum(st.values()))
     _lib.check(r.finish(), "stage timing")
     stages = {k: float(np.mean([s[k] for s in stage_runs])) for k in stage_runs[0]}
-    dominant = max(("raster_low", "raster_high"), key=lambda k: stages[k])
+    dominant = "block_raster"
 
     ab = algorithmic_bytes(scene, r, info)
     dom_bytes = ab[dominant]

@@ -113,7 +113,8 @@ int lucid_read_setup(lucid_renderer *r, uint32_t *tri_records, size_t num_tris,
 /* Per-stage GPU times (ms) of the last frame, measured with HIP events on the frame's
  * stream when the renderer was created with LUCID_OPT_TIMERS; replaces the reference's
  * in-shader timers (LucidRenderOpt::timers, src/lucid_renderer.cpp:582-596,754-762).
- * Stages: 0 setup, 1 binning, 2 raster_low, 3 raster_high, 4 exact fallback.
+ * Stages: 0 setup, 1 binning, 2 row generation of low bins, 3 row generation of high bins
+ * plus sorting of long block lists, 4 block rasterisation, 5 exact per-pixel fallback.
  * Returns the number of stages written, or a negative status. */
 int lucid_stage_times(lucid_renderer *r, float *ms, int max_stages);
 

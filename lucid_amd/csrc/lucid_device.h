@@ -35,8 +35,23 @@ enum CtrlWord : int {
 	kCtrlTotalBinQuads = 5,
 	kCtrlTotalBinTris = 6,
 	kCtrlFallbackPixels = 7,
+	kCtrlListCursor = 8,   // bump allocator of per-block index lists of high bins (u32 units)
+	kCtrlBigCount = 9,     // blocks whose lists are pre-sorted by block_sort_kernel
+	kCtrlSortedCursor = 10, // bump allocator of pre-sorted block arrays (u32 units)
 	kCtrlWords = 16,
 };
+
+// Overflow bits of kCtrlOverflow
+constexpr uint32_t kOvfBinLists = 1u, kOvfRecords = 2u, kOvfFallback = 4u, kOvfSpill = 8u,
+				   kOvfLists = 16u, kOvfSorted = 32u;
+
+// Per-block slot flags (blk_flags)
+constexpr uint32_t kSlotLevelMask = 0xffu; // LUCID_BIN_LEVEL_*; 0 = empty bin
+constexpr uint32_t kSlotPresorted = 1u << 8;
+constexpr uint32_t kSlotPromoted = 1u << 9;  // low bin handed to the high path
+constexpr uint32_t kSlotWhole = 1u << 10;    // list too long: every pixel takes the fallback
+constexpr int kRegSortCap = 256;  // lists sorted in registers by the raster wave itself
+constexpr int kBigSortCap = 8192; // lists sorted in LDS by block_sort_kernel
 
 enum ColorMode : int { kColorNone = 0, kColorRGBA8 = 1, kColorF32 = 2 };
 
@@ -90,6 +105,14 @@ struct FrameBufs {
 	uint4 *low_records;  // bin_count * 4 * kLowRowCap
 	uint4 *high_records; // pool
 	uint64_t high_records_cap;
+	uint4 *blk_slot;     // bin_count * 16: list offset, list length, record offset, record count
+	uint32_t *blk_flags; // bin_count * 16
+	uint32_t *blk_sorted; // bin_count * 16: offset of the pre-sorted arrays
+	uint32_t *blk_lists;  // low bins: fixed bin_count * 16 * kRegSortCap; then high bump region
+	uint64_t blk_lists_cap;
+	uint32_t *big_queue;  // bin_count * 16
+	uint32_t *sorted_pool;
+	uint64_t sorted_cap;
 	uint4 *fallback_queue; // 2 uint4 per entry
 	uint32_t fallback_cap;
 	uint64_t *spill; // deep-pixel spill pool

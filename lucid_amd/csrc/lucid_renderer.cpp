@@ -76,9 +76,11 @@ struct lucid_renderer {
 		tri_color, large_list;
 	DevBuf wg_hist, bin_quads, bin_tris;
 	DevBuf low_records, high_records, fallback_queue, spill;
+	DevBuf blk_slot, blk_flags, blk_sorted, blk_lists, big_queue, sorted_pool;
 
 	int64_t quad_cap = 0;
 	uint64_t bin_quads_cap = 0, bin_tris_cap = 0, high_records_cap = 0, spill_cap = 0;
+	uint64_t blk_lists_cap = 0, sorted_cap = 0;
 	uint32_t fallback_cap = 0;
 
 	std::vector<lucid_instance_data> host_instances;
@@ -87,7 +89,7 @@ struct lucid_renderer {
 
 	uint32_t *status = nullptr; // pinned copy of the control words of the last frame
 	hipEvent_t done = nullptr;
-	static constexpr int kStages = 5;
+	static constexpr int kStages = 6;
 	hipEvent_t stage_ev[kStages + 1] = {};
 	bool timers = false;
 	bool frame_pending = false;
@@ -151,21 +153,33 @@ int grow_after_overflow(lucid_renderer *r) {
 	uint32_t flags = r->status[kCtrlOverflow];
 	if(!flags)
 		return LUCID_OK;
-	if(flags & 1u) {
+	if(flags & kOvfBinLists) {
 		uint64_t need = std::max<uint64_t>(r->status[kCtrlTotalBinTris], r->bin_tris_cap * 2);
 		LUCID_CHECK(r->bin_tris.reserve(need * 4));
 		r->bin_tris_cap = need;
 	}
-	if(flags & 2u) {
+	if(flags & kOvfRecords) {
 		uint64_t need = std::max<uint64_t>((uint64_t)r->status[kCtrlRecordCursor],
 										   r->high_records_cap * 2);
 		LUCID_CHECK(r->high_records.reserve(need * 16));
 		r->high_records_cap = need;
 	}
-	if(flags & 8u) {
+	if(flags & kOvfSpill) {
 		uint64_t need = std::max<uint64_t>((uint64_t)r->status[kCtrlSpillCursor], r->spill_cap * 2);
 		LUCID_CHECK(r->spill.reserve(need * 8));
 		r->spill_cap = need;
+	}
+	if(flags & kOvfLists) {
+		uint64_t low = (uint64_t)r->bin_count * 16 * kRegSortCap;
+		uint64_t need = std::max<uint64_t>(low + (uint64_t)r->status[kCtrlListCursor],
+										   r->blk_lists_cap * 2);
+		LUCID_CHECK(r->blk_lists.reserve(need * 4));
+		r->blk_lists_cap = need;
+	}
+	if(flags & kOvfSorted) {
+		uint64_t need = std::max<uint64_t>((uint64_t)r->status[kCtrlSortedCursor], r->sorted_cap * 2);
+		LUCID_CHECK(r->sorted_pool.reserve(need * 4));
+		r->sorted_cap = need;
 	}
 	return LUCID_ERR_CAPACITY;
 }
@@ -259,6 +273,14 @@ int lucid_create(const lucid_create_info *ci, lucid_renderer **out) {
 	chk(r->high_records.reserve(r->high_records_cap * 16));
 	r->spill_cap = 1u << 20;
 	chk(r->spill.reserve(r->spill_cap * 8));
+	chk(r->blk_slot.reserve((size_t)NB * 16 * 16));
+	chk(r->blk_flags.reserve((size_t)NB * 16 * 4));
+	chk(r->blk_sorted.reserve((size_t)NB * 16 * 4));
+	chk(r->big_queue.reserve((size_t)NB * 16 * 4));
+	r->blk_lists_cap = (uint64_t)NB * 16 * kRegSortCap + (1u << 22);
+	chk(r->blk_lists.reserve(r->blk_lists_cap * 4));
+	r->sorted_cap = 1u << 22;
+	chk(r->sorted_pool.reserve(r->sorted_cap * 4));
 	chk(hipHostMalloc((void **)&r->status, kCtrlWords * 4, hipHostMallocDefault));
 	chk(hipEventCreateWithFlags(&r->done, hipEventDisableTiming));
 	r->timers = (ci->opts & LUCID_OPT_TIMERS) != 0;
@@ -403,6 +425,14 @@ int lucid_render(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_fr
 	b.fallback_cap = r->fallback_cap;
 	b.spill = r->spill.as<uint64_t>();
 	b.spill_cap = r->spill_cap;
+	b.blk_slot = r->blk_slot.as<uint4>();
+	b.blk_flags = r->blk_flags.as<uint32_t>();
+	b.blk_sorted = r->blk_sorted.as<uint32_t>();
+	b.blk_lists = r->blk_lists.as<uint32_t>();
+	b.blk_lists_cap = r->blk_lists_cap;
+	b.big_queue = r->big_queue.as<uint32_t>();
+	b.sorted_pool = r->sorted_pool.as<uint32_t>();
+	b.sorted_cap = r->sorted_cap;
 	b.image = out->image_rgba8;
 	b.image_f = out->image_rgba32f;
 	b.frag_counts = out->frag_counts;

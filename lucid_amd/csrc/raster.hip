@@ -1,45 +1,50 @@
 // Tile rasterisation, per-pixel depth ordering and front-to-back blending
 // (≙ data/shaders/raster_low.glsl, raster_high.glsl, shared/raster.glsl, shared/shading.glsl).
 //
-// Work decomposition (wave64, one lane per pixel of an 8x8 block):
-//   raster_low   one 256-thread work-group per bin with fewer than 1024 triangles. Builds
-//                block-row records with the reference's 8-row scanline rule, promotes the bin
-//                to raster_high when an 8x8 block holds more than 256 triangles
-//                (raster_low.glsl:90-106,229-237), otherwise each wave rasterises 4 blocks.
-//   raster_high  one 256-thread work-group per (high bin, 8-row band) with the 4-row rule of
-//                raster_high.glsl:54-90; each wave rasterises one 8x8 block.
-//   fallback     exact per-pixel sort for pixels whose speculative ordering was violated.
+// Kernels (wave64; an 8x8 block of pixels is one wave, one lane per pixel):
+//   rowgen_low   one work-group per bin with fewer than 1024 triangles: block-row records with
+//                the reference's 8-row scanline rule and per-block index lists; promotes the bin
+//                to the high path when an 8x8 block holds more than 256 triangles
+//                (raster_low.glsl:39-106,229-237).
+//   rowgen_high  one work-group per (high bin, 8-row band) with the 4-row rule of
+//                raster_high.glsl:54-90.
+//   block_sort   one work-group per block with more than 256 triangles: centroid-depth keys
+//                sorted by an LDS bitonic sort, written back as sorted (tri, mask, prefix) arrays.
+//   block_raster one wave per block: sorts short lists itself in registers, expands coverage
+//                masks into dense 64-sample chunks, shades, routes samples to pixel lanes and
+//                blends front to back.
+//   fallback     exact per-pixel sort for pixels whose speculative order was violated.
 //
-// Exact ordering: a block's triangles are sorted by the depth at their covered-pixel centroid
-// (wave bitonic in registers / LDS). Each pixel lane then consumes its fragments in that
-// order through a K-entry insertion window keyed by (depth at the pixel, triangle id) and
-// checks every arrival against the last emitted key. If no arrival is ever smaller than the
-// last emitted key, the emitted sequence is exactly the per-pixel sorted order; otherwise the
-// pixel is queued for the fallback kernel, which sorts the pixel's full fragment list
-// (LDS bitonic, global-memory spill for deep pixels). Ties in depth are ordered by triangle id.
+// Exact ordering: triangles of a block are pre-sorted by the depth at their covered-pixel
+// centroid. Each pixel lane consumes its fragments in that order through a K-entry insertion
+// window keyed by (depth at the pixel, triangle id) and checks every arrival against the last
+// emitted key. If no arrival is ever smaller than the last emitted key, the emitted sequence is
+// exactly the per-pixel sorted order; otherwise the pixel is queued for the fallback kernel,
+// which sorts its full fragment list (LDS bitonic, global-memory spill for deep pixels).
+// Ties in depth are ordered by triangle id.
 #include "lucid_device.h"
 
 namespace lucid {
 
-constexpr int kRasterThreads = 256;
+constexpr int kRowgenThreads = 256;
+constexpr int kSortThreads = 256;
+constexpr int kFallbackThreads = 256;
 constexpr int kWindow = 4;
-constexpr int kLowCap = 256;    // == kMaxBlockTris
-constexpr int kHighCap = 1024;  // block lists above this go to the fallback kernel
 constexpr int kFallbackCap = 512; // fragments per pixel sorted in LDS by the fallback kernel
 
 // ------------------------------------------------------------------ shading
 
+// GLSL specifies pow(x, y) with the precision of exp2(y * log2(x)); the hardware log2/exp2
+// evaluate exactly that.
+__device__ __forceinline__ float glsl_pow(float x, float y) {
+	return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
 __device__ __forceinline__ float srgb_to_linear(float x) {
-	return x < 0.04045f ? (1.0f / 12.92f) * x : powf((x + 0.055f) * (1.0f / 1.055f), 2.4f);
+	return x < 0.04045f ? (1.0f / 12.92f) * x : glsl_pow((x + 0.055f) * (1.0f / 1.055f), 2.4f);
 }
 __device__ __forceinline__ float linear_to_srgb(float x) {
-	return x < 0.0031308f ? 12.92f * x : 1.055f * powf(x, 1.0f / 2.4f) - 0.055f;
+	return x < 0.0031308f ? 12.92f * x : 1.055f * glsl_pow(x, 1.0f / 2.4f) - 0.055f;
 }
-
-struct Shaded {
-	uint32_t enc; // RGBA8 sample (quantised mode); 0 = contributes nothing
-	float4 col;   // fp32 sample (fp32 mode)
-};
 
 __device__ __forceinline__ float sample_inv_depth(const FrameBufs &b, uint32_t t, float px,
 												  float py) {
@@ -47,10 +52,38 @@ __device__ __forceinline__ float sample_inv_depth(const FrameBufs &b, uint32_t t
 	return bits2f(d.x) * px + (bits2f(d.y) * py + bits2f(d.z));
 }
 
+// Shaded sample colour: RGBA8 word in the reference's quantised mode (0 = contributes
+// nothing, shading.glsl:167-168,274) or fp32 with LUCID_OPT_FP32_SAMPLES.
+template <bool FP32> struct SampleColor;
+template <> struct SampleColor<false> {
+	uint32_t enc;
+	__device__ __forceinline__ bool contributes() const { return enc != 0; }
+	__device__ __forceinline__ float4 rgba() const { return decode_rgba8(enc); }
+	__device__ __forceinline__ void zero() { enc = 0; }
+	__device__ __forceinline__ SampleColor shfl(int src) const {
+		SampleColor o;
+		o.enc = __shfl(enc, src, 64);
+		return o;
+	}
+};
+template <> struct SampleColor<true> {
+	float4 col;
+	__device__ __forceinline__ bool contributes() const { return col.w != 0.0f; }
+	__device__ __forceinline__ float4 rgba() const { return col; }
+	__device__ __forceinline__ void zero() { col = make_float4(0, 0, 0, 0); }
+	__device__ __forceinline__ SampleColor shfl(int src) const {
+		SampleColor o;
+		o.col.x = __shfl(col.x, src, 64), o.col.y = __shfl(col.y, src, 64);
+		o.col.z = __shfl(col.z, src, 64), o.col.w = __shfl(col.w, src, 64);
+		return o;
+	}
+};
+
 // shadeSample (shading.glsl:107-184) without textures.
 template <bool FP32>
-__device__ __forceinline__ Shaded shade_sample(const FrameConst &c, const FrameBufs &b, uint32_t t,
-											   float px, float py, float inv_ray) {
+__device__ __forceinline__ SampleColor<FP32> shade_sample(const FrameConst &c, const FrameBufs &b,
+														  uint32_t t, float px, float py,
+														  float inv_ray) {
 	uint4 d = b.tri_depth[t];
 	uint32_t flags = d.w & 0xffffu, inst = d.w >> 16;
 	uint4 b0 = b.tri_bary[2 * (size_t)t], b1 = b.tri_bary[2 * (size_t)t + 1];
@@ -84,10 +117,9 @@ __device__ __forceinline__ Shaded shade_sample(const FrameConst &c, const FrameB
 			color.w = color.w * c0.w;
 		}
 	}
-	Shaded out;
+	SampleColor<FP32> out;
 	if(color.w == 0.0f) {
-		out.enc = 0;
-		out.col = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		out.zero();
 		return out;
 	}
 	V3 normal;
@@ -111,13 +143,10 @@ __device__ __forceinline__ Shaded shade_sample(const FrameConst &c, const FrameB
 		float dif = (c.sun_color[k] * c.sun_power) * light;
 		rgb[k] = sat(linear_to_srgb(srgb_to_linear(rgb[k]) * (amb + dif)));
 	}
-	if(FP32) {
-		out.enc = 1;
+	if constexpr(FP32)
 		out.col = make_float4(rgb[0], rgb[1], rgb[2], color.w);
-	} else {
+	else
 		out.enc = encode_rgba8(rgb[0], rgb[1], rgb[2], color.w);
-		out.col = decode_rgba8(out.enc);
-	}
 	return out;
 }
 
@@ -127,24 +156,11 @@ __device__ __forceinline__ uint64_t pixel_key(float inv_ray, uint32_t t) {
 
 // ------------------------------------------------------------------ per-pixel state
 
-template <bool FP32>
-struct PixelState {
-	uint64_t wk[kWindow];
-	float4 wc[kWindow];
-	int n;
-	uint64_t last;
-	bool bad;
+struct Accum {
 	float cr, cg, cb, tr;
-
-	__device__ __forceinline__ void init() {
-#pragma unroll
-		for(int i = 0; i < kWindow; i++)
-			wk[i] = ~0ull, wc[i] = make_float4(0, 0, 0, 0);
-		n = 0, last = 0, bad = false;
-		cr = cg = cb = 0.0f, tr = 1.0f;
-	}
-
-	__device__ __forceinline__ void emit(float4 col, bool additive) {
+	__device__ __forceinline__ void init() { cr = cg = cb = 0.0f, tr = 1.0f; }
+	// shading.glsl:276-283 (front-to-back "over"; additive variant)
+	__device__ __forceinline__ void blend(float4 col, bool additive) {
 		if(additive) {
 			cr = cr + col.x * col.w;
 			cg = cg + col.y * col.w;
@@ -156,8 +172,26 @@ struct PixelState {
 			tr = tr * (1.0f - col.w);
 		}
 	}
+};
 
-	__device__ __forceinline__ void insert(uint64_t key, float4 col, bool additive) {
+template <bool FP32>
+struct PixelState {
+	uint64_t wk[kWindow];
+	SampleColor<FP32> wc[kWindow];
+	int n;
+	uint64_t last;
+	bool bad;
+	Accum acc;
+
+	__device__ __forceinline__ void init() {
+#pragma unroll
+		for(int i = 0; i < kWindow; i++)
+			wk[i] = ~0ull, wc[i].zero();
+		n = 0, last = 0, bad = false;
+		acc.init();
+	}
+
+	__device__ __forceinline__ void insert(uint64_t key, SampleColor<FP32> col, bool additive) {
 		if(key < last) {
 			bad = true;
 			return;
@@ -176,14 +210,14 @@ struct PixelState {
 		for(int i = 0; i < kWindow; i++)
 			if(wk[i] < mk)
 				mk = wk[i], mi = i;
-		float4 out = col;
+		SampleColor<FP32> out = col;
 #pragma unroll
 		for(int i = 0; i < kWindow; i++)
 			if(i == mi) {
 				out = wc[i];
 				wk[i] = key, wc[i] = col;
 			}
-		emit(out, additive);
+		acc.blend(out.rgba(), additive);
 		last = mk;
 	}
 
@@ -197,25 +231,25 @@ struct PixelState {
 				if(wk[i] < mk)
 					mk = wk[i], mi = i;
 			if(mi >= 0) {
-				float4 out = make_float4(0, 0, 0, 0);
+				SampleColor<FP32> out;
+				out.zero();
 #pragma unroll
 				for(int i = 0; i < kWindow; i++)
 					if(i == mi)
 						out = wc[i], wk[i] = ~0ull;
-				emit(out, additive);
+				acc.blend(out.rgba(), additive);
 			}
 		}
 	}
 };
 
 __device__ __forceinline__ void write_pixel(const FrameConst &c, const FrameBufs &b, int px,
-											int py, float cr, float cg, float cb, float tr,
-											uint32_t count) {
+											int py, const Accum &a, uint32_t count) {
 	if(px >= c.width || py >= c.height)
 		return;
-	float r = sat(cr + tr * c.background[0]);
-	float g = sat(cg + tr * c.background[1]);
-	float bl = sat(cb + tr * c.background[2]);
+	float r = sat(a.cr + a.tr * c.background[0]);
+	float g = sat(a.cg + a.tr * c.background[1]);
+	float bl = sat(a.cb + a.tr * c.background[2]);
 	size_t o = (size_t)py * c.width + px;
 	if(b.image) {
 		uint32_t v = f2u(r * 255.0f + 0.5f) | (f2u(g * 255.0f + 0.5f) << 8) |
@@ -228,7 +262,7 @@ __device__ __forceinline__ void write_pixel(const FrameConst &c, const FrameBufs
 		b.frag_counts[o] = count;
 }
 
-// ------------------------------------------------------------------ wave sorts
+// ------------------------------------------------------------------ sorts
 
 __device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
 	uint32_t lo = __shfl((uint32_t)v, src, 64), hi = __shfl((uint32_t)(v >> 32), src, 64);
@@ -275,8 +309,8 @@ __device__ __forceinline__ void bitonic_regs(uint64_t (&v)[E]) {
 	}
 }
 
-// Ascending bitonic sort of P (power of two, >= 2) keys in LDS by one wave.
-__device__ void bitonic_lds(uint64_t *key, uint32_t P) {
+// Ascending bitonic sort of P (power of two) keys in LDS by one wave.
+__device__ void bitonic_lds_wave(uint64_t *key, uint32_t P) {
 	const int lane = lane_id();
 	for(uint32_t k = 2; k <= P; k <<= 1) {
 		for(uint32_t j = k >> 1; j > 0; j >>= 1) {
@@ -296,7 +330,7 @@ __device__ void bitonic_lds(uint64_t *key, uint32_t P) {
 }
 
 // Same on global memory (deep pixels); agent-scope fences make each stage visible.
-__device__ void bitonic_global(uint64_t *key, uint32_t P) {
+__device__ void bitonic_global_wave(uint64_t *key, uint32_t P) {
 	const int lane = lane_id();
 	for(uint32_t k = 2; k <= P; k <<= 1) {
 		for(uint32_t j = k >> 1; j > 0; j >>= 1) {
@@ -321,26 +355,15 @@ __device__ __forceinline__ uint32_t next_pow2(uint32_t x) {
 	return x <= 1 ? 1u : 1u << (32 - __clz(x - 1));
 }
 
-// ------------------------------------------------------------------ block raster (one wave)
+// ------------------------------------------------------------------ records and keys
 
-struct WaveLds {
-	uint64_t *key; // [cap]: sort keys, then block masks in sorted order
-	uint32_t *tri; // [cap]
-	uint32_t *prefix; // [cap + 1]
-	uint32_t *own;  // [64]
-	uint64_t *pm;   // [64]
-};
+__device__ __forceinline__ uint4 make_record(uint32_t t, const uint32_t sp[8], uint32_t bxm) {
+	return make_uint4(t, sp[0] | (sp[1] << 10) | (sp[2] << 20), sp[3] | (sp[4] << 10) | (sp[5] << 20),
+					  sp[6] | (sp[7] << 10) | (bxm << 20));
+}
 
-struct BlockRef {
-	int bin, by, bx;    // bin id, block row/col inside the bin
-	int px0, py0;       // absolute pixel of the block's top-left corner
-	const uint4 *recs;  // block-row records
-	uint32_t rec_offset;
-	uint32_t nrec;
-	int level;          // LUCID_BIN_LEVEL_LOW or _HIGH (selects the record buffer)
-};
-
-// Centroid key of a record inside block column bx, plus its coverage mask.
+// Sort key of a record inside block column bx: depth at the covered-pixel centroid (nearest
+// first), then the list position (ties are resolved exactly per pixel later).
 __device__ __forceinline__ uint64_t block_key(const FrameBufs &b, uint4 rec, int bx, int px0,
 											  int py0, uint32_t idx) {
 	int n = 0, sx = 0, sy = 0;
@@ -363,176 +386,6 @@ __device__ __forceinline__ uint64_t block_key(const FrameBufs &b, uint4 rec, int
 	return ((uint64_t)(~ord_f32(inv)) << 32) | idx;
 }
 
-// `whole` marks a block whose fragments were not counted by the raster kernel.
-__device__ __forceinline__ void enqueue_fallback(const FrameBufs &b, const BlockRef &br,
-												 uint64_t mask, bool whole = false) {
-	uint32_t slot = atomicAdd(&b.ctrl[kCtrlFallbackCount], 1u);
-	if(slot >= b.fallback_cap) {
-		atomicOr(&b.ctrl[kCtrlOverflow], 4u);
-		return;
-	}
-	b.fallback_queue[2 * slot] =
-		make_uint4((uint32_t)br.bin,
-				   (uint32_t)(br.by * 4 + br.bx) | ((uint32_t)br.level << 8) | (whole ? 1u << 16 : 0u),
-				   br.rec_offset, br.nrec);
-	b.fallback_queue[2 * slot + 1] =
-		make_uint4((uint32_t)mask, (uint32_t)(mask >> 32), (uint32_t)br.px0, (uint32_t)br.py0);
-}
-
-// Rasterises one 8x8 block with one wave. `sel` lists the record indices of the block
-// (m entries, m <= cap). Returns the number of fragments (for statistics).
-template <bool FP32, int RegE>
-__device__ uint32_t raster_block(const FrameConst &c, const FrameBufs &b, const WaveLds &L,
-								 const BlockRef &br, const uint16_t *sel16, const uint32_t *sel32,
-								 uint32_t m) {
-	const int lane = lane_id();
-	const bool additive = (c.opts & (1u << 5)) != 0;
-	// --- 1. keys + sort
-	uint32_t P = next_pow2(m < 64 ? 64 : m);
-	if(RegE > 0 && P <= 64u * RegE) {
-		// register sort; pick the smallest E that holds P
-		auto load = [&](uint32_t j) -> uint64_t {
-			if(j >= m)
-				return ~0ull;
-			uint32_t ri = sel16 ? sel16[j] : sel32[j];
-			return block_key(b, br.recs[ri], br.bx, br.px0, br.py0, j);
-		};
-		if(P <= 64) {
-			uint64_t v[1] = {load(lane)};
-			bitonic_regs<1>(v);
-			L.key[lane] = v[0];
-		} else if(P <= 128) {
-			uint64_t v[2] = {load(lane), load(64 + lane)};
-			bitonic_regs<2>(v);
-			L.key[lane] = v[0], L.key[64 + lane] = v[1];
-		} else {
-			uint64_t v[4] = {load(lane), load(64 + lane), load(128 + lane), load(192 + lane)};
-			bitonic_regs<4>(v);
-#pragma unroll
-			for(int e = 0; e < 4; e++)
-				L.key[e * 64 + lane] = v[e];
-		}
-	} else {
-		for(uint32_t j = lane; j < P; j += 64) {
-			uint64_t k = ~0ull;
-			if(j < m) {
-				uint32_t ri = sel16 ? sel16[j] : sel32[j];
-				k = block_key(b, br.recs[ri], br.bx, br.px0, br.py0, j);
-			}
-			L.key[j] = k;
-		}
-		wave_sync_lds();
-		bitonic_lds(L.key, P);
-	}
-	wave_sync_lds();
-	// --- 2. sorted masks, tri ids and sample prefix sums
-	uint32_t carry = 0;
-	for(uint32_t base = 0; base < m; base += 64) {
-		uint32_t i = base + lane;
-		uint32_t cnt = 0;
-		if(i < m) {
-			uint32_t j = (uint32_t)L.key[i];
-			uint32_t ri = sel16 ? sel16[j] : sel32[j];
-			uint4 rec = br.recs[ri];
-			uint64_t mask = record_block_mask(rec, br.bx);
-			cnt = (uint32_t)__popcll(mask);
-			L.key[i] = mask;
-			L.tri[i] = rec.x;
-		}
-		uint32_t inc = wave_inclusive_add(cnt);
-		if(i < m)
-			L.prefix[i] = carry + inc - cnt;
-		carry += __shfl(inc, 63, 64);
-	}
-	if(lane == 0)
-		L.prefix[m] = carry;
-	wave_sync_lds();
-	const uint32_t total = carry;
-
-	// --- 3. sample chunks
-	PixelState<FP32> ps;
-	ps.init();
-	uint32_t count = 0;
-	uint32_t e_cur = 0;
-	for(uint32_t s0 = 0; s0 < total; s0 += 64) {
-		L.own[lane] = 0;
-		L.pm[lane] = 0;
-		wave_sync_lds();
-		uint32_t e = e_cur + lane;
-		if(e < m) {
-			uint32_t st = L.prefix[e];
-			if(st < s0 + 64u)
-				L.own[st > s0 ? st - s0 : 0] = e;
-		}
-		wave_sync_lds();
-		uint32_t own = wave_inclusive_max(L.own[lane]);
-		uint32_t s = s0 + lane;
-		bool valid = s < total;
-		uint64_t key = ~0ull;
-		float4 col = make_float4(0, 0, 0, 0);
-		uint32_t pix = 0;
-		bool contributes = false;
-		if(valid) {
-			uint32_t k = s - L.prefix[own];
-			uint64_t mask = L.key[own];
-			pix = (uint32_t)select_bit64(mask, k);
-			uint32_t t = L.tri[own];
-			float px = (float)(br.px0 + (int)(pix & 7u)), py = (float)(br.py0 + (int)(pix >> 3));
-			float inv = sample_inv_depth(b, t, px, py);
-			Shaded sh = shade_sample<FP32>(c, b, t, px, py, inv);
-			contributes = FP32 ? sh.col.w != 0.0f : sh.enc != 0;
-			key = pixel_key(inv, t);
-			col = sh.col;
-			atomicOr((unsigned long long *)&L.pm[pix], 1ull << lane);
-		}
-		uint32_t own63 = __shfl(own, 63, 64);
-		wave_sync_lds();
-		uint64_t bits = L.pm[lane];
-		count += (uint32_t)__popcll(bits);
-		// only contributing samples take part in ordering
-		uint64_t contrib = ballot(contributes);
-		bits &= contrib;
-		while(ballot(bits != 0)) {
-			int src = bits ? (int)__ffsll((unsigned long long)bits) - 1 : lane;
-			uint64_t k2 = shfl64(key, src);
-			float4 c2;
-			c2.x = __shfl(col.x, src, 64);
-			c2.y = __shfl(col.y, src, 64);
-			c2.z = __shfl(col.z, src, 64);
-			c2.w = __shfl(col.w, src, 64);
-			if(bits) {
-				bits &= bits - 1;
-				ps.insert(k2, c2, additive);
-			}
-		}
-		e_cur = (L.prefix[own63 + 1] <= s0 + 64u) ? own63 + 1 : own63;
-	}
-	ps.flush(additive);
-	int px = br.px0 + (lane & 7), py = br.py0 + (lane >> 3);
-	write_pixel(c, b, px, py, ps.cr, ps.cg, ps.cb, ps.tr, count);
-	bool on_screen = px < c.width && py < c.height;
-	uint64_t bad = ballot(ps.bad && on_screen);
-	if(bad && lane == 0)
-		enqueue_fallback(b, br, bad);
-	// fragments of on-screen pixels (stats[0] counts what reaches the image)
-	uint32_t visible = on_screen ? count : 0u;
-	return __shfl(wave_inclusive_add(visible), 63, 64);
-}
-
-// ------------------------------------------------------------------ record generation
-
-// Spans of rows [row_lo, row_hi] (inclusive, inside the bin) for a triangle whose scan
-// starts at row `start` (group start of the reference's rule), packed as block-row records.
-struct RowGen {
-	RowScan scan;
-	int row; // next row the scan state corresponds to
-};
-
-__device__ __forceinline__ uint4 make_record(uint32_t t, const uint32_t sp[8], uint32_t bxm) {
-	return make_uint4(t, sp[0] | (sp[1] << 10) | (sp[2] << 20), sp[3] | (sp[4] << 10) | (sp[5] << 20),
-					  sp[6] | (sp[7] << 10) | (bxm << 20));
-}
-
 __device__ __forceinline__ uint32_t bin_tri_at(const FrameBufs &b, uint32_t i, uint32_t qc,
 											   uint32_t qo, uint32_t to) {
 	if(i < 2 * qc) {
@@ -545,62 +398,79 @@ __device__ __forceinline__ uint32_t bin_tri_at(const FrameBufs &b, uint32_t i, u
 	return b.bin_tris[to + (i - 2 * qc)];
 }
 
-__device__ __forceinline__ void background_bin(const FrameConst &c, const FrameBufs &b, int bx0,
-											   int by0) {
-	for(int i = threadIdx.x; i < 32 * 32; i += kRasterThreads)
-		write_pixel(c, b, bx0 + (i & 31), by0 + (i >> 5), 0.0f, 0.0f, 0.0f, 1.0f, 0u);
+struct BinInfo {
+	uint32_t qc, tc, qo, to, nlevel;
+	int bin_x, bin_y;
+};
+
+__device__ __forceinline__ BinInfo load_bin(const FrameConst &c, const FrameBufs &b, int bin) {
+	const int NB = c.bin_count;
+	const uint32_t *counts = b.info + LUCID_INFO_U32_SIZE;
+	BinInfo bi;
+	bi.qc = counts[LUCID_COUNTS_QUAD_COUNTS * NB + bin];
+	bi.tc = counts[LUCID_COUNTS_TRI_COUNTS * NB + bin];
+	bi.qo = counts[LUCID_COUNTS_QUAD_OFFSETS * NB + bin];
+	bi.to = counts[LUCID_COUNTS_TRI_OFFSETS * NB + bin];
+	bi.nlevel = bi.tc + 2 * bi.qc;
+	bi.bin_x = (bin % c.bin_count_x) * kBinSize;
+	bi.bin_y = (bin / c.bin_count_x) * kBinSize;
+	return bi;
 }
 
-// ------------------------------------------------------------------ raster_low
-
-template <bool FP32>
-__global__ __launch_bounds__(kRasterThreads) void raster_low_kernel(FrameConst c, FrameBufs b) {
-	if(b.ctrl[kCtrlOverflow] & 1u)
-		return;
-	const int NB = c.bin_count;
-	const int bin = blockIdx.x;
-	const uint32_t *counts = b.info + LUCID_INFO_U32_SIZE;
-	const uint32_t qc = counts[LUCID_COUNTS_QUAD_COUNTS * NB + bin];
-	const uint32_t tc = counts[LUCID_COUNTS_TRI_COUNTS * NB + bin];
-	const uint32_t qo = counts[LUCID_COUNTS_QUAD_OFFSETS * NB + bin];
-	const uint32_t to = counts[LUCID_COUNTS_TRI_OFFSETS * NB + bin];
-	const uint32_t nlevel = tc + 2 * qc;
-	const int bin_x = (bin % c.bin_count_x) * kBinSize, bin_y = (bin / c.bin_count_x) * kBinSize;
-	if(nlevel == 0) {
-		background_bin(c, b, bin_x, bin_y);
+// Fallback queue entry: {bin, block | level << 8 | whole << 16, record offset, record count},
+// {pixel mask lo, hi, px0, py0}.
+__device__ __forceinline__ void enqueue_fallback(const FrameBufs &b, int bin, int blk, int level,
+												 uint32_t rec_off, uint32_t nrec, int px0,
+												 int py0, uint64_t mask, bool whole) {
+	uint32_t slot = atomicAdd(&b.ctrl[kCtrlFallbackCount], 1u);
+	if(slot >= b.fallback_cap) {
+		atomicOr(&b.ctrl[kCtrlOverflow], kOvfFallback);
 		return;
 	}
-	if(nlevel >= (uint32_t)kLowLevelLimit)
+	b.fallback_queue[2 * slot] = make_uint4(
+		(uint32_t)bin, (uint32_t)blk | ((uint32_t)level << 8) | (whole ? 1u << 16 : 0u), rec_off,
+		nrec);
+	b.fallback_queue[2 * slot + 1] =
+		make_uint4((uint32_t)mask, (uint32_t)(mask >> 32), (uint32_t)px0, (uint32_t)py0);
+}
+
+// ------------------------------------------------------------------ rowgen_low
+
+__global__ __launch_bounds__(kRowgenThreads) void rowgen_low_kernel(FrameConst c, FrameBufs b) {
+	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
 		return;
+	const int bin = blockIdx.x;
+	const BinInfo bi = load_bin(c, b, bin);
+	uint4 *slots = b.blk_slot + (size_t)bin * 16;
+	uint32_t *flags = b.blk_flags + (size_t)bin * 16;
+	if(bi.nlevel == 0) {
+		if(threadIdx.x < 16)
+			flags[threadIdx.x] = LUCID_BIN_LEVEL_EMPTY;
+		return;
+	}
+	if(bi.nlevel >= (uint32_t)kLowLevelLimit)
+		return; // rowgen_high writes this bin's slots
 
 	__shared__ uint32_t s_row_n[4];
 	__shared__ uint32_t s_blk_n[16];
-	__shared__ uint16_t s_blk_idx[16][kLowCap];
-	__shared__ uint64_t s_key[4][kLowCap];
-	__shared__ uint32_t s_tri[4][kLowCap];
-	__shared__ uint32_t s_prefix[4][kLowCap + 1];
-	__shared__ uint32_t s_own[4][64];
-	__shared__ uint64_t s_pm[4][64];
-	__shared__ uint32_t s_stats[2];
 	if(threadIdx.x < 16)
 		s_blk_n[threadIdx.x] = 0;
 	if(threadIdx.x < 4)
 		s_row_n[threadIdx.x] = 0;
-	if(threadIdx.x < 2)
-		s_stats[threadIdx.x] = 0;
 	__syncthreads();
 
 	uint4 *recs = b.low_records + (size_t)bin * 4 * kLowRowCap;
-	// --- block-row records, 8-row rule (raster_low.glsl:39-79)
-	for(uint32_t i = threadIdx.x; i < nlevel; i += kRasterThreads) {
-		uint32_t t = bin_tri_at(b, i, qc, qo, to);
+	uint32_t *lists = b.blk_lists + (size_t)bin * 16 * kRegSortCap;
+	// block-row records, 8-row rule (raster_low.glsl:39-79)
+	for(uint32_t i = threadIdx.x; i < bi.nlevel; i += kRowgenThreads) {
+		uint32_t t = bin_tri_at(b, i, bi.qc, bi.qo, bi.to);
 		if(t == 0xffffffffu)
 			continue;
 		uint4 s0 = b.tri_scan[2 * (size_t)t], s1 = b.tri_scan[2 * (size_t)t + 1];
-		int min_by = min(max((int)(s0.w & 0xffffu) - bin_y, 0), kBinMask) >> 3;
-		int max_by = min(max((int)(s0.w >> 16) - bin_y, 0), kBinMask) >> 3;
+		int min_by = min(max((int)(s0.w & 0xffffu) - bi.bin_y, 0), kBinMask) >> 3;
+		int max_by = min(max((int)(s0.w >> 16) - bi.bin_y, 0), kBinMask) >> 3;
 		RowScan rs;
-		row_scan_init(rs, s0, s1, (float)bin_x, (float)(bin_y + min_by * 8));
+		row_scan_init(rs, s0, s1, (float)bi.bin_x, (float)(bi.bin_y + min_by * 8));
 		for(int by = min_by; by <= max_by; by++) {
 			uint32_t sp[8], bxm = 0;
 #pragma unroll
@@ -615,8 +485,8 @@ __global__ __launch_bounds__(kRasterThreads) void raster_low_kernel(FrameConst c
 			for(int bx = 0; bx < 4; bx++)
 				if(bxm & (1u << bx)) {
 					uint32_t k = atomicAdd(&s_blk_n[by * 4 + bx], 1u);
-					if(k < (uint32_t)kLowCap)
-						s_blk_idx[by * 4 + bx][k] = (uint16_t)idx;
+					if(k < (uint32_t)kRegSortCap)
+						lists[(by * 4 + bx) * kRegSortCap + k] = idx;
 				}
 		}
 	}
@@ -628,99 +498,81 @@ __global__ __launch_bounds__(kRasterThreads) void raster_low_kernel(FrameConst c
 		if(threadIdx.x == 0) {
 			lucid_info *info = (lucid_info *)b.info;
 			uint32_t id = atomicAdd((uint32_t *)&info->bin_level_counts[LUCID_BIN_LEVEL_HIGH], 1u);
-			b.info[LUCID_INFO_U32_SIZE + LUCID_COUNTS_HIGH_BINS * NB + id] = (uint32_t)bin;
+			b.info[LUCID_INFO_U32_SIZE + LUCID_COUNTS_HIGH_BINS * c.bin_count + id] = (uint32_t)bin;
 		}
+		if(threadIdx.x < 16)
+			flags[threadIdx.x] = kSlotPromoted;
 		return;
 	}
-	const int w = threadIdx.x >> 6;
-	WaveLds L{s_key[w], s_tri[w], s_prefix[w], s_own[w], s_pm[w]};
-	uint32_t frags = 0, pairs = 0;
-	for(int by = 0; by < 4; by++) {
-		BlockRef br;
-		br.bin = bin, br.by = by, br.bx = w;
-		br.px0 = bin_x + w * 8, br.py0 = bin_y + by * 8;
-		br.recs = recs + by * kLowRowCap;
-		br.rec_offset = (uint32_t)((size_t)bin * 4 * kLowRowCap + by * kLowRowCap);
-		br.nrec = s_row_n[by];
-		br.level = LUCID_BIN_LEVEL_LOW;
-		uint32_t m = s_blk_n[by * 4 + w];
-		frags += raster_block<FP32, 4>(c, b, L, br, s_blk_idx[by * 4 + w], nullptr, m);
-		pairs += m;
-	}
-	if(lane_id() == 0) {
-		atomicAdd(&s_stats[0], frags);
-		atomicAdd(&s_stats[1], pairs);
-	}
-	__syncthreads();
-	if(threadIdx.x == 0) {
-		lucid_info *info = (lucid_info *)b.info;
-		atomicAdd(&info->stats[0], s_stats[0]);
-		atomicAdd(&info->stats[1], s_stats[1]);
+	if(threadIdx.x < 16) {
+		int blk = threadIdx.x, by = blk >> 2;
+		slots[blk] = make_uint4((uint32_t)((size_t)bin * 16 * kRegSortCap + blk * kRegSortCap),
+								s_blk_n[blk],
+								(uint32_t)((size_t)bin * 4 * kLowRowCap + by * kLowRowCap),
+								s_row_n[by]);
+		flags[blk] = LUCID_BIN_LEVEL_LOW;
 	}
 }
 
-// ------------------------------------------------------------------ raster_high
+// ------------------------------------------------------------------ rowgen_high
 
-template <bool FP32>
-__global__ __launch_bounds__(kRasterThreads) void raster_high_kernel(FrameConst c, FrameBufs b) {
-	if(b.ctrl[kCtrlOverflow] & 1u)
+__global__ __launch_bounds__(kRowgenThreads) void rowgen_high_kernel(FrameConst c, FrameBufs b) {
+	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
 		return;
-	const int NB = c.bin_count;
 	const lucid_info *info = (const lucid_info *)b.info;
 	const uint32_t slot = blockIdx.x >> 2;
 	const int by = blockIdx.x & 3;
 	if(slot >= (uint32_t)info->bin_level_counts[LUCID_BIN_LEVEL_HIGH])
 		return;
 	const uint32_t *counts = b.info + LUCID_INFO_U32_SIZE;
-	const int bin = (int)counts[LUCID_COUNTS_HIGH_BINS * NB + slot];
-	const uint32_t qc = counts[LUCID_COUNTS_QUAD_COUNTS * NB + bin];
-	const uint32_t tc = counts[LUCID_COUNTS_TRI_COUNTS * NB + bin];
-	const uint32_t qo = counts[LUCID_COUNTS_QUAD_OFFSETS * NB + bin];
-	const uint32_t to = counts[LUCID_COUNTS_TRI_OFFSETS * NB + bin];
-	const uint32_t nlevel = tc + 2 * qc;
-	const int bin_x = (bin % c.bin_count_x) * kBinSize, bin_y = (bin / c.bin_count_x) * kBinSize;
+	const int bin = (int)counts[LUCID_COUNTS_HIGH_BINS * c.bin_count + slot];
+	const BinInfo bi = load_bin(c, b, bin);
 
 	__shared__ uint32_t s_row_n;
 	__shared__ uint32_t s_blk_n[4];
-	__shared__ uint32_t s_base;
-	__shared__ uint32_t s_blk_idx[4][kHighCap];
-	__shared__ uint64_t s_key[4][kHighCap];
-	__shared__ uint32_t s_tri[4][kHighCap];
-	__shared__ uint32_t s_prefix[4][kHighCap + 1];
-	__shared__ uint32_t s_own[4][64];
-	__shared__ uint64_t s_pm[4][64];
-	__shared__ uint32_t s_stats[2];
+	__shared__ uint32_t s_rec_base, s_list_base;
 	if(threadIdx.x < 4)
 		s_blk_n[threadIdx.x] = 0;
-	if(threadIdx.x < 2)
-		s_stats[threadIdx.x] = 0;
 	if(threadIdx.x == 0) {
 		s_row_n = 0;
-		uint32_t base = atomicAdd(&b.ctrl[kCtrlRecordCursor], nlevel);
-		if((uint64_t)base + nlevel > b.high_records_cap) {
-			atomicOr(&b.ctrl[kCtrlOverflow], 2u);
-			base = 0xffffffffu;
+		uint32_t rb = atomicAdd(&b.ctrl[kCtrlRecordCursor], bi.nlevel);
+		if((uint64_t)rb + bi.nlevel > b.high_records_cap) {
+			atomicOr(&b.ctrl[kCtrlOverflow], kOvfRecords);
+			rb = 0xffffffffu;
 		}
-		s_base = base;
+		// high-bin lists follow the fixed region of the low bins
+		uint64_t lb = (uint64_t)c.bin_count * 16 * kRegSortCap +
+					  atomicAdd(&b.ctrl[kCtrlListCursor], 4 * bi.nlevel);
+		if(lb + 4ull * bi.nlevel > b.blk_lists_cap) {
+			atomicOr(&b.ctrl[kCtrlOverflow], kOvfLists);
+			lb = 0xffffffffu;
+		}
+		s_rec_base = rb, s_list_base = (uint32_t)lb;
 	}
 	__syncthreads();
-	if(s_base == 0xffffffffu)
+	uint32_t *flags = b.blk_flags + (size_t)bin * 16 + by * 4;
+	if(s_rec_base == 0xffffffffu || s_list_base == 0xffffffffu) {
+		if(threadIdx.x < 4)
+			flags[threadIdx.x] = kSlotPromoted; // nothing rendered; the frame reports overflow
 		return;
-	uint4 *recs = b.high_records + s_base;
+	}
+	uint4 *recs = b.high_records + s_rec_base;
+	uint32_t *lists = b.blk_lists + s_list_base;
 	const int band0 = by * 8;
-	// --- records of this 8-row band with the 4-row rule (raster_high.glsl:54-90)
-	for(uint32_t i = threadIdx.x; i < nlevel; i += kRasterThreads) {
-		uint32_t t = bin_tri_at(b, i, qc, qo, to);
+	// records of this 8-row band with the 4-row rule (raster_high.glsl:54-90)
+	for(uint32_t i = threadIdx.x; i < bi.nlevel; i += kRowgenThreads) {
+		uint32_t t = bin_tri_at(b, i, bi.qc, bi.qo, bi.to);
 		if(t == 0xffffffffu)
 			continue;
-		uint4 s0 = b.tri_scan[2 * (size_t)t], s1 = b.tri_scan[2 * (size_t)t + 1];
-		int min_g = min(max((int)(s0.w & 0xffffu) - bin_y, 0), kBinMask) >> 2;
-		int max_g = min(max((int)(s0.w >> 16) - bin_y, 0), kBinMask) >> 2;
+		uint4 s0 = b.tri_scan[2 * (size_t)t];
+		int min_g = min(max((int)(s0.w & 0xffffu) - bi.bin_y, 0), kBinMask) >> 2;
+		int max_g = min(max((int)(s0.w >> 16) - bi.bin_y, 0), kBinMask) >> 2;
 		int row_lo = min_g * 4, row_hi = max_g * 4 + 3;
 		if(row_hi < band0 || row_lo > band0 + 7)
 			continue;
+		uint4 s1 = b.tri_scan[2 * (size_t)t + 1];
 		RowScan rs;
-		row_scan_init(rs, s0, s1, (float)bin_x, (float)(bin_y + row_lo));
+		row_scan_init(rs, s0, s1, (float)bi.bin_x, (float)(bi.bin_y + row_lo));
 		uint32_t sp[8], bxm = 0;
 #pragma unroll
 		for(int r = 0; r < 8; r++)
@@ -744,38 +596,286 @@ __global__ __launch_bounds__(kRasterThreads) void raster_high_kernel(FrameConst 
 		for(int bx = 0; bx < 4; bx++)
 			if(bxm & (1u << bx)) {
 				uint32_t k = atomicAdd(&s_blk_n[bx], 1u);
-				if(k < (uint32_t)kHighCap)
-					s_blk_idx[bx][k] = idx;
+				lists[bx * bi.nlevel + k] = idx;
 			}
 	}
 	__syncthreads();
-	const int w = threadIdx.x >> 6;
-	BlockRef br;
-	br.bin = bin, br.by = by, br.bx = w;
-	br.px0 = bin_x + w * 8, br.py0 = bin_y + band0;
-	br.recs = recs;
-	br.rec_offset = s_base;
-	br.nrec = s_row_n;
-	br.level = LUCID_BIN_LEVEL_HIGH;
-	uint32_t m = s_blk_n[w];
-	uint32_t frags = 0;
-	if(m > (uint32_t)kHighCap) {
-		// too many triangles for LDS: every pixel of the block takes the exact per-pixel path
-		if(lane_id() == 0)
-			enqueue_fallback(b, br, ~0ull, true);
-	} else {
-		WaveLds L{s_key[w], s_tri[w], s_prefix[w], s_own[w], s_pm[w]};
-		frags = raster_block<FP32, 4>(c, b, L, br, nullptr, s_blk_idx[w], m);
+	if(threadIdx.x < 4) {
+		int bx = threadIdx.x, blk = by * 4 + bx;
+		uint32_t m = s_blk_n[bx];
+		b.blk_slot[(size_t)bin * 16 + blk] =
+			make_uint4(s_list_base + bx * bi.nlevel, m, s_rec_base, s_row_n);
+		uint32_t f = LUCID_BIN_LEVEL_HIGH;
+		if(m > (uint32_t)kBigSortCap) {
+			f |= kSlotWhole;
+			enqueue_fallback(b, bin, blk, LUCID_BIN_LEVEL_HIGH, s_rec_base, s_row_n,
+							 bi.bin_x + bx * 8, bi.bin_y + band0, ~0ull, true);
+		} else if(m > (uint32_t)kRegSortCap) {
+			f |= kSlotPresorted;
+			uint32_t q = atomicAdd(&b.ctrl[kCtrlBigCount], 1u);
+			b.big_queue[q] = (uint32_t)(bin * 16 + blk); // at most 16 * bin_count entries
+		}
+		flags[bx] = f;
 	}
-	if(lane_id() == 0) {
-		atomicAdd(&s_stats[0], frags);
-		atomicAdd(&s_stats[1], m);
+}
+
+// ------------------------------------------------------------------ block_sort
+
+// Sorts one long block list in LDS with the whole work-group and stores, in sorted order,
+// the triangle ids, the block coverage masks and the exclusive prefix of sample counts.
+__global__ __launch_bounds__(kSortThreads) void block_sort_kernel(FrameConst c, FrameBufs b) {
+	if(blockIdx.x >= b.ctrl[kCtrlBigCount])
+		return;
+	__shared__ uint64_t s_key[kBigSortCap];
+	__shared__ uint32_t s_sum[kSortThreads];
+	__shared__ uint32_t s_base;
+	const uint32_t slot = b.big_queue[blockIdx.x];
+	const int bin = slot >> 4, blk = slot & 15, by = blk >> 2, bx = blk & 3;
+	const uint4 sl = b.blk_slot[slot];
+	const uint32_t m = sl.y;
+	const uint32_t *list = b.blk_lists + sl.x;
+	const uint4 *recs = b.high_records + sl.z;
+	const int bin_x = (bin % c.bin_count_x) * kBinSize, bin_y = (bin / c.bin_count_x) * kBinSize;
+	const int px0 = bin_x + bx * 8, py0 = bin_y + by * 8;
+	const uint32_t P = next_pow2(m);
+	if(threadIdx.x == 0) {
+		uint32_t need = 4 * m + 4;
+		uint32_t base = atomicAdd(&b.ctrl[kCtrlSortedCursor], need);
+		if((uint64_t)base + need > b.sorted_cap) {
+			atomicOr(&b.ctrl[kCtrlOverflow], kOvfSorted);
+			base = 0xffffffffu;
+		}
+		s_base = base;
+	}
+	for(uint32_t j = threadIdx.x; j < P; j += kSortThreads)
+		s_key[j] = j < m ? block_key(b, recs[list[j]], bx, px0, py0, j) : ~0ull;
+	__syncthreads();
+	if(s_base == 0xffffffffu) {
+		if(threadIdx.x == 0)
+			b.blk_flags[slot] = kSlotPromoted; // nothing rendered; the frame reports overflow
+		return;
+	}
+	for(uint32_t k = 2; k <= P; k <<= 1) {
+		for(uint32_t j = k >> 1; j > 0; j >>= 1) {
+			for(uint32_t i = threadIdx.x; i < P / 2; i += kSortThreads) {
+				uint32_t lo = ((i & ~(j - 1)) << 1) | (i & (j - 1));
+				uint32_t hi = lo | j;
+				bool asc = (lo & k) == 0;
+				uint64_t a = s_key[lo], bb = s_key[hi];
+				if((a > bb) == asc) {
+					s_key[lo] = bb;
+					s_key[hi] = a;
+				}
+			}
+			__syncthreads();
+		}
+	}
+	// Layout in the pool: tri[m], prefix[m + 1], pad to 8 bytes, mask[m] (u64).
+	const uint32_t base = (s_base + 1u) & ~1u;
+	uint32_t *tri = b.sorted_pool + base;
+	uint32_t *prefix = tri + m;
+	uint64_t *mask = (uint64_t *)(b.sorted_pool + ((base + 2 * m + 2) & ~1u));
+	// each thread owns a contiguous run of sorted elements
+	const uint32_t per = (m + kSortThreads - 1) / kSortThreads;
+	const uint32_t i0 = min(m, threadIdx.x * per), i1 = min(m, i0 + per);
+	uint32_t run = 0;
+	for(uint32_t i = i0; i < i1; i++) {
+		uint32_t j = (uint32_t)s_key[i];
+		uint4 rec = recs[list[j]];
+		uint64_t msk = record_block_mask(rec, bx);
+		tri[i] = rec.x;
+		mask[i] = msk;
+		run += (uint32_t)__popcll(msk);
+	}
+	s_sum[threadIdx.x] = run;
+	__syncthreads();
+	if(threadIdx.x < 64) {
+		// exclusive scan of the 256 per-thread sums by one wave (4 per lane)
+		uint32_t v[4], tot = 0;
+#pragma unroll
+		for(int q = 0; q < 4; q++)
+			v[q] = s_sum[threadIdx.x * 4 + q], tot += v[q];
+		uint32_t inc = wave_inclusive_add(tot);
+		uint32_t ex = inc - tot;
+#pragma unroll
+		for(int q = 0; q < 4; q++) {
+			uint32_t t = v[q];
+			s_sum[threadIdx.x * 4 + q] = ex;
+			ex += t;
+		}
+		if(threadIdx.x == 63)
+			prefix[m] = inc;
 	}
 	__syncthreads();
-	if(threadIdx.x == 0) {
-		lucid_info *inf = (lucid_info *)b.info;
-		atomicAdd(&inf->stats[0], s_stats[0]);
-		atomicAdd(&inf->stats[1], s_stats[1]);
+	run = s_sum[threadIdx.x];
+	for(uint32_t i = i0; i < i1; i++) {
+		prefix[i] = run;
+		run += (uint32_t)__popcll(mask[i]);
+	}
+	if(threadIdx.x == 0)
+		b.blk_sorted[slot] = base;
+}
+
+// ------------------------------------------------------------------ block_raster
+
+// Sorted block arrays, in LDS (short lists) or global memory (pre-sorted long lists).
+struct SortedBlock {
+	const uint32_t *tri;
+	const uint64_t *mask;
+	const uint32_t *prefix; // m + 1 entries
+};
+
+template <bool FP32>
+__device__ __forceinline__ void raster_chunks(const FrameConst &c, const FrameBufs &b,
+											  const SortedBlock &sb, uint32_t m, int px0, int py0,
+											  uint32_t *s_own, uint64_t *s_pm,
+											  PixelState<FP32> &ps, uint32_t &count) {
+	const int lane = lane_id();
+	const bool additive = (c.opts & (1u << 5)) != 0;
+	const uint32_t total = sb.prefix[m];
+	uint32_t e_cur = 0;
+	for(uint32_t s0 = 0; s0 < total; s0 += 64) {
+		s_own[lane] = 0;
+		s_pm[lane] = 0;
+		wave_sync_lds();
+		// mark the first sample of every entry that starts inside this chunk
+		uint32_t e = e_cur + lane;
+		if(e < m) {
+			uint32_t st = sb.prefix[e];
+			if(st < s0 + 64u)
+				s_own[st > s0 ? st - s0 : 0] = e;
+		}
+		wave_sync_lds();
+		uint32_t own = wave_inclusive_max(s_own[lane]);
+		uint32_t s = s0 + lane;
+		bool valid = s < total;
+		uint64_t key = ~0ull;
+		SampleColor<FP32> col;
+		col.zero();
+		if(valid) {
+			uint32_t k = s - sb.prefix[own];
+			uint32_t pix = (uint32_t)select_bit64(sb.mask[own], k);
+			uint32_t t = sb.tri[own];
+			float px = (float)(px0 + (int)(pix & 7u)), py = (float)(py0 + (int)(pix >> 3));
+			float inv = sample_inv_depth(b, t, px, py);
+			col = shade_sample<FP32>(c, b, t, px, py, inv);
+			key = pixel_key(inv, t);
+			atomicOr((unsigned long long *)&s_pm[pix], 1ull << lane);
+		}
+		uint32_t own63 = __shfl(own, 63, 64);
+		wave_sync_lds();
+		uint64_t bits = s_pm[lane];
+		count += (uint32_t)__popcll(bits);
+		bits &= ballot(col.contributes()); // only contributing samples are ordered and blended
+		while(ballot(bits != 0)) {
+			int src = bits ? (int)__ffsll((unsigned long long)bits) - 1 : lane;
+			uint64_t k2 = shfl64(key, src);
+			SampleColor<FP32> c2 = col.shfl(src);
+			if(bits) {
+				bits &= bits - 1;
+				ps.insert(k2, c2, additive);
+			}
+		}
+		e_cur = (sb.prefix[own63 + 1] <= s0 + 64u) ? own63 + 1 : own63;
+	}
+}
+
+template <bool FP32>
+__global__ __launch_bounds__(64) void block_raster_kernel(FrameConst c, FrameBufs b) {
+	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
+		return;
+	__shared__ uint64_t s_mask[kRegSortCap];
+	__shared__ uint32_t s_tri[kRegSortCap];
+	__shared__ uint32_t s_prefix[kRegSortCap + 1];
+	__shared__ uint32_t s_own[64];
+	__shared__ uint64_t s_pm[64];
+	const int lane = lane_id();
+	const uint32_t slot = blockIdx.x;
+	const int bin = slot >> 4, blk = slot & 15, by = blk >> 2, bx = blk & 3;
+	const int bin_x = (bin % c.bin_count_x) * kBinSize, bin_y = (bin / c.bin_count_x) * kBinSize;
+	const int px0 = bin_x + bx * 8, py0 = bin_y + by * 8;
+	const int px = px0 + (lane & 7), py = py0 + (lane >> 3);
+	const uint32_t flags = b.blk_flags[slot];
+	const uint32_t level = flags & kSlotLevelMask;
+	if(flags & (kSlotWhole | kSlotPromoted))
+		return; // pixels written by the fallback kernel, or the frame overflowed
+	Accum bg;
+	bg.init();
+	if(level == LUCID_BIN_LEVEL_EMPTY) {
+		write_pixel(c, b, px, py, bg, 0u);
+		return;
+	}
+	const uint4 sl = b.blk_slot[slot];
+	const uint32_t m = sl.y;
+	if(m == 0) {
+		write_pixel(c, b, px, py, bg, 0u);
+		return;
+	}
+	const uint4 *recs = (level == LUCID_BIN_LEVEL_LOW ? b.low_records : b.high_records) + sl.z;
+	SortedBlock sb;
+	if(flags & kSlotPresorted) {
+		uint32_t base = b.blk_sorted[slot];
+		sb.tri = b.sorted_pool + base;
+		sb.prefix = sb.tri + m;
+		sb.mask = (const uint64_t *)(b.sorted_pool + ((base + 2 * m + 2) & ~1u));
+	} else {
+		// short list: keys + register bitonic sort, then sorted arrays in LDS
+		const uint32_t *list = b.blk_lists + sl.x;
+		auto load = [&](uint32_t j) -> uint64_t {
+			return j < m ? block_key(b, recs[list[j]], bx, px0, py0, j) : ~0ull;
+		};
+		uint64_t v[4];
+		uint32_t P = next_pow2(m < 64 ? 64 : m);
+		if(P <= 64) {
+			uint64_t w1[1] = {load(lane)};
+			bitonic_regs<1>(w1);
+			v[0] = w1[0], v[1] = v[2] = v[3] = ~0ull;
+		} else if(P <= 128) {
+			uint64_t w2[2] = {load(lane), load(64 + lane)};
+			bitonic_regs<2>(w2);
+			v[0] = w2[0], v[1] = w2[1], v[2] = v[3] = ~0ull;
+		} else {
+			v[0] = load(lane), v[1] = load(64 + lane), v[2] = load(128 + lane), v[3] = load(192 + lane);
+			bitonic_regs<4>(v);
+		}
+		uint32_t carry = 0;
+#pragma unroll
+		for(int e = 0; e < 4; e++) {
+			uint32_t i = e * 64 + lane;
+			uint32_t cnt = 0;
+			if(i < m) {
+				uint4 rec = recs[list[(uint32_t)v[e]]];
+				uint64_t msk = record_block_mask(rec, bx);
+				cnt = (uint32_t)__popcll(msk);
+				s_mask[i] = msk;
+				s_tri[i] = rec.x;
+			}
+			uint32_t inc = wave_inclusive_add(cnt);
+			if(i < m)
+				s_prefix[i] = carry + inc - cnt;
+			carry += __shfl(inc, 63, 64);
+		}
+		if(lane == 0)
+			s_prefix[m] = carry;
+		wave_sync_lds();
+		sb.tri = s_tri, sb.mask = s_mask, sb.prefix = s_prefix;
+	}
+	PixelState<FP32> ps;
+	ps.init();
+	uint32_t count = 0;
+	raster_chunks<FP32>(c, b, sb, m, px0, py0, s_own, s_pm, ps, count);
+	const bool additive = (c.opts & (1u << 5)) != 0;
+	ps.flush(additive);
+	write_pixel(c, b, px, py, ps.acc, count);
+	bool on_screen = px < c.width && py < c.height;
+	uint64_t bad = ballot(ps.bad && on_screen);
+	if(bad && lane == 0)
+		enqueue_fallback(b, bin, blk, (int)level, sl.z, sl.w, px0, py0, bad, false);
+	uint32_t frags = __shfl(wave_inclusive_add(on_screen ? count : 0u), 63, 64);
+	if(lane == 0) {
+		lucid_info *info = (lucid_info *)b.info;
+		atomicAdd(&info->stats[0], frags);
+		atomicAdd(&info->stats[1], m);
 	}
 }
 
@@ -815,54 +915,39 @@ __device__ uint32_t gather_pixel_keys(const FrameBufs &b, const uint4 *recs, uin
 }
 
 template <bool FP32>
-__device__ void blend_sorted(const FrameConst &c, const FrameBufs &b, const uint64_t *keys,
-							 uint32_t n, float px, float py, bool global_src, float &cr, float &cg,
-							 float &cb, float &tr) {
+__device__ Accum blend_sorted(const FrameConst &c, const FrameBufs &b, const uint64_t *keys,
+							  uint32_t n, float px, float py, bool global_src) {
 	const int lane = lane_id();
 	const bool additive = (c.opts & (1u << 5)) != 0;
-	cr = cg = cb = 0.0f, tr = 1.0f;
+	Accum acc;
+	acc.init();
 	for(uint32_t base = 0; base < n; base += 64) {
 		uint32_t i = base + lane;
-		float4 col = make_float4(0, 0, 0, 0);
-		bool contributes = false;
+		SampleColor<FP32> col;
+		col.zero();
 		if(i < n) {
 			uint64_t k = global_src
 							 ? __hip_atomic_load(&keys[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
 							 : keys[i];
 			uint32_t t = (uint32_t)k;
-			float inv = sample_inv_depth(b, t, px, py);
-			Shaded sh = shade_sample<FP32>(c, b, t, px, py, inv);
-			col = sh.col;
-			contributes = FP32 ? sh.col.w != 0.0f : sh.enc != 0;
+			col = shade_sample<FP32>(c, b, t, px, py, sample_inv_depth(b, t, px, py));
 		}
 		uint32_t cnt = min(64u, n - base);
 		for(uint32_t j = 0; j < cnt; j++) {
-			float4 c2;
-			c2.x = __shfl(col.x, j, 64);
-			c2.y = __shfl(col.y, j, 64);
-			c2.z = __shfl(col.z, j, 64);
-			c2.w = __shfl(col.w, j, 64);
-			int ok = __shfl((int)contributes, j, 64);
-			if(!ok)
-				continue;
-			if(additive) {
-				cr = cr + c2.x * c2.w, cg = cg + c2.y * c2.w, cb = cb + c2.z * c2.w;
-			} else {
-				cr = cr + (c2.x * c2.w) * tr;
-				cg = cg + (c2.y * c2.w) * tr;
-				cb = cb + (c2.z * c2.w) * tr;
-				tr = tr * (1.0f - c2.w);
-			}
+			SampleColor<FP32> c2 = col.shfl((int)j);
+			if(c2.contributes())
+				acc.blend(c2.rgba(), additive);
 		}
 	}
+	return acc;
 }
 
 template <bool FP32>
-__global__ __launch_bounds__(kRasterThreads) void fallback_kernel(FrameConst c, FrameBufs b) {
+__global__ __launch_bounds__(kFallbackThreads) void fallback_kernel(FrameConst c, FrameBufs b) {
 	uint32_t nq = min(b.ctrl[kCtrlFallbackCount], b.fallback_cap);
 	if(blockIdx.x >= nq)
 		return;
-	__shared__ uint64_t s_keys[4][kFallbackCap];
+	__shared__ uint64_t s_keys[kFallbackThreads / 64][kFallbackCap];
 	uint4 q0 = b.fallback_queue[2 * blockIdx.x], q1 = b.fallback_queue[2 * blockIdx.x + 1];
 	int blk = (int)(q0.y & 0xffu), level = (int)((q0.y >> 8) & 0xffu);
 	bool whole = (q0.y >> 16) & 1u;
@@ -872,12 +957,13 @@ __global__ __launch_bounds__(kRasterThreads) void fallback_kernel(FrameConst c, 
 	uint64_t mask = (uint64_t)q1.x | ((uint64_t)q1.y << 32);
 	int px0 = (int)q1.z, py0 = (int)q1.w;
 	const int w = threadIdx.x >> 6, lane = lane_id();
+	const int nw = kFallbackThreads / 64;
 	uint32_t pixels = 0, frags = 0;
 	int rank = 0;
 	for(int p = 0; p < 64; p++) {
 		if(!((mask >> p) & 1ull))
 			continue;
-		if((rank++ & 3) != w)
+		if((rank++ % nw) != w)
 			continue;
 		int px = px0 + (p & 7), py = py0 + (p >> 3);
 		if(px >= c.width || py >= c.height)
@@ -886,14 +972,14 @@ __global__ __launch_bounds__(kRasterThreads) void fallback_kernel(FrameConst c, 
 		float fx = (float)px, fy = (float)py;
 		uint64_t *keys = s_keys[w];
 		uint32_t n = gather_pixel_keys(b, recs, nrec, row, xb, fx, fy, keys, kFallbackCap, false);
-		float cr, cg, cb, tr;
+		Accum acc;
 		if(n <= (uint32_t)kFallbackCap) {
 			uint32_t P = next_pow2(n < 2 ? 2 : n);
 			for(uint32_t i = n + lane; i < P; i += 64)
 				keys[i] = ~0ull;
 			wave_sync_lds();
-			bitonic_lds(keys, P);
-			blend_sorted<FP32>(c, b, keys, n, fx, fy, false, cr, cg, cb, tr);
+			bitonic_lds_wave(keys, P);
+			acc = blend_sorted<FP32>(c, b, keys, n, fx, fy, false);
 		} else {
 			// deep pixel: spill its keys to global memory
 			uint32_t P = next_pow2(n);
@@ -901,7 +987,7 @@ __global__ __launch_bounds__(kRasterThreads) void fallback_kernel(FrameConst c, 
 			if(lane == 0) {
 				off = (uint32_t)atomicAdd(&b.ctrl[kCtrlSpillCursor], P);
 				if((uint64_t)off + P > b.spill_cap) {
-					atomicOr(&b.ctrl[kCtrlOverflow], 8u);
+					atomicOr(&b.ctrl[kCtrlOverflow], kOvfSpill);
 					off = 0xffffffffu;
 				}
 			}
@@ -914,11 +1000,11 @@ __global__ __launch_bounds__(kRasterThreads) void fallback_kernel(FrameConst c, 
 				__hip_atomic_store(&g[i], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 			__threadfence();
 			__builtin_amdgcn_wave_barrier();
-			bitonic_global(g, P);
-			blend_sorted<FP32>(c, b, g, n, fx, fy, true, cr, cg, cb, tr);
+			bitonic_global_wave(g, P);
+			acc = blend_sorted<FP32>(c, b, g, n, fx, fy, true);
 		}
 		if(lane == 0)
-			write_pixel(c, b, px, py, cr, cg, cb, tr, n);
+			write_pixel(c, b, px, py, acc, n);
 		pixels++;
 		frags += n;
 	}
@@ -931,7 +1017,7 @@ __global__ __launch_bounds__(kRasterThreads) void fallback_kernel(FrameConst c, 
 		if(whole) {
 			atomicAdd(&info->stats[0], frags);
 			if(w == 0)
-				atomicAdd(&info->stats[3], 1u); // stats[3]: blocks too large for LDS
+				atomicAdd(&info->stats[3], 1u); // stats[3]: blocks too large to pre-sort
 		}
 	}
 }
@@ -941,21 +1027,26 @@ __global__ __launch_bounds__(kRasterThreads) void fallback_kernel(FrameConst c, 
 template <bool FP32>
 static void launch_raster_t(const FrameConst &c, const FrameBufs &b, hipStream_t stream,
 							hipEvent_t *ev) {
-	hipLaunchKernelGGL(raster_low_kernel<FP32>, dim3(c.bin_count), dim3(kRasterThreads), 0, stream,
-					   c, b);
+	hipLaunchKernelGGL(rowgen_low_kernel, dim3(c.bin_count), dim3(kRowgenThreads), 0, stream, c, b);
 	if(ev)
 		(void)hipEventRecord(ev[0], stream);
-	hipLaunchKernelGGL(raster_high_kernel<FP32>, dim3(c.bin_count * 4), dim3(kRasterThreads), 0,
-					   stream, c, b);
+	hipLaunchKernelGGL(rowgen_high_kernel, dim3(c.bin_count * 4), dim3(kRowgenThreads), 0, stream,
+					   c, b);
+	hipLaunchKernelGGL(block_sort_kernel, dim3(c.bin_count * 16), dim3(kSortThreads), 0, stream, c,
+					   b);
 	if(ev)
 		(void)hipEventRecord(ev[1], stream);
-	hipLaunchKernelGGL(fallback_kernel<FP32>, dim3(b.fallback_cap), dim3(kRasterThreads), 0, stream,
-					   c, b);
+	hipLaunchKernelGGL(block_raster_kernel<FP32>, dim3(c.bin_count * 16), dim3(64), 0, stream, c, b);
 	if(ev)
 		(void)hipEventRecord(ev[2], stream);
+	hipLaunchKernelGGL(fallback_kernel<FP32>, dim3(b.fallback_cap), dim3(kFallbackThreads), 0,
+					   stream, c, b);
+	if(ev)
+		(void)hipEventRecord(ev[3], stream);
 }
 
-// events (optional): recorded after raster_low, raster_high and the fallback kernel.
+// events (optional, 4): after rowgen_low, after rowgen_high + block_sort, after block_raster,
+// after the fallback kernel.
 hipError_t launch_raster(const FrameConst &c, const FrameBufs &b, hipStream_t stream,
 						 hipEvent_t *events) {
 	if(c.opts & (1u << 16))

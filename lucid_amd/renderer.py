@@ -171,7 +171,7 @@ class LucidRenderer:
                                              aabb.ctypes.data, num_quads), "lucid_read_setup")
         return recs, aabb
 
-    STAGES = ("setup", "binning", "raster_low", "raster_high", "fallback")
+    STAGES = ("setup", "binning", "rowgen_low", "rowgen_high_sort", "block_raster", "fallback")
 
     def stage_times(self) -> dict:
         """Per-stage GPU ms of the last frame (needs opts & OPT_TIMERS)."""
