@@ -49,49 +49,21 @@ def algorithmic_bytes(scene, r, info):
     cnt = info[INFO:]
     qc, qo = cnt[0:nb].astype(np.int64), cnt[nb:2 * nb].astype(np.int64)
     tc, to = cnt[3 * nb:4 * nb].astype(np.int64), cnt[4 * nb:5 * nb].astype(np.int64)
-    bin_quads, bin_tris = r.read_bin_lists()
-    high = set(int(b) for b in cnt[9 * nb: 9 * nb + int(info[9])])
-    ntri = tc + 2 * qc
     w, h = scene.width, scene.height
-    bcx = (w + 31) // 32
-
-    def pixels(bins):
-        tot = 0
-        for b in bins:
-            x0, y0 = (b % bcx) * 32, (b // bcx) * 32
-            tot += max(0, min(32, w - x0)) * max(0, min(32, h - y0))
-        return tot
-
-    low_bins = [b for b in range(nb) if b not in high]  # raster_low also writes empty bins
-    def tris_of(bins):
-        qs, ts = [], []
-        for b in bins:
-            qs.append(bin_quads[qo[b]:qo[b] + qc[b]] & 0x0FFFFFFF)
-            ts.append(bin_tris[to[b]:to[b] + tc[b]])
-        q = np.unique(np.concatenate(qs)) if qs else np.zeros(0, np.uint32)
-        t = np.unique(np.concatenate(ts)) if ts else np.zeros(0, np.uint32)
-        return len(q), len(t), int(sum(qc[b] + tc[b] for b in bins))
-
-    qlow, tlow, elow = tris_of(low_bins)
-    qhigh, thigh, ehigh = tris_of(sorted(high))
-    # setup records: 84 B per visible triangle + 20 B per visible quad (read once here)
+    # setup records: 84 B per visible triangle + 20 B per visible quad
     vis_q = int(info[1]) + int(info[2])
-    t_vis = 2 * vis_q  # upper bound; refined below from the soup structure
+    t_vis = 2 * vis_q
     if scene.quad_indices.shape[0] and np.all(scene.quad_indices[:, 2] == scene.quad_indices[:, 3]):
-        t_vis = vis_q  # degenerate quads: one triangle each
-    per_tri = 84
-    tpq = 1 if t_vis == vis_q else 2  # triangles per visible quad
+        t_vis = vis_q  # degenerate quads (a, b, c, c): one triangle each
     b_in = scene.input_bytes()
     e_total = int(qc.sum() + tc.sum())
     P = w * h
-    frame = b_in + 2 * (per_tri * t_vis + 20 * vis_q) + 2 * 4 * e_total + 8 * P
-    # raster_low: records of the distinct tris/quads its bins reference, its list entries,
-    # and the pixels it writes (low + empty bins)
-    low = (per_tri * (tpq * qlow + tlow) + 20 * qlow) + 4 * elow + 8 * pixels(low_bins)
-    hi = (per_tri * (tpq * qhigh + thigh) + 20 * qhigh) + 4 * ehigh + 8 * pixels(sorted(high))
-    return {"frame": int(frame), "raster_low": int(low), "raster_high": int(hi),
-            "t_vis": int(t_vis), "q_vis": vis_q, "e_quad": int(qc.sum()), "e_tri": int(tc.sum()),
-            "pixels": P, "b_in": int(b_in)}
+    frame = b_in + 2 * (84 * t_vis + 20 * vis_q) + 2 * 4 * e_total + 8 * P
+    # the rasteriser's share: setup records and bin lists read once, image + counts written once
+    raster = 84 * t_vis + 20 * vis_q + 4 * e_total + 8 * P
+    return {"frame": int(frame), "block_raster": int(raster), "t_vis": int(t_vis),
+            "q_vis": vis_q, "e_quad": int(qc.sum()), "e_tri": int(tc.sum()), "pixels": P,
+            "b_in": int(b_in)}
 
 
 def cpu_baseline(scene, threads: int):

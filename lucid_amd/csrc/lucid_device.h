@@ -36,7 +36,6 @@ enum CtrlWord : int {
 	kCtrlTotalBinTris = 6,
 	kCtrlFallbackPixels = 7,
 	kCtrlListCursor = 8,   // bump allocator of per-block index lists of high bins (u32 units)
-	kCtrlBigCount = 9,     // blocks whose lists are pre-sorted by block_sort_kernel
 	kCtrlSortedCursor = 10, // bump allocator of pre-sorted block arrays (u32 units)
 	kCtrlWords = 16,
 };
@@ -50,8 +49,10 @@ constexpr uint32_t kSlotLevelMask = 0xffu; // LUCID_BIN_LEVEL_*; 0 = empty bin
 constexpr uint32_t kSlotPresorted = 1u << 8;
 constexpr uint32_t kSlotPromoted = 1u << 9;  // low bin handed to the high path
 constexpr uint32_t kSlotWhole = 1u << 10;    // list too long: every pixel takes the fallback
+constexpr uint32_t kSlotPartShift = 12;      // log2 of the waves sharing a pre-sorted block
 constexpr int kRegSortCap = 256;  // lists sorted in registers by the raster wave itself
-constexpr int kBigSortCap = 8192; // lists sorted in LDS by block_sort_kernel
+constexpr int kBigSortCap = 8192; // lists sorted in LDS by high_finalize_kernel
+constexpr int kRowgenSplit = 8;   // work-groups per (high bin, band) in rowgen_high
 
 enum ColorMode : int { kColorNone = 0, kColorRGBA8 = 1, kColorF32 = 2 };
 
@@ -110,7 +111,8 @@ struct FrameBufs {
 	uint32_t *blk_sorted; // bin_count * 16: offset of the pre-sorted arrays
 	uint32_t *blk_lists;  // low bins: fixed bin_count * 16 * kRegSortCap; then high bump region
 	uint64_t blk_lists_cap;
-	uint32_t *big_queue;  // bin_count * 16
+	uint32_t *blk_count;  // bin_count * 16: list lengths of high bins (atomic counters)
+	uint32_t *band_count; // bin_count * 4: record counts of high-bin bands (atomic counters)
 	uint32_t *sorted_pool;
 	uint64_t sorted_cap;
 	uint4 *fallback_queue; // 2 uint4 per entry

@@ -76,7 +76,7 @@ struct lucid_renderer {
 		tri_color, large_list;
 	DevBuf wg_hist, bin_quads, bin_tris;
 	DevBuf low_records, high_records, fallback_queue, spill;
-	DevBuf blk_slot, blk_flags, blk_sorted, blk_lists, big_queue, sorted_pool;
+	DevBuf blk_slot, blk_flags, blk_sorted, blk_lists, blk_count, band_count, sorted_pool;
 
 	int64_t quad_cap = 0;
 	uint64_t bin_quads_cap = 0, bin_tris_cap = 0, high_records_cap = 0, spill_cap = 0;
@@ -181,6 +181,12 @@ int grow_after_overflow(lucid_renderer *r) {
 		LUCID_CHECK(r->sorted_pool.reserve(need * 4));
 		r->sorted_cap = need;
 	}
+	if(flags & kOvfFallback) {
+		uint64_t need = std::max<uint64_t>((uint64_t)r->status[kCtrlFallbackCount],
+										   (uint64_t)r->fallback_cap * 2);
+		LUCID_CHECK(r->fallback_queue.reserve(need * 32));
+		r->fallback_cap = (uint32_t)need;
+	}
 	return LUCID_ERR_CAPACITY;
 }
 
@@ -267,7 +273,8 @@ int lucid_create(const lucid_create_info *ci, lucid_renderer **out) {
 	chk(r->ctrl.reserve(kCtrlWords * 4));
 	chk(r->wg_hist.reserve((size_t)kNumBinWgs * 2 * NB * 4));
 	chk(r->low_records.reserve((size_t)NB * 4 * kLowRowCap * 16));
-	r->fallback_cap = (uint32_t)NB * 16;
+	// at most one entry per wave of a block: 8 row parts, or one whole-block entry
+	r->fallback_cap = (uint32_t)NB * 16 * 8;
 	chk(r->fallback_queue.reserve((size_t)r->fallback_cap * 32));
 	r->high_records_cap = 1u << 22;
 	chk(r->high_records.reserve(r->high_records_cap * 16));
@@ -276,7 +283,8 @@ int lucid_create(const lucid_create_info *ci, lucid_renderer **out) {
 	chk(r->blk_slot.reserve((size_t)NB * 16 * 16));
 	chk(r->blk_flags.reserve((size_t)NB * 16 * 4));
 	chk(r->blk_sorted.reserve((size_t)NB * 16 * 4));
-	chk(r->big_queue.reserve((size_t)NB * 16 * 4));
+	chk(r->blk_count.reserve((size_t)NB * 16 * 4));
+	chk(r->band_count.reserve((size_t)NB * 4 * 4));
 	r->blk_lists_cap = (uint64_t)NB * 16 * kRegSortCap + (1u << 22);
 	chk(r->blk_lists.reserve(r->blk_lists_cap * 4));
 	r->sorted_cap = 1u << 22;
@@ -430,7 +438,8 @@ int lucid_render(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_fr
 	b.blk_sorted = r->blk_sorted.as<uint32_t>();
 	b.blk_lists = r->blk_lists.as<uint32_t>();
 	b.blk_lists_cap = r->blk_lists_cap;
-	b.big_queue = r->big_queue.as<uint32_t>();
+	b.blk_count = r->blk_count.as<uint32_t>();
+	b.band_count = r->band_count.as<uint32_t>();
 	b.sorted_pool = r->sorted_pool.as<uint32_t>();
 	b.sorted_cap = r->sorted_cap;
 	b.image = out->image_rgba8;

@@ -8,11 +8,12 @@
 //                (raster_low.glsl:39-106,229-237).
 //   rowgen_high  one work-group per (high bin, 8-row band) with the 4-row rule of
 //                raster_high.glsl:54-90.
-//   block_sort   one work-group per block with more than 256 triangles: centroid-depth keys
-//                sorted by an LDS bitonic sort, written back as sorted (tri, mask, prefix) arrays.
+//   high_prep / high_finalize  allocation for high bins; blocks with more than 256 triangles
+//                are sorted by centroid depth (LDS bitonic, whole work-group) and split into
+//                pixel-row parts so that several waves share a dense block.
 //   block_raster one wave per block: sorts short lists itself in registers, expands coverage
 //                masks into dense 64-sample chunks, shades, routes samples to pixel lanes and
-//                blends front to back.
+//                blends front to back (block_raster_big: one wave per part of a sorted block).
 //   fallback     exact per-pixel sort for pixels whose speculative order was violated.
 //
 // Exact ordering: triangles of a block are pre-sorted by the depth at their covered-pixel
@@ -514,147 +515,192 @@ __global__ __launch_bounds__(kRowgenThreads) void rowgen_low_kernel(FrameConst c
 	}
 }
 
-// ------------------------------------------------------------------ rowgen_high
+// ------------------------------------------------------------------ high bins
 
-__global__ __launch_bounds__(kRowgenThreads) void rowgen_high_kernel(FrameConst c, FrameBufs b) {
+// Allocates record and list space of every high bin (promotions included) with an exclusive
+// scan over the high list, and clears the counters rowgen_high accumulates into.
+__global__ __launch_bounds__(1024) void high_prep_kernel(FrameConst c, FrameBufs b) {
 	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
 		return;
 	const lucid_info *info = (const lucid_info *)b.info;
-	const uint32_t slot = blockIdx.x >> 2;
-	const int by = blockIdx.x & 3;
+	const uint32_t nh = (uint32_t)info->bin_level_counts[LUCID_BIN_LEVEL_HIGH];
+	const uint32_t *high = b.info + LUCID_INFO_U32_SIZE + LUCID_COUNTS_HIGH_BINS * c.bin_count;
+	__shared__ uint32_t s_wave[16];
+	__shared__ uint32_t s_carry;
+	if(threadIdx.x == 0)
+		s_carry = 0;
+	__syncthreads();
+	const uint64_t list0 = (uint64_t)c.bin_count * 16 * kRegSortCap;
+	const int wave = threadIdx.x >> 6;
+	for(uint32_t base = 0; base < nh; base += 1024) {
+		uint32_t i = base + threadIdx.x;
+		int bin = i < nh ? (int)high[i] : 0;
+		uint32_t n = i < nh ? load_bin(c, b, bin).nlevel : 0u;
+		uint32_t inc = wave_inclusive_add(n);
+		if(lane_id() == 63)
+			s_wave[wave] = inc;
+		__syncthreads();
+		if(threadIdx.x == 0) {
+			uint32_t run = s_carry;
+			for(int w = 0; w < 16; w++) {
+				uint32_t v = s_wave[w];
+				s_wave[w] = run;
+				run += v;
+			}
+			s_carry = run;
+		}
+		__syncthreads();
+		if(i < nh) {
+			uint64_t ex = (uint64_t)s_wave[wave] + inc - n; // tris of earlier high bins
+			uint64_t rec = 4 * ex, lst = list0 + 16 * ex;
+			bool ok = rec + 4ull * n <= b.high_records_cap && lst + 16ull * n <= b.blk_lists_cap;
+			if(!ok)
+				atomicOr(&b.ctrl[kCtrlOverflow], kOvfRecords | kOvfLists);
+			for(int blk = 0; blk < 16; blk++) {
+				int band = blk >> 2, bx = blk & 3;
+				b.blk_slot[(size_t)bin * 16 + blk] =
+					make_uint4((uint32_t)(lst + (uint64_t)(band * 4 + bx) * n), 0u,
+							   (uint32_t)(rec + (uint64_t)band * n), 0u);
+				b.blk_count[(size_t)bin * 16 + blk] = 0;
+				b.blk_flags[(size_t)bin * 16 + blk] = ok ? 0u : kSlotPromoted;
+			}
+			for(int band = 0; band < 4; band++)
+				b.band_count[(size_t)bin * 4 + band] = 0;
+		}
+		__syncthreads();
+	}
+}
+
+// Records of one 8-row band of a high bin with the 4-row rule (raster_high.glsl:54-90).
+// kRowgenSplit work-groups share a band; positions come from wave-aggregated atomics.
+__global__ __launch_bounds__(kRowgenThreads) void rowgen_high_kernel(FrameConst c, FrameBufs b) {
+	if(b.ctrl[kCtrlOverflow] & (kOvfBinLists | kOvfRecords | kOvfLists))
+		return;
+	const lucid_info *info = (const lucid_info *)b.info;
+	const uint32_t slot = blockIdx.x / (4 * kRowgenSplit);
+	const int by = (blockIdx.x / kRowgenSplit) & 3, split = blockIdx.x % kRowgenSplit;
 	if(slot >= (uint32_t)info->bin_level_counts[LUCID_BIN_LEVEL_HIGH])
 		return;
 	const uint32_t *counts = b.info + LUCID_INFO_U32_SIZE;
 	const int bin = (int)counts[LUCID_COUNTS_HIGH_BINS * c.bin_count + slot];
 	const BinInfo bi = load_bin(c, b, bin);
-
-	__shared__ uint32_t s_row_n;
-	__shared__ uint32_t s_blk_n[4];
-	__shared__ uint32_t s_rec_base, s_list_base;
-	if(threadIdx.x < 4)
-		s_blk_n[threadIdx.x] = 0;
-	if(threadIdx.x == 0) {
-		s_row_n = 0;
-		uint32_t rb = atomicAdd(&b.ctrl[kCtrlRecordCursor], bi.nlevel);
-		if((uint64_t)rb + bi.nlevel > b.high_records_cap) {
-			atomicOr(&b.ctrl[kCtrlOverflow], kOvfRecords);
-			rb = 0xffffffffu;
-		}
-		// high-bin lists follow the fixed region of the low bins
-		uint64_t lb = (uint64_t)c.bin_count * 16 * kRegSortCap +
-					  atomicAdd(&b.ctrl[kCtrlListCursor], 4 * bi.nlevel);
-		if(lb + 4ull * bi.nlevel > b.blk_lists_cap) {
-			atomicOr(&b.ctrl[kCtrlOverflow], kOvfLists);
-			lb = 0xffffffffu;
-		}
-		s_rec_base = rb, s_list_base = (uint32_t)lb;
-	}
-	__syncthreads();
-	uint32_t *flags = b.blk_flags + (size_t)bin * 16 + by * 4;
-	if(s_rec_base == 0xffffffffu || s_list_base == 0xffffffffu) {
-		if(threadIdx.x < 4)
-			flags[threadIdx.x] = kSlotPromoted; // nothing rendered; the frame reports overflow
-		return;
-	}
-	uint4 *recs = b.high_records + s_rec_base;
-	uint32_t *lists = b.blk_lists + s_list_base;
+	const uint4 *slot4 = b.blk_slot + (size_t)bin * 16 + by * 4;
+	uint4 *recs = b.high_records + slot4[0].z;
+	uint32_t *band_n = b.band_count + (size_t)bin * 4 + by;
+	uint32_t *blk_n = b.blk_count + (size_t)bin * 16 + by * 4;
 	const int band0 = by * 8;
-	// records of this 8-row band with the 4-row rule (raster_high.glsl:54-90)
-	for(uint32_t i = threadIdx.x; i < bi.nlevel; i += kRowgenThreads) {
-		uint32_t t = bin_tri_at(b, i, bi.qc, bi.qo, bi.to);
-		if(t == 0xffffffffu)
-			continue;
-		uint4 s0 = b.tri_scan[2 * (size_t)t];
-		int min_g = min(max((int)(s0.w & 0xffffu) - bi.bin_y, 0), kBinMask) >> 2;
-		int max_g = min(max((int)(s0.w >> 16) - bi.bin_y, 0), kBinMask) >> 2;
-		int row_lo = min_g * 4, row_hi = max_g * 4 + 3;
-		if(row_hi < band0 || row_lo > band0 + 7)
-			continue;
-		uint4 s1 = b.tri_scan[2 * (size_t)t + 1];
-		RowScan rs;
-		row_scan_init(rs, s0, s1, (float)bi.bin_x, (float)(bi.bin_y + row_lo));
+	const int lane = lane_id();
+	const uint32_t stride = kRowgenThreads * kRowgenSplit;
+	const uint32_t wave_base = split * kRowgenThreads + (threadIdx.x & ~63u);
+	for(uint32_t base = wave_base; base < bi.nlevel; base += stride) {
+		uint32_t i = base + lane;
+		uint32_t t = i < bi.nlevel ? bin_tri_at(b, i, bi.qc, bi.qo, bi.to) : 0xffffffffu;
 		uint32_t sp[8], bxm = 0;
+		if(t != 0xffffffffu) {
+			uint4 s0 = b.tri_scan[2 * (size_t)t];
+			int min_g = min(max((int)(s0.w & 0xffffu) - bi.bin_y, 0), kBinMask) >> 2;
+			int max_g = min(max((int)(s0.w >> 16) - bi.bin_y, 0), kBinMask) >> 2;
+			int row_lo = min_g * 4, row_hi = max_g * 4 + 3;
+			if(row_hi >= band0 && row_lo <= band0 + 7) {
+				uint4 s1 = b.tri_scan[2 * (size_t)t + 1];
+				RowScan rs;
+				row_scan_init(rs, s0, s1, (float)bi.bin_x, (float)(bi.bin_y + row_lo));
 #pragma unroll
-		for(int r = 0; r < 8; r++)
-			sp[r] = kBinMask; // empty (31, 0)
-		int last = min(row_hi, band0 + 7);
-		for(int row = row_lo; row <= last; row++) {
-			uint32_t v = row_scan_step(rs);
-			if(row >= band0) {
-				int r = row - band0;
+				for(int r = 0; r < 8; r++)
+					sp[r] = kBinMask; // empty (31, 0)
+				int last = min(row_hi, band0 + 7);
+				for(int row = row_lo; row <= last; row++) {
+					uint32_t v = row_scan_step(rs);
+					if(row >= band0) {
+						int r = row - band0;
 #pragma unroll
-				for(int k = 0; k < 8; k++)
-					if(k == r)
-						sp[k] = v;
-				bxm |= span_bx_mask(v);
+						for(int k = 0; k < 8; k++)
+							if(k == r)
+								sp[k] = v;
+						bxm |= span_bx_mask(v);
+					}
+				}
 			}
 		}
-		if(!bxm)
+		uint64_t emit = ballot(bxm != 0);
+		if(!emit)
 			continue;
-		uint32_t idx = atomicAdd(&s_row_n, 1u);
-		recs[idx] = make_record(t, sp, bxm);
-		for(int bx = 0; bx < 4; bx++)
-			if(bxm & (1u << bx)) {
-				uint32_t k = atomicAdd(&s_blk_n[bx], 1u);
-				lists[bx * bi.nlevel + k] = idx;
-			}
-	}
-	__syncthreads();
-	if(threadIdx.x < 4) {
-		int bx = threadIdx.x, blk = by * 4 + bx;
-		uint32_t m = s_blk_n[bx];
-		b.blk_slot[(size_t)bin * 16 + blk] =
-			make_uint4(s_list_base + bx * bi.nlevel, m, s_rec_base, s_row_n);
-		uint32_t f = LUCID_BIN_LEVEL_HIGH;
-		if(m > (uint32_t)kBigSortCap) {
-			f |= kSlotWhole;
-			enqueue_fallback(b, bin, blk, LUCID_BIN_LEVEL_HIGH, s_rec_base, s_row_n,
-							 bi.bin_x + bx * 8, bi.bin_y + band0, ~0ull, true);
-		} else if(m > (uint32_t)kRegSortCap) {
-			f |= kSlotPresorted;
-			uint32_t q = atomicAdd(&b.ctrl[kCtrlBigCount], 1u);
-			b.big_queue[q] = (uint32_t)(bin * 16 + blk); // at most 16 * bin_count entries
+		uint32_t rbase = 0;
+		if(lane == 0)
+			rbase = atomicAdd(band_n, (uint32_t)__popcll(emit));
+		rbase = __shfl(rbase, 0, 64);
+		uint32_t idx = rbase + (uint32_t)__popcll(emit & ((1ull << lane) - 1ull));
+		if(bxm)
+			recs[idx] = make_record(t, sp, bxm);
+#pragma unroll
+		for(int bx = 0; bx < 4; bx++) {
+			uint64_t sel = ballot((bxm >> bx) & 1u);
+			if(!sel)
+				continue;
+			uint32_t lb = 0;
+			if(lane == 0)
+				lb = atomicAdd(&blk_n[bx], (uint32_t)__popcll(sel));
+			lb = __shfl(lb, 0, 64);
+			if((bxm >> bx) & 1u)
+				b.blk_lists[slot4[bx].x + lb + (uint32_t)__popcll(sel & ((1ull << lane) - 1ull))] = idx;
 		}
-		flags[bx] = f;
 	}
 }
 
-// ------------------------------------------------------------------ block_sort
+// Pixel bits of part p when a block's 8 rows are split evenly over `parts` waves.
+__device__ __forceinline__ uint64_t part_bits(int p, uint32_t parts) {
+	if(parts == 1)
+		return ~0ull;
+	int bits = 64 / (int)parts;
+	return ((1ull << bits) - 1ull) << (p * bits);
+}
 
-// Sorts one long block list in LDS with the whole work-group and stores, in sorted order,
-// the triangle ids, the block coverage masks and the exclusive prefix of sample counts.
-__global__ __launch_bounds__(kSortThreads) void block_sort_kernel(FrameConst c, FrameBufs b) {
-	if(blockIdx.x >= b.ctrl[kCtrlBigCount])
+// Finalises every block of the high bins: copies the list lengths into the slot table and
+// pre-sorts long lists. A long list is sorted once by centroid depth (LDS bitonic by the
+// whole work-group) and split into `parts` pixel-row groups, each a compacted list of the
+// entries that cover it, so that several waves can rasterise a dense block.
+__global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst c, FrameBufs b) {
+	if(b.ctrl[kCtrlOverflow] & (kOvfBinLists | kOvfRecords | kOvfLists))
 		return;
-	__shared__ uint64_t s_key[kBigSortCap];
-	__shared__ uint32_t s_sum[kSortThreads];
-	__shared__ uint32_t s_base;
-	const uint32_t slot = b.big_queue[blockIdx.x];
-	const int bin = slot >> 4, blk = slot & 15, by = blk >> 2, bx = blk & 3;
+	const lucid_info *info = (const lucid_info *)b.info;
+	const uint32_t hslot = blockIdx.x >> 4;
+	const int blk = blockIdx.x & 15, by = blk >> 2, bx = blk & 3;
+	if(hslot >= (uint32_t)info->bin_level_counts[LUCID_BIN_LEVEL_HIGH])
+		return;
+	const uint32_t *counts = b.info + LUCID_INFO_U32_SIZE;
+	const int bin = (int)counts[LUCID_COUNTS_HIGH_BINS * c.bin_count + hslot];
+	const uint32_t slot = (uint32_t)bin * 16 + blk;
+	if(b.blk_flags[slot] & kSlotPromoted)
+		return; // allocation failed: the frame reports an overflow
+	const uint32_t m = b.blk_count[slot];
+	const uint32_t nrec = b.band_count[(size_t)bin * 4 + by];
 	const uint4 sl = b.blk_slot[slot];
-	const uint32_t m = sl.y;
-	const uint32_t *list = b.blk_lists + sl.x;
-	const uint4 *recs = b.high_records + sl.z;
 	const int bin_x = (bin % c.bin_count_x) * kBinSize, bin_y = (bin / c.bin_count_x) * kBinSize;
 	const int px0 = bin_x + bx * 8, py0 = bin_y + by * 8;
-	const uint32_t P = next_pow2(m);
-	if(threadIdx.x == 0) {
-		uint32_t need = 4 * m + 4;
-		uint32_t base = atomicAdd(&b.ctrl[kCtrlSortedCursor], need);
-		if((uint64_t)base + need > b.sorted_cap) {
-			atomicOr(&b.ctrl[kCtrlOverflow], kOvfSorted);
-			base = 0xffffffffu;
-		}
-		s_base = base;
+	if(threadIdx.x == 0)
+		b.blk_slot[slot] = make_uint4(sl.x, m, sl.z, nrec);
+	if(m <= (uint32_t)kRegSortCap) {
+		if(threadIdx.x == 0)
+			b.blk_flags[slot] = LUCID_BIN_LEVEL_HIGH;
+		return;
 	}
+	if(m > (uint32_t)kBigSortCap) {
+		if(threadIdx.x == 0) {
+			b.blk_flags[slot] = LUCID_BIN_LEVEL_HIGH | kSlotWhole;
+			enqueue_fallback(b, bin, blk, LUCID_BIN_LEVEL_HIGH, sl.z, nrec, px0, py0, ~0ull, true);
+		}
+		return;
+	}
+	__shared__ uint64_t s_key[kBigSortCap];
+	__shared__ uint32_t s_part[8][kSortThreads];
+	__shared__ uint32_t s_samples[kSortThreads];
+	__shared__ uint32_t s_base, s_parts, s_total;
+	const uint32_t *list = b.blk_lists + sl.x;
+	const uint4 *recs = b.high_records + sl.z;
+	const uint32_t P = next_pow2(m);
 	for(uint32_t j = threadIdx.x; j < P; j += kSortThreads)
 		s_key[j] = j < m ? block_key(b, recs[list[j]], bx, px0, py0, j) : ~0ull;
 	__syncthreads();
-	if(s_base == 0xffffffffu) {
-		if(threadIdx.x == 0)
-			b.blk_flags[slot] = kSlotPromoted; // nothing rendered; the frame reports overflow
-		return;
-	}
 	for(uint32_t k = 2; k <= P; k <<= 1) {
 		for(uint32_t j = k >> 1; j > 0; j >>= 1) {
 			for(uint32_t i = threadIdx.x; i < P / 2; i += kSortThreads) {
@@ -670,99 +716,162 @@ __global__ __launch_bounds__(kSortThreads) void block_sort_kernel(FrameConst c, 
 			__syncthreads();
 		}
 	}
-	// Layout in the pool: tri[m], prefix[m + 1], pad to 8 bytes, mask[m] (u64).
-	const uint32_t base = (s_base + 1u) & ~1u;
-	uint32_t *tri = b.sorted_pool + base;
-	uint32_t *prefix = tri + m;
-	uint64_t *mask = (uint64_t *)(b.sorted_pool + ((base + 2 * m + 2) & ~1u));
-	// each thread owns a contiguous run of sorted elements
+	// s_key[i] now holds the list position of the i-th entry in depth order.
 	const uint32_t per = (m + kSortThreads - 1) / kSortThreads;
 	const uint32_t i0 = min(m, threadIdx.x * per), i1 = min(m, i0 + per);
-	uint32_t run = 0;
-	for(uint32_t i = i0; i < i1; i++) {
-		uint32_t j = (uint32_t)s_key[i];
-		uint4 rec = recs[list[j]];
-		uint64_t msk = record_block_mask(rec, bx);
-		tri[i] = rec.x;
-		mask[i] = msk;
-		run += (uint32_t)__popcll(msk);
-	}
-	s_sum[threadIdx.x] = run;
+	uint32_t samples = 0;
+	for(uint32_t i = i0; i < i1; i++)
+		samples += (uint32_t)__popcll(record_block_mask(recs[list[(uint32_t)s_key[i]]], bx));
+	s_samples[threadIdx.x] = samples;
 	__syncthreads();
-	if(threadIdx.x < 64) {
-		// exclusive scan of the 256 per-thread sums by one wave (4 per lane)
+	if(threadIdx.x == 0) {
+		uint32_t tot = 0;
+		for(int t = 0; t < kSortThreads; t++)
+			tot += s_samples[t];
+		// waves per block: about 4k samples or more each
+		s_parts = tot <= 4096u ? 1u : tot <= 8192u ? 2u : tot <= 16384u ? 4u : 8u;
+	}
+	__syncthreads();
+	const uint32_t parts = s_parts;
+	uint32_t pc[8];
+#pragma unroll
+	for(int p = 0; p < 8; p++)
+		pc[p] = 0;
+	for(uint32_t i = i0; i < i1; i++) {
+		uint64_t msk = record_block_mask(recs[list[(uint32_t)s_key[i]]], bx);
+#pragma unroll
+		for(int p = 0; p < 8; p++)
+			if(p < (int)parts && (msk & part_bits(p, parts)))
+				pc[p]++;
+	}
+#pragma unroll
+	for(int p = 0; p < 8; p++)
+		s_part[p][threadIdx.x] = pc[p];
+	__syncthreads();
+	// exclusive scan of each part's per-thread counts (wave w scans parts w, w + 4)
+	const int wave = threadIdx.x >> 6, lane = lane_id();
+	for(int p = wave; p < (int)parts; p += kSortThreads / 64) {
 		uint32_t v[4], tot = 0;
 #pragma unroll
 		for(int q = 0; q < 4; q++)
-			v[q] = s_sum[threadIdx.x * 4 + q], tot += v[q];
+			v[q] = s_part[p][lane * 4 + q], tot += v[q];
 		uint32_t inc = wave_inclusive_add(tot);
 		uint32_t ex = inc - tot;
 #pragma unroll
 		for(int q = 0; q < 4; q++) {
 			uint32_t t = v[q];
-			s_sum[threadIdx.x * 4 + q] = ex;
+			s_part[p][lane * 4 + q] = ex;
 			ex += t;
 		}
-		if(threadIdx.x == 63)
-			prefix[m] = inc;
+		if(lane == 63)
+			s_samples[p] = inc; // part sizes (s_samples is free again)
 	}
 	__syncthreads();
-	run = s_sum[threadIdx.x];
-	for(uint32_t i = i0; i < i1; i++) {
-		prefix[i] = run;
-		run += (uint32_t)__popcll(mask[i]);
+	if(threadIdx.x == 0) {
+		uint32_t total = 0;
+		for(uint32_t p = 0; p < parts; p++)
+			total += s_samples[p];
+		// layout: 18-word header {parts, count[8], offset[8], pad}, tri[total], pad, mask[total]
+		uint32_t need = 18 + total + 2 + 2 * total + 2;
+		uint32_t base = atomicAdd(&b.ctrl[kCtrlSortedCursor], need);
+		if((uint64_t)base + need > b.sorted_cap) {
+			atomicOr(&b.ctrl[kCtrlOverflow], kOvfSorted);
+			base = 0xffffffffu;
+		} else {
+			base = (base + 1u) & ~1u;
+			uint32_t *hdr = b.sorted_pool + base;
+			hdr[0] = parts;
+			uint32_t off = 0;
+			for(uint32_t p = 0; p < parts; p++) {
+				hdr[1 + p] = s_samples[p];
+				hdr[9 + p] = off;
+				off += s_samples[p];
+			}
+		}
+		s_base = base;
+		s_total = total;
 	}
-	if(threadIdx.x == 0)
+	__syncthreads();
+	if(s_base == 0xffffffffu) {
+		if(threadIdx.x == 0)
+			b.blk_flags[slot] = kSlotPromoted; // nothing rendered; the frame reports overflow
+		return;
+	}
+	const uint32_t base = s_base, total = s_total;
+	const uint32_t *hdr = b.sorted_pool + base;
+	uint32_t *tri_out = b.sorted_pool + base + 18;
+	uint64_t *mask_out = (uint64_t *)(b.sorted_pool + ((base + 18 + total + 2) & ~1u));
+	uint32_t pos[8];
+#pragma unroll
+	for(int p = 0; p < 8; p++)
+		pos[p] = p < (int)parts ? hdr[9 + p] + s_part[p][threadIdx.x] : 0u;
+	for(uint32_t i = i0; i < i1; i++) {
+		uint4 rec = recs[list[(uint32_t)s_key[i]]];
+		uint64_t msk = record_block_mask(rec, bx);
+#pragma unroll
+		for(int p = 0; p < 8; p++) {
+			uint64_t part_mask = p < (int)parts ? msk & part_bits(p, parts) : 0ull;
+			if(part_mask) {
+				tri_out[pos[p]] = rec.x;
+				mask_out[pos[p]] = part_mask;
+				pos[p]++;
+			}
+		}
+	}
+	if(threadIdx.x == 0) {
 		b.blk_sorted[slot] = base;
+		b.blk_flags[slot] =
+			LUCID_BIN_LEVEL_HIGH | kSlotPresorted | ((uint32_t)(31 - __clz(parts)) << kSlotPartShift);
+	}
 }
 
 // ------------------------------------------------------------------ block_raster
 
-// Sorted block arrays, in LDS (short lists) or global memory (pre-sorted long lists).
-struct SortedBlock {
-	const uint32_t *tri;
-	const uint64_t *mask;
-	const uint32_t *prefix; // m + 1 entries
-};
-
+// Streams the sorted entries of one (sub-)block through the sample chunks. Entries are
+// read 64 at a time; each lane holds one entry's mask and triangle, a wave scan gives sample
+// offsets, and each sample finds its entry by a max-scan over marked start positions.
+// `pix0` is the first pixel of the sub-block; lanes [0, npix) own its pixels.
 template <bool FP32>
-__device__ __forceinline__ void raster_chunks(const FrameConst &c, const FrameBufs &b,
-											  const SortedBlock &sb, uint32_t m, int px0, int py0,
+__device__ __forceinline__ void raster_stream(const FrameConst &c, const FrameBufs &b,
+											  const uint32_t *tri, const uint64_t *mask,
+											  uint32_t m, int px0, int py0, int pix0,
 											  uint32_t *s_own, uint64_t *s_pm,
 											  PixelState<FP32> &ps, uint32_t &count) {
 	const int lane = lane_id();
 	const bool additive = (c.opts & (1u << 5)) != 0;
-	const uint32_t total = sb.prefix[m];
-	uint32_t e_cur = 0;
-	for(uint32_t s0 = 0; s0 < total; s0 += 64) {
+	uint32_t e_cur = 0, s_start = 0, s0 = 0;
+	while(e_cur < m) {
+		uint32_t e = e_cur + lane;
+		uint64_t msk = e < m ? mask[e] : 0ull;
+		uint32_t tr = e < m ? tri[e] : 0u;
+		uint32_t cnt = (uint32_t)__popcll(msk);
+		uint32_t inc = wave_inclusive_add(cnt);
+		uint32_t st = s_start + inc - cnt;
+		uint32_t win_end = s_start + __shfl(inc, 63, 64);
+		uint32_t s_end = min(s0 + 64u, win_end);
 		s_own[lane] = 0;
 		s_pm[lane] = 0;
 		wave_sync_lds();
-		// mark the first sample of every entry that starts inside this chunk
-		uint32_t e = e_cur + lane;
-		if(e < m) {
-			uint32_t st = sb.prefix[e];
-			if(st < s0 + 64u)
-				s_own[st > s0 ? st - s0 : 0] = e;
-		}
+		if(cnt && st < s_end)
+			s_own[st > s0 ? st - s0 : 0] = (uint32_t)lane;
 		wave_sync_lds();
 		uint32_t own = wave_inclusive_max(s_own[lane]);
 		uint32_t s = s0 + lane;
-		bool valid = s < total;
+		bool valid = s < s_end;
+		uint32_t o_st = __shfl(st, (int)own, 64);
+		uint64_t o_msk = shfl64(msk, (int)own);
+		uint32_t o_tri = __shfl(tr, (int)own, 64);
 		uint64_t key = ~0ull;
 		SampleColor<FP32> col;
 		col.zero();
 		if(valid) {
-			uint32_t k = s - sb.prefix[own];
-			uint32_t pix = (uint32_t)select_bit64(sb.mask[own], k);
-			uint32_t t = sb.tri[own];
+			uint32_t pix = (uint32_t)select_bit64(o_msk, s - o_st);
 			float px = (float)(px0 + (int)(pix & 7u)), py = (float)(py0 + (int)(pix >> 3));
-			float inv = sample_inv_depth(b, t, px, py);
-			col = shade_sample<FP32>(c, b, t, px, py, inv);
-			key = pixel_key(inv, t);
-			atomicOr((unsigned long long *)&s_pm[pix], 1ull << lane);
+			float inv = sample_inv_depth(b, o_tri, px, py);
+			col = shade_sample<FP32>(c, b, o_tri, px, py, inv);
+			key = pixel_key(inv, o_tri);
+			atomicOr((unsigned long long *)&s_pm[pix - pix0], 1ull << lane);
 		}
-		uint32_t own63 = __shfl(own, 63, 64);
 		wave_sync_lds();
 		uint64_t bits = s_pm[lane];
 		count += (uint32_t)__popcll(bits);
@@ -776,17 +885,50 @@ __device__ __forceinline__ void raster_chunks(const FrameConst &c, const FrameBu
 				ps.insert(k2, c2, additive);
 			}
 		}
-		e_cur = (sb.prefix[own63 + 1] <= s0 + 64u) ? own63 + 1 : own63;
+		// advance to the entry holding sample s_end
+		uint32_t done = (uint32_t)__popcll(ballot(st + cnt <= s_end));
+		if(done >= 64u) {
+			e_cur += 64u;
+			s_start = win_end;
+		} else {
+			s_start = __shfl(st, (int)done, 64);
+			e_cur += done;
+		}
+		s0 = s_end;
 	}
 }
 
+// Pixel lane i of the wave owns block pixel pix0 + i.
+template <bool FP32>
+__device__ __forceinline__ void finish_block(const FrameConst &c, const FrameBufs &b,
+											 PixelState<FP32> &ps, uint32_t count, int px, int py,
+											 bool own_pixel, int pix0, int bin, int blk, int level,
+											 uint4 sl, int px0, int py0, uint32_t m) {
+	const int lane = lane_id();
+	const bool additive = (c.opts & (1u << 5)) != 0;
+	ps.flush(additive);
+	bool on_screen = own_pixel && px < c.width && py < c.height;
+	if(own_pixel)
+		write_pixel(c, b, px, py, ps.acc, count);
+	uint64_t bad = ballot(ps.bad && on_screen);
+	uint32_t frags = __shfl(wave_inclusive_add(on_screen ? count : 0u), 63, 64);
+	if(lane == 0) {
+		if(bad)
+			enqueue_fallback(b, bin, blk, level, sl.z, sl.w, px0, py0, bad << pix0, false);
+		lucid_info *info = (lucid_info *)b.info;
+		atomicAdd(&info->stats[0], frags);
+		if(m)
+			atomicAdd(&info->stats[1], m);
+	}
+}
+
+// One wave per block whose list fits the register sort (every low bin, most high blocks).
 template <bool FP32>
 __global__ __launch_bounds__(64) void block_raster_kernel(FrameConst c, FrameBufs b) {
 	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
 		return;
 	__shared__ uint64_t s_mask[kRegSortCap];
 	__shared__ uint32_t s_tri[kRegSortCap];
-	__shared__ uint32_t s_prefix[kRegSortCap + 1];
 	__shared__ uint32_t s_own[64];
 	__shared__ uint64_t s_pm[64];
 	const int lane = lane_id();
@@ -797,8 +939,8 @@ __global__ __launch_bounds__(64) void block_raster_kernel(FrameConst c, FrameBuf
 	const int px = px0 + (lane & 7), py = py0 + (lane >> 3);
 	const uint32_t flags = b.blk_flags[slot];
 	const uint32_t level = flags & kSlotLevelMask;
-	if(flags & (kSlotWhole | kSlotPromoted))
-		return; // pixels written by the fallback kernel, or the frame overflowed
+	if(flags & (kSlotWhole | kSlotPromoted | kSlotPresorted))
+		return; // fallback kernel, overflowed frame, or block_raster_big
 	Accum bg;
 	bg.init();
 	if(level == LUCID_BIN_LEVEL_EMPTY) {
@@ -812,71 +954,84 @@ __global__ __launch_bounds__(64) void block_raster_kernel(FrameConst c, FrameBuf
 		return;
 	}
 	const uint4 *recs = (level == LUCID_BIN_LEVEL_LOW ? b.low_records : b.high_records) + sl.z;
-	SortedBlock sb;
-	if(flags & kSlotPresorted) {
-		uint32_t base = b.blk_sorted[slot];
-		sb.tri = b.sorted_pool + base;
-		sb.prefix = sb.tri + m;
-		sb.mask = (const uint64_t *)(b.sorted_pool + ((base + 2 * m + 2) & ~1u));
+	const uint32_t *list = b.blk_lists + sl.x;
+	auto load = [&](uint32_t j) -> uint64_t {
+		return j < m ? block_key(b, recs[list[j]], bx, px0, py0, j) : ~0ull;
+	};
+	uint64_t v[4];
+	uint32_t P = next_pow2(m < 64 ? 64 : m);
+	if(P <= 64) {
+		uint64_t w1[1] = {load(lane)};
+		bitonic_regs<1>(w1);
+		v[0] = w1[0], v[1] = v[2] = v[3] = ~0ull;
+	} else if(P <= 128) {
+		uint64_t w2[2] = {load(lane), load(64 + lane)};
+		bitonic_regs<2>(w2);
+		v[0] = w2[0], v[1] = w2[1], v[2] = v[3] = ~0ull;
 	} else {
-		// short list: keys + register bitonic sort, then sorted arrays in LDS
-		const uint32_t *list = b.blk_lists + sl.x;
-		auto load = [&](uint32_t j) -> uint64_t {
-			return j < m ? block_key(b, recs[list[j]], bx, px0, py0, j) : ~0ull;
-		};
-		uint64_t v[4];
-		uint32_t P = next_pow2(m < 64 ? 64 : m);
-		if(P <= 64) {
-			uint64_t w1[1] = {load(lane)};
-			bitonic_regs<1>(w1);
-			v[0] = w1[0], v[1] = v[2] = v[3] = ~0ull;
-		} else if(P <= 128) {
-			uint64_t w2[2] = {load(lane), load(64 + lane)};
-			bitonic_regs<2>(w2);
-			v[0] = w2[0], v[1] = w2[1], v[2] = v[3] = ~0ull;
-		} else {
-			v[0] = load(lane), v[1] = load(64 + lane), v[2] = load(128 + lane), v[3] = load(192 + lane);
-			bitonic_regs<4>(v);
-		}
-		uint32_t carry = 0;
-#pragma unroll
-		for(int e = 0; e < 4; e++) {
-			uint32_t i = e * 64 + lane;
-			uint32_t cnt = 0;
-			if(i < m) {
-				uint4 rec = recs[list[(uint32_t)v[e]]];
-				uint64_t msk = record_block_mask(rec, bx);
-				cnt = (uint32_t)__popcll(msk);
-				s_mask[i] = msk;
-				s_tri[i] = rec.x;
-			}
-			uint32_t inc = wave_inclusive_add(cnt);
-			if(i < m)
-				s_prefix[i] = carry + inc - cnt;
-			carry += __shfl(inc, 63, 64);
-		}
-		if(lane == 0)
-			s_prefix[m] = carry;
-		wave_sync_lds();
-		sb.tri = s_tri, sb.mask = s_mask, sb.prefix = s_prefix;
+		v[0] = load(lane), v[1] = load(64 + lane), v[2] = load(128 + lane), v[3] = load(192 + lane);
+		bitonic_regs<4>(v);
 	}
+#pragma unroll
+	for(int e = 0; e < 4; e++) {
+		uint32_t i = e * 64 + lane;
+		if(i < m) {
+			uint4 rec = recs[list[(uint32_t)v[e]]];
+			s_mask[i] = record_block_mask(rec, bx);
+			s_tri[i] = rec.x;
+		}
+	}
+	wave_sync_lds();
 	PixelState<FP32> ps;
 	ps.init();
 	uint32_t count = 0;
-	raster_chunks<FP32>(c, b, sb, m, px0, py0, s_own, s_pm, ps, count);
-	const bool additive = (c.opts & (1u << 5)) != 0;
-	ps.flush(additive);
-	write_pixel(c, b, px, py, ps.acc, count);
-	bool on_screen = px < c.width && py < c.height;
-	uint64_t bad = ballot(ps.bad && on_screen);
-	if(bad && lane == 0)
-		enqueue_fallback(b, bin, blk, (int)level, sl.z, sl.w, px0, py0, bad, false);
-	uint32_t frags = __shfl(wave_inclusive_add(on_screen ? count : 0u), 63, 64);
-	if(lane == 0) {
-		lucid_info *info = (lucid_info *)b.info;
-		atomicAdd(&info->stats[0], frags);
-		atomicAdd(&info->stats[1], m);
-	}
+	raster_stream<FP32>(c, b, s_tri, s_mask, m, px0, py0, 0, s_own, s_pm, ps, count);
+	finish_block<FP32>(c, b, ps, count, px, py, true, 0, bin, blk, (int)level, sl, px0, py0, m);
+}
+
+// Pre-sorted long lists: one work-group of 8 waves per block; wave p rasterises the pixel
+// rows of part p (the block's rows split evenly over `parts` waves).
+template <bool FP32>
+__global__ __launch_bounds__(512) void block_raster_big_kernel(FrameConst c, FrameBufs b) {
+	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
+		return;
+	const lucid_info *info = (const lucid_info *)b.info;
+	const uint32_t hslot = blockIdx.x >> 4;
+	if(hslot >= (uint32_t)info->bin_level_counts[LUCID_BIN_LEVEL_HIGH])
+		return;
+	const int blk = blockIdx.x & 15, by = blk >> 2, bx = blk & 3;
+	const int bin = (int)b.info[LUCID_INFO_U32_SIZE + LUCID_COUNTS_HIGH_BINS * c.bin_count + hslot];
+	const uint32_t slot = (uint32_t)bin * 16 + blk;
+	const uint32_t flags = b.blk_flags[slot];
+	if(!(flags & kSlotPresorted) || (flags & kSlotPromoted))
+		return;
+	__shared__ uint32_t s_own[8][64];
+	__shared__ uint64_t s_pm[8][64];
+	const int wave = threadIdx.x >> 6, lane = lane_id();
+	const uint32_t parts = 1u << ((flags >> kSlotPartShift) & 3u);
+	if((uint32_t)wave >= parts)
+		return;
+	const int bin_x = (bin % c.bin_count_x) * kBinSize, bin_y = (bin / c.bin_count_x) * kBinSize;
+	const int px0 = bin_x + bx * 8, py0 = bin_y + by * 8;
+	const uint4 sl = b.blk_slot[slot];
+	const uint32_t base = b.blk_sorted[slot];
+	const uint32_t *hdr = b.sorted_pool + base;
+	uint32_t total = 0;
+	for(uint32_t p = 0; p < parts; p++)
+		total += hdr[1 + p];
+	const uint32_t mp = hdr[1 + wave], off = hdr[9 + wave];
+	const uint32_t *tri = b.sorted_pool + base + 18 + off;
+	const uint64_t *mask = (const uint64_t *)(b.sorted_pool + ((base + 18 + total + 2) & ~1u)) + off;
+	const int npix = 64 / (int)parts, pix0 = wave * npix;
+	const bool own_pixel = lane < npix;
+	const int pix = pix0 + (own_pixel ? lane : 0);
+	const int px = px0 + (pix & 7), py = py0 + (pix >> 3);
+	PixelState<FP32> ps;
+	ps.init();
+	uint32_t count = 0;
+	raster_stream<FP32>(c, b, tri, mask, mp, px0, py0, pix0, s_own[wave], s_pm[wave], ps, count);
+	finish_block<FP32>(c, b, ps, count, px, py, own_pixel, pix0, bin, blk, LUCID_BIN_LEVEL_HIGH, sl,
+					   px0, py0, wave == 0 ? sl.y : 0u);
 }
 
 // ------------------------------------------------------------------ exact fallback
@@ -943,12 +1098,9 @@ __device__ Accum blend_sorted(const FrameConst &c, const FrameBufs &b, const uin
 }
 
 template <bool FP32>
-__global__ __launch_bounds__(kFallbackThreads) void fallback_kernel(FrameConst c, FrameBufs b) {
-	uint32_t nq = min(b.ctrl[kCtrlFallbackCount], b.fallback_cap);
-	if(blockIdx.x >= nq)
-		return;
-	__shared__ uint64_t s_keys[kFallbackThreads / 64][kFallbackCap];
-	uint4 q0 = b.fallback_queue[2 * blockIdx.x], q1 = b.fallback_queue[2 * blockIdx.x + 1];
+__device__ void fallback_entry(const FrameConst &c, const FrameBufs &b, uint32_t qi,
+							   uint64_t (*s_keys)[kFallbackCap]) {
+	uint4 q0 = b.fallback_queue[2 * qi], q1 = b.fallback_queue[2 * qi + 1];
 	int blk = (int)(q0.y & 0xffu), level = (int)((q0.y >> 8) & 0xffu);
 	bool whole = (q0.y >> 16) & 1u;
 	int bx = blk & 3;
@@ -1022,6 +1174,15 @@ __global__ __launch_bounds__(kFallbackThreads) void fallback_kernel(FrameConst c
 	}
 }
 
+// Persistent over the queue: work-group g handles entries g, g + grid, ...
+template <bool FP32>
+__global__ __launch_bounds__(kFallbackThreads) void fallback_kernel(FrameConst c, FrameBufs b) {
+	__shared__ uint64_t s_keys[kFallbackThreads / 64][kFallbackCap];
+	uint32_t nq = min(b.ctrl[kCtrlFallbackCount], b.fallback_cap);
+	for(uint32_t qi = blockIdx.x; qi < nq; qi += gridDim.x)
+		fallback_entry<FP32>(c, b, qi, s_keys);
+}
+
 // ------------------------------------------------------------------ launch
 
 template <bool FP32>
@@ -1030,23 +1191,25 @@ static void launch_raster_t(const FrameConst &c, const FrameBufs &b, hipStream_t
 	hipLaunchKernelGGL(rowgen_low_kernel, dim3(c.bin_count), dim3(kRowgenThreads), 0, stream, c, b);
 	if(ev)
 		(void)hipEventRecord(ev[0], stream);
-	hipLaunchKernelGGL(rowgen_high_kernel, dim3(c.bin_count * 4), dim3(kRowgenThreads), 0, stream,
+	hipLaunchKernelGGL(high_prep_kernel, dim3(1), dim3(1024), 0, stream, c, b);
+	hipLaunchKernelGGL(rowgen_high_kernel, dim3(c.bin_count * 4 * kRowgenSplit),
+					   dim3(kRowgenThreads), 0, stream, c, b);
+	hipLaunchKernelGGL(high_finalize_kernel, dim3(c.bin_count * 16), dim3(kSortThreads), 0, stream,
 					   c, b);
-	hipLaunchKernelGGL(block_sort_kernel, dim3(c.bin_count * 16), dim3(kSortThreads), 0, stream, c,
-					   b);
 	if(ev)
 		(void)hipEventRecord(ev[1], stream);
 	hipLaunchKernelGGL(block_raster_kernel<FP32>, dim3(c.bin_count * 16), dim3(64), 0, stream, c, b);
+	hipLaunchKernelGGL(block_raster_big_kernel<FP32>, dim3(c.bin_count * 16), dim3(512), 0, stream,
+					   c, b);
 	if(ev)
 		(void)hipEventRecord(ev[2], stream);
-	hipLaunchKernelGGL(fallback_kernel<FP32>, dim3(b.fallback_cap), dim3(kFallbackThreads), 0,
-					   stream, c, b);
+	hipLaunchKernelGGL(fallback_kernel<FP32>, dim3(1024), dim3(kFallbackThreads), 0, stream, c, b);
 	if(ev)
 		(void)hipEventRecord(ev[3], stream);
 }
 
-// events (optional, 4): after rowgen_low, after rowgen_high + block_sort, after block_raster,
-// after the fallback kernel.
+// events (optional, 4): after rowgen_low, after the high-bin row generation and sorting,
+// after block rasterisation, after the fallback kernel.
 hipError_t launch_raster(const FrameConst &c, const FrameBufs &b, hipStream_t stream,
 						 hipEvent_t *events) {
 	if(c.opts & (1u << 16))

@@ -153,6 +153,15 @@ def test_block_over_lds_capacity(oracle_mod):
     _check(sc, gpu, ref, oracle_mod)
 
 
+def test_block_over_sort_capacity(oracle_mod):
+    # more than 8192 triangles in one 8x8 block: every pixel takes the exact per-pixel path,
+    # and pixels deeper than the fallback's LDS capacity spill their keys to global memory
+    sc = scenes.block_stack(32, 32, 8300, center_px=(12, 12), size_px=4.0, seed=13)
+    r, gpu, ref = _render_both(sc, oracle_mod)
+    _check(sc, gpu, ref, oracle_mod)
+    assert gpu["stats"]["oversized_blocks"] >= 1
+
+
 def test_ordering_violations_fallback(oracle_mod):
     sc = scenes.tilted_planes(128, 96, 24)
     r, gpu, ref = _render_both(sc, oracle_mod)
