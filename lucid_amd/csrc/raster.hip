@@ -175,72 +175,68 @@ struct Accum {
 	}
 };
 
+// Per-pixel ordering window: the kWindow nearest pending samples, kept sorted (k0 nearest,
+// ~0 = empty). A sample that arrives when the window is full pushes out the nearest of the
+// five, which is blended; the pixel is exact iff the blended keys strictly increase
+// (keys are unique: depth, then triangle id), otherwise `bad` marks it for the exact
+// per-pixel sort. Branch-free: every slot is a select, no dynamic register indexing.
 template <bool FP32>
 struct PixelState {
-	uint64_t wk[kWindow];
-	SampleColor<FP32> wc[kWindow];
-	int n;
-	uint64_t last;
+	uint64_t k0, k1, k2, k3;
+	SampleColor<FP32> c0, c1, c2, c3;
+	uint64_t first, last; // first and last blended key (~0 / 0 before the first)
 	bool bad;
 	Accum acc;
 
 	__device__ __forceinline__ void init() {
-#pragma unroll
-		for(int i = 0; i < kWindow; i++)
-			wk[i] = ~0ull, wc[i].zero();
-		n = 0, last = 0, bad = false;
+		k0 = k1 = k2 = k3 = ~0ull;
+		c0.zero(), c1.zero(), c2.zero(), c3.zero();
+		first = ~0ull, last = 0, bad = false;
 		acc.init();
 	}
 
+	__device__ __forceinline__ void emit(uint64_t k, SampleColor<FP32> col, bool additive) {
+		bad |= k < last;
+		first = first == ~0ull ? k : first;
+		last = k;
+		acc.blend(col.rgba(), additive);
+	}
+
 	__device__ __forceinline__ void insert(uint64_t key, SampleColor<FP32> col, bool additive) {
-		if(key < last) {
-			bad = true;
-			return;
+		static_assert(kWindow == 4, "window slots are spelled out");
+		const bool full = k3 != ~0ull;
+		const bool g0 = key < k0, g1 = key < k1, g2 = key < k2, g3 = key < k3;
+		// the five keys in order: s0 <= s1 <= s2 <= s3 <= s4
+		uint64_t s0 = g0 ? key : k0;
+		uint64_t s1 = g0 ? k0 : (g1 ? key : k1);
+		uint64_t s2 = g1 ? k1 : (g2 ? key : k2);
+		uint64_t s3 = g2 ? k2 : (g3 ? key : k3);
+		uint64_t s4 = g3 ? k3 : key;
+		SampleColor<FP32> t0 = g0 ? col : c0;
+		SampleColor<FP32> t1 = g0 ? c0 : (g1 ? col : c1);
+		SampleColor<FP32> t2 = g1 ? c1 : (g2 ? col : c2);
+		SampleColor<FP32> t3 = g2 ? c2 : (g3 ? col : c3);
+		SampleColor<FP32> t4 = g3 ? c3 : col;
+		if(full) {
+			emit(s0, t0, additive);
+			k0 = s1, k1 = s2, k2 = s3, k3 = s4;
+			c0 = t1, c1 = t2, c2 = t3, c3 = t4;
+		} else {
+			k0 = s0, k1 = s1, k2 = s2, k3 = s3;
+			c0 = t0, c1 = t1, c2 = t2, c3 = t3;
 		}
-		if(n < kWindow) {
-#pragma unroll
-			for(int i = 0; i < kWindow; i++)
-				if(i == n)
-					wk[i] = key, wc[i] = col;
-			n++;
-			return;
-		}
-		uint64_t mk = key;
-		int mi = -1;
-#pragma unroll
-		for(int i = 0; i < kWindow; i++)
-			if(wk[i] < mk)
-				mk = wk[i], mi = i;
-		SampleColor<FP32> out = col;
-#pragma unroll
-		for(int i = 0; i < kWindow; i++)
-			if(i == mi) {
-				out = wc[i];
-				wk[i] = key, wc[i] = col;
-			}
-		acc.blend(out.rgba(), additive);
-		last = mk;
 	}
 
 	__device__ __forceinline__ void flush(bool additive) {
-#pragma unroll
-		for(int r = 0; r < kWindow; r++) {
-			uint64_t mk = ~0ull;
-			int mi = -1;
-#pragma unroll
-			for(int i = 0; i < kWindow; i++)
-				if(wk[i] < mk)
-					mk = wk[i], mi = i;
-			if(mi >= 0) {
-				SampleColor<FP32> out;
-				out.zero();
-#pragma unroll
-				for(int i = 0; i < kWindow; i++)
-					if(i == mi)
-						out = wc[i], wk[i] = ~0ull;
-				acc.blend(out.rgba(), additive);
-			}
-		}
+		if(k0 != ~0ull)
+			emit(k0, c0, additive);
+		if(k1 != ~0ull)
+			emit(k1, c1, additive);
+		if(k2 != ~0ull)
+			emit(k2, c2, additive);
+		if(k3 != ~0ull)
+			emit(k3, c3, additive);
+		k0 = k1 = k2 = k3 = ~0ull;
 	}
 };
 
@@ -647,12 +643,19 @@ __global__ __launch_bounds__(kRowgenThreads) void rowgen_high_kernel(FrameConst 
 	}
 }
 
-// Pixel bits of part p when a block's 8 rows are split evenly over `parts` waves.
-__device__ __forceinline__ uint64_t part_bits(int p, uint32_t parts) {
-	if(parts == 1)
-		return ~0ull;
-	int bits = 64 / (int)parts;
-	return ((1ull << bits) - 1ull) << (p * bits);
+// Entries (triangles of one block in depth order) are handed to the rasteriser in slices
+// of up to kSliceEntries: entry lane + 64 k of the slice sits in register k of that lane.
+constexpr int kSliceEntries = 256;
+constexpr int kSliceWords = kSliceEntries / 64;
+constexpr uint32_t kSamplesPerWave = 2048; // target samples per wave of a pre-sorted block
+constexpr int kBigWaves = 8;               // waves of block_raster_big_kernel
+
+// Pre-sorted block layout in the sorted pool (offsets in words from an even base):
+//   [0] number of waves W, [1 .. W + 1] first entry of each wave's range (then m),
+//   [kSortedHeader .. + m) triangle ids, [sorted_mask_offset(m) ..) 64-bit pixel masks.
+constexpr uint32_t kSortedHeader = 10;
+__device__ __forceinline__ uint32_t sorted_mask_offset(uint32_t m) {
+	return (kSortedHeader + m + 1u) & ~1u;
 }
 
 // Finalises every block of the high bins: copies the list lengths into the slot table and
@@ -692,8 +695,8 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 		return;
 	}
 	__shared__ uint64_t s_key[kBigSortCap];
-	__shared__ uint32_t s_part[8][kSortThreads];
 	__shared__ uint32_t s_samples[kSortThreads];
+	__shared__ uint32_t s_bound[kBigWaves + 1];
 	__shared__ uint32_t s_base, s_parts, s_total;
 	const uint32_t *list = b.blk_lists + sl.x;
 	const uint4 *recs = b.high_records + sl.z;
@@ -726,70 +729,24 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 	__syncthreads();
 	if(threadIdx.x == 0) {
 		uint32_t tot = 0;
-		for(int t = 0; t < kSortThreads; t++)
-			tot += s_samples[t];
-		// waves per block: about 4k samples or more each
-		s_parts = tot <= 4096u ? 1u : tot <= 8192u ? 2u : tot <= 16384u ? 4u : 8u;
-	}
-	__syncthreads();
-	const uint32_t parts = s_parts;
-	uint32_t pc[8];
-#pragma unroll
-	for(int p = 0; p < 8; p++)
-		pc[p] = 0;
-	for(uint32_t i = i0; i < i1; i++) {
-		uint64_t msk = record_block_mask(recs[list[(uint32_t)s_key[i]]], bx);
-#pragma unroll
-		for(int p = 0; p < 8; p++)
-			if(p < (int)parts && (msk & part_bits(p, parts)))
-				pc[p]++;
-	}
-#pragma unroll
-	for(int p = 0; p < 8; p++)
-		s_part[p][threadIdx.x] = pc[p];
-	__syncthreads();
-	// exclusive scan of each part's per-thread counts (wave w scans parts w, w + 4)
-	const int wave = threadIdx.x >> 6, lane = lane_id();
-	for(int p = wave; p < (int)parts; p += kSortThreads / 64) {
-		uint32_t v[4], tot = 0;
-#pragma unroll
-		for(int q = 0; q < 4; q++)
-			v[q] = s_part[p][lane * 4 + q], tot += v[q];
-		uint32_t inc = wave_inclusive_add(tot);
-		uint32_t ex = inc - tot;
-#pragma unroll
-		for(int q = 0; q < 4; q++) {
-			uint32_t t = v[q];
-			s_part[p][lane * 4 + q] = ex;
-			ex += t;
+		for(int t = 0; t < kSortThreads; t++) {
+			uint32_t v = s_samples[t];
+			s_samples[t] = tot; // exclusive prefix
+			tot += v;
 		}
-		if(lane == 63)
-			s_samples[p] = inc; // part sizes (s_samples is free again)
-	}
-	__syncthreads();
-	if(threadIdx.x == 0) {
-		uint32_t total = 0;
-		for(uint32_t p = 0; p < parts; p++)
-			total += s_samples[p];
-		// layout: 18-word header {parts, count[8], offset[8], pad}, tri[total], pad, mask[total]
-		uint32_t need = 18 + total + 2 + 2 * total + 2;
+		uint32_t nw = (tot + kSamplesPerWave - 1) / kSamplesPerWave;
+		nw = nw < 1u ? 1u : nw > (uint32_t)kBigWaves ? (uint32_t)kBigWaves : nw;
+		s_parts = nw;
+		s_total = tot;
+		for(int w = 0; w <= kBigWaves; w++)
+			s_bound[w] = w == 0 ? 0u : m;
+		const uint32_t need = sorted_mask_offset(m) + 2 * m + 1;
 		uint32_t base = atomicAdd(&b.ctrl[kCtrlSortedCursor], need);
 		if((uint64_t)base + need > b.sorted_cap) {
 			atomicOr(&b.ctrl[kCtrlOverflow], kOvfSorted);
 			base = 0xffffffffu;
-		} else {
-			base = (base + 1u) & ~1u;
-			uint32_t *hdr = b.sorted_pool + base;
-			hdr[0] = parts;
-			uint32_t off = 0;
-			for(uint32_t p = 0; p < parts; p++) {
-				hdr[1 + p] = s_samples[p];
-				hdr[9 + p] = off;
-				off += s_samples[p];
-			}
 		}
-		s_base = base;
-		s_total = total;
+		s_base = base == 0xffffffffu ? base : (base + 1u) & ~1u;
 	}
 	__syncthreads();
 	if(s_base == 0xffffffffu) {
@@ -797,124 +754,127 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 			b.blk_flags[slot] = kSlotPromoted; // nothing rendered; the frame reports overflow
 		return;
 	}
-	const uint32_t base = s_base, total = s_total;
-	const uint32_t *hdr = b.sorted_pool + base;
-	uint32_t *tri_out = b.sorted_pool + base + 18;
-	uint64_t *mask_out = (uint64_t *)(b.sorted_pool + ((base + 18 + total + 2) & ~1u));
-	uint32_t pos[8];
-#pragma unroll
-	for(int p = 0; p < 8; p++)
-		pos[p] = p < (int)parts ? hdr[9 + p] + s_part[p][threadIdx.x] : 0u;
+	const uint32_t nw = s_parts, total = s_total;
+	uint32_t *hdr = b.sorted_pool + s_base;
+	uint32_t *tri_out = hdr + kSortedHeader;
+	uint64_t *mask_out = (uint64_t *)(hdr + sorted_mask_offset(m));
+	// wave w starts at the first entry with at least w * total / nw samples before it
+	uint32_t run = s_samples[threadIdx.x], found = 0;
 	for(uint32_t i = i0; i < i1; i++) {
 		uint4 rec = recs[list[(uint32_t)s_key[i]]];
 		uint64_t msk = record_block_mask(rec, bx);
-#pragma unroll
-		for(int p = 0; p < 8; p++) {
-			uint64_t part_mask = p < (int)parts ? msk & part_bits(p, parts) : 0ull;
-			if(part_mask) {
-				tri_out[pos[p]] = rec.x;
-				mask_out[pos[p]] = part_mask;
-				pos[p]++;
+		for(uint32_t w = 1; w < nw; w++) {
+			uint32_t target = (uint32_t)(((uint64_t)w * total) / nw);
+			if(!((found >> w) & 1u) && run >= target) {
+				atomicMin(&s_bound[w], i);
+				found |= 1u << w;
 			}
 		}
+		tri_out[i] = rec.x;
+		mask_out[i] = msk;
+		run += (uint32_t)__popcll(msk);
 	}
+	__syncthreads();
 	if(threadIdx.x == 0) {
-		b.blk_sorted[slot] = base;
-		b.blk_flags[slot] =
-			LUCID_BIN_LEVEL_HIGH | kSlotPresorted | ((uint32_t)(31 - __clz(parts)) << kSlotPartShift);
+		hdr[0] = nw;
+		for(uint32_t w = 0; w < nw; w++)
+			hdr[1 + w] = s_bound[w];
+		hdr[1 + nw] = m;
+		b.blk_sorted[slot] = s_base;
+		b.blk_flags[slot] = LUCID_BIN_LEVEL_HIGH | kSlotPresorted;
 	}
 }
 
 // ------------------------------------------------------------------ block_raster
 
-// Streams the sorted entries of one (sub-)block through the sample chunks. Entries are
-// read 64 at a time; each lane holds one entry's mask and triangle, a wave scan gives sample
-// offsets, and each sample finds its entry by a max-scan over marked start positions.
-// `pix0` is the first pixel of the sub-block; lanes [0, npix) own its pixels.
-template <bool FP32>
-__device__ __forceinline__ void raster_stream(const FrameConst &c, const FrameBufs &b,
-											  const uint32_t *tri, const uint64_t *mask,
-											  uint32_t m, int px0, int py0, int pix0,
-											  uint32_t *s_own, uint64_t *s_pm,
-											  PixelState<FP32> &ps, uint32_t &count) {
-	const int lane = lane_id();
-	const bool additive = (c.opts & (1u << 5)) != 0;
-	uint32_t e_cur = 0, s_start = 0, s0 = 0;
-	while(e_cur < m) {
-		uint32_t e = e_cur + lane;
-		uint64_t msk = e < m ? mask[e] : 0ull;
-		uint32_t tr = e < m ? tri[e] : 0u;
-		uint32_t cnt = (uint32_t)__popcll(msk);
-		uint32_t inc = wave_inclusive_add(cnt);
-		uint32_t st = s_start + inc - cnt;
-		uint32_t win_end = s_start + __shfl(inc, 63, 64);
-		uint32_t s_end = min(s0 + 64u, win_end);
-		s_own[lane] = 0;
-		s_pm[lane] = 0;
-		wave_sync_lds();
-		if(cnt && st < s_end)
-			s_own[st > s0 ? st - s0 : 0] = (uint32_t)lane;
-		wave_sync_lds();
-		uint32_t own = wave_inclusive_max(s_own[lane]);
-		uint32_t s = s0 + lane;
-		bool valid = s < s_end;
-		uint32_t o_st = __shfl(st, (int)own, 64);
-		uint64_t o_msk = shfl64(msk, (int)own);
-		uint32_t o_tri = __shfl(tr, (int)own, 64);
-		uint64_t key = ~0ull;
-		SampleColor<FP32> col;
-		col.zero();
-		if(valid) {
-			uint32_t pix = (uint32_t)select_bit64(o_msk, s - o_st);
-			float px = (float)(px0 + (int)(pix & 7u)), py = (float)(py0 + (int)(pix >> 3));
-			float inv = sample_inv_depth(b, o_tri, px, py);
-			col = shade_sample<FP32>(c, b, o_tri, px, py, inv);
-			key = pixel_key(inv, o_tri);
-			atomicOr((unsigned long long *)&s_pm[pix - pix0], 1ull << lane);
-		}
-		wave_sync_lds();
-		uint64_t bits = s_pm[lane];
-		count += (uint32_t)__popcll(bits);
-		bits &= ballot(col.contributes()); // only contributing samples are ordered and blended
-		while(ballot(bits != 0)) {
-			int src = bits ? (int)__ffsll((unsigned long long)bits) - 1 : lane;
-			uint64_t k2 = shfl64(key, src);
-			SampleColor<FP32> c2 = col.shfl(src);
-			if(bits) {
-				bits &= bits - 1;
-				ps.insert(k2, c2, additive);
-			}
-		}
-		// advance to the entry holding sample s_end
-		uint32_t done = (uint32_t)__popcll(ballot(st + cnt <= s_end));
-		if(done >= 64u) {
-			e_cur += 64u;
-			s_start = win_end;
-		} else {
-			s_start = __shfl(st, (int)done, 64);
-			e_cur += done;
-		}
-		s0 = s_end;
+// v_writelane_b32 x2: lane L of (lo, hi) takes the wave-uniform pair (x_lo, x_hi). The pair
+// usually comes straight from a VALU compare; a VALU read of an SGPR written by a VALU needs
+// two wait states, which the s_nop provides (the compiler does not look inside the asm).
+template <int L>
+__device__ __forceinline__ void write_lane2(uint32_t &lo, uint32_t &hi, uint32_t x_lo,
+											uint32_t x_hi) {
+	asm volatile("s_nop 1\n\tv_writelane_b32 %0, %2, %4\n\tv_writelane_b32 %1, %3, %4"
+				 : "+v"(lo), "+v"(hi)
+				 : "s"(x_lo), "s"(x_hi), "n"(L));
+}
+
+// Pixels P.. of one transpose word: ballot of mask bit P over the entry lanes lands in lane P.
+template <int P>
+__device__ __forceinline__ void transpose_bits(uint64_t msk, uint32_t &lo, uint32_t &hi) {
+	if constexpr(P < 64) {
+		const uint64_t w = ballot((msk >> P) & 1ull);
+		write_lane2<P>(lo, hi, (uint32_t)w, (uint32_t)(w >> 32));
+		transpose_bits<P + 1>(msk, lo, hi);
 	}
 }
 
-// Pixel lane i of the wave owns block pixel pix0 + i.
+// Transposes the coverage masks of a slice (64-bit pixel masks, one per entry) into
+// per-pixel entry bitmaps: after the call bit e of bm[k] in lane p is set iff entry
+// 64 k + e covers pixel p. One ballot per (word, pixel).
+__device__ __forceinline__ void transpose_masks(const uint64_t (&msk)[kSliceWords], int nwords,
+												uint64_t (&bm)[kSliceWords]) {
+#pragma unroll
+	for(int k = 0; k < kSliceWords; k++) {
+		uint32_t lo = 0, hi = 0;
+		if(k < nwords)
+			transpose_bits<0>(msk[k], lo, hi);
+		bm[k] = (uint64_t)lo | ((uint64_t)hi << 32);
+	}
+}
+
+// Walks this lane's pixel through the slice in entry order: each step takes the nearest
+// remaining entry that covers the pixel, shades the sample and feeds the ordering window.
+// `s_tri` holds the slice's triangle ids in LDS.
 template <bool FP32>
-__device__ __forceinline__ void finish_block(const FrameConst &c, const FrameBufs &b,
-											 PixelState<FP32> &ps, uint32_t count, int px, int py,
-											 bool own_pixel, int pix0, int bin, int blk, int level,
-											 uint4 sl, int px0, int py0, uint32_t m) {
+__device__ __forceinline__ void walk_pixel(const FrameConst &c, const FrameBufs &b,
+										   const uint32_t *s_tri, uint64_t (&bm)[kSliceWords],
+										   float px, float py, PixelState<FP32> &ps, bool additive) {
+	static_assert(kSliceWords == 4, "word selection is spelled out");
+	while(true) {
+		const bool nz0 = bm[0] != 0ull, nz1 = bm[1] != 0ull, nz2 = bm[2] != 0ull,
+				   nz3 = bm[3] != 0ull;
+		const bool any = nz0 || nz1 || nz2 || nz3;
+		if(!ballot(any))
+			break;
+		if(any) {
+			const uint64_t cur = nz0 ? bm[0] : nz1 ? bm[1] : nz2 ? bm[2] : bm[3];
+			const int word = nz0 ? 0 : nz1 ? 1 : nz2 ? 2 : 3;
+			const uint64_t rest = cur & (cur - 1ull);
+			bm[0] = nz0 ? rest : bm[0];
+			bm[1] = (!nz0 && nz1) ? rest : bm[1];
+			bm[2] = (!nz0 && !nz1 && nz2) ? rest : bm[2];
+			bm[3] = (!nz0 && !nz1 && !nz2) ? rest : bm[3];
+			const uint32_t t = s_tri[word * 64 + (int)__builtin_ctzll(cur)];
+			const float inv = sample_inv_depth(b, t, px, py);
+			const SampleColor<FP32> col = shade_sample<FP32>(c, b, t, px, py, inv);
+			if(col.contributes())
+				ps.insert(pixel_key(inv, t), col, additive);
+		}
+	}
+}
+
+__device__ __forceinline__ uint32_t bitmap_count(const uint64_t (&bm)[kSliceWords]) {
+	uint32_t n = 0;
+#pragma unroll
+	for(int k = 0; k < kSliceWords; k++)
+		n += (uint32_t)__popcll(bm[k]);
+	return n;
+}
+
+// Writes the pixel, queues the block's mis-ordered pixels for the exact sort and adds the
+// frame statistics (one lane per block pixel).
+__device__ __forceinline__ void finish_pixels(const FrameConst &c, const FrameBufs &b,
+											  const Accum &acc, bool bad, uint32_t count, int px,
+											  int py, int bin, int blk, int level, uint4 sl,
+											  int px0, int py0, uint32_t m) {
 	const int lane = lane_id();
-	const bool additive = (c.opts & (1u << 5)) != 0;
-	ps.flush(additive);
-	bool on_screen = own_pixel && px < c.width && py < c.height;
-	if(own_pixel)
-		write_pixel(c, b, px, py, ps.acc, count);
-	uint64_t bad = ballot(ps.bad && on_screen);
-	uint32_t frags = __shfl(wave_inclusive_add(on_screen ? count : 0u), 63, 64);
+	const bool on_screen = px < c.width && py < c.height;
+	write_pixel(c, b, px, py, acc, count);
+	const uint64_t bad_px = ballot(bad && on_screen);
+	const uint32_t frags = __shfl(wave_inclusive_add(on_screen ? count : 0u), 63, 64);
 	if(lane == 0) {
-		if(bad)
-			enqueue_fallback(b, bin, blk, level, sl.z, sl.w, px0, py0, bad << pix0, false);
+		if(bad_px)
+			enqueue_fallback(b, bin, blk, level, sl.z, sl.w, px0, py0, bad_px, false);
 		lucid_info *info = (lucid_info *)b.info;
 		atomicAdd(&info->stats[0], frags);
 		if(m)
@@ -922,15 +882,14 @@ __device__ __forceinline__ void finish_block(const FrameConst &c, const FrameBuf
 	}
 }
 
-// One wave per block whose list fits the register sort (every low bin, most high blocks).
+// One wave per block whose list fits one slice (every low bin, most high blocks): sorts
+// the list by centroid depth in registers, then walks every pixel through it.
 template <bool FP32>
 __global__ __launch_bounds__(64) void block_raster_kernel(FrameConst c, FrameBufs b) {
 	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
 		return;
-	__shared__ uint64_t s_mask[kRegSortCap];
-	__shared__ uint32_t s_tri[kRegSortCap];
-	__shared__ uint32_t s_own[64];
-	__shared__ uint64_t s_pm[64];
+	static_assert(kRegSortCap == kSliceEntries, "one register-sorted list is one slice");
+	__shared__ uint32_t s_tri[kSliceEntries];
 	const int lane = lane_id();
 	const uint32_t slot = blockIdx.x;
 	const int bin = slot >> 4, blk = slot & 15, by = blk >> 2, bx = blk & 3;
@@ -972,27 +931,37 @@ __global__ __launch_bounds__(64) void block_raster_kernel(FrameConst c, FrameBuf
 		v[0] = load(lane), v[1] = load(64 + lane), v[2] = load(128 + lane), v[3] = load(192 + lane);
 		bitonic_regs<4>(v);
 	}
+	uint64_t msk[kSliceWords];
 #pragma unroll
-	for(int e = 0; e < 4; e++) {
+	for(int e = 0; e < kSliceWords; e++) {
 		uint32_t i = e * 64 + lane;
+		msk[e] = 0;
 		if(i < m) {
 			uint4 rec = recs[list[(uint32_t)v[e]]];
-			s_mask[i] = record_block_mask(rec, bx);
+			msk[e] = record_block_mask(rec, bx);
 			s_tri[i] = rec.x;
 		}
 	}
 	wave_sync_lds();
+	uint64_t bm[kSliceWords];
+	transpose_masks(msk, (int)((m + 63) / 64), bm);
+	const uint32_t count = bitmap_count(bm);
+	const bool additive = (c.opts & (1u << 5)) != 0;
 	PixelState<FP32> ps;
 	ps.init();
-	uint32_t count = 0;
-	raster_stream<FP32>(c, b, s_tri, s_mask, m, px0, py0, 0, s_own, s_pm, ps, count);
-	finish_block<FP32>(c, b, ps, count, px, py, true, 0, bin, blk, (int)level, sl, px0, py0, m);
+	walk_pixel<FP32>(c, b, s_tri, bm, (float)px, (float)py, ps, additive);
+	ps.flush(additive);
+	finish_pixels(c, b, ps.acc, ps.bad, count, px, py, bin, blk, (int)level, sl, px0, py0, m);
 }
 
-// Pre-sorted long lists: one work-group of 8 waves per block; wave p rasterises the pixel
-// rows of part p (the block's rows split evenly over `parts` waves).
+// Pre-sorted lists longer than one slice: one work-group per block. Wave w takes the
+// contiguous depth range [start[w], start[w + 1]) of the sorted list (ranges balanced by
+// sample count), walks it slice by slice with one ordering window per pixel, and leaves a
+// partial composite; wave 0 chains the partials front to back. A pixel is exact iff each
+// range's blended keys increase and every range starts behind the previous one's last key.
 template <bool FP32>
-__global__ __launch_bounds__(512) void block_raster_big_kernel(FrameConst c, FrameBufs b) {
+__global__ __launch_bounds__(kBigWaves * 64) void block_raster_big_kernel(FrameConst c,
+																		   FrameBufs b) {
 	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
 		return;
 	const lucid_info *info = (const lucid_info *)b.info;
@@ -1005,33 +974,73 @@ __global__ __launch_bounds__(512) void block_raster_big_kernel(FrameConst c, Fra
 	const uint32_t flags = b.blk_flags[slot];
 	if(!(flags & kSlotPresorted) || (flags & kSlotPromoted))
 		return;
-	__shared__ uint32_t s_own[8][64];
-	__shared__ uint64_t s_pm[8][64];
+	__shared__ uint32_t s_tri[kBigWaves][kSliceEntries];
+	__shared__ float4 s_part[kBigWaves][64];
+	__shared__ uint64_t s_first[kBigWaves][64], s_last[kBigWaves][64];
+	__shared__ uint32_t s_cnt[kBigWaves][64];
 	const int wave = threadIdx.x >> 6, lane = lane_id();
-	const uint32_t parts = 1u << ((flags >> kSlotPartShift) & 3u);
-	if((uint32_t)wave >= parts)
-		return;
+	const uint4 sl = b.blk_slot[slot];
+	const uint32_t m = sl.y;
+	const uint32_t *hdr = b.sorted_pool + b.blk_sorted[slot];
+	const uint32_t nw = hdr[0];
+	const uint32_t *tri = hdr + kSortedHeader;
+	const uint64_t *mask = (const uint64_t *)(hdr + sorted_mask_offset(m));
 	const int bin_x = (bin % c.bin_count_x) * kBinSize, bin_y = (bin / c.bin_count_x) * kBinSize;
 	const int px0 = bin_x + bx * 8, py0 = bin_y + by * 8;
-	const uint4 sl = b.blk_slot[slot];
-	const uint32_t base = b.blk_sorted[slot];
-	const uint32_t *hdr = b.sorted_pool + base;
-	uint32_t total = 0;
-	for(uint32_t p = 0; p < parts; p++)
-		total += hdr[1 + p];
-	const uint32_t mp = hdr[1 + wave], off = hdr[9 + wave];
-	const uint32_t *tri = b.sorted_pool + base + 18 + off;
-	const uint64_t *mask = (const uint64_t *)(b.sorted_pool + ((base + 18 + total + 2) & ~1u)) + off;
-	const int npix = 64 / (int)parts, pix0 = wave * npix;
-	const bool own_pixel = lane < npix;
-	const int pix = pix0 + (own_pixel ? lane : 0);
-	const int px = px0 + (pix & 7), py = py0 + (pix >> 3);
-	PixelState<FP32> ps;
-	ps.init();
+	const int px = px0 + (lane & 7), py = py0 + (lane >> 3);
+	const bool additive = (c.opts & (1u << 5)) != 0;
+	if((uint32_t)wave < nw) {
+		const uint32_t e0 = hdr[1 + wave], e1 = hdr[2 + wave];
+		PixelState<FP32> ps;
+		ps.init();
+		uint32_t count = 0;
+		for(uint32_t s0 = e0; s0 < e1; s0 += kSliceEntries) {
+			const uint32_t n = min(e1 - s0, (uint32_t)kSliceEntries);
+			uint64_t msk[kSliceWords];
+#pragma unroll
+			for(int k = 0; k < kSliceWords; k++) {
+				uint32_t i = k * 64 + lane;
+				msk[k] = i < n ? mask[s0 + i] : 0ull;
+				if(i < n)
+					s_tri[wave][i] = tri[s0 + i];
+			}
+			wave_sync_lds();
+			uint64_t bm[kSliceWords];
+			transpose_masks(msk, (int)((n + 63) / 64), bm);
+			count += bitmap_count(bm);
+			walk_pixel<FP32>(c, b, s_tri[wave], bm, (float)px, (float)py, ps, additive);
+			wave_sync_lds(); // every lane is done with this slice's triangle ids
+		}
+		ps.flush(additive);
+		s_part[wave][lane] = make_float4(ps.acc.cr, ps.acc.cg, ps.acc.cb, ps.acc.tr);
+		s_first[wave][lane] = ps.first;
+		s_last[wave][lane] = ps.last;
+		s_cnt[wave][lane] = count | (ps.bad ? 0x80000000u : 0u);
+	}
+	__syncthreads();
+	if(wave != 0)
+		return;
+	Accum acc;
+	acc.init();
+	uint64_t last = 0;
+	bool bad = false;
 	uint32_t count = 0;
-	raster_stream<FP32>(c, b, tri, mask, mp, px0, py0, pix0, s_own[wave], s_pm[wave], ps, count);
-	finish_block<FP32>(c, b, ps, count, px, py, own_pixel, pix0, bin, blk, LUCID_BIN_LEVEL_HIGH, sl,
-					   px0, py0, wave == 0 ? sl.y : 0u);
+	for(uint32_t w = 0; w < nw; w++) {
+		const uint32_t cw = s_cnt[w][lane];
+		count += cw & 0x7fffffffu;
+		bad |= (cw >> 31) != 0;
+		const uint64_t f = s_first[w][lane];
+		if(f != ~0ull) {
+			bad |= f < last;
+			last = s_last[w][lane];
+			const float4 p = s_part[w][lane];
+			acc.cr = acc.cr + acc.tr * p.x;
+			acc.cg = acc.cg + acc.tr * p.y;
+			acc.cb = acc.cb + acc.tr * p.z;
+			acc.tr = acc.tr * p.w;
+		}
+	}
+	finish_pixels(c, b, acc, bad, count, px, py, bin, blk, LUCID_BIN_LEVEL_HIGH, sl, px0, py0, m);
 }
 
 // ------------------------------------------------------------------ exact fallback
