@@ -29,7 +29,7 @@ constexpr int kLowRowCap = 1024;     // records per block row of a low bin (< 10
 enum CtrlWord : int {
 	kCtrlNumLarge = 0,      // entries in the large-quad list
 	kCtrlRecordCursor = 1,  // bump allocator of the raster_high record pool (uint4 units)
-	kCtrlFallbackCount = 2, // entries in the exact-fallback queue
+	kCtrlFallbackCount = 2, // pixels in the exact-fallback queue
 	kCtrlOverflow = 3,      // bit 0 bin lists, 1 record pool, 2 fallback queue, 3 spill pool
 	kCtrlSpillCursor = 4,   // bump allocator of the deep-pixel spill pool (u64 units)
 	kCtrlTotalBinQuads = 5,
@@ -52,7 +52,7 @@ constexpr uint32_t kSlotWhole = 1u << 10;    // list too long: every pixel takes
 constexpr uint32_t kSlotPartShift = 12;      // log2 of the waves sharing a pre-sorted block
 constexpr int kRegSortCap = 256;  // lists sorted in registers by the raster wave itself
 constexpr int kBigSortCap = 8192; // lists sorted in LDS by high_finalize_kernel
-constexpr int kRowgenSplit = 8;   // work-groups per (high bin, band) in rowgen_high
+constexpr int kRowgenSplit = 8;   // work-groups per high bin in rowgen_high
 
 enum ColorMode : int { kColorNone = 0, kColorRGBA8 = 1, kColorF32 = 2 };
 
@@ -115,7 +115,7 @@ struct FrameBufs {
 	uint32_t *band_count; // bin_count * 4: record counts of high-bin bands (atomic counters)
 	uint32_t *sorted_pool;
 	uint64_t sorted_cap;
-	uint4 *fallback_queue; // 2 uint4 per entry
+	uint4 *fallback_queue; // one {slot, px, py, whole} per pixel
 	uint32_t fallback_cap;
 	uint64_t *spill; // deep-pixel spill pool
 	uint64_t spill_cap;

@@ -184,7 +184,7 @@ int grow_after_overflow(lucid_renderer *r) {
 	if(flags & kOvfFallback) {
 		uint64_t need = std::max<uint64_t>((uint64_t)r->status[kCtrlFallbackCount],
 										   (uint64_t)r->fallback_cap * 2);
-		LUCID_CHECK(r->fallback_queue.reserve(need * 32));
+		LUCID_CHECK(r->fallback_queue.reserve(need * 16));
 		r->fallback_cap = (uint32_t)need;
 	}
 	return LUCID_ERR_CAPACITY;
@@ -273,9 +273,9 @@ int lucid_create(const lucid_create_info *ci, lucid_renderer **out) {
 	chk(r->ctrl.reserve(kCtrlWords * 4));
 	chk(r->wg_hist.reserve((size_t)kNumBinWgs * 2 * NB * 4));
 	chk(r->low_records.reserve((size_t)NB * 4 * kLowRowCap * 16));
-	// at most one entry per wave of a block: 8 row parts, or one whole-block entry
+	// one 16-byte item per pixel sent to the exact sort; grows after an overflow
 	r->fallback_cap = (uint32_t)NB * 16 * 8;
-	chk(r->fallback_queue.reserve((size_t)r->fallback_cap * 32));
+	chk(r->fallback_queue.reserve((size_t)r->fallback_cap * 16));
 	r->high_records_cap = 1u << 22;
 	chk(r->high_records.reserve(r->high_records_cap * 16));
 	r->spill_cap = 1u << 20;

@@ -58,6 +58,12 @@ __device__ __forceinline__ float sample_inv_depth(const FrameBufs &b, uint32_t t
 template <bool FP32> struct SampleColor;
 template <> struct SampleColor<false> {
 	uint32_t enc;
+	// component-wise select (keeps struct selects in registers)
+	static __device__ __forceinline__ SampleColor sel(bool p, SampleColor a, SampleColor b) {
+		SampleColor o;
+		o.enc = p ? a.enc : b.enc;
+		return o;
+	}
 	__device__ __forceinline__ bool contributes() const { return enc != 0; }
 	__device__ __forceinline__ float4 rgba() const { return decode_rgba8(enc); }
 	__device__ __forceinline__ void zero() { enc = 0; }
@@ -69,6 +75,14 @@ template <> struct SampleColor<false> {
 };
 template <> struct SampleColor<true> {
 	float4 col;
+	static __device__ __forceinline__ SampleColor sel(bool p, SampleColor a, SampleColor b) {
+		SampleColor o;
+		o.col.x = p ? a.col.x : b.col.x;
+		o.col.y = p ? a.col.y : b.col.y;
+		o.col.z = p ? a.col.z : b.col.z;
+		o.col.w = p ? a.col.w : b.col.w;
+		return o;
+	}
 	__device__ __forceinline__ bool contributes() const { return col.w != 0.0f; }
 	__device__ __forceinline__ float4 rgba() const { return col; }
 	__device__ __forceinline__ void zero() { col = make_float4(0, 0, 0, 0); }
@@ -80,75 +94,154 @@ template <> struct SampleColor<true> {
 	}
 };
 
-// shadeSample (shading.glsl:107-184) without textures.
+// Per-channel lighting factor of finalShading (funcs.glsl:261-271): ambient + sun * light,
+// light = max(0, dot(-sun_dir, normal) * 0.7 + 0.3).
+__device__ __forceinline__ void light_factors(const FrameConst &c, V3 normal, float (&f)[3]) {
+	float light = fmx(0.0f, dot3(neg3(ld3(c.sun_dir)), normal) * 0.7f + 0.3f);
+#pragma unroll
+	for(int k = 0; k < 3; k++) {
+		float amb = c.ambient[k] * c.ambient_power;
+		float dif = (c.sun_color[k] * c.sun_power) * light;
+		f[k] = amb + dif;
+	}
+}
+
+// Shading record of one triangle, 7 x uint4 (the rasteriser stages one per list entry in LDS):
+//   q0  depth plane x, y, z | flags | instance << 16
+//   q1  barycentric plane 0 x, y, z | offset;  q2 the same for plane 1
+//   q3  RGBA8 colours: the triangle's three vertex colours | 0
+//       fp32 colours:  c0.rgb | triangle alpha
+//   q4  c1.rgb | L0;  q5 c2.rgb | L1
+//   q6  instance colour (RGBA8; white without one) | L2 | triangle id | 0
+// L0-L2 are the lighting factors of a flat-shaded triangle, or its three packed vertex
+// normals with LUCID_INST_HAS_VERTEX_NORMALS.
+constexpr int kRecQuads = 7;
+
+__device__ __forceinline__ void load_shading_record(const FrameConst &c, const FrameBufs &b,
+													uint32_t t, uint4 (&q)[kRecQuads]) {
+	// every slot is computed into a named local and stored once, so that no part of q
+	// is written under a branch (which would push q to scratch memory)
+	const uint4 d = b.tri_depth[t];
+	const uint32_t flags = d.w & 0xffffu, inst = d.w >> 16;
+	const bool s = t & 1u;
+	const bool vcol = (flags & LUCID_INST_HAS_VERTEX_COLORS) != 0;
+	uint4 q3 = make_uint4(0u, 0u, 0u, 0u), q4 = q3, q5 = q3;
+	if(vcol && c.color_mode == kColorRGBA8) {
+		uint4 qc = b.quad_color[t >> 1];
+		q3 = make_uint4(qc.x, s ? qc.z : qc.y, s ? qc.w : qc.z, 0u);
+	}
+	if(vcol && c.color_mode == kColorF32) {
+		float4 c0 = b.tri_color[3 * (size_t)t], c1 = b.tri_color[3 * (size_t)t + 1],
+			   c2 = b.tri_color[3 * (size_t)t + 2];
+		q3 = make_uint4(f2bits(c0.x), f2bits(c0.y), f2bits(c0.z), f2bits(c0.w));
+		q4 = make_uint4(f2bits(c1.x), f2bits(c1.y), f2bits(c1.z), 0u);
+		q5 = make_uint4(f2bits(c2.x), f2bits(c2.y), f2bits(c2.z), 0u);
+	}
+	const uint32_t icol =
+		((flags & LUCID_INST_HAS_COLOR) && b.inst_colors) ? b.inst_colors[inst] : 0xffffffffu;
+	uint32_t l0, l1, l2;
+	if(flags & LUCID_INST_HAS_VERTEX_NORMALS) {
+		uint4 nq = b.quad_normal[t >> 1];
+		l0 = nq.x, l1 = s ? nq.z : nq.y, l2 = s ? nq.w : nq.z;
+	} else {
+		float f[3];
+		light_factors(c, decode_normal(b.tri_normal[t]), f);
+		l0 = f2bits(f[0]), l1 = f2bits(f[1]), l2 = f2bits(f[2]);
+	}
+	q4.w = l0;
+	q5.w = l1;
+	q[0] = d;
+	q[1] = b.tri_bary[2 * (size_t)t];
+	q[2] = b.tri_bary[2 * (size_t)t + 1];
+	q[3] = q3;
+	q[4] = q4;
+	q[5] = q5;
+	q[6] = make_uint4(icol, l2, t, 0u);
+}
+
+// Depth plane at a pixel: inverse distance along the pixel's ray (shading.glsl:108).
+__device__ __forceinline__ float record_inv_depth(uint4 q0, float px, float py) {
+	return bits2f(q0.x) * px + (bits2f(q0.y) * py + bits2f(q0.z));
+}
+
+// shadeSample (shading.glsl:107-184) without textures, from a shading record (LDS or
+// registers). `inv_ray` is record_inv_depth at the same pixel.
 template <bool FP32>
-__device__ __forceinline__ SampleColor<FP32> shade_sample(const FrameConst &c, const FrameBufs &b,
-														  uint32_t t, float px, float py,
-														  float inv_ray) {
-	uint4 d = b.tri_depth[t];
-	uint32_t flags = d.w & 0xffffu, inst = d.w >> 16;
-	uint4 b0 = b.tri_bary[2 * (size_t)t], b1 = b.tri_bary[2 * (size_t)t + 1];
-	float ray_pos = 1.0f / inv_ray;
+__device__ __forceinline__ SampleColor<FP32> shade_record(const FrameConst &c, const uint4 *q,
+														  float px, float py, float inv_ray) {
+	const uint32_t flags = q[0].w & 0xffffu;
+	const uint4 b0 = q[1], b1 = q[2];
+	// GLSL's division is specified to 2.5 ulp; v_rcp_f32 (1 ulp) meets that
+	float ray_pos = __builtin_amdgcn_rcpf(inv_ray);
 	float e0 = bits2f(b0.x) * px + (bits2f(b0.y) * py + bits2f(b0.z));
 	float e1 = bits2f(b1.x) * px + (bits2f(b1.y) * py + bits2f(b1.z));
 	float bary0 = e0 * ray_pos, bary1 = e1 * ray_pos;
 	bary0 = bary0 - bits2f(b0.w);
 	bary1 = bary1 - bits2f(b1.w);
 
+	// vertex colour (or white), then times the instance colour; products by an exact 1.0
+	// are skipped, which leaves every result bit-identical
 	float4 color = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-	if((flags & LUCID_INST_HAS_COLOR) && b.inst_colors)
-		color = decode_rgba8(b.inst_colors[inst]);
+	const uint4 q6 = q[6];
 	if(flags & LUCID_INST_HAS_VERTEX_COLORS) {
 		float w0 = (1.0f - bary0) - bary1;
+		const uint4 q3 = q[3];
 		if(c.color_mode == kColorRGBA8) {
-			uint4 qc = b.quad_color[t >> 1];
-			bool s = t & 1u;
-			float4 c0 = decode_rgba8(qc.x), c1 = decode_rgba8(s ? qc.z : qc.y),
-				   c2 = decode_rgba8(s ? qc.w : qc.z);
-			color.x = color.x * (w0 * c0.x + (bary0 * c1.x + bary1 * c2.x));
-			color.y = color.y * (w0 * c0.y + (bary0 * c1.y + bary1 * c2.y));
-			color.z = color.z * (w0 * c0.z + (bary0 * c1.z + bary1 * c2.z));
-			color.w = color.w * (w0 * c0.w + (bary0 * c1.w + bary1 * c2.w));
+			float4 c0 = decode_rgba8(q3.x), c1 = decode_rgba8(q3.y), c2 = decode_rgba8(q3.z);
+			color.x = w0 * c0.x + (bary0 * c1.x + bary1 * c2.x);
+			color.y = w0 * c0.y + (bary0 * c1.y + bary1 * c2.y);
+			color.z = w0 * c0.z + (bary0 * c1.z + bary1 * c2.z);
+			color.w = w0 * c0.w + (bary0 * c1.w + bary1 * c2.w);
 		} else if(c.color_mode == kColorF32) {
-			float4 c0 = b.tri_color[3 * (size_t)t], c1 = b.tri_color[3 * (size_t)t + 1],
-				   c2 = b.tri_color[3 * (size_t)t + 2];
-			color.x = color.x * (w0 * c0.x + (bary0 * c1.x + bary1 * c2.x));
-			color.y = color.y * (w0 * c0.y + (bary0 * c1.y + bary1 * c2.y));
-			color.z = color.z * (w0 * c0.z + (bary0 * c1.z + bary1 * c2.z));
-			color.w = color.w * c0.w;
+			const uint4 q4 = q[4], q5 = q[5];
+			color.x = w0 * bits2f(q3.x) + (bary0 * bits2f(q4.x) + bary1 * bits2f(q5.x));
+			color.y = w0 * bits2f(q3.y) + (bary0 * bits2f(q4.y) + bary1 * bits2f(q5.y));
+			color.z = w0 * bits2f(q3.z) + (bary0 * bits2f(q4.z) + bary1 * bits2f(q5.z));
+			color.w = bits2f(q3.w);
 		}
+	}
+	if(flags & LUCID_INST_HAS_COLOR) {
+		const float4 ic = decode_rgba8(q6.x);
+		color.x = ic.x * color.x;
+		color.y = ic.y * color.y;
+		color.z = ic.z * color.z;
+		color.w = ic.w * color.w;
 	}
 	SampleColor<FP32> out;
 	if(color.w == 0.0f) {
 		out.zero();
 		return out;
 	}
-	V3 normal;
+	float f[3];
 	if(flags & LUCID_INST_HAS_VERTEX_NORMALS) {
-		uint4 nq = b.quad_normal[t >> 1];
-		bool s = t & 1u;
-		V3 n0 = decode_normal(nq.x), n1 = decode_normal(s ? nq.z : nq.y),
-		   n2 = decode_normal(s ? nq.w : nq.z);
+		V3 n0 = decode_normal(q[4].w), n1 = decode_normal(q[5].w), n2 = decode_normal(q6.y);
 		n1 = sub3(n1, n0);
 		n2 = sub3(n2, n0);
-		normal = mk3(bary0 * n1.x + (bary1 * n2.x + n0.x), bary0 * n1.y + (bary1 * n2.y + n0.y),
-					 bary0 * n1.z + (bary1 * n2.z + n0.z));
+		V3 normal = mk3(bary0 * n1.x + (bary1 * n2.x + n0.x), bary0 * n1.y + (bary1 * n2.y + n0.y),
+						bary0 * n1.z + (bary1 * n2.z + n0.z));
+		light_factors(c, normal, f);
 	} else {
-		normal = decode_normal(b.tri_normal[t]);
+		f[0] = bits2f(q[4].w), f[1] = bits2f(q[5].w), f[2] = bits2f(q6.y);
 	}
-	float light = fmx(0.0f, dot3(neg3(ld3(c.sun_dir)), normal) * 0.7f + 0.3f);
 	float rgb[3] = {color.x, color.y, color.z};
 #pragma unroll
-	for(int k = 0; k < 3; k++) {
-		float amb = c.ambient[k] * c.ambient_power;
-		float dif = (c.sun_color[k] * c.sun_power) * light;
-		rgb[k] = sat(linear_to_srgb(srgb_to_linear(rgb[k]) * (amb + dif)));
-	}
+	for(int k = 0; k < 3; k++)
+		rgb[k] = sat(linear_to_srgb(srgb_to_linear(rgb[k]) * f[k]));
 	if constexpr(FP32)
 		out.col = make_float4(rgb[0], rgb[1], rgb[2], color.w);
 	else
 		out.enc = encode_rgba8(rgb[0], rgb[1], rgb[2], color.w);
 	return out;
+}
+
+// Shades triangle t at a pixel straight from the setup records (exact fallback path).
+template <bool FP32>
+__device__ __forceinline__ SampleColor<FP32> shade_sample(const FrameConst &c, const FrameBufs &b,
+														  uint32_t t, float px, float py,
+														  float inv_ray) {
+	uint4 q[kRecQuads];
+	load_shading_record(c, b, t, q);
+	return shade_record<FP32>(c, q, px, py, inv_ray);
 }
 
 __device__ __forceinline__ uint64_t pixel_key(float inv_ray, uint32_t t) {
@@ -175,68 +268,80 @@ struct Accum {
 	}
 };
 
-// Per-pixel ordering window: the kWindow nearest pending samples, kept sorted (k0 nearest,
-// ~0 = empty). A sample that arrives when the window is full pushes out the nearest of the
-// five, which is blended; the pixel is exact iff the blended keys strictly increase
-// (keys are unique: depth, then triangle id), otherwise `bad` marks it for the exact
-// per-pixel sort. Branch-free: every slot is a select, no dynamic register indexing.
+// Sample order key: nearer (larger inverse depth) first, equal depths by triangle id.
+constexpr uint64_t kEmptyKey = ~0ull;
+
+// Per-pixel ordering window of kWindow pending samples in fixed slots. A sample that
+// arrives when the window is full pushes out the nearest of the five, which is blended
+// front to back. The pixel is exact iff the blended keys strictly increase; otherwise `bad`
+// marks it for the exact per-pixel sort. Slots are updated in place by selects: no
+// register rotation and no dynamic indexing.
 template <bool FP32>
 struct PixelState {
+	using SC = SampleColor<FP32>;
 	uint64_t k0, k1, k2, k3;
 	SampleColor<FP32> c0, c1, c2, c3;
-	uint64_t first, last; // first and last blended key (~0 / 0 before the first)
+	uint32_t n;           // filled slots
+	uint64_t first, last; // first and last blended key (kEmptyKey / 0 before the first)
 	bool bad;
 	Accum acc;
 
 	__device__ __forceinline__ void init() {
-		k0 = k1 = k2 = k3 = ~0ull;
+		k0 = k1 = k2 = k3 = kEmptyKey;
 		c0.zero(), c1.zero(), c2.zero(), c3.zero();
-		first = ~0ull, last = 0, bad = false;
+		n = 0, first = kEmptyKey, last = 0, bad = false;
 		acc.init();
 	}
 
 	__device__ __forceinline__ void emit(uint64_t k, SampleColor<FP32> col, bool additive) {
-		bad |= k < last;
-		first = first == ~0ull ? k : first;
+		bad |= k <= last;
+		first = first == kEmptyKey ? k : first;
 		last = k;
 		acc.blend(col.rgba(), additive);
 	}
 
 	__device__ __forceinline__ void insert(uint64_t key, SampleColor<FP32> col, bool additive) {
 		static_assert(kWindow == 4, "window slots are spelled out");
-		const bool full = k3 != ~0ull;
-		const bool g0 = key < k0, g1 = key < k1, g2 = key < k2, g3 = key < k3;
-		// the five keys in order: s0 <= s1 <= s2 <= s3 <= s4
-		uint64_t s0 = g0 ? key : k0;
-		uint64_t s1 = g0 ? k0 : (g1 ? key : k1);
-		uint64_t s2 = g1 ? k1 : (g2 ? key : k2);
-		uint64_t s3 = g2 ? k2 : (g3 ? key : k3);
-		uint64_t s4 = g3 ? k3 : key;
-		SampleColor<FP32> t0 = g0 ? col : c0;
-		SampleColor<FP32> t1 = g0 ? c0 : (g1 ? col : c1);
-		SampleColor<FP32> t2 = g1 ? c1 : (g2 ? col : c2);
-		SampleColor<FP32> t3 = g2 ? c2 : (g3 ? col : c3);
-		SampleColor<FP32> t4 = g3 ? c3 : col;
-		if(full) {
-			emit(s0, t0, additive);
-			k0 = s1, k1 = s2, k2 = s3, k3 = s4;
-			c0 = t1, c1 = t2, c2 = t3, c3 = t4;
-		} else {
-			k0 = s0, k1 = s1, k2 = s2, k3 = s3;
-			c0 = t0, c1 = t1, c2 = t2, c3 = t3;
+		if(n < (uint32_t)kWindow) {
+			k0 = n == 0 ? key : k0, c0 = SC::sel(n == 0, col, c0);
+			k1 = n == 1 ? key : k1, c1 = SC::sel(n == 1, col, c1);
+			k2 = n == 2 ? key : k2, c2 = SC::sel(n == 2, col, c2);
+			k3 = n == 3 ? key : k3, c3 = SC::sel(n == 3, col, c3);
+			n++;
+			return;
 		}
+		// nearest pending sample by a two-level tournament
+		const bool a = k1 < k0, bq = k3 < k2;
+		const uint64_t ka = a ? k1 : k0, kb = bq ? k3 : k2;
+		const SC ca = SC::sel(a, c1, c0), cb = SC::sel(bq, c3, c2);
+		const bool m = kb < ka;
+		const uint64_t km = m ? kb : ka;
+		const SC cm = SC::sel(m, cb, ca);
+		const bool take_new = key < km;
+		emit(take_new ? key : km, SC::sel(take_new, col, cm), additive);
+		// the new sample takes the emitted slot
+		const bool s0 = !take_new && !m && !a, s1 = !take_new && !m && a;
+		const bool s2 = !take_new && m && !bq, s3 = !take_new && m && bq;
+		k0 = s0 ? key : k0, c0 = SC::sel(s0, col, c0);
+		k1 = s1 ? key : k1, c1 = SC::sel(s1, col, c1);
+		k2 = s2 ? key : k2, c2 = SC::sel(s2, col, c2);
+		k3 = s3 ? key : k3, c3 = SC::sel(s3, col, c3);
 	}
 
 	__device__ __forceinline__ void flush(bool additive) {
-		if(k0 != ~0ull)
-			emit(k0, c0, additive);
-		if(k1 != ~0ull)
-			emit(k1, c1, additive);
-		if(k2 != ~0ull)
-			emit(k2, c2, additive);
-		if(k3 != ~0ull)
-			emit(k3, c3, additive);
-		k0 = k1 = k2 = k3 = ~0ull;
+		for(uint32_t r = 0; r < n; r++) {
+			const bool a = k1 < k0, bq = k3 < k2;
+			const uint64_t ka = a ? k1 : k0, kb = bq ? k3 : k2;
+			const SC ca = SC::sel(a, c1, c0), cb = SC::sel(bq, c3, c2);
+			const bool m = kb < ka;
+			emit(m ? kb : ka, SC::sel(m, cb, ca), additive);
+			const bool s0 = !m && !a, s1 = !m && a, s2 = m && !bq, s3 = m && bq;
+			k0 = s0 ? kEmptyKey : k0;
+			k1 = s1 ? kEmptyKey : k1;
+			k2 = s2 ? kEmptyKey : k2;
+			k3 = s3 ? kEmptyKey : k3;
+		}
+		n = 0;
 	}
 };
 
@@ -414,21 +519,25 @@ __device__ __forceinline__ BinInfo load_bin(const FrameConst &c, const FrameBufs
 	return bi;
 }
 
-// Fallback queue entry: {bin, block | level << 8 | whole << 16, record offset, record count},
-// {pixel mask lo, hi, px0, py0}.
-__device__ __forceinline__ void enqueue_fallback(const FrameBufs &b, int bin, int blk, int level,
-												 uint32_t rec_off, uint32_t nrec, int px0,
-												 int py0, uint64_t mask, bool whole) {
-	uint32_t slot = atomicAdd(&b.ctrl[kCtrlFallbackCount], 1u);
-	if(slot >= b.fallback_cap) {
-		atomicOr(&b.ctrl[kCtrlOverflow], kOvfFallback);
+// Exact-fallback queue: one uint4 item per pixel, {block slot, px, py, whole}. Called by
+// a whole wave; lanes with `mine` set queue their pixel (px, py).
+__device__ __forceinline__ void enqueue_pixels(const FrameBufs &b, uint32_t slot, bool mine, int px,
+											   int py, bool whole) {
+	const int lane = lane_id();
+	const uint64_t sel = ballot(mine);
+	if(!sel)
 		return;
+	uint32_t base = 0;
+	if(lane == 0)
+		base = atomicAdd(&b.ctrl[kCtrlFallbackCount], (uint32_t)__popcll(sel));
+	base = __shfl(base, 0, 64);
+	if(mine) {
+		const uint32_t i = base + (uint32_t)__popcll(sel & ((1ull << lane) - 1ull));
+		if(i < b.fallback_cap)
+			b.fallback_queue[i] = make_uint4(slot, (uint32_t)px, (uint32_t)py, whole ? 1u : 0u);
+		else
+			atomicOr(&b.ctrl[kCtrlOverflow], kOvfFallback);
 	}
-	b.fallback_queue[2 * slot] = make_uint4(
-		(uint32_t)bin, (uint32_t)blk | ((uint32_t)level << 8) | (whole ? 1u << 16 : 0u), rec_off,
-		nrec);
-	b.fallback_queue[2 * slot + 1] =
-		make_uint4((uint32_t)mask, (uint32_t)(mask >> 32), (uint32_t)px0, (uint32_t)py0);
 }
 
 // ------------------------------------------------------------------ rowgen_low
@@ -567,88 +676,112 @@ __global__ __launch_bounds__(1024) void high_prep_kernel(FrameConst c, FrameBufs
 	}
 }
 
-// Records of one 8-row band of a high bin with the 4-row rule (raster_high.glsl:54-90).
-// kRowgenSplit work-groups share a band; positions come from wave-aggregated atomics.
+// Block-row records of a high bin with the 4-row rule (raster_high.glsl:54-90). Each lane
+// takes one triangle of the bin and scans its rows once, from the first row of its 4-row
+// group (the start row fixes the scanline arithmetic) to its last row, emitting one record
+// per 8-row band it covers; kRowgenSplit work-groups share a bin. Record and list
+// positions come from wave-aggregated atomics.
 __global__ __launch_bounds__(kRowgenThreads) void rowgen_high_kernel(FrameConst c, FrameBufs b) {
 	if(b.ctrl[kCtrlOverflow] & (kOvfBinLists | kOvfRecords | kOvfLists))
 		return;
 	const lucid_info *info = (const lucid_info *)b.info;
-	const uint32_t slot = blockIdx.x / (4 * kRowgenSplit);
-	const int by = (blockIdx.x / kRowgenSplit) & 3, split = blockIdx.x % kRowgenSplit;
+	const uint32_t slot = blockIdx.x / kRowgenSplit;
+	const int split = blockIdx.x % kRowgenSplit;
 	if(slot >= (uint32_t)info->bin_level_counts[LUCID_BIN_LEVEL_HIGH])
 		return;
 	const uint32_t *counts = b.info + LUCID_INFO_U32_SIZE;
 	const int bin = (int)counts[LUCID_COUNTS_HIGH_BINS * c.bin_count + slot];
 	const BinInfo bi = load_bin(c, b, bin);
-	const uint4 *slot4 = b.blk_slot + (size_t)bin * 16 + by * 4;
-	uint4 *recs = b.high_records + slot4[0].z;
-	uint32_t *band_n = b.band_count + (size_t)bin * 4 + by;
-	uint32_t *blk_n = b.blk_count + (size_t)bin * 16 + by * 4;
-	const int band0 = by * 8;
+	const uint4 *slot16 = b.blk_slot + (size_t)bin * 16;
+	uint32_t *band_n = b.band_count + (size_t)bin * 4;
+	uint32_t *blk_n = b.blk_count + (size_t)bin * 16;
 	const int lane = lane_id();
 	const uint32_t stride = kRowgenThreads * kRowgenSplit;
 	const uint32_t wave_base = split * kRowgenThreads + (threadIdx.x & ~63u);
 	for(uint32_t base = wave_base; base < bi.nlevel; base += stride) {
-		uint32_t i = base + lane;
-		uint32_t t = i < bi.nlevel ? bin_tri_at(b, i, bi.qc, bi.qo, bi.to) : 0xffffffffu;
-		uint32_t sp[8], bxm = 0;
+		const uint32_t i = base + lane;
+		const uint32_t t = i < bi.nlevel ? bin_tri_at(b, i, bi.qc, bi.qo, bi.to) : 0xffffffffu;
+		int row = 0, row_lo = 0, row_hi = -1;
+		RowScan rs;
 		if(t != 0xffffffffu) {
-			uint4 s0 = b.tri_scan[2 * (size_t)t];
-			int min_g = min(max((int)(s0.w & 0xffffu) - bi.bin_y, 0), kBinMask) >> 2;
-			int max_g = min(max((int)(s0.w >> 16) - bi.bin_y, 0), kBinMask) >> 2;
-			int row_lo = min_g * 4, row_hi = max_g * 4 + 3;
-			if(row_hi >= band0 && row_lo <= band0 + 7) {
-				uint4 s1 = b.tri_scan[2 * (size_t)t + 1];
-				RowScan rs;
-				row_scan_init(rs, s0, s1, (float)bi.bin_x, (float)(bi.bin_y + row_lo));
-#pragma unroll
-				for(int r = 0; r < 8; r++)
-					sp[r] = kBinMask; // empty (31, 0)
-				int last = min(row_hi, band0 + 7);
-				for(int row = row_lo; row <= last; row++) {
-					uint32_t v = row_scan_step(rs);
-					if(row >= band0) {
-						int r = row - band0;
-#pragma unroll
-						for(int k = 0; k < 8; k++)
-							if(k == r)
-								sp[k] = v;
-						bxm |= span_bx_mask(v);
-					}
+			const uint4 s0 = b.tri_scan[2 * (size_t)t], s1 = b.tri_scan[2 * (size_t)t + 1];
+			const int min_g = min(max((int)(s0.w & 0xffffu) - bi.bin_y, 0), kBinMask) >> 2;
+			const int max_g = min(max((int)(s0.w >> 16) - bi.bin_y, 0), kBinMask) >> 2;
+			row_lo = min_g * 4, row_hi = max_g * 4 + 3;
+			row_scan_init(rs, s0, s1, (float)bi.bin_x, (float)(bi.bin_y + row_lo));
+			row = row_lo;
+		}
+		for(int by = 0; by < 4; by++) {
+			const int band0 = by * 8;
+			const bool in_band = row_hi >= band0 && row_lo <= band0 + 7;
+			if(!ballot(in_band))
+				continue;
+			// rows 0-5 of the band in lo (10 bits each), rows 6-7 in hi; rows the triangle
+			// does not reach stay empty (31, 0)
+			uint64_t lo = 0;
+			uint32_t hi = 0, bxm = 0;
+			int r0 = 8, r1 = -1;
+			if(in_band) {
+				// rows before band0 were scanned with the previous band
+				const int last = min(row_hi, band0 + 7);
+				r0 = row - band0, r1 = last - band0;
+				for(; row <= last; row++) {
+					const uint32_t v = row_scan_step(rs);
+					const int r = row - band0;
+					bxm |= span_bx_mask(v);
+					if(r < 6)
+						lo |= (uint64_t)v << (10 * r);
+					else
+						hi |= v << (10 * (r - 6));
 				}
 			}
-		}
-		uint64_t emit = ballot(bxm != 0);
-		if(!emit)
-			continue;
-		uint32_t rbase = 0;
-		if(lane == 0)
-			rbase = atomicAdd(band_n, (uint32_t)__popcll(emit));
-		rbase = __shfl(rbase, 0, 64);
-		uint32_t idx = rbase + (uint32_t)__popcll(emit & ((1ull << lane) - 1ull));
-		if(bxm)
-			recs[idx] = make_record(t, sp, bxm);
 #pragma unroll
-		for(int bx = 0; bx < 4; bx++) {
-			uint64_t sel = ballot((bxm >> bx) & 1u);
-			if(!sel)
+			for(int k = 0; k < 8; k++) {
+				if(k >= r0 && k <= r1)
+					continue;
+				if(k < 6)
+					lo |= (uint64_t)kBinMask << (10 * k);
+				else
+					hi |= (uint32_t)kBinMask << (10 * (k - 6));
+			}
+			const uint64_t emit = ballot(bxm != 0);
+			if(!emit)
 				continue;
-			uint32_t lb = 0;
+			uint32_t rbase = 0;
 			if(lane == 0)
-				lb = atomicAdd(&blk_n[bx], (uint32_t)__popcll(sel));
-			lb = __shfl(lb, 0, 64);
-			if((bxm >> bx) & 1u)
-				b.blk_lists[slot4[bx].x + lb + (uint32_t)__popcll(sel & ((1ull << lane) - 1ull))] = idx;
+				rbase = atomicAdd(&band_n[by], (uint32_t)__popcll(emit));
+			rbase = __shfl(rbase, 0, 64);
+			const uint32_t idx = rbase + (uint32_t)__popcll(emit & ((1ull << lane) - 1ull));
+			if(bxm)
+				b.high_records[slot16[by * 4].z + idx] =
+					make_uint4(t, (uint32_t)lo & 0x3fffffffu, (uint32_t)(lo >> 30) & 0x3fffffffu,
+							   hi | (bxm << 20));
+#pragma unroll
+			for(int bx = 0; bx < 4; bx++) {
+				const uint64_t sel = ballot((bxm >> bx) & 1u);
+				if(!sel)
+					continue;
+				uint32_t lb = 0;
+				if(lane == 0)
+					lb = atomicAdd(&blk_n[by * 4 + bx], (uint32_t)__popcll(sel));
+				lb = __shfl(lb, 0, 64);
+				if((bxm >> bx) & 1u)
+					b.blk_lists[slot16[by * 4 + bx].x + lb +
+								(uint32_t)__popcll(sel & ((1ull << lane) - 1ull))] = idx;
+			}
 		}
 	}
 }
 
 // Entries (triangles of one block in depth order) are handed to the rasteriser in slices
 // of up to kSliceEntries: entry lane + 64 k of the slice sits in register k of that lane.
-constexpr int kSliceEntries = 256;
-constexpr int kSliceWords = kSliceEntries / 64;
+#ifndef LUCID_SLICE_WORDS
+#define LUCID_SLICE_WORDS 1
+#endif
+constexpr int kSliceWords = LUCID_SLICE_WORDS;
+constexpr int kSliceEntries = 64 * kSliceWords;
 constexpr uint32_t kSamplesPerWave = 2048; // target samples per wave of a pre-sorted block
-constexpr int kBigWaves = 8;               // waves of block_raster_big_kernel
+constexpr int kBigWaves = 4;               // waves of block_raster_big_kernel
 
 // Pre-sorted block layout in the sorted pool (offsets in words from an even base):
 //   [0] number of waves W, [1 .. W + 1] first entry of each wave's range (then m),
@@ -688,9 +821,13 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 		return;
 	}
 	if(m > (uint32_t)kBigSortCap) {
-		if(threadIdx.x == 0) {
-			b.blk_flags[slot] = LUCID_BIN_LEVEL_HIGH | kSlotWhole;
-			enqueue_fallback(b, bin, blk, LUCID_BIN_LEVEL_HIGH, sl.z, nrec, px0, py0, ~0ull, true);
+		if(threadIdx.x < 64) {
+			const int px = px0 + (threadIdx.x & 7), py = py0 + (threadIdx.x >> 3);
+			if(threadIdx.x == 0) {
+				b.blk_flags[slot] = LUCID_BIN_LEVEL_HIGH | kSlotWhole;
+				atomicAdd(&((lucid_info *)b.info)->stats[3], 1u); // blocks too long to pre-sort
+			}
+			enqueue_pixels(b, slot, px < c.width && py < c.height, px, py, true);
 		}
 		return;
 	}
@@ -822,33 +959,58 @@ __device__ __forceinline__ void transpose_masks(const uint64_t (&msk)[kSliceWord
 	}
 }
 
+// Stages the shading records of a slice in LDS: lane l writes entries l + 64 k.
+__device__ __forceinline__ void stage_slice(const FrameConst &c, const FrameBufs &b,
+											const uint32_t (&tri)[kSliceWords], uint32_t n,
+											uint4 *s_rec) {
+	const int lane = lane_id();
+#pragma unroll
+	for(int k = 0; k < kSliceWords; k++) {
+		const uint32_t i = k * 64 + lane;
+		if(i < n) {
+			uint4 q[kRecQuads];
+			load_shading_record(c, b, tri[k], q);
+#pragma unroll
+			for(int j = 0; j < kRecQuads; j++)
+				s_rec[i * kRecQuads + j] = q[j];
+		}
+	}
+}
+
 // Walks this lane's pixel through the slice in entry order: each step takes the nearest
-// remaining entry that covers the pixel, shades the sample and feeds the ordering window.
-// `s_tri` holds the slice's triangle ids in LDS.
+// remaining entry that covers the pixel, shades the sample from the entry's LDS record and
+// feeds the ordering window.
 template <bool FP32>
-__device__ __forceinline__ void walk_pixel(const FrameConst &c, const FrameBufs &b,
-										   const uint32_t *s_tri, uint64_t (&bm)[kSliceWords],
-										   float px, float py, PixelState<FP32> &ps, bool additive) {
-	static_assert(kSliceWords == 4, "word selection is spelled out");
+__device__ __forceinline__ void walk_pixel(const FrameConst &c, const uint4 *s_rec,
+										   uint64_t (&bm)[kSliceWords], float px, float py,
+										   PixelState<FP32> &ps, bool additive) {
 	while(true) {
-		const bool nz0 = bm[0] != 0ull, nz1 = bm[1] != 0ull, nz2 = bm[2] != 0ull,
-				   nz3 = bm[3] != 0ull;
-		const bool any = nz0 || nz1 || nz2 || nz3;
+		bool nz[kSliceWords], any = false;
+#pragma unroll
+		for(int k = 0; k < kSliceWords; k++) {
+			nz[k] = bm[k] != 0ull;
+			any = any || nz[k];
+		}
 		if(!ballot(any))
 			break;
 		if(any) {
-			const uint64_t cur = nz0 ? bm[0] : nz1 ? bm[1] : nz2 ? bm[2] : bm[3];
-			const int word = nz0 ? 0 : nz1 ? 1 : nz2 ? 2 : 3;
+			uint64_t cur = bm[kSliceWords - 1];
+			int word = kSliceWords - 1;
+#pragma unroll
+			for(int k = kSliceWords - 2; k >= 0; k--) {
+				cur = nz[k] ? bm[k] : cur;
+				word = nz[k] ? k : word;
+			}
 			const uint64_t rest = cur & (cur - 1ull);
-			bm[0] = nz0 ? rest : bm[0];
-			bm[1] = (!nz0 && nz1) ? rest : bm[1];
-			bm[2] = (!nz0 && !nz1 && nz2) ? rest : bm[2];
-			bm[3] = (!nz0 && !nz1 && !nz2) ? rest : bm[3];
-			const uint32_t t = s_tri[word * 64 + (int)__builtin_ctzll(cur)];
-			const float inv = sample_inv_depth(b, t, px, py);
-			const SampleColor<FP32> col = shade_sample<FP32>(c, b, t, px, py, inv);
+#pragma unroll
+			for(int k = 0; k < kSliceWords; k++)
+				bm[k] = word == k ? rest : bm[k];
+			const uint4 *rec = s_rec + (word * 64 + (int)__builtin_ctzll(cur)) * kRecQuads;
+			const uint4 q0 = rec[0];
+			const float inv = record_inv_depth(q0, px, py);
+			const SampleColor<FP32> col = shade_record<FP32>(c, rec, px, py, inv);
 			if(col.contributes())
-				ps.insert(pixel_key(inv, t), col, additive);
+				ps.insert(pixel_key(inv, rec[6].z), col, additive);
 		}
 	}
 }
@@ -865,16 +1027,13 @@ __device__ __forceinline__ uint32_t bitmap_count(const uint64_t (&bm)[kSliceWord
 // frame statistics (one lane per block pixel).
 __device__ __forceinline__ void finish_pixels(const FrameConst &c, const FrameBufs &b,
 											  const Accum &acc, bool bad, uint32_t count, int px,
-											  int py, int bin, int blk, int level, uint4 sl,
-											  int px0, int py0, uint32_t m) {
+											  int py, uint32_t slot, uint32_t m) {
 	const int lane = lane_id();
 	const bool on_screen = px < c.width && py < c.height;
 	write_pixel(c, b, px, py, acc, count);
-	const uint64_t bad_px = ballot(bad && on_screen);
+	enqueue_pixels(b, slot, bad && on_screen, px, py, false);
 	const uint32_t frags = __shfl(wave_inclusive_add(on_screen ? count : 0u), 63, 64);
 	if(lane == 0) {
-		if(bad_px)
-			enqueue_fallback(b, bin, blk, level, sl.z, sl.w, px0, py0, bad_px, false);
 		lucid_info *info = (lucid_info *)b.info;
 		atomicAdd(&info->stats[0], frags);
 		if(m)
@@ -882,14 +1041,14 @@ __device__ __forceinline__ void finish_pixels(const FrameConst &c, const FrameBu
 	}
 }
 
-// One wave per block whose list fits one slice (every low bin, most high blocks): sorts
-// the list by centroid depth in registers, then walks every pixel through it.
+// One wave per block whose list fits the register sort (every low bin, most high blocks):
+// sorts the list by centroid depth in registers, then walks every pixel through it slice by
+// slice (each slice's shading records staged in LDS).
 template <bool FP32>
 __global__ __launch_bounds__(64) void block_raster_kernel(FrameConst c, FrameBufs b) {
 	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
 		return;
-	static_assert(kRegSortCap == kSliceEntries, "one register-sorted list is one slice");
-	__shared__ uint32_t s_tri[kSliceEntries];
+	__shared__ uint4 s_rec[kSliceEntries * kRecQuads];
 	const int lane = lane_id();
 	const uint32_t slot = blockIdx.x;
 	const int bin = slot >> 4, blk = slot & 15, by = blk >> 2, bx = blk & 3;
@@ -931,33 +1090,52 @@ __global__ __launch_bounds__(64) void block_raster_kernel(FrameConst c, FrameBuf
 		v[0] = load(lane), v[1] = load(64 + lane), v[2] = load(128 + lane), v[3] = load(192 + lane);
 		bitonic_regs<4>(v);
 	}
-	uint64_t msk[kSliceWords];
+	// sorted entry 64 j + lane: list position (uint32_t)v[j]
+	uint64_t msk_all[4];
+	uint32_t tri_all[4];
 #pragma unroll
-	for(int e = 0; e < kSliceWords; e++) {
-		uint32_t i = e * 64 + lane;
-		msk[e] = 0;
-		if(i < m) {
-			uint4 rec = recs[list[(uint32_t)v[e]]];
-			msk[e] = record_block_mask(rec, bx);
-			s_tri[i] = rec.x;
+	for(int j = 0; j < 4; j++) {
+		msk_all[j] = 0, tri_all[j] = 0;
+		if(j * 64 + lane < (int)m) {
+			uint4 rec = recs[list[(uint32_t)v[j]]];
+			msk_all[j] = record_block_mask(rec, bx);
+			tri_all[j] = rec.x;
 		}
 	}
-	wave_sync_lds();
-	uint64_t bm[kSliceWords];
-	transpose_masks(msk, (int)((m + 63) / 64), bm);
-	const uint32_t count = bitmap_count(bm);
 	const bool additive = (c.opts & (1u << 5)) != 0;
 	PixelState<FP32> ps;
 	ps.init();
-	walk_pixel<FP32>(c, b, s_tri, bm, (float)px, (float)py, ps, additive);
+	uint32_t count = 0;
+	static_assert(4 % kSliceWords == 0, "register-sorted lists split into whole slices");
+	for(uint32_t s0 = 0; s0 < m; s0 += kSliceEntries) {
+		const int j0 = (int)(s0 / 64);
+		uint64_t msk[kSliceWords];
+		uint32_t tri[kSliceWords];
+#pragma unroll
+		for(int k = 0; k < kSliceWords; k++) {
+			msk[k] = msk_all[k], tri[k] = tri_all[k];
+#pragma unroll
+			for(int j = kSliceWords; j < 4; j++)
+				if(j0 + k == j)
+					msk[k] = msk_all[j], tri[k] = tri_all[j];
+		}
+		const uint32_t n = min(m - s0, (uint32_t)kSliceEntries);
+		stage_slice(c, b, tri, n, s_rec);
+		wave_sync_lds();
+		uint64_t bm[kSliceWords];
+		transpose_masks(msk, (int)((n + 63) / 64), bm);
+		count += bitmap_count(bm);
+		walk_pixel<FP32>(c, s_rec, bm, (float)px, (float)py, ps, additive);
+		wave_sync_lds(); // every lane is done with this slice's records
+	}
 	ps.flush(additive);
-	finish_pixels(c, b, ps.acc, ps.bad, count, px, py, bin, blk, (int)level, sl, px0, py0, m);
+	finish_pixels(c, b, ps.acc, ps.bad, count, px, py, slot, m);
 }
 
-// Pre-sorted lists longer than one slice: one work-group per block. Wave w takes the
-// contiguous depth range [start[w], start[w + 1]) of the sorted list (ranges balanced by
-// sample count), walks it slice by slice with one ordering window per pixel, and leaves a
-// partial composite; wave 0 chains the partials front to back. A pixel is exact iff each
+// Pre-sorted lists longer than one register sort: one work-group per block. Wave w takes
+// the contiguous depth range [start[w], start[w + 1]) of the sorted list (ranges balanced
+// by sample count), walks it slice by slice with one ordering window per pixel, and leaves
+// a partial composite; wave 0 chains the partials front to back. A pixel is exact iff each
 // range's blended keys increase and every range starts behind the previous one's last key.
 template <bool FP32>
 __global__ __launch_bounds__(kBigWaves * 64) void block_raster_big_kernel(FrameConst c,
@@ -974,7 +1152,7 @@ __global__ __launch_bounds__(kBigWaves * 64) void block_raster_big_kernel(FrameC
 	const uint32_t flags = b.blk_flags[slot];
 	if(!(flags & kSlotPresorted) || (flags & kSlotPromoted))
 		return;
-	__shared__ uint32_t s_tri[kBigWaves][kSliceEntries];
+	__shared__ uint4 s_rec[kBigWaves][kSliceEntries * kRecQuads];
 	__shared__ float4 s_part[kBigWaves][64];
 	__shared__ uint64_t s_first[kBigWaves][64], s_last[kBigWaves][64];
 	__shared__ uint32_t s_cnt[kBigWaves][64];
@@ -997,19 +1175,20 @@ __global__ __launch_bounds__(kBigWaves * 64) void block_raster_big_kernel(FrameC
 		for(uint32_t s0 = e0; s0 < e1; s0 += kSliceEntries) {
 			const uint32_t n = min(e1 - s0, (uint32_t)kSliceEntries);
 			uint64_t msk[kSliceWords];
+			uint32_t t[kSliceWords];
 #pragma unroll
 			for(int k = 0; k < kSliceWords; k++) {
 				uint32_t i = k * 64 + lane;
 				msk[k] = i < n ? mask[s0 + i] : 0ull;
-				if(i < n)
-					s_tri[wave][i] = tri[s0 + i];
+				t[k] = i < n ? tri[s0 + i] : 0u;
 			}
+			stage_slice(c, b, t, n, s_rec[wave]);
 			wave_sync_lds();
 			uint64_t bm[kSliceWords];
 			transpose_masks(msk, (int)((n + 63) / 64), bm);
 			count += bitmap_count(bm);
-			walk_pixel<FP32>(c, b, s_tri[wave], bm, (float)px, (float)py, ps, additive);
-			wave_sync_lds(); // every lane is done with this slice's triangle ids
+			walk_pixel<FP32>(c, s_rec[wave], bm, (float)px, (float)py, ps, additive);
+			wave_sync_lds(); // every lane is done with this slice's records
 		}
 		ps.flush(additive);
 		s_part[wave][lane] = make_float4(ps.acc.cr, ps.acc.cg, ps.acc.cb, ps.acc.tr);
@@ -1030,8 +1209,8 @@ __global__ __launch_bounds__(kBigWaves * 64) void block_raster_big_kernel(FrameC
 		count += cw & 0x7fffffffu;
 		bad |= (cw >> 31) != 0;
 		const uint64_t f = s_first[w][lane];
-		if(f != ~0ull) {
-			bad |= f < last;
+		if(f != kEmptyKey) {
+			bad |= f <= last;
 			last = s_last[w][lane];
 			const float4 p = s_part[w][lane];
 			acc.cr = acc.cr + acc.tr * p.x;
@@ -1040,44 +1219,62 @@ __global__ __launch_bounds__(kBigWaves * 64) void block_raster_big_kernel(FrameC
 			acc.tr = acc.tr * p.w;
 		}
 	}
-	finish_pixels(c, b, acc, bad, count, px, py, bin, blk, LUCID_BIN_LEVEL_HIGH, sl, px0, py0, m);
+	finish_pixels(c, b, acc, bad, count, px, py, slot, m);
 }
 
 // ------------------------------------------------------------------ exact fallback
 
-// Collects the keys of one pixel's fragments into `dst` (LDS or global), returns the count.
-__device__ uint32_t gather_pixel_keys(const FrameBufs &b, const uint4 *recs, uint32_t nrec, int row,
-									  int xb, float px, float py, uint64_t *dst, uint32_t cap,
-									  bool global_dst) {
+// Collects the keys (depth key << 32 | triangle id) of one pixel's fragments into `dst`
+// (LDS or global) from the block's list; returns the count. Entries beyond `cap` are
+// counted but not stored.
+__device__ uint32_t gather_pixel_keys(const FrameBufs &b, const uint4 *recs, const uint32_t *list,
+									  uint32_t m, int row, int xb, float px, float py,
+									  uint64_t *dst, uint32_t cap, bool global_dst) {
+	constexpr int kU = 4; // entries per lane per step: their loads are issued together
 	const int lane = lane_id();
 	uint32_t n = 0;
-	for(uint32_t base = 0; base < nrec; base += 64) {
-		uint32_t i = base + lane;
-		bool hit = false;
-		uint64_t key = 0;
-		if(i < nrec) {
-			uint4 rec = recs[i];
-			uint32_t sp = record_span(rec, row);
-			int a = (int)(sp & 31u), e = (int)((sp >> 5) & 31u);
-			hit = a <= xb && xb <= e;
-			if(hit)
-				key = pixel_key(sample_inv_depth(b, rec.x, px, py), rec.x);
+	for(uint32_t base = 0; base < m; base += 64 * kU) {
+		uint32_t idx[kU];
+#pragma unroll
+		for(int k = 0; k < kU; k++) {
+			const uint32_t j = base + k * 64 + lane;
+			idx[k] = j < m ? list[j] : 0xffffffffu;
 		}
-		uint64_t bal = ballot(hit);
-		if(hit) {
-			uint32_t pos = n + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-			if(pos < cap) {
-				if(global_dst)
-					__hip_atomic_store(&dst[pos], key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-				else
-					dst[pos] = key;
+		uint4 rec[kU];
+#pragma unroll
+		for(int k = 0; k < kU; k++)
+			rec[k] = idx[k] != 0xffffffffu ? recs[idx[k]] : make_uint4(0u, kBinMask, kBinMask, kBinMask);
+		bool hit[kU];
+		uint4 d[kU];
+#pragma unroll
+		for(int k = 0; k < kU; k++) {
+			const uint32_t sp = record_span(rec[k], row);
+			const int a = (int)(sp & 31u), e = (int)((sp >> 5) & 31u);
+			hit[k] = idx[k] != 0xffffffffu && a <= xb && xb <= e;
+			d[k] = hit[k] ? b.tri_depth[rec[k].x] : make_uint4(0u, 0u, 0u, 0u);
+		}
+#pragma unroll
+		for(int k = 0; k < kU; k++) {
+			const uint64_t key = pixel_key(record_inv_depth(d[k], px, py), rec[k].x);
+			const uint64_t bal = ballot(hit[k]);
+			if(hit[k]) {
+				const uint32_t pos = n + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+				if(pos < cap) {
+					if(global_dst)
+						__hip_atomic_store(&dst[pos], key, __ATOMIC_RELAXED,
+										   __HIP_MEMORY_SCOPE_AGENT);
+					else
+						dst[pos] = key;
+				}
 			}
+			n += (uint32_t)__popcll(bal);
 		}
-		n += (uint32_t)__popcll(bal);
 	}
 	return n;
 }
 
+// Blends n sorted keys front to back: 64 samples are shaded in parallel, then composited
+// in order.
 template <bool FP32>
 __device__ Accum blend_sorted(const FrameConst &c, const FrameBufs &b, const uint64_t *keys,
 							  uint32_t n, float px, float py, bool global_src) {
@@ -1086,19 +1283,19 @@ __device__ Accum blend_sorted(const FrameConst &c, const FrameBufs &b, const uin
 	Accum acc;
 	acc.init();
 	for(uint32_t base = 0; base < n; base += 64) {
-		uint32_t i = base + lane;
+		const uint32_t i = base + lane;
 		SampleColor<FP32> col;
 		col.zero();
 		if(i < n) {
-			uint64_t k = global_src
-							 ? __hip_atomic_load(&keys[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-							 : keys[i];
-			uint32_t t = (uint32_t)k;
+			const uint64_t k = global_src ? __hip_atomic_load(&keys[i], __ATOMIC_RELAXED,
+															  __HIP_MEMORY_SCOPE_AGENT)
+										  : keys[i];
+			const uint32_t t = (uint32_t)k;
 			col = shade_sample<FP32>(c, b, t, px, py, sample_inv_depth(b, t, px, py));
 		}
-		uint32_t cnt = min(64u, n - base);
+		const uint32_t cnt = min(64u, n - base);
 		for(uint32_t j = 0; j < cnt; j++) {
-			SampleColor<FP32> c2 = col.shfl((int)j);
+			const SampleColor<FP32> c2 = col.shfl((int)j);
 			if(c2.contributes())
 				acc.blend(c2.rgba(), additive);
 		}
@@ -1106,93 +1303,77 @@ __device__ Accum blend_sorted(const FrameConst &c, const FrameBufs &b, const uin
 	return acc;
 }
 
+// Exact path for one queued pixel: gathers its fragments from the block list, sorts the
+// keys (LDS bitonic up to kFallbackCap, else a global spill), shades and blends in order.
 template <bool FP32>
-__device__ void fallback_entry(const FrameConst &c, const FrameBufs &b, uint32_t qi,
-							   uint64_t (*s_keys)[kFallbackCap]) {
-	uint4 q0 = b.fallback_queue[2 * qi], q1 = b.fallback_queue[2 * qi + 1];
-	int blk = (int)(q0.y & 0xffu), level = (int)((q0.y >> 8) & 0xffu);
-	bool whole = (q0.y >> 16) & 1u;
-	int bx = blk & 3;
-	const uint4 *recs = (level == LUCID_BIN_LEVEL_LOW ? b.low_records : b.high_records) + q0.z;
-	uint32_t nrec = q0.w;
-	uint64_t mask = (uint64_t)q1.x | ((uint64_t)q1.y << 32);
-	int px0 = (int)q1.z, py0 = (int)q1.w;
-	const int w = threadIdx.x >> 6, lane = lane_id();
-	const int nw = kFallbackThreads / 64;
-	uint32_t pixels = 0, frags = 0;
-	int rank = 0;
-	for(int p = 0; p < 64; p++) {
-		if(!((mask >> p) & 1ull))
-			continue;
-		if((rank++ % nw) != w)
-			continue;
-		int px = px0 + (p & 7), py = py0 + (p >> 3);
-		if(px >= c.width || py >= c.height)
-			continue;
-		int row = p >> 3, xb = bx * 8 + (p & 7);
-		float fx = (float)px, fy = (float)py;
-		uint64_t *keys = s_keys[w];
-		uint32_t n = gather_pixel_keys(b, recs, nrec, row, xb, fx, fy, keys, kFallbackCap, false);
-		Accum acc;
-		if(n <= (uint32_t)kFallbackCap) {
-			uint32_t P = next_pow2(n < 2 ? 2 : n);
-			for(uint32_t i = n + lane; i < P; i += 64)
-				keys[i] = ~0ull;
-			wave_sync_lds();
-			bitonic_lds_wave(keys, P);
-			acc = blend_sorted<FP32>(c, b, keys, n, fx, fy, false);
-		} else {
-			// deep pixel: spill its keys to global memory
-			uint32_t P = next_pow2(n);
-			uint32_t off = 0;
-			if(lane == 0) {
-				off = (uint32_t)atomicAdd(&b.ctrl[kCtrlSpillCursor], P);
-				if((uint64_t)off + P > b.spill_cap) {
-					atomicOr(&b.ctrl[kCtrlOverflow], kOvfSpill);
-					off = 0xffffffffu;
-				}
+__device__ void fallback_pixel(const FrameConst &c, const FrameBufs &b, uint4 item,
+							   uint64_t *s_keys) {
+	const int lane = lane_id();
+	const uint32_t slot = item.x;
+	const int px = (int)item.y, py = (int)item.z;
+	const bool whole = item.w != 0;
+	const int blk = (int)(slot & 15u), bx = blk & 3;
+	const uint32_t level = b.blk_flags[slot] & kSlotLevelMask;
+	const uint4 sl = b.blk_slot[slot];
+	const uint4 *recs = (level == LUCID_BIN_LEVEL_LOW ? b.low_records : b.high_records) + sl.z;
+	const uint32_t *list = b.blk_lists + sl.x;
+	const uint32_t m = sl.y;
+	const int row = py & 7, xb = bx * 8 + (px & 7);
+	const float fx = (float)px, fy = (float)py;
+	const uint32_t n = gather_pixel_keys(b, recs, list, m, row, xb, fx, fy, s_keys, kFallbackCap, false);
+	Accum acc;
+	if(n <= (uint32_t)kFallbackCap) {
+		const uint32_t P = next_pow2(n < 2 ? 2 : n);
+		for(uint32_t i = n + lane; i < P; i += 64)
+			s_keys[i] = ~0ull;
+		wave_sync_lds();
+		bitonic_lds_wave(s_keys, P);
+		acc = blend_sorted<FP32>(c, b, s_keys, n, fx, fy, false);
+	} else {
+		// deep pixel: sort its keys in a global spill region
+		const uint32_t P = next_pow2(n);
+		uint32_t off = 0;
+		if(lane == 0) {
+			off = (uint32_t)atomicAdd(&b.ctrl[kCtrlSpillCursor], P);
+			if((uint64_t)off + P > b.spill_cap) {
+				atomicOr(&b.ctrl[kCtrlOverflow], kOvfSpill);
+				off = 0xffffffffu;
 			}
-			off = __shfl(off, 0, 64);
-			if(off == 0xffffffffu)
-				continue;
-			uint64_t *g = b.spill + off;
-			gather_pixel_keys(b, recs, nrec, row, xb, fx, fy, g, P, true);
-			for(uint32_t i = n + lane; i < P; i += 64)
-				__hip_atomic_store(&g[i], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-			__threadfence();
-			__builtin_amdgcn_wave_barrier();
-			bitonic_global_wave(g, P);
-			acc = blend_sorted<FP32>(c, b, g, n, fx, fy, true);
 		}
-		if(lane == 0)
-			write_pixel(c, b, px, py, acc, n);
-		pixels++;
-		frags += n;
+		off = __shfl(off, 0, 64);
+		if(off == 0xffffffffu)
+			return; // the frame reports an overflow and is rendered again
+		uint64_t *g = b.spill + off;
+		gather_pixel_keys(b, recs, list, m, row, xb, fx, fy, g, P, true);
+		for(uint32_t i = n + lane; i < P; i += 64)
+			__hip_atomic_store(&g[i], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		__threadfence();
+		__builtin_amdgcn_wave_barrier();
+		bitonic_global_wave(g, P);
+		acc = blend_sorted<FP32>(c, b, g, n, fx, fy, true);
 	}
-	if(lane == 0 && pixels) {
+	if(lane == 0) {
+		write_pixel(c, b, px, py, acc, n);
 		lucid_info *info = (lucid_info *)b.info;
-		atomicAdd(&b.ctrl[kCtrlFallbackPixels], pixels);
+		atomicAdd(&b.ctrl[kCtrlFallbackPixels], 1u);
 		// stats[2]: pixels resolved by the exact per-pixel sort (the reference counts the
 		// pixels its 3-deep window gets wrong there, shading.glsl:258-265)
-		atomicAdd(&info->stats[2], pixels);
-		if(whole) {
-			atomicAdd(&info->stats[0], frags);
-			if(w == 0)
-				atomicAdd(&info->stats[3], 1u); // stats[3]: blocks too large to pre-sort
-		}
+		atomicAdd(&info->stats[2], 1u);
+		if(whole)
+			atomicAdd(&info->stats[0], n); // these pixels skipped the block rasteriser
 	}
 }
 
-// Persistent over the queue: work-group g handles entries g, g + grid, ...
+// Persistent over the per-pixel queue: wave w of the grid takes items w, w + waves, ...
 template <bool FP32>
 __global__ __launch_bounds__(kFallbackThreads) void fallback_kernel(FrameConst c, FrameBufs b) {
 	__shared__ uint64_t s_keys[kFallbackThreads / 64][kFallbackCap];
-	uint32_t nq = min(b.ctrl[kCtrlFallbackCount], b.fallback_cap);
-	for(uint32_t qi = blockIdx.x; qi < nq; qi += gridDim.x)
-		fallback_entry<FP32>(c, b, qi, s_keys);
+	const uint32_t nq = min(b.ctrl[kCtrlFallbackCount], b.fallback_cap);
+	const uint32_t waves = gridDim.x * (kFallbackThreads / 64);
+	const uint32_t w = blockIdx.x * (kFallbackThreads / 64) + (threadIdx.x >> 6);
+	for(uint32_t qi = w; qi < nq; qi += waves)
+		fallback_pixel<FP32>(c, b, b.fallback_queue[qi], s_keys[threadIdx.x >> 6]);
 }
-
-// ------------------------------------------------------------------ launch
 
 template <bool FP32>
 static void launch_raster_t(const FrameConst &c, const FrameBufs &b, hipStream_t stream,
@@ -1201,15 +1382,15 @@ static void launch_raster_t(const FrameConst &c, const FrameBufs &b, hipStream_t
 	if(ev)
 		(void)hipEventRecord(ev[0], stream);
 	hipLaunchKernelGGL(high_prep_kernel, dim3(1), dim3(1024), 0, stream, c, b);
-	hipLaunchKernelGGL(rowgen_high_kernel, dim3(c.bin_count * 4 * kRowgenSplit),
-					   dim3(kRowgenThreads), 0, stream, c, b);
+	hipLaunchKernelGGL(rowgen_high_kernel, dim3(c.bin_count * kRowgenSplit), dim3(kRowgenThreads),
+					   0, stream, c, b);
 	hipLaunchKernelGGL(high_finalize_kernel, dim3(c.bin_count * 16), dim3(kSortThreads), 0, stream,
 					   c, b);
 	if(ev)
 		(void)hipEventRecord(ev[1], stream);
 	hipLaunchKernelGGL(block_raster_kernel<FP32>, dim3(c.bin_count * 16), dim3(64), 0, stream, c, b);
-	hipLaunchKernelGGL(block_raster_big_kernel<FP32>, dim3(c.bin_count * 16), dim3(512), 0, stream,
-					   c, b);
+	hipLaunchKernelGGL(block_raster_big_kernel<FP32>, dim3(c.bin_count * 16), dim3(kBigWaves * 64), 0,
+					   stream, c, b);
 	if(ev)
 		(void)hipEventRecord(ev[2], stream);
 	hipLaunchKernelGGL(fallback_kernel<FP32>, dim3(1024), dim3(kFallbackThreads), 0, stream, c, b);
