@@ -99,7 +99,7 @@ __global__ __launch_bounds__(kBinThreads) void bin_count_kernel(FrameConst c, Fr
 		uint32_t t = 2 * q + s;
 		int bsx = (int)(a & 0x7fu), bex = (int)((a >> 14) & 0x7fu);
 		BinScan bs;
-		bin_scan_init(bs, b.tri_scan[2 * (size_t)t], b.tri_scan[2 * (size_t)t + 1]);
+		bin_scan_init(bs, b.tri_scan[2 * (size_t)tri_rec(b, t)], b.tri_scan[2 * (size_t)tri_rec(b, t) + 1]);
 		for(int by = bs.min_by; by <= bs.max_by; by++) {
 			int bmin, bmax;
 			bin_scan_step(bs, bmin, bmax);
@@ -132,21 +132,54 @@ __global__ __launch_bounds__(kBinThreads) void bin_count_kernel(FrameConst c, Fr
 }
 
 // Per bin: totals over work-groups and each work-group's exclusive offset inside the bin.
-__global__ __launch_bounds__(256) void bin_wg_scan_kernel(FrameConst c, FrameBufs b) {
-	const int NB = c.bin_count;
-	int i = blockIdx.x * 256 + threadIdx.x;
-	if(i >= 2 * NB)
-		return;
-	int k = i / NB, bin = i - k * NB;
-	uint32_t run = 0;
-	for(int w = 0; w < c.num_bin_wgs; w++) {
-		uint32_t *p = b.wg_hist + (size_t)w * 2 * NB + (size_t)k * NB + bin;
-		uint32_t v = *p;
-		*p = run;
-		run += v;
+// Exclusive scan of the per-work-group bin histograms down each (kind, bin) column, and the
+// column totals (the bin counts). A work-group takes 16 columns; its 16 row-parts each sum
+// a run of rows with independent loads, an LDS scan joins the parts, then each part writes
+// its exclusive prefixes.
+constexpr int kScanCols = 16, kScanParts = 16;
+__global__ __launch_bounds__(kScanCols * kScanParts) void bin_wg_scan_kernel(FrameConst c,
+																			  FrameBufs b) {
+	const int NB = c.bin_count, nw = c.num_bin_wgs;
+	const int col_l = threadIdx.x % kScanCols, part = threadIdx.x / kScanCols;
+	const int col = blockIdx.x * kScanCols + col_l; // 0 .. 2 NB: quads then tris
+	const int rows = (nw + kScanParts - 1) / kScanParts; // <= 16: num_bin_wgs <= 256
+	const int w0 = part * rows, w1 = min(nw, w0 + rows);
+	__shared__ uint32_t s_part[kScanParts][kScanCols];
+	uint32_t sum = 0;
+	if(col < 2 * NB)
+		for(int w = w0; w < w1; w++)
+			sum += b.wg_hist[(size_t)w * 2 * NB + col];
+	s_part[part][col_l] = sum;
+	__syncthreads();
+	if(part == 0) {
+		uint32_t run = 0;
+		for(int p = 0; p < kScanParts; p++) {
+			uint32_t v = s_part[p][col_l];
+			s_part[p][col_l] = run;
+			run += v;
+		}
+		if(col < 2 * NB) {
+			const int k = col / NB, bin = col - k * NB;
+			uint32_t *counts = b.info + LUCID_INFO_U32_SIZE;
+			counts[(k == 0 ? LUCID_COUNTS_QUAD_COUNTS : LUCID_COUNTS_TRI_COUNTS) * NB + bin] = run;
+		}
 	}
-	uint32_t *counts = b.info + LUCID_INFO_U32_SIZE;
-	counts[(k == 0 ? LUCID_COUNTS_QUAD_COUNTS : LUCID_COUNTS_TRI_COUNTS) * NB + bin] = run;
+	__syncthreads();
+	if(col >= 2 * NB)
+		return;
+	// all loads first (independent), then the stores: a part spans at most kMaxRows rows
+	constexpr int kMaxRows = 16;
+	uint32_t v[kMaxRows];
+#pragma unroll
+	for(int j = 0; j < kMaxRows; j++)
+		v[j] = w0 + j < w1 ? b.wg_hist[(size_t)(w0 + j) * 2 * NB + col] : 0u;
+	uint32_t run = s_part[part][col_l];
+#pragma unroll
+	for(int j = 0; j < kMaxRows; j++) {
+		if(w0 + j < w1)
+			b.wg_hist[(size_t)(w0 + j) * 2 * NB + col] = run;
+		run += v[j];
+	}
 }
 
 // bin_categorizer.glsl: exclusive offsets over bins, bin levels and level lists.
@@ -252,7 +285,7 @@ __global__ __launch_bounds__(kBinThreads) void bin_dispatch_kernel(FrameConst c,
 		uint32_t t = 2 * q + s;
 		int bsx = (int)(a & 0x7fu), bex = (int)((a >> 14) & 0x7fu);
 		BinScan bs;
-		bin_scan_init(bs, b.tri_scan[2 * (size_t)t], b.tri_scan[2 * (size_t)t + 1]);
+		bin_scan_init(bs, b.tri_scan[2 * (size_t)tri_rec(b, t)], b.tri_scan[2 * (size_t)tri_rec(b, t) + 1]);
 		for(int by = bs.min_by; by <= bs.max_by; by++) {
 			int bmin, bmax;
 			bin_scan_step(bs, bmin, bmax);
@@ -266,8 +299,8 @@ __global__ __launch_bounds__(kBinThreads) void bin_dispatch_kernel(FrameConst c,
 hipError_t launch_binning(const FrameConst &c, const FrameBufs &b, hipStream_t stream) {
 	size_t lds = (size_t)2 * c.bin_count * sizeof(uint32_t);
 	hipLaunchKernelGGL(bin_count_kernel, dim3(c.num_bin_wgs), dim3(kBinThreads), lds, stream, c, b);
-	hipLaunchKernelGGL(bin_wg_scan_kernel, dim3((2 * c.bin_count + 255) / 256), dim3(256), 0, stream,
-					   c, b);
+	hipLaunchKernelGGL(bin_wg_scan_kernel, dim3((2 * c.bin_count + kScanCols - 1) / kScanCols),
+					   dim3(kScanCols * kScanParts), 0, stream, c, b);
 	hipLaunchKernelGGL(bin_categorize_kernel, dim3(1), dim3(1024), 0, stream, c, b);
 	hipLaunchKernelGGL(bin_dispatch_kernel, dim3(c.num_bin_wgs), dim3(kBinThreads), lds, stream, c,
 					   b);

@@ -88,7 +88,10 @@ struct FrameBufs {
 	// info + control
 	uint32_t *info;   // LucidInfo followed by 10 * bin_count counters
 	uint32_t *ctrl;   // kCtrlWords
-	// setup records
+	// setup records; per-triangle arrays are indexed by tri_rec(): the first triangles of
+	// all quads, then the second ones (soup scenes cull every second triangle, so their
+	// records stay dense)
+	uint32_t tri_half;    // quad capacity: record index of triangle 2 q + 1 is q + tri_half
 	uint32_t *quad_aabb;  // per quad
 	uint4 *tri_scan;      // 2 per tri
 	uint4 *tri_depth;     // 1 per tri
@@ -124,6 +127,11 @@ struct FrameBufs {
 	float *image_f;
 	uint32_t *frag_counts;
 };
+
+// Record index of triangle t (2 q + s) in the per-triangle setup arrays.
+__device__ __forceinline__ uint32_t tri_rec(const FrameBufs &b, uint32_t t) {
+	return (t >> 1) + (t & 1u) * b.tri_half;
+}
 
 // ------------------------------------------------------------------ conversions
 

@@ -35,7 +35,7 @@ namespace {
 constexpr int kMaxWidth = 2560, kMaxHeight = 2048;   // src/lucid_renderer.h:27
 constexpr int kMaxInstances = 64 * 1024;            // src/lucid_renderer.h:28
 constexpr int64_t kMaxQuads = (int64_t)1 << 28;     // quad ids share a word with 4 flag bits
-constexpr int kNumBinWgs = 256;
+constexpr int kNumBinWgs = 256; // bin_wg_scan_kernel assumes at most 256 (16 rows per part)
 
 struct DevBuf {
 	void *ptr = nullptr;
@@ -412,6 +412,7 @@ int lucid_render(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_fr
 	b.normals = in->normals;
 	b.info = r->info.as<uint32_t>();
 	b.ctrl = r->ctrl.as<uint32_t>();
+	b.tri_half = (uint32_t)r->quad_cap;
 	b.quad_aabb = r->quad_aabb.as<uint32_t>();
 	b.tri_scan = r->tri_scan.as<uint4>();
 	b.tri_depth = r->tri_depth.as<uint4>();
@@ -552,16 +553,24 @@ int lucid_read_setup(lucid_renderer *r, uint32_t *tri_records, size_t num_tris,
 	if(quad_aabbs && q)
 		LUCID_CHECK(hipMemcpy(quad_aabbs, r->quad_aabb.ptr, q * 4, hipMemcpyDeviceToHost));
 	if(tri_records && q) {
-		size_t t = 2 * q;
-		std::vector<uint32_t> scan(t * 8), depth(t * 4), bary(t * 8);
-		LUCID_CHECK(hipMemcpy(scan.data(), r->tri_scan.ptr, t * 32, hipMemcpyDeviceToHost));
-		LUCID_CHECK(hipMemcpy(depth.data(), r->tri_depth.ptr, t * 16, hipMemcpyDeviceToHost));
-		LUCID_CHECK(hipMemcpy(bary.data(), r->tri_bary.ptr, t * 32, hipMemcpyDeviceToHost));
-		for(size_t i = 0; i < t; i++) {
+		// record arrays hold the first triangles of all quads, then the second ones
+		const size_t half = (size_t)r->quad_cap;
+		std::vector<uint32_t> scan(2 * q * 8), depth(2 * q * 4), bary(2 * q * 8);
+		for(size_t h = 0; h < 2; h++) {
+			const size_t src = h * half, dst = h * q;
+			LUCID_CHECK(hipMemcpy(&scan[dst * 8], (char *)r->tri_scan.ptr + src * 32, q * 32,
+								  hipMemcpyDeviceToHost));
+			LUCID_CHECK(hipMemcpy(&depth[dst * 4], (char *)r->tri_depth.ptr + src * 16, q * 16,
+								  hipMemcpyDeviceToHost));
+			LUCID_CHECK(hipMemcpy(&bary[dst * 8], (char *)r->tri_bary.ptr + src * 32, q * 32,
+								  hipMemcpyDeviceToHost));
+		}
+		for(size_t i = 0; i < 2 * q; i++) {
+			const size_t k = (i >> 1) + (i & 1) * q;
 			uint32_t *o = tri_records + i * 20;
-			memcpy(o, &bary[i * 8], 32);
-			memcpy(o + 8, &scan[i * 8], 32);
-			memcpy(o + 16, &depth[i * 4], 16);
+			memcpy(o, &bary[k * 8], 32);
+			memcpy(o + 8, &scan[k * 8], 32);
+			memcpy(o + 16, &depth[k * 4], 16);
 		}
 	}
 	return LUCID_OK;

@@ -49,7 +49,7 @@ __device__ __forceinline__ float linear_to_srgb(float x) {
 
 __device__ __forceinline__ float sample_inv_depth(const FrameBufs &b, uint32_t t, float px,
 												  float py) {
-	uint4 d = b.tri_depth[t];
+	uint4 d = b.tri_depth[tri_rec(b, t)];
 	return bits2f(d.x) * px + (bits2f(d.y) * py + bits2f(d.z));
 }
 
@@ -121,7 +121,7 @@ __device__ __forceinline__ void load_shading_record(const FrameConst &c, const F
 													uint32_t t, uint4 (&q)[kRecQuads]) {
 	// every slot is computed into a named local and stored once, so that no part of q
 	// is written under a branch (which would push q to scratch memory)
-	const uint4 d = b.tri_depth[t];
+	const uint4 d = b.tri_depth[tri_rec(b, t)];
 	const uint32_t flags = d.w & 0xffffu, inst = d.w >> 16;
 	const bool s = t & 1u;
 	const bool vcol = (flags & LUCID_INST_HAS_VERTEX_COLORS) != 0;
@@ -131,8 +131,8 @@ __device__ __forceinline__ void load_shading_record(const FrameConst &c, const F
 		q3 = make_uint4(qc.x, s ? qc.z : qc.y, s ? qc.w : qc.z, 0u);
 	}
 	if(vcol && c.color_mode == kColorF32) {
-		float4 c0 = b.tri_color[3 * (size_t)t], c1 = b.tri_color[3 * (size_t)t + 1],
-			   c2 = b.tri_color[3 * (size_t)t + 2];
+		float4 c0 = b.tri_color[3 * (size_t)tri_rec(b, t)], c1 = b.tri_color[3 * (size_t)tri_rec(b, t) + 1],
+			   c2 = b.tri_color[3 * (size_t)tri_rec(b, t) + 2];
 		q3 = make_uint4(f2bits(c0.x), f2bits(c0.y), f2bits(c0.z), f2bits(c0.w));
 		q4 = make_uint4(f2bits(c1.x), f2bits(c1.y), f2bits(c1.z), 0u);
 		q5 = make_uint4(f2bits(c2.x), f2bits(c2.y), f2bits(c2.z), 0u);
@@ -145,14 +145,14 @@ __device__ __forceinline__ void load_shading_record(const FrameConst &c, const F
 		l0 = nq.x, l1 = s ? nq.z : nq.y, l2 = s ? nq.w : nq.z;
 	} else {
 		float f[3];
-		light_factors(c, decode_normal(b.tri_normal[t]), f);
+		light_factors(c, decode_normal(b.tri_normal[tri_rec(b, t)]), f);
 		l0 = f2bits(f[0]), l1 = f2bits(f[1]), l2 = f2bits(f[2]);
 	}
 	q4.w = l0;
 	q5.w = l1;
 	q[0] = d;
-	q[1] = b.tri_bary[2 * (size_t)t];
-	q[2] = b.tri_bary[2 * (size_t)t + 1];
+	q[1] = b.tri_bary[2 * (size_t)tri_rec(b, t)];
+	q[2] = b.tri_bary[2 * (size_t)tri_rec(b, t) + 1];
 	q[3] = q3;
 	q[4] = q4;
 	q[5] = q5;
@@ -572,7 +572,7 @@ __global__ __launch_bounds__(kRowgenThreads) void rowgen_low_kernel(FrameConst c
 		uint32_t t = bin_tri_at(b, i, bi.qc, bi.qo, bi.to);
 		if(t == 0xffffffffu)
 			continue;
-		uint4 s0 = b.tri_scan[2 * (size_t)t], s1 = b.tri_scan[2 * (size_t)t + 1];
+		uint4 s0 = b.tri_scan[2 * (size_t)tri_rec(b, t)], s1 = b.tri_scan[2 * (size_t)tri_rec(b, t) + 1];
 		int min_by = min(max((int)(s0.w & 0xffffu) - bi.bin_y, 0), kBinMask) >> 3;
 		int max_by = min(max((int)(s0.w >> 16) - bi.bin_y, 0), kBinMask) >> 3;
 		RowScan rs;
@@ -704,7 +704,7 @@ __global__ __launch_bounds__(kRowgenThreads) void rowgen_high_kernel(FrameConst 
 		int row = 0, row_lo = 0, row_hi = -1;
 		RowScan rs;
 		if(t != 0xffffffffu) {
-			const uint4 s0 = b.tri_scan[2 * (size_t)t], s1 = b.tri_scan[2 * (size_t)t + 1];
+			const uint4 s0 = b.tri_scan[2 * (size_t)tri_rec(b, t)], s1 = b.tri_scan[2 * (size_t)tri_rec(b, t) + 1];
 			const int min_g = min(max((int)(s0.w & 0xffffu) - bi.bin_y, 0), kBinMask) >> 2;
 			const int max_g = min(max((int)(s0.w >> 16) - bi.bin_y, 0), kBinMask) >> 2;
 			row_lo = min_g * 4, row_hi = max_g * 4 + 3;
@@ -1251,7 +1251,7 @@ __device__ uint32_t gather_pixel_keys(const FrameBufs &b, const uint4 *recs, con
 			const uint32_t sp = record_span(rec[k], row);
 			const int a = (int)(sp & 31u), e = (int)((sp >> 5) & 31u);
 			hit[k] = idx[k] != 0xffffffffu && a <= xb && xb <= e;
-			d[k] = hit[k] ? b.tri_depth[rec[k].x] : make_uint4(0u, 0u, 0u, 0u);
+			d[k] = hit[k] ? b.tri_depth[tri_rec(b, rec[k].x)] : make_uint4(0u, 0u, 0u, 0u);
 		}
 #pragma unroll
 		for(int k = 0; k < kU; k++) {

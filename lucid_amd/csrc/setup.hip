@@ -90,7 +90,7 @@ __device__ void store_tri(const FrameConst &c, const FrameBufs &b, uint32_t t, u
 	float mult = 1.0f / sqrtf(dot3(normal, normal));
 	normal = mul3(normal, mult);
 	if((flags_id & LUCID_INST_HAS_VERTEX_NORMALS) == 0)
-		b.tri_normal[t] = encode_normal(normal);
+		b.tri_normal[tri_rec(b, t)] = encode_normal(normal);
 
 	V3 edge0 = mul3(sub3(tri0, tri2), mult);
 	V3 edge1 = mul3(sub3(tri1, tri0), mult);
@@ -105,10 +105,10 @@ __device__ void store_tri(const FrameConst &c, const FrameBufs &b, uint32_t t, u
 	V3 pn = mul3(normal, 1.0f / plane_dist);
 	V3 deq = mk3(dot3(pn, dirx), dot3(pn, diry), dot3(pn, rd0));
 
-	b.tri_depth[t] = make_uint4(f2bits(deq.x), f2bits(deq.y), f2bits(deq.z), flags_id);
-	b.tri_bary[2 * (size_t)t + 0] =
+	b.tri_depth[tri_rec(b, t)] = make_uint4(f2bits(deq.x), f2bits(deq.y), f2bits(deq.z), flags_id);
+	b.tri_bary[2 * (size_t)tri_rec(b, t) + 0] =
 		make_uint4(f2bits(edge0.x), f2bits(edge0.y), f2bits(edge0.z), f2bits(param0));
-	b.tri_bary[2 * (size_t)t + 1] =
+	b.tri_bary[2 * (size_t)tri_rec(b, t) + 1] =
 		make_uint4(f2bits(edge1.x), f2bits(edge1.y), f2bits(edge1.z), f2bits(param1));
 
 	V3 n0 = cross3(tri2, sub3(tri1, tri2));
@@ -130,8 +130,8 @@ __device__ void store_tri(const FrameConst &c, const FrameBufs &b, uint32_t t, u
 	uint32_t signs = (e[0].x < 0.0f ? 1u : 0u) | (e[1].x < 0.0f ? 2u : 0u) |
 					 (e[2].x < 0.0f ? 4u : 0u) | (e[0].y < 0.0f ? 8u : 0u) |
 					 (e[1].y < 0.0f ? 16u : 0u) | (e[2].y < 0.0f ? 32u : 0u);
-	b.tri_scan[2 * (size_t)t + 0] = make_uint4(f2bits(scan[0]), f2bits(scan[1]), f2bits(scan[2]), y_aabb);
-	b.tri_scan[2 * (size_t)t + 1] = make_uint4(f2bits(step[0]), f2bits(step[1]), f2bits(step[2]), signs);
+	b.tri_scan[2 * (size_t)tri_rec(b, t) + 0] = make_uint4(f2bits(scan[0]), f2bits(scan[1]), f2bits(scan[2]), y_aabb);
+	b.tri_scan[2 * (size_t)tri_rec(b, t) + 1] = make_uint4(f2bits(step[0]), f2bits(step[1]), f2bits(step[2]), signs);
 }
 
 // Returns 0 small, 1 large, 2 + rejection type when rejected.
@@ -242,14 +242,16 @@ __device__ int process_quad(const FrameConst &c, const FrameBufs &b, uint32_t q,
 		} else if(c.color_mode == kColorF32) {
 #pragma unroll
 			for(int s = 0; s < 2; s++) {
+				if((cull_flags >> s) & 1u)
+					continue; // no record is read for a culled triangle
 				uint32_t t = 2 * q + s;
 				const float *c0 = b.colors_rgb + (size_t)vi[0] * 3;
 				const float *c1 = b.colors_rgb + (size_t)vi[1 + s] * 3;
 				const float *c2 = b.colors_rgb + (size_t)vi[2 + s] * 3;
 				float alpha = b.tri_alpha ? b.tri_alpha[t] : 1.0f;
-				b.tri_color[3 * (size_t)t + 0] = make_float4(c0[0], c0[1], c0[2], alpha);
-				b.tri_color[3 * (size_t)t + 1] = make_float4(c1[0], c1[1], c1[2], 0.0f);
-				b.tri_color[3 * (size_t)t + 2] = make_float4(c2[0], c2[1], c2[2], 0.0f);
+				b.tri_color[3 * (size_t)tri_rec(b, t) + 0] = make_float4(c0[0], c0[1], c0[2], alpha);
+				b.tri_color[3 * (size_t)tri_rec(b, t) + 1] = make_float4(c1[0], c1[1], c1[2], 0.0f);
+				b.tri_color[3 * (size_t)tri_rec(b, t) + 2] = make_float4(c2[0], c2[1], c2[2], 0.0f);
 			}
 		}
 	}
