@@ -91,6 +91,18 @@ void lucid_destroy(lucid_renderer *r);
 
 int lucid_render(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_frame_outputs *out,
 				 void *stream);
+/* Adjoint of the fp32-sample forward ("over" blending, fp32 vertex colours colors_rgb and
+ * per-triangle alpha tri_alpha): given dL/d(image_rgba32f) (device, width*height float4;
+ * the alpha channel is ignored), writes dL/d(colors_rgb) (device, num_verts float3) and
+ * dL/d(tri_alpha) (device, one float per triangle id, 2 per quad); either output may be NULL.
+ * The frame's fragment lists are recomputed, not stored. Synchronous on `stream`. Returns
+ * LUCID_ERR_CAPACITY after growing an internal buffer (call again), LUCID_ERR_UNSUPPORTED
+ * for RGBA8 colours or additive blending.
+ * No counterpart in the reference (which has no backward pass): this is the differentiable
+ * extension of the drop-in renderer. */
+int lucid_backward(lucid_renderer *r, const lucid_frame_inputs *in, const float *grad_image,
+				   float *grad_colors_rgb, float *grad_tri_alpha, void *stream);
+
 /* Waits for the last frame. Returns LUCID_ERR_CAPACITY if it overflowed an internal buffer
  * (the buffers have then been grown and the frame must be rendered again). */
 int lucid_finish(lucid_renderer *r);

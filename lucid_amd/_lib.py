@@ -33,7 +33,7 @@ INFO_U32 = 1152
 EXPORTED = (
     "lucid_error_string", "lucid_create", "lucid_destroy", "lucid_render", "lucid_finish",
     "lucid_read_info", "lucid_verify_info", "lucid_read_bin_lists", "lucid_read_setup",
-    "lucid_stage_times",
+    "lucid_stage_times", "lucid_backward",
     "lucid_bin_count", "lucid_bin_size", "lucid_block_size", "lucid_max_visible_quads",
     "lucid_max_scene_quads",
 )
@@ -111,6 +111,8 @@ def load(path: str = LIB_PATH):
     lib.lucid_read_setup.argtypes = [vp, vp, sz, vp, sz]
     lib.lucid_stage_times.restype = i32
     lib.lucid_stage_times.argtypes = [vp, vp, i32]
+    lib.lucid_backward.restype = i32
+    lib.lucid_backward.argtypes = [vp, ctypes.POINTER(FrameInputs), vp, vp, vp, vp]
     for name in ("lucid_bin_count", "lucid_bin_size", "lucid_block_size"):
         getattr(lib, name).restype = i32
         getattr(lib, name).argtypes = [vp]

@@ -37,12 +37,24 @@ enum CtrlWord : int {
 	kCtrlFallbackPixels = 7,
 	kCtrlListCursor = 8,   // bump allocator of per-block index lists of high bins (u32 units)
 	kCtrlSortedCursor = 10, // bump allocator of pre-sorted block arrays (u32 units)
+	kCtrlBwdRows = 11,      // gradient rows of the block kernel (set by the row scan)
+	kCtrlBwdExtra = 12,     // bump allocator of per-fragment gradient rows of exact pixels
 	kCtrlWords = 16,
 };
 
 // Overflow bits of kCtrlOverflow
 constexpr uint32_t kOvfBinLists = 1u, kOvfRecords = 2u, kOvfFallback = 4u, kOvfSpill = 8u,
-				   kOvfLists = 16u, kOvfSorted = 32u;
+				   kOvfLists = 16u, kOvfSorted = 32u, kOvfGradRows = 64u;
+
+// Buffers of the backward pass (passed by value beside FrameBufs).
+constexpr int kGradWords = 10; // per row: corner 0 rgb, corner 1 rgb, corner 2 rgb, alpha
+struct BwdBufs {
+	const float4 *grad_image; // dL/d(rgba32f image); alpha ignored
+	uint32_t *row_base;       // bin_count * 16: first gradient row of each block
+	float *rows;              // kGradWords floats per row
+	uint32_t *row_tri;        // triangle id of each row
+	uint64_t rows_cap;
+};
 
 // Per-block slot flags (blk_flags)
 constexpr uint32_t kSlotLevelMask = 0xffu; // LUCID_BIN_LEVEL_*; 0 = empty bin

@@ -26,6 +26,16 @@ hipError_t launch_setup(const FrameConst &c, const FrameBufs &b, uint32_t n_info
 hipError_t launch_binning(const FrameConst &c, const FrameBufs &b, hipStream_t stream);
 hipError_t launch_raster(const FrameConst &c, const FrameBufs &b, hipStream_t stream,
 						 hipEvent_t *events);
+hipError_t launch_backward_prepare(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
+								   hipStream_t stream);
+hipError_t launch_backward_blocks(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
+								  hipStream_t stream);
+size_t reduce_sort_temp_bytes(uint32_t n);
+hipError_t reduce_gradients(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
+							uint32_t nrows, uint32_t ntri, uint32_t nverts, uint32_t *keys_out,
+							uint32_t *vals_in, uint32_t *vals_out, uint32_t *keys_in2,
+							float *tri_grad, void *temp, size_t temp_bytes, float *grad_colors,
+							float *grad_alpha, hipStream_t stream);
 } // namespace lucid
 
 using namespace lucid;
@@ -77,6 +87,10 @@ struct lucid_renderer {
 	DevBuf wg_hist, bin_quads, bin_tris;
 	DevBuf low_records, high_records, fallback_queue, spill;
 	DevBuf blk_slot, blk_flags, blk_sorted, blk_lists, blk_count, band_count, sorted_pool;
+	// backward
+	DevBuf grad_row_base, grad_rows, grad_row_tri, red_keys_out, red_vals_in, red_vals_out,
+		red_keys_in2, tri_grad, red_temp;
+	uint64_t grad_rows_cap = 0;
 
 	int64_t quad_cap = 0;
 	uint64_t bin_quads_cap = 0, bin_tris_cap = 0, high_records_cap = 0, spill_cap = 0;
@@ -186,6 +200,13 @@ int grow_after_overflow(lucid_renderer *r) {
 										   (uint64_t)r->fallback_cap * 2);
 		LUCID_CHECK(r->fallback_queue.reserve(need * 16));
 		r->fallback_cap = (uint32_t)need;
+	}
+	if(flags & kOvfGradRows) {
+		uint64_t need = std::max<uint64_t>(
+			(uint64_t)r->status[kCtrlBwdRows] + r->status[kCtrlBwdExtra], r->grad_rows_cap * 2);
+		LUCID_CHECK(r->grad_rows.reserve(need * kGradWords * 4));
+		LUCID_CHECK(r->grad_row_tri.reserve(need * 4));
+		r->grad_rows_cap = need;
 	}
 	return LUCID_ERR_CAPACITY;
 }
@@ -318,12 +339,15 @@ void lucid_destroy(lucid_renderer *r) {
 	delete r;
 }
 
-int lucid_render(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_frame_outputs *out,
-				 void *stream_ptr) {
-	if(!r || !in || !out || !in->config || in->num_instances < 0 ||
-	   in->num_instances > kMaxInstances || (in->num_instances > 0 && !in->instances))
+namespace {
+
+// Validates the frame inputs, uploads instance data when it changed, grows capacities and
+// fills the per-frame constants and buffer table (shared by the forward and backward).
+int prepare_frame(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_frame_outputs *out,
+				  FrameConst &c, FrameBufs &b, int64_t &nq_out) {
+	if(!in->config || in->num_instances < 0 || in->num_instances > kMaxInstances ||
+	   (in->num_instances > 0 && !in->instances))
 		return LUCID_ERR_INVALID_ARG;
-	hipStream_t stream = (hipStream_t)stream_ptr;
 	LUCID_CHECK(hipSetDevice(r->device));
 
 	// --- instances (uploadInstances, src/lucid_renderer.cpp:352-429)
@@ -397,9 +421,7 @@ int lucid_render(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_fr
 			r->host_inst_colors.assign(in->instance_colors, in->instance_colors + in->num_instances);
 	}
 
-	FrameConst c;
 	fill_const(r, *in->config, nq, in->num_instances, color_mode, in->normals != nullptr, c);
-	FrameBufs b;
 	memset(&b, 0, sizeof(b));
 	b.instances = r->instances.as<lucid_instance_data>();
 	b.inst_prefix = r->inst_prefix.as<int>();
@@ -443,9 +465,26 @@ int lucid_render(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_fr
 	b.band_count = r->band_count.as<uint32_t>();
 	b.sorted_pool = r->sorted_pool.as<uint32_t>();
 	b.sorted_cap = r->sorted_cap;
-	b.image = out->image_rgba8;
-	b.image_f = out->image_rgba32f;
-	b.frag_counts = out->frag_counts;
+	b.image = out ? out->image_rgba8 : nullptr;
+	b.image_f = out ? out->image_rgba32f : nullptr;
+	b.frag_counts = out ? out->frag_counts : nullptr;
+	nq_out = nq;
+	return LUCID_OK;
+}
+
+} // namespace
+
+int lucid_render(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_frame_outputs *out,
+				 void *stream_ptr) {
+	if(!r || !in || !out)
+		return LUCID_ERR_INVALID_ARG;
+	hipStream_t stream = (hipStream_t)stream_ptr;
+	FrameConst c;
+	FrameBufs b;
+	int64_t nq = 0;
+	int pe = prepare_frame(r, in, out, c, b, nq);
+	if(pe != LUCID_OK)
+		return pe;
 
 	uint32_t n_info = LUCID_INFO_U32_SIZE + 10u * (uint32_t)r->bin_count;
 	if(r->timers)
@@ -463,6 +502,98 @@ int lucid_render(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_fr
 	r->frame_pending = true;
 	r->last_num_quads = nq;
 	r->last_stream = stream;
+	return LUCID_OK;
+}
+
+int lucid_backward(lucid_renderer *r, const lucid_frame_inputs *in, const float *grad_image,
+				   float *grad_colors_rgb, float *grad_tri_alpha, void *stream_ptr) {
+	if(!r || !in || !grad_image || (!grad_colors_rgb && !grad_tri_alpha))
+		return LUCID_ERR_INVALID_ARG;
+	// the adjoint is of the fp32-colour, fp32-sample, "over" forward
+	if(!in->colors_rgb || !in->tri_alpha || in->colors_rgba8 || (r->opts & LUCID_OPT_ADDITIVE_BLENDING))
+		return LUCID_ERR_UNSUPPORTED;
+	hipStream_t stream = (hipStream_t)stream_ptr;
+	LUCID_CHECK(hipSetDevice(r->device));
+	if(r->frame_pending) {
+		int f = lucid_finish(r);
+		if(f != LUCID_OK && f != LUCID_ERR_CAPACITY)
+			return f;
+	}
+	FrameConst c;
+	FrameBufs b;
+	int64_t nq = 0;
+	int pe = prepare_frame(r, in, nullptr, c, b, nq);
+	if(pe != LUCID_OK)
+		return pe;
+	const size_t slots = (size_t)r->bin_count * 16;
+	LUCID_CHECK(r->grad_row_base.reserve(slots * 4));
+	if(r->grad_rows_cap == 0) {
+		r->grad_rows_cap = 1u << 20;
+		LUCID_CHECK(r->grad_rows.reserve(r->grad_rows_cap * kGradWords * 4));
+		LUCID_CHECK(r->grad_row_tri.reserve(r->grad_rows_cap * 4));
+	}
+	BwdBufs g;
+	g.grad_image = (const float4 *)grad_image;
+	g.row_base = r->grad_row_base.as<uint32_t>();
+	g.rows = r->grad_rows.as<float>();
+	g.row_tri = r->grad_row_tri.as<uint32_t>();
+	g.rows_cap = r->grad_rows_cap;
+
+	// forward stages (the fragment lists are recomputed), then the gradient row layout
+	const uint32_t n_info = LUCID_INFO_U32_SIZE + 10u * (uint32_t)r->bin_count;
+	LUCID_CHECK(launch_setup(c, b, n_info, stream));
+	LUCID_CHECK(launch_binning(c, b, stream));
+	LUCID_CHECK(launch_backward_prepare(c, b, g, stream));
+	LUCID_CHECK(hipMemcpyAsync(r->status, r->ctrl.ptr, kCtrlWords * 4, hipMemcpyDeviceToHost,
+							   stream));
+	LUCID_CHECK(hipStreamSynchronize(stream));
+	r->last_num_quads = nq;
+	r->last_stream = stream;
+	if(r->status[kCtrlOverflow]) {
+		int gr = grow_after_overflow(r);
+		memset(r->status, 0, kCtrlWords * 4);
+		return gr;
+	}
+	// rows of the block kernel plus room for the exact per-pixel rows
+	const uint64_t block_rows = r->status[kCtrlBwdRows];
+	const uint64_t want = block_rows + std::max<uint64_t>(1u << 16, block_rows / 8);
+	if(want > r->grad_rows_cap) {
+		LUCID_CHECK(r->grad_rows.reserve(want * kGradWords * 4));
+		LUCID_CHECK(r->grad_row_tri.reserve(want * 4));
+		r->grad_rows_cap = want;
+		g.rows = r->grad_rows.as<float>();
+		g.row_tri = r->grad_row_tri.as<uint32_t>();
+		g.rows_cap = r->grad_rows_cap;
+	}
+	LUCID_CHECK(launch_backward_blocks(c, b, g, stream));
+	LUCID_CHECK(hipMemcpyAsync(r->status, r->ctrl.ptr, kCtrlWords * 4, hipMemcpyDeviceToHost,
+							   stream));
+	LUCID_CHECK(hipStreamSynchronize(stream));
+	if(r->status[kCtrlOverflow]) {
+		int gr = grow_after_overflow(r);
+		memset(r->status, 0, kCtrlWords * 4);
+		return gr;
+	}
+	const uint64_t nrows = block_rows + r->status[kCtrlBwdExtra];
+	memset(r->status, 0, kCtrlWords * 4);
+	const uint64_t ntri = 2 * (uint64_t)nq;
+	const uint64_t nverts = (uint64_t)std::max<int64_t>(in->num_verts, 0);
+	if(nrows >= (1ull << 32) || 3 * ntri >= (1ull << 32))
+		return LUCID_ERR_UNSUPPORTED;
+	const uint64_t npairs = std::max<uint64_t>(std::max<uint64_t>(nrows, 3 * ntri), 1);
+	LUCID_CHECK(r->red_keys_out.reserve(npairs * 4));
+	LUCID_CHECK(r->red_vals_in.reserve(npairs * 4));
+	LUCID_CHECK(r->red_vals_out.reserve(npairs * 4));
+	LUCID_CHECK(r->red_keys_in2.reserve(npairs * 4));
+	LUCID_CHECK(r->tri_grad.reserve(std::max<uint64_t>(ntri, 1) * kGradWords * 4));
+	const size_t temp_bytes = std::max<size_t>(reduce_sort_temp_bytes((uint32_t)npairs), 16);
+	LUCID_CHECK(r->red_temp.reserve(temp_bytes));
+	LUCID_CHECK(reduce_gradients(c, b, g, (uint32_t)nrows, (uint32_t)ntri, (uint32_t)nverts,
+								 r->red_keys_out.as<uint32_t>(), r->red_vals_in.as<uint32_t>(),
+								 r->red_vals_out.as<uint32_t>(), r->red_keys_in2.as<uint32_t>(),
+								 r->tri_grad.as<float>(), r->red_temp.ptr, r->red_temp.bytes,
+								 grad_colors_rgb, grad_tri_alpha, stream));
+	LUCID_CHECK(hipStreamSynchronize(stream));
 	return LUCID_OK;
 }
 

@@ -164,11 +164,19 @@ __device__ __forceinline__ float record_inv_depth(uint4 q0, float px, float py) 
 	return bits2f(q0.x) * px + (bits2f(q0.y) * py + bits2f(q0.z));
 }
 
-// shadeSample (shading.glsl:107-184) without textures, from a shading record (LDS or
-// registers). `inv_ray` is record_inv_depth at the same pixel.
-template <bool FP32>
-__device__ __forceinline__ SampleColor<FP32> shade_record(const FrameConst &c, const uint4 *q,
-														  float px, float py, float inv_ray) {
+// Everything shadeSample (shading.glsl:107-184, no textures) computes before lighting,
+// from a shading record (LDS or registers): barycentrics, the vertex colour times the
+// instance colour, and the per-channel lighting factors. `inv_ray` is record_inv_depth at
+// the same pixel. Returns false when the sample's alpha is 0 (it contributes nothing).
+struct ShadeState {
+	float bary0, bary1;
+	float4 color; // interpolated vertex colour (white without one) times the instance colour
+	float4 inst;  // instance colour (white without one)
+	float f[3];   // lighting factors
+};
+
+__device__ __forceinline__ bool shade_prepare(const FrameConst &c, const uint4 *q, float px,
+											  float py, float inv_ray, ShadeState &st) {
 	const uint32_t flags = q[0].w & 0xffffu;
 	const uint4 b0 = q[1], b1 = q[2];
 	// GLSL's division is specified to 2.5 ulp; v_rcp_f32 (1 ulp) meets that
@@ -178,6 +186,7 @@ __device__ __forceinline__ SampleColor<FP32> shade_record(const FrameConst &c, c
 	float bary0 = e0 * ray_pos, bary1 = e1 * ray_pos;
 	bary0 = bary0 - bits2f(b0.w);
 	bary1 = bary1 - bits2f(b1.w);
+	st.bary0 = bary0, st.bary1 = bary1;
 
 	// vertex colour (or white), then times the instance colour; products by an exact 1.0
 	// are skipped, which leaves every result bit-identical
@@ -200,37 +209,50 @@ __device__ __forceinline__ SampleColor<FP32> shade_record(const FrameConst &c, c
 			color.w = bits2f(q3.w);
 		}
 	}
+	st.inst = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
 	if(flags & LUCID_INST_HAS_COLOR) {
 		const float4 ic = decode_rgba8(q6.x);
+		st.inst = ic;
 		color.x = ic.x * color.x;
 		color.y = ic.y * color.y;
 		color.z = ic.z * color.z;
 		color.w = ic.w * color.w;
 	}
-	SampleColor<FP32> out;
-	if(color.w == 0.0f) {
-		out.zero();
-		return out;
-	}
-	float f[3];
+	st.color = color;
+	if(color.w == 0.0f)
+		return false;
 	if(flags & LUCID_INST_HAS_VERTEX_NORMALS) {
 		V3 n0 = decode_normal(q[4].w), n1 = decode_normal(q[5].w), n2 = decode_normal(q6.y);
 		n1 = sub3(n1, n0);
 		n2 = sub3(n2, n0);
 		V3 normal = mk3(bary0 * n1.x + (bary1 * n2.x + n0.x), bary0 * n1.y + (bary1 * n2.y + n0.y),
 						bary0 * n1.z + (bary1 * n2.z + n0.z));
-		light_factors(c, normal, f);
+		light_factors(c, normal, st.f);
 	} else {
-		f[0] = bits2f(q[4].w), f[1] = bits2f(q[5].w), f[2] = bits2f(q6.y);
+		st.f[0] = bits2f(q[4].w), st.f[1] = bits2f(q[5].w), st.f[2] = bits2f(q6.y);
 	}
-	float rgb[3] = {color.x, color.y, color.z};
+	return true;
+}
+
+// shadeSample (shading.glsl:107-184) without textures: lighting through the sRGB round trip
+// of finalShading (funcs.glsl:261-271), then RGBA8 quantisation unless FP32.
+template <bool FP32>
+__device__ __forceinline__ SampleColor<FP32> shade_record(const FrameConst &c, const uint4 *q,
+														  float px, float py, float inv_ray) {
+	ShadeState st;
+	SampleColor<FP32> out;
+	if(!shade_prepare(c, q, px, py, inv_ray, st)) {
+		out.zero();
+		return out;
+	}
+	float rgb[3] = {st.color.x, st.color.y, st.color.z};
 #pragma unroll
 	for(int k = 0; k < 3; k++)
-		rgb[k] = sat(linear_to_srgb(srgb_to_linear(rgb[k]) * f[k]));
+		rgb[k] = sat(linear_to_srgb(srgb_to_linear(rgb[k]) * st.f[k]));
 	if constexpr(FP32)
-		out.col = make_float4(rgb[0], rgb[1], rgb[2], color.w);
+		out.col = make_float4(rgb[0], rgb[1], rgb[2], st.color.w);
 	else
-		out.enc = encode_rgba8(rgb[0], rgb[1], rgb[2], color.w);
+		out.enc = encode_rgba8(rgb[0], rgb[1], rgb[2], st.color.w);
 	return out;
 }
 
@@ -271,70 +293,68 @@ struct Accum {
 // Sample order key: nearer (larger inverse depth) first, equal depths by triangle id.
 constexpr uint64_t kEmptyKey = ~0ull;
 
-// Per-pixel ordering window of kWindow pending samples in fixed slots. A sample that
-// arrives when the window is full pushes out the nearest of the five, which is blended
-// front to back. The pixel is exact iff the blended keys strictly increase; otherwise `bad`
-// marks it for the exact per-pixel sort. Slots are updated in place by selects: no
-// register rotation and no dynamic indexing.
-template <bool FP32>
-struct PixelState {
-	using SC = SampleColor<FP32>;
+// Per-pixel ordering window of kWindow pending samples (payload P) in fixed slots. A
+// sample that arrives when the window is full pushes out the nearest of the five, which is
+// handed to the emit callback, so samples leave front to back. The pixel is exact iff the
+// emitted keys strictly increase; otherwise `bad` marks it for the exact per-pixel sort.
+// Slots are updated in place by selects: no register rotation and no dynamic indexing.
+// P provides zero() and a component-wise static sel(bool, P, P).
+template <class P>
+struct OrderWindow {
 	uint64_t k0, k1, k2, k3;
-	SampleColor<FP32> c0, c1, c2, c3;
+	P c0, c1, c2, c3;
 	uint32_t n;           // filled slots
-	uint64_t first, last; // first and last blended key (kEmptyKey / 0 before the first)
+	uint64_t first, last; // first and last emitted key (kEmptyKey / 0 before the first)
 	bool bad;
-	Accum acc;
 
 	__device__ __forceinline__ void init() {
 		k0 = k1 = k2 = k3 = kEmptyKey;
 		c0.zero(), c1.zero(), c2.zero(), c3.zero();
 		n = 0, first = kEmptyKey, last = 0, bad = false;
-		acc.init();
 	}
 
-	__device__ __forceinline__ void emit(uint64_t k, SampleColor<FP32> col, bool additive) {
+	template <class F> __device__ __forceinline__ void emit(uint64_t k, const P &col, F &&f) {
 		bad |= k <= last;
 		first = first == kEmptyKey ? k : first;
 		last = k;
-		acc.blend(col.rgba(), additive);
+		f(col);
 	}
 
-	__device__ __forceinline__ void insert(uint64_t key, SampleColor<FP32> col, bool additive) {
+	template <class F> __device__ __forceinline__ void insert(uint64_t key, const P &col, F &&f) {
 		static_assert(kWindow == 4, "window slots are spelled out");
 		if(n < (uint32_t)kWindow) {
-			k0 = n == 0 ? key : k0, c0 = SC::sel(n == 0, col, c0);
-			k1 = n == 1 ? key : k1, c1 = SC::sel(n == 1, col, c1);
-			k2 = n == 2 ? key : k2, c2 = SC::sel(n == 2, col, c2);
-			k3 = n == 3 ? key : k3, c3 = SC::sel(n == 3, col, c3);
+			k0 = n == 0 ? key : k0, c0 = P::sel(n == 0, col, c0);
+			k1 = n == 1 ? key : k1, c1 = P::sel(n == 1, col, c1);
+			k2 = n == 2 ? key : k2, c2 = P::sel(n == 2, col, c2);
+			k3 = n == 3 ? key : k3, c3 = P::sel(n == 3, col, c3);
 			n++;
 			return;
 		}
 		// nearest pending sample by a two-level tournament
 		const bool a = k1 < k0, bq = k3 < k2;
 		const uint64_t ka = a ? k1 : k0, kb = bq ? k3 : k2;
-		const SC ca = SC::sel(a, c1, c0), cb = SC::sel(bq, c3, c2);
+		const P ca = P::sel(a, c1, c0), cb = P::sel(bq, c3, c2);
 		const bool m = kb < ka;
 		const uint64_t km = m ? kb : ka;
-		const SC cm = SC::sel(m, cb, ca);
+		const P cm = P::sel(m, cb, ca);
 		const bool take_new = key < km;
-		emit(take_new ? key : km, SC::sel(take_new, col, cm), additive);
+		emit(take_new ? key : km, P::sel(take_new, col, cm), f);
 		// the new sample takes the emitted slot
 		const bool s0 = !take_new && !m && !a, s1 = !take_new && !m && a;
 		const bool s2 = !take_new && m && !bq, s3 = !take_new && m && bq;
-		k0 = s0 ? key : k0, c0 = SC::sel(s0, col, c0);
-		k1 = s1 ? key : k1, c1 = SC::sel(s1, col, c1);
-		k2 = s2 ? key : k2, c2 = SC::sel(s2, col, c2);
-		k3 = s3 ? key : k3, c3 = SC::sel(s3, col, c3);
+		k0 = s0 ? key : k0, c0 = P::sel(s0, col, c0);
+		k1 = s1 ? key : k1, c1 = P::sel(s1, col, c1);
+		k2 = s2 ? key : k2, c2 = P::sel(s2, col, c2);
+		k3 = s3 ? key : k3, c3 = P::sel(s3, col, c3);
 	}
 
-	__device__ __forceinline__ void flush(bool additive) {
+	template <class F> __device__ __forceinline__ void flush(F &&f) {
 		for(uint32_t r = 0; r < n; r++) {
 			const bool a = k1 < k0, bq = k3 < k2;
 			const uint64_t ka = a ? k1 : k0, kb = bq ? k3 : k2;
-			const SC ca = SC::sel(a, c1, c0), cb = SC::sel(bq, c3, c2);
+			const P ca = P::sel(a, c1, c0), cb = P::sel(bq, c3, c2);
 			const bool m = kb < ka;
-			emit(m ? kb : ka, SC::sel(m, cb, ca), additive);
+			emit(m ? kb : ka, P::sel(m, cb, ca), f);
 			const bool s0 = !m && !a, s1 = !m && a, s2 = m && !bq, s3 = m && bq;
 			k0 = s0 ? kEmptyKey : k0;
 			k1 = s1 ? kEmptyKey : k1;
@@ -342,6 +362,24 @@ struct PixelState {
 			k3 = s3 ? kEmptyKey : k3;
 		}
 		n = 0;
+	}
+};
+
+// Forward per-pixel state: the ordering window feeding the front-to-back blend.
+template <bool FP32>
+struct PixelState {
+	OrderWindow<SampleColor<FP32>> w;
+	Accum acc;
+
+	__device__ __forceinline__ void init() {
+		w.init();
+		acc.init();
+	}
+	__device__ __forceinline__ void insert(uint64_t key, SampleColor<FP32> col, bool additive) {
+		w.insert(key, col, [&](const SampleColor<FP32> &s) { acc.blend(s.rgba(), additive); });
+	}
+	__device__ __forceinline__ void flush(bool additive) {
+		w.flush([&](const SampleColor<FP32> &s) { acc.blend(s.rgba(), additive); });
 	}
 };
 
@@ -977,13 +1015,10 @@ __device__ __forceinline__ void stage_slice(const FrameConst &c, const FrameBufs
 	}
 }
 
-// Walks this lane's pixel through the slice in entry order: each step takes the nearest
-// remaining entry that covers the pixel, shades the sample from the entry's LDS record and
-// feeds the ordering window.
-template <bool FP32>
-__device__ __forceinline__ void walk_pixel(const FrameConst &c, const uint4 *s_rec,
-										   uint64_t (&bm)[kSliceWords], float px, float py,
-										   PixelState<FP32> &ps, bool additive) {
+// Visits this lane's set bitmap bits in increasing entry order, one per step; the wave
+// steps until every lane is done. visit(e) runs for the lanes that still have entries.
+template <class Visit>
+__device__ __forceinline__ void walk_bits(uint64_t (&bm)[kSliceWords], Visit &&visit) {
 	while(true) {
 		bool nz[kSliceWords], any = false;
 #pragma unroll
@@ -1005,14 +1040,26 @@ __device__ __forceinline__ void walk_pixel(const FrameConst &c, const uint4 *s_r
 #pragma unroll
 			for(int k = 0; k < kSliceWords; k++)
 				bm[k] = word == k ? rest : bm[k];
-			const uint4 *rec = s_rec + (word * 64 + (int)__builtin_ctzll(cur)) * kRecQuads;
-			const uint4 q0 = rec[0];
-			const float inv = record_inv_depth(q0, px, py);
-			const SampleColor<FP32> col = shade_record<FP32>(c, rec, px, py, inv);
-			if(col.contributes())
-				ps.insert(pixel_key(inv, rec[6].z), col, additive);
+			visit(word * 64 + (int)__builtin_ctzll(cur));
 		}
 	}
+}
+
+// Walks this lane's pixel through the slice in entry order: each step takes the nearest
+// remaining entry that covers the pixel, shades the sample from the entry's LDS record and
+// feeds the ordering window.
+template <bool FP32>
+__device__ __forceinline__ void walk_pixel(const FrameConst &c, const uint4 *s_rec,
+										   uint64_t (&bm)[kSliceWords], float px, float py,
+										   PixelState<FP32> &ps, bool additive) {
+	walk_bits(bm, [&](int e) {
+		const uint4 *rec = s_rec + e * kRecQuads;
+		const uint4 q0 = rec[0];
+		const float inv = record_inv_depth(q0, px, py);
+		const SampleColor<FP32> col = shade_record<FP32>(c, rec, px, py, inv);
+		if(col.contributes())
+			ps.insert(pixel_key(inv, rec[6].z), col, additive);
+	});
 }
 
 __device__ __forceinline__ uint32_t bitmap_count(const uint64_t (&bm)[kSliceWords]) {
@@ -1129,7 +1176,7 @@ __global__ __launch_bounds__(64) void block_raster_kernel(FrameConst c, FrameBuf
 		wave_sync_lds(); // every lane is done with this slice's records
 	}
 	ps.flush(additive);
-	finish_pixels(c, b, ps.acc, ps.bad, count, px, py, slot, m);
+	finish_pixels(c, b, ps.acc, ps.w.bad, count, px, py, slot, m);
 }
 
 // Pre-sorted lists longer than one register sort: one work-group per block. Wave w takes
@@ -1192,9 +1239,9 @@ __global__ __launch_bounds__(kBigWaves * 64) void block_raster_big_kernel(FrameC
 		}
 		ps.flush(additive);
 		s_part[wave][lane] = make_float4(ps.acc.cr, ps.acc.cg, ps.acc.cb, ps.acc.tr);
-		s_first[wave][lane] = ps.first;
-		s_last[wave][lane] = ps.last;
-		s_cnt[wave][lane] = count | (ps.bad ? 0x80000000u : 0u);
+		s_first[wave][lane] = ps.w.first;
+		s_last[wave][lane] = ps.w.last;
+		s_cnt[wave][lane] = count | (ps.w.bad ? 0x80000000u : 0u);
 	}
 	__syncthreads();
 	if(wave != 0)
@@ -1375,6 +1422,476 @@ __global__ __launch_bounds__(kFallbackThreads) void fallback_kernel(FrameConst c
 		fallback_pixel<FP32>(c, b, b.fallback_queue[qi], s_keys[threadIdx.x >> 6]);
 }
 
+// ------------------------------------------------------------------ backward (adjoint)
+//
+// The adjoint of sorted front-to-back compositing for the fp32-sample forward. Fragment
+// lists are recomputed (the forward stages run again), never stored. For every block a
+// wave replays the forward walk twice with the same ordering window, so both passes see the
+// identical fragment order:
+//   pass A accumulates, per pixel, the composite in fp64 (C, T) and the fp32 composite the
+//          forward wrote (for the saturation mask of the output);
+//   pass B walks again with T_i and the remainder R_i = out - C_{<i} (fp64, so deep pixels
+//          whose transmittance underflows fp32 stay exact), and per fragment forms
+//            dout/dc_i = a_i T_i,   dout/da_i = T_i c_i - R_{i+1} / (1 - a_i),
+//          chains them through the sRGB lighting stage and the barycentric weights, and
+//          sums them per (block, entry) in LDS.
+// Each (block, entry) row is then reduced per triangle and per vertex by sorted
+// segmented sums (reduce.hip): no global atomics on gradient values. Pixels whose window
+// order is not exact take the per-pixel exact sort, as in the forward.
+
+__device__ __forceinline__ float srgb_to_linear_d(float x) {
+	return x < 0.04045f ? (1.0f / 12.92f)
+						: (2.4f / 1.055f) * glsl_pow((x + 0.055f) * (1.0f / 1.055f), 1.4f);
+}
+__device__ __forceinline__ float linear_to_srgb_d(float y) {
+	return y < 0.0031308f ? 12.92f : (1.055f / 2.4f) * glsl_pow(y, 1.0f / 2.4f - 1.0f);
+}
+
+// One sample of the backward walk.
+struct BwdSample {
+	float r, g, b, a;  // shaded sample (fp32, as the fp32-sample forward)
+	float b0, b1;      // barycentrics: corner weights (1 - b0) - b1, b0, b1
+	float dr, dg, db;  // d(sample channel) / d(vertex colour channel) per unit corner weight
+	float da;          // d(sample alpha) / d(triangle alpha)
+	uint32_t e;        // entry index within the current chunk
+	__device__ __forceinline__ void zero() {
+		r = g = b = a = b0 = b1 = dr = dg = db = da = 0.0f;
+		e = 0;
+	}
+	static __device__ __forceinline__ BwdSample sel(bool p, const BwdSample &x,
+													const BwdSample &y) {
+		BwdSample o;
+		o.r = p ? x.r : y.r, o.g = p ? x.g : y.g, o.b = p ? x.b : y.b, o.a = p ? x.a : y.a;
+		o.b0 = p ? x.b0 : y.b0, o.b1 = p ? x.b1 : y.b1;
+		o.dr = p ? x.dr : y.dr, o.dg = p ? x.dg : y.dg, o.db = p ? x.db : y.db;
+		o.da = p ? x.da : y.da;
+		o.e = p ? x.e : y.e;
+		return o;
+	}
+};
+
+// The fp32-sample shading of shade_record with the derivatives of the lighting stage.
+__device__ __forceinline__ bool shade_grad(const FrameConst &c, const uint4 *q, float px, float py,
+										   float inv_ray, uint32_t e, BwdSample &o) {
+	ShadeState st;
+	if(!shade_prepare(c, q, px, py, inv_ray, st)) {
+		o.zero();
+		return false;
+	}
+	const float x[3] = {st.color.x, st.color.y, st.color.z};
+	const float ic[3] = {st.inst.x, st.inst.y, st.inst.z};
+	float sv[3], dv[3];
+#pragma unroll
+	for(int k = 0; k < 3; k++) {
+		const float y = srgb_to_linear(x[k]) * st.f[k];
+		const float s = linear_to_srgb(y);
+		sv[k] = sat(s);
+		// clamp passes the gradient on [0, 1] (as torch.clamp does)
+		dv[k] = (s >= 0.0f && s <= 1.0f)
+					? linear_to_srgb_d(y) * st.f[k] * srgb_to_linear_d(x[k]) * ic[k]
+					: 0.0f;
+	}
+	o.r = sv[0], o.g = sv[1], o.b = sv[2], o.a = st.color.w;
+	o.b0 = st.bary0, o.b1 = st.bary1;
+	o.dr = dv[0], o.dg = dv[1], o.db = dv[2];
+	o.da = st.inst.w;
+	o.e = e;
+	return true;
+}
+
+// Pass A accumulator: fp64 composite and the fp32 composite of the forward.
+struct BwdTotals {
+	double t, cr, cg, cb;
+	Accum acc;
+	__device__ __forceinline__ void init() {
+		t = 1.0, cr = cg = cb = 0.0;
+		acc.init();
+	}
+	__device__ __forceinline__ void add(const BwdSample &s) {
+		const double a = s.a;
+		cr += (double)s.r * a * t;
+		cg += (double)s.g * a * t;
+		cb += (double)s.b * a * t;
+		t *= 1.0 - a;
+		acc.blend(make_float4(s.r, s.g, s.b, s.a), false);
+	}
+};
+
+// Rows of one chunk of a block's sorted list are summed in LDS.
+constexpr int kBwdChunk = 256;
+
+__device__ __forceinline__ void bwd_fragment(double &t, double &rr, double &rg, double &rb,
+											 float gr, float gg, float gb, const BwdSample &s,
+											 float (&out)[kGradWords]) {
+	const double a = s.a;
+	const double cr = (double)s.r * a * t, cg = (double)s.g * a * t, cb = (double)s.b * a * t;
+	const double ar = rr - cr, ag = rg - cg, ab = rb - cb; // remainder behind this fragment
+	const double om = 1.0 - a;
+	const double inv_om = om > 0.0 ? 1.0 / om : 0.0;
+	const double ga = (double)gr * ((double)s.r * t - ar * inv_om) +
+					  (double)gg * ((double)s.g * t - ag * inv_om) +
+					  (double)gb * ((double)s.b * t - ab * inv_om);
+	const float w = (float)(a * t);
+	const float xr = gr * w * s.dr, xg = gg * w * s.dg, xb = gb * w * s.db;
+	const float w0 = (1.0f - s.b0) - s.b1;
+	out[0] = xr * w0, out[1] = xg * w0, out[2] = xb * w0;
+	out[3] = xr * s.b0, out[4] = xg * s.b0, out[5] = xb * s.b0;
+	out[6] = xr * s.b1, out[7] = xg * s.b1, out[8] = xb * s.b1;
+	out[9] = (float)ga * s.da;
+	t *= om;
+	rr = ar, rg = ag, rb = ab;
+}
+
+// Entries of one block in depth order: from the pre-sorted pool, or sorted here in
+// registers (lists of at most kRegSortCap) and kept in LDS.
+struct BlockEntries {
+	const uint32_t *tri;  // depth order
+	const uint64_t *mask; // depth order
+};
+
+// One wave per block: passes A and B over the block's entries (chunks of kBwdChunk, the
+// ordering window flushed at chunk ends in both passes, so they see the same order).
+__global__ __launch_bounds__(64) void block_backward_kernel(FrameConst c, FrameBufs b, BwdBufs g) {
+	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
+		return;
+	__shared__ uint4 s_rec[kSliceEntries * kRecQuads];
+	__shared__ float s_grad[kBwdChunk * kGradWords];
+	__shared__ uint32_t s_tri_all[kRegSortCap];
+	__shared__ uint64_t s_mask_all[kRegSortCap];
+	const int lane = lane_id();
+	const uint32_t slot = blockIdx.x;
+	const int bin = slot >> 4, blk = slot & 15, by = blk >> 2, bx = blk & 3;
+	const uint32_t flags = b.blk_flags[slot];
+	const uint32_t level = flags & kSlotLevelMask;
+	if(level == LUCID_BIN_LEVEL_EMPTY || (flags & (kSlotWhole | kSlotPromoted)))
+		return;
+	const uint4 sl = b.blk_slot[slot];
+	const uint32_t m = sl.y;
+	if(m == 0)
+		return;
+	const int bin_x = (bin % c.bin_count_x) * kBinSize, bin_y = (bin / c.bin_count_x) * kBinSize;
+	const int px0 = bin_x + bx * 8, py0 = bin_y + by * 8;
+	const int px = px0 + (lane & 7), py = py0 + (lane >> 3);
+	const bool on_screen = px < c.width && py < c.height;
+	const float fx = (float)px, fy = (float)py;
+
+	BlockEntries ent;
+	if(flags & kSlotPresorted) {
+		const uint32_t *hdr = b.sorted_pool + b.blk_sorted[slot];
+		ent.tri = hdr + kSortedHeader;
+		ent.mask = (const uint64_t *)(hdr + sorted_mask_offset(m));
+	} else {
+		const uint4 *recs =
+			(level == LUCID_BIN_LEVEL_LOW ? b.low_records : b.high_records) + sl.z;
+		const uint32_t *list = b.blk_lists + sl.x;
+		auto load = [&](uint32_t j) -> uint64_t {
+			return j < m ? block_key(b, recs[list[j]], bx, px0, py0, j) : ~0ull;
+		};
+		uint64_t v[4];
+		uint32_t P = next_pow2(m < 64 ? 64 : m);
+		if(P <= 64) {
+			uint64_t w1[1] = {load(lane)};
+			bitonic_regs<1>(w1);
+			v[0] = w1[0], v[1] = v[2] = v[3] = ~0ull;
+		} else if(P <= 128) {
+			uint64_t w2[2] = {load(lane), load(64 + lane)};
+			bitonic_regs<2>(w2);
+			v[0] = w2[0], v[1] = w2[1], v[2] = v[3] = ~0ull;
+		} else {
+			v[0] = load(lane), v[1] = load(64 + lane), v[2] = load(128 + lane),
+			v[3] = load(192 + lane);
+			bitonic_regs<4>(v);
+		}
+#pragma unroll
+		for(int j = 0; j < 4; j++)
+			if(j * 64 + lane < (int)m) {
+				uint4 rec = recs[list[(uint32_t)v[j]]];
+				s_mask_all[j * 64 + lane] = record_block_mask(rec, bx);
+				s_tri_all[j * 64 + lane] = rec.x;
+			}
+		wave_sync_lds();
+		ent.tri = s_tri_all;
+		ent.mask = s_mask_all;
+	}
+
+	// one pass over the block: visit(chunk0) before each chunk, then every emitted sample
+	auto run_pass = [&](auto &&before_chunk, auto &&after_chunk, auto &&emit) {
+		OrderWindow<BwdSample> w;
+		w.init();
+		for(uint32_t chunk0 = 0; chunk0 < m; chunk0 += kBwdChunk) {
+			const uint32_t chunk1 = min(m, chunk0 + (uint32_t)kBwdChunk);
+			before_chunk(chunk0, chunk1);
+			for(uint32_t s0 = chunk0; s0 < chunk1; s0 += kSliceEntries) {
+				const uint32_t n = min(chunk1 - s0, (uint32_t)kSliceEntries);
+				uint64_t msk[kSliceWords];
+				uint32_t t[kSliceWords];
+#pragma unroll
+				for(int k = 0; k < kSliceWords; k++) {
+					const uint32_t i = k * 64 + lane;
+					msk[k] = i < n ? ent.mask[s0 + i] : 0ull;
+					t[k] = i < n ? ent.tri[s0 + i] : 0u;
+				}
+				stage_slice(c, b, t, n, s_rec);
+				wave_sync_lds();
+				uint64_t bm[kSliceWords];
+				transpose_masks(msk, (int)((n + 63) / 64), bm);
+				const uint32_t e_base = s0 - chunk0;
+				walk_bits(bm, [&](int e) {
+					const uint4 *rec = s_rec + e * kRecQuads;
+					const float inv = record_inv_depth(rec[0], fx, fy);
+					BwdSample smp;
+					if(shade_grad(c, rec, fx, fy, inv, e_base + (uint32_t)e, smp))
+						w.insert(pixel_key(inv, rec[6].z), smp, emit);
+				});
+				wave_sync_lds();
+			}
+			w.flush(emit);
+			after_chunk(chunk0, chunk1);
+		}
+		return w.bad;
+	};
+	auto nothing = [](uint32_t, uint32_t) {};
+
+	// pass A
+	BwdTotals tot;
+	tot.init();
+	const bool bad = run_pass(nothing, nothing, [&](const BwdSample &s) { tot.add(s); });
+	const bool exact = on_screen && !bad;
+	enqueue_pixels(b, slot, on_screen && bad, px, py, false);
+	const float bg[3] = {c.background[0], c.background[1], c.background[2]};
+	const float o32[3] = {tot.acc.cr + tot.acc.tr * bg[0], tot.acc.cg + tot.acc.tr * bg[1],
+						  tot.acc.cb + tot.acc.tr * bg[2]};
+	float gsel[3] = {0.0f, 0.0f, 0.0f};
+	if(exact) {
+		const float4 gi = g.grad_image[(size_t)py * c.width + px];
+		const float gv[3] = {gi.x, gi.y, gi.z};
+#pragma unroll
+		for(int k = 0; k < 3; k++)
+			gsel[k] = (o32[k] >= 0.0f && o32[k] <= 1.0f) ? gv[k] : 0.0f;
+	}
+
+	// pass B
+	double t = 1.0;
+	double rr = tot.cr + tot.t * (double)bg[0], rg = tot.cg + tot.t * (double)bg[1],
+		   rb = tot.cb + tot.t * (double)bg[2];
+	const uint32_t row0 = g.row_base[slot];
+	auto zero_grad = [&](uint32_t, uint32_t) {
+		for(int i = lane; i < kBwdChunk * kGradWords; i += 64)
+			s_grad[i] = 0.0f;
+		wave_sync_lds();
+	};
+	auto write_rows = [&](uint32_t chunk0, uint32_t chunk1) {
+		wave_sync_lds();
+		for(uint32_t e = lane; e < chunk1 - chunk0; e += 64) {
+			const size_t row = (size_t)row0 + chunk0 + e;
+			if(row < g.rows_cap) {
+				g.row_tri[row] = ent.tri[chunk0 + e];
+#pragma unroll
+				for(int k = 0; k < kGradWords; k++)
+					g.rows[row * kGradWords + k] = s_grad[e * kGradWords + k];
+			}
+		}
+		wave_sync_lds();
+	};
+	run_pass(zero_grad, write_rows, [&](const BwdSample &s) {
+		float out[kGradWords];
+		bwd_fragment(t, rr, rg, rb, gsel[0], gsel[1], gsel[2], s, out);
+		if(gsel[0] != 0.0f || gsel[1] != 0.0f || gsel[2] != 0.0f) {
+#pragma unroll
+			for(int k = 0; k < kGradWords; k++)
+				atomicAdd(&s_grad[s.e * kGradWords + k], out[k]);
+		}
+	});
+}
+
+// Exclusive scan of the per-block gradient row counts (blocks the block kernel processes).
+__global__ __launch_bounds__(1024) void grad_row_scan_kernel(FrameConst c, FrameBufs b, BwdBufs g) {
+	const uint32_t n = (uint32_t)c.bin_count * 16;
+	__shared__ uint32_t s_wave[16];
+	__shared__ uint32_t s_carry;
+	if(threadIdx.x == 0)
+		s_carry = 0;
+	__syncthreads();
+	const int wave = threadIdx.x >> 6;
+	for(uint32_t base = 0; base < n; base += 1024) {
+		const uint32_t slot = base + threadIdx.x;
+		uint32_t v = 0;
+		if(slot < n) {
+			const uint32_t flags = b.blk_flags[slot];
+			const uint32_t level = flags & kSlotLevelMask;
+			if(level != LUCID_BIN_LEVEL_EMPTY && !(flags & (kSlotWhole | kSlotPromoted)))
+				v = b.blk_slot[slot].y;
+		}
+		const uint32_t inc = wave_inclusive_add(v);
+		if(lane_id() == 63)
+			s_wave[wave] = inc;
+		__syncthreads();
+		if(threadIdx.x == 0) {
+			uint32_t run = s_carry;
+			for(int w = 0; w < 16; w++) {
+				uint32_t x = s_wave[w];
+				s_wave[w] = run;
+				run += x;
+			}
+			s_carry = run;
+		}
+		__syncthreads();
+		if(slot < n)
+			g.row_base[slot] = s_wave[wave] + inc - v;
+		__syncthreads();
+	}
+	if(threadIdx.x == 0) {
+		b.ctrl[kCtrlBwdRows] = s_carry;
+		b.ctrl[kCtrlBwdExtra] = 0;
+	}
+}
+
+// Exact adjoint of one queued pixel: its fragments sorted exactly (as fallback_pixel), then
+// both passes over the sorted list; one gradient row per fragment, appended after the
+// block rows.
+__device__ void bwd_fallback_pixel(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
+								   uint4 item, uint64_t *s_keys) {
+	const int lane = lane_id();
+	const uint32_t slot = item.x;
+	const int px = (int)item.y, py = (int)item.z;
+	const int blk = (int)(slot & 15u), bx = blk & 3;
+	const uint32_t level = b.blk_flags[slot] & kSlotLevelMask;
+	const uint4 sl = b.blk_slot[slot];
+	const uint4 *recs = (level == LUCID_BIN_LEVEL_LOW ? b.low_records : b.high_records) + sl.z;
+	const uint32_t *list = b.blk_lists + sl.x;
+	const uint32_t m = sl.y;
+	const int row = py & 7, xb = bx * 8 + (px & 7);
+	const float fx = (float)px, fy = (float)py;
+	const uint32_t n =
+		gather_pixel_keys(b, recs, list, m, row, xb, fx, fy, s_keys, kFallbackCap, false);
+	const uint64_t *keys = s_keys;
+	bool global_keys = false;
+	if(n <= (uint32_t)kFallbackCap) {
+		const uint32_t P = next_pow2(n < 2 ? 2 : n);
+		for(uint32_t i = n + lane; i < P; i += 64)
+			s_keys[i] = ~0ull;
+		wave_sync_lds();
+		bitonic_lds_wave(s_keys, P);
+	} else {
+		const uint32_t P = next_pow2(n);
+		uint32_t off = 0;
+		if(lane == 0) {
+			off = (uint32_t)atomicAdd(&b.ctrl[kCtrlSpillCursor], P);
+			if((uint64_t)off + P > b.spill_cap) {
+				atomicOr(&b.ctrl[kCtrlOverflow], kOvfSpill);
+				off = 0xffffffffu;
+			}
+		}
+		off = __shfl(off, 0, 64);
+		if(off == 0xffffffffu)
+			return;
+		uint64_t *gk = b.spill + off;
+		gather_pixel_keys(b, recs, list, m, row, xb, fx, fy, gk, P, true);
+		for(uint32_t i = n + lane; i < P; i += 64)
+			__hip_atomic_store(&gk[i], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		__threadfence();
+		__builtin_amdgcn_wave_barrier();
+		bitonic_global_wave(gk, P);
+		keys = gk;
+		global_keys = true;
+	}
+	auto key_at = [&](uint32_t i) -> uint64_t {
+		return global_keys ? __hip_atomic_load(&keys[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+						   : keys[i];
+	};
+	auto shade_at = [&](uint32_t i, BwdSample &smp) -> bool {
+		const uint32_t t = (uint32_t)key_at(i);
+		uint4 q[kRecQuads];
+		load_shading_record(c, b, t, q);
+		return shade_grad(c, q, fx, fy, record_inv_depth(q[0], fx, fy), 0u, smp);
+	};
+	// pass A (serial over the sorted list, 64 samples shaded at a time)
+	BwdTotals tot;
+	tot.init();
+	for(uint32_t base = 0; base < n; base += 64) {
+		BwdSample smp;
+		smp.zero();
+		const bool ok = base + lane < n && shade_at(base + lane, smp);
+		const uint32_t cnt = min(64u, n - base);
+		for(uint32_t j = 0; j < cnt; j++) {
+			BwdSample s;
+			s.r = __shfl(smp.r, (int)j, 64), s.g = __shfl(smp.g, (int)j, 64);
+			s.b = __shfl(smp.b, (int)j, 64), s.a = __shfl(smp.a, (int)j, 64);
+			if(__shfl((int)ok, (int)j, 64))
+				tot.add(s);
+		}
+	}
+	const float bg[3] = {c.background[0], c.background[1], c.background[2]};
+	const float o32[3] = {tot.acc.cr + tot.acc.tr * bg[0], tot.acc.cg + tot.acc.tr * bg[1],
+						  tot.acc.cb + tot.acc.tr * bg[2]};
+	const float4 gi = g.grad_image[(size_t)py * c.width + px];
+	const float gv[3] = {gi.x, gi.y, gi.z};
+	float gsel[3];
+#pragma unroll
+	for(int k = 0; k < 3; k++)
+		gsel[k] = (o32[k] >= 0.0f && o32[k] <= 1.0f) ? gv[k] : 0.0f;
+	// pass B: rows [base_row, base_row + n)
+	uint32_t base_row = 0;
+	if(lane == 0) {
+		base_row = atomicAdd(&b.ctrl[kCtrlBwdExtra], n);
+		if((uint64_t)b.ctrl[kCtrlBwdRows] + base_row + n > g.rows_cap) {
+			atomicOr(&b.ctrl[kCtrlOverflow], kOvfGradRows);
+			base_row = 0xffffffffu;
+		}
+	}
+	base_row = __shfl(base_row, 0, 64);
+	if(base_row == 0xffffffffu)
+		return;
+	const size_t row0 = (size_t)b.ctrl[kCtrlBwdRows] + base_row;
+	double t = 1.0;
+	double rr = tot.cr + tot.t * (double)bg[0], rg = tot.cg + tot.t * (double)bg[1],
+		   rb = tot.cb + tot.t * (double)bg[2];
+	for(uint32_t base = 0; base < n; base += 64) {
+		BwdSample smp;
+		smp.zero();
+		const bool ok = base + lane < n && shade_at(base + lane, smp);
+		const uint32_t cnt = min(64u, n - base);
+		float mine[kGradWords];
+#pragma unroll
+		for(int k = 0; k < kGradWords; k++)
+			mine[k] = 0.0f;
+		for(uint32_t j = 0; j < cnt; j++) {
+			BwdSample s;
+			s.r = __shfl(smp.r, (int)j, 64), s.g = __shfl(smp.g, (int)j, 64);
+			s.b = __shfl(smp.b, (int)j, 64), s.a = __shfl(smp.a, (int)j, 64);
+			s.b0 = __shfl(smp.b0, (int)j, 64), s.b1 = __shfl(smp.b1, (int)j, 64);
+			s.dr = __shfl(smp.dr, (int)j, 64), s.dg = __shfl(smp.dg, (int)j, 64);
+			s.db = __shfl(smp.db, (int)j, 64), s.da = __shfl(smp.da, (int)j, 64);
+			if(__shfl((int)ok, (int)j, 64)) {
+				float out[kGradWords];
+				bwd_fragment(t, rr, rg, rb, gsel[0], gsel[1], gsel[2], s, out);
+				if((uint32_t)lane == j) {
+#pragma unroll
+					for(int k = 0; k < kGradWords; k++)
+						mine[k] = out[k];
+				}
+			}
+		}
+		if(base + lane < n) {
+			const size_t r = row0 + base + lane;
+			g.row_tri[r] = (uint32_t)key_at(base + lane);
+#pragma unroll
+			for(int k = 0; k < kGradWords; k++)
+				g.rows[r * kGradWords + k] = mine[k];
+		}
+	}
+}
+
+__global__ __launch_bounds__(kFallbackThreads) void bwd_fallback_kernel(FrameConst c, FrameBufs b,
+																		 BwdBufs g) {
+	__shared__ uint64_t s_keys[kFallbackThreads / 64][kFallbackCap];
+	const uint32_t nq = min(b.ctrl[kCtrlFallbackCount], b.fallback_cap);
+	const uint32_t waves = gridDim.x * (kFallbackThreads / 64);
+	const uint32_t w = blockIdx.x * (kFallbackThreads / 64) + (threadIdx.x >> 6);
+	for(uint32_t qi = w; qi < nq; qi += waves)
+		bwd_fallback_pixel(c, b, g, b.fallback_queue[qi], s_keys[threadIdx.x >> 6]);
+}
+
 template <bool FP32>
 static void launch_raster_t(const FrameConst &c, const FrameBufs &b, hipStream_t stream,
 							hipEvent_t *ev) {
@@ -1406,6 +1923,29 @@ hipError_t launch_raster(const FrameConst &c, const FrameBufs &b, hipStream_t st
 		launch_raster_t<true>(c, b, stream, events);
 	else
 		launch_raster_t<false>(c, b, stream, events);
+	return hipGetLastError();
+}
+
+// Backward, first part: the forward's row generation and block sorting (fragment lists are
+// recomputed, not stored), then the gradient row allocation. The row counts are then read
+// back by the host to size the row buffers.
+hipError_t launch_backward_prepare(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
+								   hipStream_t stream) {
+	hipLaunchKernelGGL(rowgen_low_kernel, dim3(c.bin_count), dim3(kRowgenThreads), 0, stream, c, b);
+	hipLaunchKernelGGL(high_prep_kernel, dim3(1), dim3(1024), 0, stream, c, b);
+	hipLaunchKernelGGL(rowgen_high_kernel, dim3(c.bin_count * kRowgenSplit), dim3(kRowgenThreads),
+					   0, stream, c, b);
+	hipLaunchKernelGGL(high_finalize_kernel, dim3(c.bin_count * 16), dim3(kSortThreads), 0, stream,
+					   c, b);
+	hipLaunchKernelGGL(grad_row_scan_kernel, dim3(1), dim3(1024), 0, stream, c, b, g);
+	return hipGetLastError();
+}
+
+// Backward, second part: per-block adjoints and the exact per-pixel adjoints.
+hipError_t launch_backward_blocks(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
+								  hipStream_t stream) {
+	hipLaunchKernelGGL(block_backward_kernel, dim3(c.bin_count * 16), dim3(64), 0, stream, c, b, g);
+	hipLaunchKernelGGL(bwd_fallback_kernel, dim3(1024), dim3(kFallbackThreads), 0, stream, c, b, g);
 	return hipGetLastError();
 }
 

@@ -55,7 +55,9 @@ class GpuScene:
                                 np.ascontiguousarray(scene.instance_colors, np.uint32))
         self.config = np.frombuffer(scene.config, np.uint8).copy()
 
-    def inputs(self) -> _lib.FrameInputs:
+    def inputs(self, colors_rgb=None, tri_alpha=None) -> _lib.FrameInputs:
+        """The C ABI's frame inputs; colors_rgb / tri_alpha (contiguous fp32 device
+        tensors) replace the scene's own, as the differentiable path does."""
         def p(t):
             return None if t is None else t.data_ptr()
         return _lib.FrameInputs(
@@ -63,7 +65,9 @@ class GpuScene:
             None if self.instance_colors is None else self.instance_colors.ctypes.data,
             self.config.ctypes.data,
             p(self.quad_indices), p(self.positions), int(self.scene.num_verts),
-            p(self.colors_rgba8), p(self.colors_rgb), p(self.tri_alpha), p(self.normals))
+            p(self.colors_rgba8),
+            p(colors_rgb if colors_rgb is not None else self.colors_rgb),
+            p(tri_alpha if tri_alpha is not None else self.tri_alpha), p(self.normals))
 
 
 class LucidRenderer:
@@ -110,7 +114,8 @@ class LucidRenderer:
             "frag_counts": torch.empty((h, w), dtype=torch.int32, device=dev) if counts else None,
         }
 
-    def render_async(self, gscene: GpuScene, outputs: dict, stream=None) -> None:
+    def render_async(self, gscene: GpuScene, outputs: dict, stream=None, colors_rgb=None,
+                     tri_alpha=None) -> None:
         """Enqueues one frame on `stream` (torch stream or None = current stream)."""
         torch = self._torch
         st = stream if stream is not None else torch.cuda.current_stream(self.device)
@@ -118,7 +123,7 @@ class LucidRenderer:
             outputs["image"].data_ptr() if outputs.get("image") is not None else None,
             outputs["image_f"].data_ptr() if outputs.get("image_f") is not None else None,
             outputs["frag_counts"].data_ptr() if outputs.get("frag_counts") is not None else None)
-        inp = gscene.inputs()
+        inp = gscene.inputs(colors_rgb, tri_alpha)
         _lib.check(self.lib.lucid_render(self.handle, ctypes.byref(inp), ctypes.byref(o),
                                          ctypes.c_void_p(st.cuda_stream)), "lucid_render")
 
@@ -126,18 +131,42 @@ class LucidRenderer:
         return self.lib.lucid_finish(self.handle)
 
     def render(self, gscene: GpuScene, outputs: Optional[dict] = None, stream=None,
-               max_retries: int = 4) -> dict:
+               max_retries: int = 4, colors_rgb=None, tri_alpha=None) -> dict:
         """Renders one frame and waits for it; re-renders after an internal buffer grew."""
         if outputs is None:
             outputs = self.alloc_outputs()
         for _ in range(max_retries + 1):
-            self.render_async(gscene, outputs, stream)
+            self.render_async(gscene, outputs, stream, colors_rgb, tri_alpha)
             rc = self.finish()
             if rc == _lib.LUCID_OK:
                 return outputs
             if rc != _lib.LUCID_ERR_CAPACITY:
                 _lib.check(rc, "lucid_finish")
         raise _lib.LucidError(_lib.LUCID_ERR_CAPACITY, "render")
+
+    def backward(self, gscene: GpuScene, grad_image, colors_rgb=None, tri_alpha=None,
+                 stream=None, max_retries: int = 4):
+        """Adjoint of the fp32-sample forward: dL/d(image_f) (H x W x 4 fp32 device tensor)
+        -> (dL/d(colors_rgb) [V, 3], dL/d(tri_alpha) [2 Q]). Recomputes the fragment lists."""
+        torch = self._torch
+        dev = torch.device("cuda", self.device)
+        st = stream if stream is not None else torch.cuda.current_stream(self.device)
+        g_img = grad_image.contiguous().to(device=dev, dtype=torch.float32)
+        if tuple(g_img.shape) != (self.height, self.width, 4):
+            raise ValueError(f"grad_image must be {(self.height, self.width, 4)}")
+        nv = int(gscene.scene.num_verts)
+        g_col = torch.empty((nv, 3), dtype=torch.float32, device=dev)
+        g_alpha = torch.empty((2 * gscene.scene.num_quads,), dtype=torch.float32, device=dev)
+        inp = gscene.inputs(colors_rgb, tri_alpha)
+        for _ in range(max_retries + 1):
+            rc = self.lib.lucid_backward(self.handle, ctypes.byref(inp), g_img.data_ptr(),
+                                         g_col.data_ptr(), g_alpha.data_ptr(),
+                                         ctypes.c_void_p(st.cuda_stream))
+            if rc == _lib.LUCID_OK:
+                return g_col, g_alpha
+            if rc != _lib.LUCID_ERR_CAPACITY:
+                _lib.check(rc, "lucid_backward")
+        raise _lib.LucidError(_lib.LUCID_ERR_CAPACITY, "lucid_backward")
 
     # ------------------------------------------------------------ readback / stats
 

@@ -246,6 +246,15 @@ def config0(seed: int = 1000, num_tris: int = 10_000, width: int = 512, height: 
                       name=f"c0_{num_tris}_{width}x{height}")
 
 
+def config4(seed: int = 1004, num_tris: int = 200_000, width: int = 512,
+            height: int = 512) -> Scene:
+    """The differentiable workload: the config0 generator at 200k triangles (mean depth
+    complexity ~160 fragments/pixel at 512x512), fp32 colours."""
+    sc = config0(seed=seed, num_tris=num_tris, width=width, height=height)
+    sc.name = f"c4_{num_tris}_{width}x{height}"
+    return sc
+
+
 def _lognormal_edges(rng, n, median_px, sigma, width):
     lengths = median_px * np.exp(rng.normal(0.0, sigma, (n, 3)))
     ang = rng.uniform(0.0, 2.0 * np.pi, (n, 3))

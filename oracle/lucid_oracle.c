@@ -625,6 +625,8 @@ typedef struct {
 	uint32_t enc;  /* RGBA8 encoded sample (0 = skipped), quantised mode */
 	float col[4];  /* fp32 sample colour, fp32 mode */
 	float inv_ray; /* depth: larger is nearer */
+	float bary[2]; /* barycentrics of corners 1 and 2 */
+	float light[3]; /* per-channel lighting factor (ambient + sun * light) */
 } sample_t;
 
 static void shade_sample(const frame *f, int px_i, int py_i, uint32_t t, sample_t *out) {
@@ -646,6 +648,8 @@ static void shade_sample(const frame *f, int px_i, int py_i, uint32_t t, sample_
 	float bary0 = e0 * ray_pos, bary1 = e1 * ray_pos;
 	bary0 = bary0 - bp[0];
 	bary1 = bary1 - bp[1];
+	out->bary[0] = bary0, out->bary[1] = bary1;
+	out->light[0] = out->light[1] = out->light[2] = 0.0f;
 
 	float color[4] = {1.0f, 1.0f, 1.0f, 1.0f};
 	if((flags & LUCID_INST_HAS_COLOR) && f->in->instance_colors) {
@@ -697,6 +701,7 @@ static void shade_sample(const frame *f, int px_i, int py_i, uint32_t t, sample_
 		float sun_c = (&cfg->lighting.sun_color.x)[k];
 		float amb = amb_c * cfg->lighting.ambient_power;
 		float dif = (sun_c * cfg->lighting.sun_power) * light_value;
+		out->light[k] = amb + dif;
 		float lin = srgb_to_linear(color[k]);
 		color[k] = satf(linear_to_srgb(lin * (amb + dif)));
 	}
@@ -719,6 +724,7 @@ typedef struct {
 	uint32_t enc;
 	uint32_t pix; /* pixel id inside the bin: row * 32 + x */
 	float col[4];
+	float inv_ray, bary[2], light[3];
 } frag_t;
 
 static inline uint32_t ord_f32(float f) {
@@ -1035,6 +1041,9 @@ static int raster_bin(const frame *f, oracle_outputs *out, int bin, int level, b
 				fr->enc = s.enc;
 				fr->pix = (uint32_t)(r * 32 + x);
 				memcpy(fr->col, s.col, sizeof(fr->col));
+				fr->inv_ray = s.inv_ray;
+				fr->bary[0] = s.bary[0], fr->bary[1] = s.bary[1];
+				memcpy(fr->light, s.light, sizeof(fr->light));
 			}
 		}
 	}
@@ -1069,6 +1078,24 @@ static int raster_bin(const frame *f, oracle_outputs *out, int bin, int level, b
 				continue;
 			frag_t *lst = sorted + start[p];
 			qsort(lst, counts[p], sizeof(frag_t), frag_cmp);
+			if(out->frags) {
+				size_t at;
+#ifdef _OPENMP
+#pragma omp atomic capture
+#endif
+				{
+					at = out->num_frags_dumped;
+					out->num_frags_dumped += counts[p];
+				}
+				for(uint32_t k = 0; k < counts[p] && at + k < out->frags_cap; k++) {
+					oracle_fragment *d = &out->frags[at + k];
+					d->x = (uint32_t)px, d->y = (uint32_t)py, d->tri = lst[k].tri;
+					d->inv_ray = lst[k].inv_ray;
+					d->bary[0] = lst[k].bary[0], d->bary[1] = lst[k].bary[1];
+					memcpy(d->light, lst[k].light, sizeof(d->light));
+					d->order = k;
+				}
+			}
 			blend_state s = {{0.0f, 0.0f, 0.0f}, 1.0f};
 			for(uint32_t k = 0; k < counts[p]; k++) {
 				if(fp32 ? lst[k].col[3] == 0.0f : lst[k].enc == 0)

@@ -46,6 +46,15 @@ typedef struct oracle_inputs {
 	uint32_t opts;
 } oracle_inputs;
 
+/* One covered sample of the exact mode, listed per pixel in blend order (front to back,
+ * depth ties by triangle id). */
+typedef struct oracle_fragment {
+	uint32_t x, y, tri, order; /* order: position in the pixel's sorted list */
+	float inv_ray;
+	float bary[2];  /* barycentrics of corners 1 and 2 (corner 0: 1 - b0 - b1) */
+	float light[3]; /* lighting factor per channel */
+} oracle_fragment;
+
 typedef struct oracle_outputs {
 	uint32_t *info;        /* LUCID_INFO_U32_SIZE + 10 * bin_count */
 	uint32_t *quad_aabb;   /* per input quad; 0xffffffff = rejected */
@@ -61,6 +70,9 @@ typedef struct oracle_outputs {
 	uint32_t *bin_levels;  /* per bin: level used by the rasterizer (low/high), may be NULL */
 	uint64_t num_fragments;
 	uint32_t num_ref_errors; /* bins the reference would paint red (raster_high limits) */
+	oracle_fragment *frags;  /* exact mode: every sample, may be NULL */
+	size_t frags_cap;
+	size_t num_frags_dumped; /* set by oracle_render (may exceed frags_cap) */
 } oracle_outputs;
 
 /* Runs the whole frame. Returns 0 on success, negative on invalid input or capacity. */

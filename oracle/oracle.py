@@ -45,7 +45,16 @@ class _Outputs(ctypes.Structure):
         ("image", ctypes.c_void_p), ("image_f", ctypes.c_void_p),
         ("frag_counts", ctypes.c_void_p), ("bin_levels", ctypes.c_void_p),
         ("num_fragments", ctypes.c_uint64), ("num_ref_errors", ctypes.c_uint32),
+        ("frags", ctypes.c_void_p), ("frags_cap", ctypes.c_size_t),
+        ("num_frags_dumped", ctypes.c_size_t),
     ]
+
+
+# One exact-mode sample (oracle_fragment): pixel, triangle, position in the pixel's sorted
+# list, depth, barycentrics of corners 1 and 2, lighting factor per channel.
+FRAGMENT_DTYPE = np.dtype([("x", np.uint32), ("y", np.uint32), ("tri", np.uint32),
+                           ("order", np.uint32), ("inv_ray", np.float32),
+                           ("bary", np.float32, 2), ("light", np.float32, 3)])
 
 
 _lib = None
@@ -87,8 +96,10 @@ def bin_count(width: int, height: int) -> int:
 
 
 def render(scene, mode: int = MODE_EXACT, opts: int = 0, bin_capacity: Optional[int] = None,
-           want_records: bool = True) -> dict:
-    """Runs the oracle on a lucid_amd.scenes.Scene; returns numpy outputs."""
+           want_records: bool = True, want_fragments: bool = False) -> dict:
+    """Runs the oracle on a lucid_amd.scenes.Scene; returns numpy outputs. With
+    want_fragments (exact mode), res["fragments"] lists every sample (FRAGMENT_DTYPE),
+    sorted by pixel (row-major) and blend order."""
     L = lib()
     w, h = scene.width, scene.height
     nb = bin_count(w, h)
@@ -136,10 +147,22 @@ def render(scene, mode: int = MODE_EXACT, opts: int = 0, bin_capacity: Optional[
         out_np["bin_quads"].ctypes.data, out_np["bin_tris"].ctypes.data, bin_capacity,
         out_np["image"].ctypes.data, out_np["image_f"].ctypes.data,
         out_np["frag_counts"].ctypes.data, out_np["bin_levels"].ctypes.data, 0, 0,
+        None, 0, 0,
     )
     rc = L.oracle_render(ctypes.byref(inp), ctypes.byref(out))
     if rc != 0:
         raise RuntimeError(f"oracle_render failed: {rc}")
+    if want_fragments:
+        # second run with a buffer sized from the first run's fragment count
+        n = int(out.num_fragments)
+        frags = np.zeros(max(n, 1), FRAGMENT_DTYPE)
+        out.frags, out.frags_cap, out.num_frags_dumped = frags.ctypes.data, n, 0
+        rc = L.oracle_render(ctypes.byref(inp), ctypes.byref(out))
+        if rc != 0 or out.num_frags_dumped != n:
+            raise RuntimeError(f"oracle_render fragment dump failed: {rc}")
+        frags = frags[:n]
+        idx = np.lexsort((frags["order"], frags["x"], frags["y"]))
+        out_np["fragments"] = frags[idx]
     out_np["num_fragments"] = int(out.num_fragments)
     out_np["num_ref_errors"] = int(out.num_ref_errors)
     out_np["bin_count"] = nb

@@ -1,0 +1,163 @@
+"""Backward pass: the GPU adjoint (lucid_backward through the C ABI) against a dense float64
+PyTorch compositing reference built from the oracle's exact fragment lists, whose own
+autograd is pinned by float64 central finite differences (h = 1e-4).
+
+Bar: gradients within 1e-4 relative (vector norm) of the float64 reference, and on 64
+random parameters within 1e-4 relative (plus a 1e-9 absolute floor for the finite
+differences' truncation error) of the finite differences.
+"""
+
+import numpy as np
+import pytest
+
+from lucid_amd import scenes
+
+from dense_ref import DenseCompositor
+
+H = 1e-4
+
+
+def _fd_check_params(rng, grad, k=64):
+    """k random parameters among those with a gradient (visible triangles)."""
+    nz = np.flatnonzero(np.abs(grad) > 1e-6 * np.abs(grad).max())
+    return rng.choice(nz, size=min(k, len(nz)), replace=False)
+
+
+def _loss64(torch, comp, colors, alpha, target):
+    img = comp.image(colors, alpha)
+    return ((img - target) ** 2).mean()
+
+
+def _setup(oracle_mod, sc, seed=5):
+    import torch
+    res = oracle_mod.render(sc, opts=oracle_mod.OPT_FP32_SAMPLES, want_fragments=True)
+    comp = DenseCompositor(sc, res["fragments"])
+    rng = np.random.default_rng(seed)
+    colors = sc.colors_rgb.astype(np.float64)
+    alpha = sc.tri_alpha.astype(np.float64)
+    # target: the image of N(0, 0.1)-perturbed parameters
+    with torch.no_grad():
+        tc = torch.from_numpy(np.clip(colors + rng.normal(0, 0.1, colors.shape), 0, 1))
+        ta = torch.from_numpy(np.clip(alpha + rng.normal(0, 0.1, alpha.shape), 0.01, 0.99))
+        target = comp.image(tc, ta)
+    return res, comp, colors, alpha, target
+
+
+def _ref_grads(oracle_mod, sc):
+    import torch
+    res, comp, colors, alpha, target = _setup(oracle_mod, sc)
+    c64 = torch.from_numpy(colors).requires_grad_(True)
+    a64 = torch.from_numpy(alpha).requires_grad_(True)
+    loss = _loss64(torch, comp, c64, a64, target)
+    loss.backward()
+    return res, comp, colors, alpha, target, c64.grad.numpy(), a64.grad.numpy()
+
+
+def test_dense_reference_matches_oracle_image(oracle_mod):
+    import torch
+    sc = scenes.config0(num_tris=600, width=64, height=64, seed=21)
+    res = oracle_mod.render(sc, opts=oracle_mod.OPT_FP32_SAMPLES, want_fragments=True)
+    comp = DenseCompositor(sc, res["fragments"])
+    img = comp.image(torch.from_numpy(sc.colors_rgb.astype(np.float64)),
+                     torch.from_numpy(sc.tri_alpha.astype(np.float64))).numpy()
+    ref = res["image_f"][..., :3]
+    assert np.abs(img - ref).max() < 1e-5
+
+
+def test_dense_reference_gradient_matches_finite_differences(oracle_mod):
+    """Pins the float64 reference's autograd with central differences (h = 1e-4)."""
+    import torch
+    sc = scenes.config0(num_tris=300, width=48, height=48, seed=22)
+    res, comp, colors, alpha, target, gc, ga = _ref_grads(oracle_mod, sc)
+    rng = np.random.default_rng(3)
+    for name, base, grad in (("colors", colors, gc), ("alpha", alpha, ga)):
+        flat = base.reshape(-1)
+        for i in _fd_check_params(rng, grad.reshape(-1), k=32):
+            vals = []
+            for sgn in (1.0, -1.0):
+                p = flat.copy()
+                p[i] += sgn * H
+                cc = torch.from_numpy(p.reshape(colors.shape) if name == "colors" else colors)
+                aa = torch.from_numpy(p if name == "alpha" else alpha)
+                vals.append(float(_loss64(torch, comp, cc, aa, target)))
+            fd = (vals[0] - vals[1]) / (2 * H)
+            g = grad.reshape(-1)[i]
+            assert abs(g - fd) <= 1e-4 * abs(fd) + 1e-9, (name, i, g, fd)
+
+
+def _gpu_grads(sc, target):
+    import torch
+    from lucid_amd import _lib
+    from lucid_amd.renderer import GpuScene, LucidRenderer
+    from lucid_amd.autograd import render_image
+    print(f"\n[case] backward {sc.name} {sc.width}x{sc.height} quads={sc.num_quads}", flush=True)
+    r = LucidRenderer((sc.width, sc.height), opts=_lib.OPT_FP32_SAMPLES)
+    gs = GpuScene(sc)
+    col = torch.tensor(sc.colors_rgb, device="cuda", requires_grad=True)
+    alpha = torch.tensor(sc.tri_alpha, device="cuda", requires_grad=True)
+    img = render_image(r, gs, col, alpha)
+    t = torch.tensor(target, dtype=torch.float32, device="cuda")
+    loss = ((img[..., :3] - t) ** 2).mean()
+    loss.backward()
+    return img.detach().cpu().numpy(), col.grad.cpu().numpy(), alpha.grad.cpu().numpy()
+
+
+def _check_grads(g, ref, what):
+    n = np.linalg.norm(ref)
+    assert n > 0, what
+    rel = np.linalg.norm(g - ref) / n
+    assert rel < 1e-4, f"{what}: relative error {rel:.3g}"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ntris,size,seed", [(400, 64, 31), (3000, 128, 32)])
+def test_backward_matches_dense_reference(oracle_mod, ntris, size, seed):
+    sc = scenes.config0(num_tris=ntris, width=size, height=size, seed=seed)
+    res, comp, colors, alpha, target, gc, ga = _ref_grads(oracle_mod, sc)
+    img, col_g, alpha_g = _gpu_grads(sc, target.numpy())
+    assert np.abs(img[..., :3] - res["image_f"][..., :3]).max() <= 1.0 / 255.0
+    _check_grads(col_g, gc, "dL/dcolors")
+    _check_grads(alpha_g[0::2], ga[0::2], "dL/dalpha")
+    assert np.all(alpha_g[1::2] == 0.0), "culled second triangles get no gradient"
+
+
+@pytest.mark.gpu
+def test_backward_finite_differences_64_params(oracle_mod):
+    """GPU gradients on 64 random parameters against float64 central differences."""
+    import torch
+    sc = scenes.config0(num_tris=2000, width=96, height=96, seed=33)
+    res, comp, colors, alpha, target = _setup(oracle_mod, sc)
+    img, col_g, alpha_g = _gpu_grads(sc, target.numpy())
+    rng = np.random.default_rng(4)
+    grads = np.concatenate([col_g.reshape(-1), alpha_g])
+    params = np.concatenate([colors.reshape(-1), alpha])
+    nc = colors.size
+    for i in _fd_check_params(rng, grads, k=64):
+        vals = []
+        for sgn in (1.0, -1.0):
+            p = params.copy()
+            p[i] += sgn * H
+            cc = torch.from_numpy(p[:nc].reshape(colors.shape))
+            aa = torch.from_numpy(p[nc:])
+            vals.append(float(_loss64(torch, comp, cc, aa, target)))
+        fd = (vals[0] - vals[1]) / (2 * H)
+        assert abs(grads[i] - fd) <= 1e-4 * abs(fd) + 1e-9, (i, grads[i], fd)
+
+
+@pytest.mark.gpu
+def test_backward_exact_fallback_and_deep_pixels(oracle_mod):
+    """Ordering violations (tilted planes) send pixels through the exact per-pixel adjoint."""
+    sc = scenes.tilted_planes(40, 32, 24, seed=11)
+    res, comp, colors, alpha, target, gc, ga = _ref_grads(oracle_mod, sc)
+    img, col_g, alpha_g = _gpu_grads(sc, target.numpy())
+    _check_grads(col_g, gc, "dL/dcolors")
+    _check_grads(alpha_g, ga, "dL/dalpha")
+
+
+@pytest.mark.gpu
+def test_backward_is_deterministic(oracle_mod):
+    sc = scenes.config0(num_tris=1500, width=96, height=96, seed=34)
+    res, comp, colors, alpha, target = _setup(oracle_mod, sc)
+    a = _gpu_grads(sc, target.numpy())
+    b = _gpu_grads(sc, target.numpy())
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
