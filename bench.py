@@ -37,7 +37,43 @@ def parse():
     p.add_argument("--copy-calibration", action="store_true",
                    help="also run a 256 MiB device copy (reference kernel for rocprof counters)")
     p.add_argument("--tris", type=int, default=1_000_000)
+    p.add_argument("--no-backward", action="store_true",
+                   help="skip the fwd+bwd / fwd ratio leg (200k triangles at 512x512)")
+    p.add_argument("--bwd-steps", type=int, default=5)
     return p.parse_args()
+
+
+def backward_ratio(steps: int):
+    """fwd+bwd vs plain fwd on the differentiable workload: 200k soup triangles at 512x512
+    (the config0 generator, ~160 fragments/pixel), fp32 samples. A fwd+bwd step is a forward
+    frame followed by lucid_backward (which recomputes the fragment lists) for a dL/dimage."""
+    import torch
+    from lucid_amd import _lib, scenes
+    from lucid_amd.renderer import GpuScene, LucidRenderer
+    sc = scenes.config4()
+    r = LucidRenderer((sc.width, sc.height), opts=_lib.OPT_FP32_SAMPLES, max_quads=sc.num_quads)
+    g = GpuScene(sc)
+    out = r.alloc_outputs(rgba8=False, rgba32f=True, counts=False)
+    grad = torch.randn((sc.height, sc.width, 4), dtype=torch.float32, device="cuda") * 1e-3
+    r.render(g, out)
+    frags = int(r.read_info()[60])
+    r.backward(g, grad)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.render(g, out)
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.render(g, out)
+        r.backward(g, grad)
+    torch.cuda.synchronize()
+    t_both = (time.perf_counter() - t0) / steps
+    return {"workload": "200k soup triangles at 512x512 (config0 generator, seed 1004), fp32 "
+                        "samples; host-timed synchronous steps",
+            "fragments": frags, "fwd_ms": t_fwd * 1e3, "fwd_bwd_ms": t_both * 1e3,
+            "fwd_bwd_ratio": t_both / t_fwd}
 
 
 def algorithmic_bytes(scene, r, info):
@@ -146,6 +182,11 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
 
+    bwd = None
+    if not args.no_backward:
+        print("[bench] fwd+bwd ratio (200k tris, 512x512)", file=sys.stderr, flush=True)
+        bwd = backward_ratio(args.bwd_steps)
+
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle)", file=sys.stderr, flush=True)
@@ -174,6 +215,7 @@ def main():
             "stages_ms": stages,
             "fallback_pixels": int(info[62]), "oversized_blocks": int(info[63]),
             "high_bins": int(info[9]), "low_bins": int(info[7]),
+            "backward": bwd,
         },
         "roofline": {
             "bound": "hbm",

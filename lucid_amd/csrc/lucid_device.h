@@ -39,6 +39,8 @@ enum CtrlWord : int {
 	kCtrlSortedCursor = 10, // bump allocator of pre-sorted block arrays (u32 units)
 	kCtrlBwdRows = 11,      // gradient rows of the block kernel (set by the row scan)
 	kCtrlBwdExtra = 12,     // bump allocator of per-fragment gradient rows of exact pixels
+	kCtrlBwdRanges = 13,    // depth ranges of the backward block kernels (set by the row scan)
+	kCtrlBwdGradMax = 14,   // max |dL/dpixel| (float bits) of the pixels the block kernels handle
 	kCtrlWords = 16,
 };
 
@@ -48,12 +50,21 @@ constexpr uint32_t kOvfBinLists = 1u, kOvfRecords = 2u, kOvfFallback = 4u, kOvfS
 
 // Buffers of the backward pass (passed by value beside FrameBufs).
 constexpr int kGradWords = 10; // per row: corner 0 rgb, corner 1 rgb, corner 2 rgb, alpha
+// Start state of one (depth range, pixel) for pass B: transmittance before the range,
+// remainder out - C_before (fp64), and the masked dL/d(pixel rgb).
+struct BwdStart {
+	double t, rr, rg, rb;
+	float gr, gg, gb, pad;
+};
 struct BwdBufs {
 	const float4 *grad_image; // dL/d(rgba32f image); alpha ignored
 	uint32_t *row_base;       // bin_count * 16: first gradient row of each block
+	uint32_t *range_base;     // bin_count * 16: first depth range of each block
 	float *rows;              // kGradWords floats per row
 	uint32_t *row_tri;        // triangle id of each row
 	uint64_t rows_cap;
+	BwdStart *start;          // 64 per depth range
+	uint8_t *seq;             // 64 bytes per gradient row: recorded blend order
 };
 
 // Per-block slot flags (blk_flags)

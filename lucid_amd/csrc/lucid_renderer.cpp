@@ -88,8 +88,8 @@ struct lucid_renderer {
 	DevBuf low_records, high_records, fallback_queue, spill;
 	DevBuf blk_slot, blk_flags, blk_sorted, blk_lists, blk_count, band_count, sorted_pool;
 	// backward
-	DevBuf grad_row_base, grad_rows, grad_row_tri, red_keys_out, red_vals_in, red_vals_out,
-		red_keys_in2, tri_grad, red_temp;
+	DevBuf grad_row_base, grad_range_base, grad_rows, grad_row_tri, grad_start, grad_seq,
+		red_keys_out, red_vals_in, red_vals_out, red_keys_in2, tri_grad, red_temp;
 	uint64_t grad_rows_cap = 0;
 
 	int64_t quad_cap = 0;
@@ -527,6 +527,7 @@ int lucid_backward(lucid_renderer *r, const lucid_frame_inputs *in, const float 
 		return pe;
 	const size_t slots = (size_t)r->bin_count * 16;
 	LUCID_CHECK(r->grad_row_base.reserve(slots * 4));
+	LUCID_CHECK(r->grad_range_base.reserve(slots * 4));
 	if(r->grad_rows_cap == 0) {
 		r->grad_rows_cap = 1u << 20;
 		LUCID_CHECK(r->grad_rows.reserve(r->grad_rows_cap * kGradWords * 4));
@@ -535,6 +536,9 @@ int lucid_backward(lucid_renderer *r, const lucid_frame_inputs *in, const float 
 	BwdBufs g;
 	g.grad_image = (const float4 *)grad_image;
 	g.row_base = r->grad_row_base.as<uint32_t>();
+	g.range_base = r->grad_range_base.as<uint32_t>();
+	g.start = nullptr;
+	g.seq = nullptr;
 	g.rows = r->grad_rows.as<float>();
 	g.row_tri = r->grad_row_tri.as<uint32_t>();
 	g.rows_cap = r->grad_rows_cap;
@@ -565,6 +569,13 @@ int lucid_backward(lucid_renderer *r, const lucid_frame_inputs *in, const float 
 		g.row_tri = r->grad_row_tri.as<uint32_t>();
 		g.rows_cap = r->grad_rows_cap;
 	}
+	// pass B start states (64 per depth range) and the recorded blend order (a byte per
+	// sample; at most 64 samples per list entry)
+	const uint64_t ranges = r->status[kCtrlBwdRanges];
+	LUCID_CHECK(r->grad_start.reserve(std::max<uint64_t>(ranges, 1) * 64 * sizeof(BwdStart)));
+	LUCID_CHECK(r->grad_seq.reserve(std::max<uint64_t>(block_rows, 1) * 64));
+	g.start = r->grad_start.as<BwdStart>();
+	g.seq = r->grad_seq.as<uint8_t>();
 	LUCID_CHECK(launch_backward_blocks(c, b, g, stream));
 	LUCID_CHECK(hipMemcpyAsync(r->status, r->ctrl.ptr, kCtrlWords * 4, hipMemcpyDeviceToHost,
 							   stream));

@@ -1425,54 +1425,37 @@ __global__ __launch_bounds__(kFallbackThreads) void fallback_kernel(FrameConst c
 // ------------------------------------------------------------------ backward (adjoint)
 //
 // The adjoint of sorted front-to-back compositing for the fp32-sample forward. Fragment
-// lists are recomputed (the forward stages run again), never stored. For every block a
-// wave replays the forward walk twice with the same ordering window, so both passes see the
-// identical fragment order:
-//   pass A accumulates, per pixel, the composite in fp64 (C, T) and the fp32 composite the
-//          forward wrote (for the saturation mask of the output);
-//   pass B walks again with T_i and the remainder R_i = out - C_{<i} (fp64, so deep pixels
-//          whose transmittance underflows fp32 stay exact), and per fragment forms
-//            dout/dc_i = a_i T_i,   dout/da_i = T_i c_i - R_{i+1} / (1 - a_i),
-//          chains them through the sRGB lighting stage and the barycentric weights, and
-//          sums them per (block, entry) in LDS.
-// Each (block, entry) row is then reduced per triangle and per vertex by sorted
-// segmented sums (reduce.hip): no global atomics on gradient values. Pixels whose window
-// order is not exact take the per-pixel exact sort, as in the forward.
+// lists are recomputed (the forward stages run again), never stored across calls. Per block
+// (its sorted list split into the depth ranges of the forward, one wave each):
+//   pass A  (bwd_pass_a_kernel) walks every pixel through its range with the forward's
+//           ordering window, records the order in which samples leave the window (one byte
+//           per sample), and accumulates the range's partial composite in fp64; the
+//           work-group then chains the ranges per pixel (exactness across ranges, the
+//           saturation mask of the output, dL/d(pixel)) and stores each range's start state:
+//           T before it and the remainder R = out - C_before;
+//   pass B  (bwd_pass_b_kernel) replays the recorded order and per fragment forms
+//             dout/dc_i = a_i T_i,   dout/da_i = T_i c_i - R_{i+1} / (1 - a_i)
+//           (T and R in fp64, so pixels whose transmittance underflows fp32 stay exact),
+//           chains them through the sRGB lighting stage and the barycentric weights, and
+//           sums them per (block, entry) in LDS.
+// The (block, entry) rows are then reduced per triangle and per vertex by sorted segmented
+// sums (reduce.hip): no global atomics on gradient values. Pixels whose window order is not
+// exact take the per-pixel exact sort, as in the forward.
 
-__device__ __forceinline__ float srgb_to_linear_d(float x) {
-	return x < 0.04045f ? (1.0f / 12.92f)
-						: (2.4f / 1.055f) * glsl_pow((x + 0.055f) * (1.0f / 1.055f), 1.4f);
-}
-__device__ __forceinline__ float linear_to_srgb_d(float y) {
-	return y < 0.0031308f ? 12.92f : (1.055f / 2.4f) * glsl_pow(y, 1.0f / 2.4f - 1.0f);
-}
-
-// One sample of the backward walk.
+// One sample with what its gradient needs.
 struct BwdSample {
 	float r, g, b, a;  // shaded sample (fp32, as the fp32-sample forward)
 	float b0, b1;      // barycentrics: corner weights (1 - b0) - b1, b0, b1
 	float dr, dg, db;  // d(sample channel) / d(vertex colour channel) per unit corner weight
 	float da;          // d(sample alpha) / d(triangle alpha)
-	uint32_t e;        // entry index within the current chunk
-	__device__ __forceinline__ void zero() {
-		r = g = b = a = b0 = b1 = dr = dg = db = da = 0.0f;
-		e = 0;
-	}
-	static __device__ __forceinline__ BwdSample sel(bool p, const BwdSample &x,
-													const BwdSample &y) {
-		BwdSample o;
-		o.r = p ? x.r : y.r, o.g = p ? x.g : y.g, o.b = p ? x.b : y.b, o.a = p ? x.a : y.a;
-		o.b0 = p ? x.b0 : y.b0, o.b1 = p ? x.b1 : y.b1;
-		o.dr = p ? x.dr : y.dr, o.dg = p ? x.dg : y.dg, o.db = p ? x.db : y.db;
-		o.da = p ? x.da : y.da;
-		o.e = p ? x.e : y.e;
-		return o;
-	}
+	__device__ __forceinline__ void zero() { r = g = b = a = b0 = b1 = dr = dg = db = da = 0.0f; }
 };
 
-// The fp32-sample shading of shade_record with the derivatives of the lighting stage.
+// The fp32-sample shading of shade_record with the derivatives of the lighting stage,
+// s = sat(L2S(S2L(x) f)): S2L'(x) = 2.4 S2L(x) / (x + 0.055) above the linear segment, and
+// L2S'(y) = (L2S(y) + 0.055) / (2.4 y), so no further powers are evaluated.
 __device__ __forceinline__ bool shade_grad(const FrameConst &c, const uint4 *q, float px, float py,
-										   float inv_ray, uint32_t e, BwdSample &o) {
+										   float inv_ray, BwdSample &o) {
 	ShadeState st;
 	if(!shade_prepare(c, q, px, py, inv_ray, st)) {
 		o.zero();
@@ -1483,23 +1466,36 @@ __device__ __forceinline__ bool shade_grad(const FrameConst &c, const uint4 *q, 
 	float sv[3], dv[3];
 #pragma unroll
 	for(int k = 0; k < 3; k++) {
-		const float y = srgb_to_linear(x[k]) * st.f[k];
+		const float lin = srgb_to_linear(x[k]);
+		const float y = lin * st.f[k];
 		const float s = linear_to_srgb(y);
 		sv[k] = sat(s);
+		const float d_lin = x[k] < 0.04045f ? (1.0f / 12.92f) : 2.4f * lin / (x[k] + 0.055f);
+		const float d_srgb = y < 0.0031308f ? 12.92f : (s + 0.055f) / (2.4f * y);
 		// clamp passes the gradient on [0, 1] (as torch.clamp does)
-		dv[k] = (s >= 0.0f && s <= 1.0f)
-					? linear_to_srgb_d(y) * st.f[k] * srgb_to_linear_d(x[k]) * ic[k]
-					: 0.0f;
+		dv[k] = (s >= 0.0f && s <= 1.0f) ? d_srgb * st.f[k] * d_lin * ic[k] : 0.0f;
 	}
 	o.r = sv[0], o.g = sv[1], o.b = sv[2], o.a = st.color.w;
 	o.b0 = st.bary0, o.b1 = st.bary1;
 	o.dr = dv[0], o.dg = dv[1], o.db = dv[2];
 	o.da = st.inst.w;
-	o.e = e;
 	return true;
 }
 
-// Pass A accumulator: fp64 composite and the fp32 composite of the forward.
+// Pass A window payload: the fp32 sample and its entry within the slice.
+struct BwdColor {
+	float r, g, b, a;
+	uint32_t e;
+	__device__ __forceinline__ void zero() { r = g = b = a = 0.0f, e = 0; }
+	static __device__ __forceinline__ BwdColor sel(bool p, const BwdColor &x, const BwdColor &y) {
+		BwdColor o;
+		o.r = p ? x.r : y.r, o.g = p ? x.g : y.g, o.b = p ? x.b : y.b, o.a = p ? x.a : y.a;
+		o.e = p ? x.e : y.e;
+		return o;
+	}
+};
+
+// fp64 composite (C, T) plus the fp32 composite the forward computes (saturation mask).
 struct BwdTotals {
 	double t, cr, cg, cb;
 	Accum acc;
@@ -1507,18 +1503,15 @@ struct BwdTotals {
 		t = 1.0, cr = cg = cb = 0.0;
 		acc.init();
 	}
-	__device__ __forceinline__ void add(const BwdSample &s) {
-		const double a = s.a;
-		cr += (double)s.r * a * t;
-		cg += (double)s.g * a * t;
-		cb += (double)s.b * a * t;
-		t *= 1.0 - a;
-		acc.blend(make_float4(s.r, s.g, s.b, s.a), false);
+	__device__ __forceinline__ void add(float r, float g, float b, float a) {
+		const double da = a;
+		cr += (double)r * da * t;
+		cg += (double)g * da * t;
+		cb += (double)b * da * t;
+		t *= 1.0 - da;
+		acc.blend(make_float4(r, g, b, a), false);
 	}
 };
-
-// Rows of one chunk of a block's sorted list are summed in LDS.
-constexpr int kBwdChunk = 256;
 
 __device__ __forceinline__ void bwd_fragment(double &t, double &rr, double &rg, double &rb,
 											 float gr, float gg, float gb, const BwdSample &s,
@@ -1542,207 +1535,387 @@ __device__ __forceinline__ void bwd_fragment(double &t, double &rr, double &rg, 
 	rr = ar, rg = ag, rb = ab;
 }
 
-// Entries of one block in depth order: from the pre-sorted pool, or sorted here in
-// registers (lists of at most kRegSortCap) and kept in LDS.
+// Entries of one block in depth order (pre-sorted pool, or a register sort kept in LDS) and
+// this wave's depth range [e0, e1) of them.
 struct BlockEntries {
-	const uint32_t *tri;  // depth order
-	const uint64_t *mask; // depth order
+	const uint32_t *tri;
+	const uint64_t *mask;
+	uint32_t nw, e0, e1;
 };
 
-// One wave per block: passes A and B over the block's entries (chunks of kBwdChunk, the
-// ordering window flushed at chunk ends in both passes, so they see the same order).
-__global__ __launch_bounds__(64) void block_backward_kernel(FrameConst c, FrameBufs b, BwdBufs g) {
-	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
-		return;
-	__shared__ uint4 s_rec[kSliceEntries * kRecQuads];
-	__shared__ float s_grad[kBwdChunk * kGradWords];
-	__shared__ uint32_t s_tri_all[kRegSortCap];
-	__shared__ uint64_t s_mask_all[kRegSortCap];
+// Valid for the blocks the block kernels process (not empty, whole or promoted, m > 0).
+// For lists sorted here only wave 0 may call it (it fills s_tri_all / s_mask_all).
+__device__ __forceinline__ BlockEntries bwd_block_entries(const FrameConst &c, const FrameBufs &b,
+														  uint32_t slot, uint32_t flags, uint4 sl,
+														  int wave, uint32_t *s_tri_all,
+														  uint64_t *s_mask_all) {
 	const int lane = lane_id();
-	const uint32_t slot = blockIdx.x;
-	const int bin = slot >> 4, blk = slot & 15, by = blk >> 2, bx = blk & 3;
-	const uint32_t flags = b.blk_flags[slot];
-	const uint32_t level = flags & kSlotLevelMask;
-	if(level == LUCID_BIN_LEVEL_EMPTY || (flags & (kSlotWhole | kSlotPromoted)))
-		return;
-	const uint4 sl = b.blk_slot[slot];
 	const uint32_t m = sl.y;
-	if(m == 0)
-		return;
-	const int bin_x = (bin % c.bin_count_x) * kBinSize, bin_y = (bin / c.bin_count_x) * kBinSize;
-	const int px0 = bin_x + bx * 8, py0 = bin_y + by * 8;
-	const int px = px0 + (lane & 7), py = py0 + (lane >> 3);
-	const bool on_screen = px < c.width && py < c.height;
-	const float fx = (float)px, fy = (float)py;
-
+	const int bin = slot >> 4, blk = slot & 15, by = blk >> 2, bx = blk & 3;
+	const int px0 = (bin % c.bin_count_x) * kBinSize + bx * 8;
+	const int py0 = (bin / c.bin_count_x) * kBinSize + by * 8;
 	BlockEntries ent;
 	if(flags & kSlotPresorted) {
 		const uint32_t *hdr = b.sorted_pool + b.blk_sorted[slot];
 		ent.tri = hdr + kSortedHeader;
 		ent.mask = (const uint64_t *)(hdr + sorted_mask_offset(m));
-	} else {
-		const uint4 *recs =
-			(level == LUCID_BIN_LEVEL_LOW ? b.low_records : b.high_records) + sl.z;
-		const uint32_t *list = b.blk_lists + sl.x;
-		auto load = [&](uint32_t j) -> uint64_t {
-			return j < m ? block_key(b, recs[list[j]], bx, px0, py0, j) : ~0ull;
-		};
-		uint64_t v[4];
-		uint32_t P = next_pow2(m < 64 ? 64 : m);
-		if(P <= 64) {
-			uint64_t w1[1] = {load(lane)};
-			bitonic_regs<1>(w1);
-			v[0] = w1[0], v[1] = v[2] = v[3] = ~0ull;
-		} else if(P <= 128) {
-			uint64_t w2[2] = {load(lane), load(64 + lane)};
-			bitonic_regs<2>(w2);
-			v[0] = w2[0], v[1] = w2[1], v[2] = v[3] = ~0ull;
-		} else {
-			v[0] = load(lane), v[1] = load(64 + lane), v[2] = load(128 + lane),
-			v[3] = load(192 + lane);
-			bitonic_regs<4>(v);
-		}
-#pragma unroll
-		for(int j = 0; j < 4; j++)
-			if(j * 64 + lane < (int)m) {
-				uint4 rec = recs[list[(uint32_t)v[j]]];
-				s_mask_all[j * 64 + lane] = record_block_mask(rec, bx);
-				s_tri_all[j * 64 + lane] = rec.x;
-			}
-		wave_sync_lds();
-		ent.tri = s_tri_all;
-		ent.mask = s_mask_all;
+		ent.nw = hdr[0];
+		ent.e0 = (uint32_t)wave < ent.nw ? hdr[1 + wave] : m;
+		ent.e1 = (uint32_t)wave < ent.nw ? hdr[2 + wave] : m;
+		return ent;
 	}
-
-	// one pass over the block: visit(chunk0) before each chunk, then every emitted sample
-	auto run_pass = [&](auto &&before_chunk, auto &&after_chunk, auto &&emit) {
-		OrderWindow<BwdSample> w;
-		w.init();
-		for(uint32_t chunk0 = 0; chunk0 < m; chunk0 += kBwdChunk) {
-			const uint32_t chunk1 = min(m, chunk0 + (uint32_t)kBwdChunk);
-			before_chunk(chunk0, chunk1);
-			for(uint32_t s0 = chunk0; s0 < chunk1; s0 += kSliceEntries) {
-				const uint32_t n = min(chunk1 - s0, (uint32_t)kSliceEntries);
-				uint64_t msk[kSliceWords];
-				uint32_t t[kSliceWords];
-#pragma unroll
-				for(int k = 0; k < kSliceWords; k++) {
-					const uint32_t i = k * 64 + lane;
-					msk[k] = i < n ? ent.mask[s0 + i] : 0ull;
-					t[k] = i < n ? ent.tri[s0 + i] : 0u;
-				}
-				stage_slice(c, b, t, n, s_rec);
-				wave_sync_lds();
-				uint64_t bm[kSliceWords];
-				transpose_masks(msk, (int)((n + 63) / 64), bm);
-				const uint32_t e_base = s0 - chunk0;
-				walk_bits(bm, [&](int e) {
-					const uint4 *rec = s_rec + e * kRecQuads;
-					const float inv = record_inv_depth(rec[0], fx, fy);
-					BwdSample smp;
-					if(shade_grad(c, rec, fx, fy, inv, e_base + (uint32_t)e, smp))
-						w.insert(pixel_key(inv, rec[6].z), smp, emit);
-				});
-				wave_sync_lds();
-			}
-			w.flush(emit);
-			after_chunk(chunk0, chunk1);
-		}
-		return w.bad;
+	const uint32_t level = flags & kSlotLevelMask;
+	const uint4 *recs = (level == LUCID_BIN_LEVEL_LOW ? b.low_records : b.high_records) + sl.z;
+	const uint32_t *list = b.blk_lists + sl.x;
+	auto load = [&](uint32_t j) -> uint64_t {
+		return j < m ? block_key(b, recs[list[j]], bx, px0, py0, j) : ~0ull;
 	};
-	auto nothing = [](uint32_t, uint32_t) {};
+	uint64_t v[4];
+	uint32_t P = next_pow2(m < 64 ? 64 : m);
+	if(P <= 64) {
+		uint64_t w1[1] = {load(lane)};
+		bitonic_regs<1>(w1);
+		v[0] = w1[0], v[1] = v[2] = v[3] = ~0ull;
+	} else if(P <= 128) {
+		uint64_t w2[2] = {load(lane), load(64 + lane)};
+		bitonic_regs<2>(w2);
+		v[0] = w2[0], v[1] = w2[1], v[2] = v[3] = ~0ull;
+	} else {
+		v[0] = load(lane), v[1] = load(64 + lane), v[2] = load(128 + lane), v[3] = load(192 + lane);
+		bitonic_regs<4>(v);
+	}
+#pragma unroll
+	for(int j = 0; j < 4; j++)
+		if(j * 64 + lane < (int)m) {
+			uint4 rec = recs[list[(uint32_t)v[j]]];
+			s_mask_all[j * 64 + lane] = record_block_mask(rec, bx);
+			s_tri_all[j * 64 + lane] = rec.x;
+		}
+	wave_sync_lds();
+	ent.tri = s_tri_all;
+	ent.mask = s_mask_all;
+	ent.nw = 1, ent.e0 = 0, ent.e1 = m;
+	return ent;
+}
 
-	// pass A
-	BwdTotals tot;
-	tot.init();
-	const bool bad = run_pass(nothing, nothing, [&](const BwdSample &s) { tot.add(s); });
-	const bool exact = on_screen && !bad;
-	enqueue_pixels(b, slot, on_screen && bad, px, py, false);
+// Loads and stages slice [s0, s0 + n) of the block's entries; returns the coverage masks.
+__device__ __forceinline__ void bwd_stage(const FrameConst &c, const FrameBufs &b,
+										  const BlockEntries &ent, uint32_t s0, uint32_t n,
+										  uint4 *s_rec, uint64_t (&msk)[kSliceWords]) {
+	const int lane = lane_id();
+	uint32_t t[kSliceWords];
+#pragma unroll
+	for(int k = 0; k < kSliceWords; k++) {
+		const uint32_t i = k * 64 + lane;
+		msk[k] = i < n ? ent.mask[s0 + i] : 0ull;
+		t[k] = i < n ? ent.tri[s0 + i] : 0u;
+	}
+	stage_slice(c, b, t, n, s_rec);
+	wave_sync_lds();
+}
+
+constexpr uint8_t kSeqEnd = 0xff; // end of a pixel's recorded order within a slice
+static_assert(kSliceEntries <= 255, "recorded entries are bytes");
+
+// Byte of the j-th sample that leaves pixel `lane`'s window in the slice starting at sorted
+// position s0 of a block whose gradient rows start at row0.
+__device__ __forceinline__ size_t seq_index(uint32_t row0, uint32_t s0, uint32_t j, int lane) {
+	return ((size_t)row0 + s0 + j) * 64 + (size_t)lane;
+}
+
+__global__ __launch_bounds__(kBigWaves * 64) void bwd_pass_a_kernel(FrameConst c, FrameBufs b,
+																	 BwdBufs g) {
+	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
+		return;
+	__shared__ uint4 s_rec[kBigWaves][kSliceEntries * kRecQuads];
+	__shared__ uint32_t s_tri_all[kRegSortCap];
+	__shared__ uint64_t s_mask_all[kRegSortCap];
+	__shared__ double s_pc[kBigWaves][4][64]; // C rgb, T per range and pixel
+	__shared__ float4 s_p32[kBigWaves][64];   // fp32 partial composite
+	__shared__ uint64_t s_first[kBigWaves][64], s_last[kBigWaves][64];
+	__shared__ uint32_t s_bad[kBigWaves][64];
+	const int lane = lane_id(), wave = threadIdx.x >> 6;
+	const uint32_t slot = blockIdx.x;
+	const uint32_t flags = b.blk_flags[slot];
+	const uint32_t level = flags & kSlotLevelMask;
+	if(level == LUCID_BIN_LEVEL_EMPTY || (flags & (kSlotWhole | kSlotPromoted)))
+		return;
+	const uint4 sl = b.blk_slot[slot];
+	if(sl.y == 0)
+		return;
+	const bool presorted = (flags & kSlotPresorted) != 0;
+	if(!presorted && wave != 0)
+		return; // one range: wave 0 alone (no work-group barrier below on this path)
+	const int bin = slot >> 4, blk = slot & 15, by = blk >> 2, bx = blk & 3;
+	const int px = (bin % c.bin_count_x) * kBinSize + bx * 8 + (lane & 7);
+	const int py = (bin / c.bin_count_x) * kBinSize + by * 8 + (lane >> 3);
+	const bool on_screen = px < c.width && py < c.height;
+	const float fx = (float)px, fy = (float)py;
+	const BlockEntries ent = bwd_block_entries(c, b, slot, flags, sl, wave, s_tri_all, s_mask_all);
+	const uint32_t row0 = g.row_base[slot];
+
+	if((uint32_t)wave < ent.nw) {
+		OrderWindow<BwdColor> w;
+		w.init();
+		BwdTotals part;
+		part.init();
+		for(uint32_t s0 = ent.e0; s0 < ent.e1; s0 += kSliceEntries) {
+			const uint32_t n = min(ent.e1 - s0, (uint32_t)kSliceEntries);
+			uint64_t msk[kSliceWords];
+			bwd_stage(c, b, ent, s0, n, s_rec[wave], msk);
+			uint64_t bm[kSliceWords];
+			transpose_masks(msk, (int)((n + 63) / 64), bm);
+			uint32_t j = 0;
+			auto emit = [&](const BwdColor &s) {
+				part.add(s.r, s.g, s.b, s.a);
+				g.seq[seq_index(row0, s0, j, lane)] = (uint8_t)s.e;
+				j++;
+			};
+			walk_bits(bm, [&](int e) {
+				const uint4 *rec = s_rec[wave] + e * kRecQuads;
+				const float inv = record_inv_depth(rec[0], fx, fy);
+				const SampleColor<true> col = shade_record<true>(c, rec, fx, fy, inv);
+				if(col.contributes()) {
+					BwdColor bc;
+					bc.r = col.col.x, bc.g = col.col.y, bc.b = col.col.z, bc.a = col.col.w;
+					bc.e = (uint32_t)e;
+					w.insert(pixel_key(inv, rec[6].z), bc, emit);
+				}
+			});
+			// the window empties at every slice end, so pass B only needs this slice staged
+			w.flush(emit);
+			if(j < n)
+				g.seq[seq_index(row0, s0, j, lane)] = kSeqEnd;
+			wave_sync_lds();
+		}
+		s_pc[wave][0][lane] = part.cr, s_pc[wave][1][lane] = part.cg;
+		s_pc[wave][2][lane] = part.cb, s_pc[wave][3][lane] = part.t;
+		s_p32[wave][lane] = make_float4(part.acc.cr, part.acc.cg, part.acc.cb, part.acc.tr);
+		s_first[wave][lane] = w.first, s_last[wave][lane] = w.last;
+		s_bad[wave][lane] = w.bad ? 1u : 0u;
+	}
+	if(presorted)
+		__syncthreads();
+	else
+		wave_sync_lds();
+	if((uint32_t)wave >= ent.nw)
+		return;
+
+	// chain the ranges front to back (each wave for its own start state)
 	const float bg[3] = {c.background[0], c.background[1], c.background[2]};
-	const float o32[3] = {tot.acc.cr + tot.acc.tr * bg[0], tot.acc.cg + tot.acc.tr * bg[1],
-						  tot.acc.cb + tot.acc.tr * bg[2]};
+	double ct[3] = {0.0, 0.0, 0.0}, tt = 1.0;   // composite of all ranges
+	double cb4[3] = {0.0, 0.0, 0.0}, tb4 = 1.0; // composite of the ranges before this one
+	Accum a32;
+	a32.init();
+	uint64_t last = 0;
+	bool bad = false;
+	for(uint32_t v = 0; v < ent.nw; v++) {
+		if(v == (uint32_t)wave)
+			cb4[0] = ct[0], cb4[1] = ct[1], cb4[2] = ct[2], tb4 = tt;
+		bad |= s_bad[v][lane] != 0;
+		const uint64_t f = s_first[v][lane];
+		if(f != kEmptyKey) {
+			bad |= f <= last;
+			last = s_last[v][lane];
+		}
+		for(int k = 0; k < 3; k++)
+			ct[k] = ct[k] + tt * s_pc[v][k][lane];
+		tt = tt * s_pc[v][3][lane];
+		const float4 p = s_p32[v][lane];
+		a32.cr = a32.cr + a32.tr * p.x;
+		a32.cg = a32.cg + a32.tr * p.y;
+		a32.cb = a32.cb + a32.tr * p.z;
+		a32.tr = a32.tr * p.w;
+	}
+	if(wave == 0)
+		enqueue_pixels(b, slot, on_screen && bad, px, py, false);
+	const float o32[3] = {a32.cr + a32.tr * bg[0], a32.cg + a32.tr * bg[1],
+						  a32.cb + a32.tr * bg[2]};
 	float gsel[3] = {0.0f, 0.0f, 0.0f};
-	if(exact) {
+	if(on_screen && !bad) {
 		const float4 gi = g.grad_image[(size_t)py * c.width + px];
 		const float gv[3] = {gi.x, gi.y, gi.z};
 #pragma unroll
 		for(int k = 0; k < 3; k++)
 			gsel[k] = (o32[k] >= 0.0f && o32[k] <= 1.0f) ? gv[k] : 0.0f;
 	}
-
-	// pass B
-	double t = 1.0;
-	double rr = tot.cr + tot.t * (double)bg[0], rg = tot.cg + tot.t * (double)bg[1],
-		   rb = tot.cb + tot.t * (double)bg[2];
-	const uint32_t row0 = g.row_base[slot];
-	auto zero_grad = [&](uint32_t, uint32_t) {
-		for(int i = lane; i < kBwdChunk * kGradWords; i += 64)
-			s_grad[i] = 0.0f;
-		wave_sync_lds();
-	};
-	auto write_rows = [&](uint32_t chunk0, uint32_t chunk1) {
-		wave_sync_lds();
-		for(uint32_t e = lane; e < chunk1 - chunk0; e += 64) {
-			const size_t row = (size_t)row0 + chunk0 + e;
-			if(row < g.rows_cap) {
-				g.row_tri[row] = ent.tri[chunk0 + e];
+	// max |dL/dpixel| of the frame: fixes pass B's fixed-point scale
+	float gm = fmaxf(fmaxf(fabsf(gsel[0]), fabsf(gsel[1])), fabsf(gsel[2]));
 #pragma unroll
-				for(int k = 0; k < kGradWords; k++)
-					g.rows[row * kGradWords + k] = s_grad[e * kGradWords + k];
+	for(int d = 1; d < 64; d <<= 1)
+		gm = fmaxf(gm, __shfl_xor(gm, d, 64));
+	if(lane == 0 && gm > 0.0f)
+		atomicMax(&b.ctrl[kCtrlBwdGradMax], f2bits(gm));
+	BwdStart st;
+	st.t = tb4;
+	st.rr = (ct[0] + tt * (double)bg[0]) - cb4[0];
+	st.rg = (ct[1] + tt * (double)bg[1]) - cb4[1];
+	st.rb = (ct[2] + tt * (double)bg[2]) - cb4[2];
+	st.gr = gsel[0], st.gg = gsel[1], st.gb = gsel[2];
+	st.pad = 0.0f;
+	g.start[((size_t)g.range_base[slot] + wave) * 64 + lane] = st;
+}
+
+// Pass B sums gradient rows in LDS as 64-bit fixed point (integer LDS atomics; integer sums
+// are associative, so a row's value does not depend on the order its pixels add in). The
+// scale is a power of two fixed per frame from max |dL/dpixel| and a bound on one
+// fragment's contribution (|d sample / d colour| <= 12.92 * 2.4 * lighting, |d out / d a|
+// <= 2 per channel), such that a row of at most 64 contributions cannot overflow; the
+// resolution is then ~1e-12 of the largest possible contribution.
+constexpr int kGradStride = kGradWords + 1; // odd stride against bank conflicts
+
+__device__ __forceinline__ double grad_fixed_scale(const FrameConst &c, float gmax) {
+	float lmax = 0.0f; // bound on the lighting factor (light in [0, 1])
+#pragma unroll
+	for(int k = 0; k < 3; k++)
+		lmax = fmaxf(lmax, fabsf(c.ambient[k] * c.ambient_power) +
+							   fabsf(c.sun_color[k] * c.sun_power));
+	const double bound = 64.0 * (double)gmax * fmax(32.0 * (double)lmax, 8.0);
+	int ex = 0;
+	(void)frexp(bound, &ex); // bound < 2^ex
+	int k = 61 - ex;
+	k = k < 0 ? 0 : k > 1000 ? 1000 : k;
+	return ldexp(1.0, k);
+}
+
+// One wave per (block, depth range); single-wave work-groups (no LDS is shared between
+// waves), so that more of them fit on a CU.
+__global__ __launch_bounds__(64) void bwd_pass_b_kernel(FrameConst c, FrameBufs b, BwdBufs g) {
+	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
+		return;
+	__shared__ uint4 s_rec[kSliceEntries * kRecQuads];
+	__shared__ unsigned long long s_grad[kSliceEntries * kGradStride];
+	__shared__ uint4 s_seq[kSliceEntries * 64 / 16]; // the slice's recorded order
+	__shared__ uint32_t s_tri_all[kRegSortCap];
+	__shared__ uint64_t s_mask_all[kRegSortCap];
+	const int lane = lane_id();
+	const uint32_t slot = blockIdx.x / kBigWaves;
+	const int wave = (int)(blockIdx.x % kBigWaves);
+	const uint32_t flags = b.blk_flags[slot];
+	const uint32_t level = flags & kSlotLevelMask;
+	if(level == LUCID_BIN_LEVEL_EMPTY || (flags & (kSlotWhole | kSlotPromoted)))
+		return;
+	const uint4 sl = b.blk_slot[slot];
+	if(sl.y == 0)
+		return;
+	if(!(flags & kSlotPresorted) && wave != 0)
+		return;
+	if(flags & kSlotPresorted) {
+		const uint32_t nw = b.sorted_pool[b.blk_sorted[slot]];
+		if((uint32_t)wave >= nw)
+			return;
+	}
+	const int bin = slot >> 4, blk = slot & 15, by = blk >> 2, bx = blk & 3;
+	const int px = (bin % c.bin_count_x) * kBinSize + bx * 8 + (lane & 7);
+	const int py = (bin / c.bin_count_x) * kBinSize + by * 8 + (lane >> 3);
+	const float fx = (float)px, fy = (float)py;
+	const BlockEntries ent = bwd_block_entries(c, b, slot, flags, sl, wave, s_tri_all, s_mask_all);
+	const uint32_t row0 = g.row_base[slot];
+	const BwdStart st = g.start[((size_t)g.range_base[slot] + wave) * 64 + lane];
+	double t = st.t, rr = st.rr, rg = st.rg, rb = st.rb;
+	const bool live = st.gr != 0.0f || st.gg != 0.0f || st.gb != 0.0f;
+	const double scale = grad_fixed_scale(c, bits2f(b.ctrl[kCtrlBwdGradMax]));
+	const double inv_scale = 1.0 / scale;
+	for(uint32_t s0 = ent.e0; s0 < ent.e1; s0 += kSliceEntries) {
+		const uint32_t n = min(ent.e1 - s0, (uint32_t)kSliceEntries);
+		uint64_t msk[kSliceWords];
+		bwd_stage(c, b, ent, s0, n, s_rec, msk);
+		for(int i = lane; i < kSliceEntries * kGradStride; i += 64)
+			s_grad[i] = 0ull;
+		// the slice's recorded order (n * 64 bytes, 64-byte aligned) into LDS in one copy
+		const uint4 *src = (const uint4 *)(g.seq + seq_index(row0, s0, 0, 0));
+		for(uint32_t i = lane; i < n * 4; i += 64)
+			s_seq[i] = src[i];
+		wave_sync_lds();
+		const uint8_t *seq = (const uint8_t *)s_seq;
+		// replay the order recorded by pass A
+		bool more = true;
+		for(uint32_t j = 0; j < n; j++) {
+			const uint8_t e = more ? seq[j * 64 + lane] : kSeqEnd;
+			more = e != kSeqEnd;
+			if(!ballot(more))
+				break;
+			if(more) {
+				const uint4 *rec = s_rec + (uint32_t)e * kRecQuads;
+				const float inv = record_inv_depth(rec[0], fx, fy);
+				BwdSample smp;
+				shade_grad(c, rec, fx, fy, inv, smp);
+				float out[kGradWords];
+				bwd_fragment(t, rr, rg, rb, st.gr, st.gg, st.gb, smp, out);
+				if(live) {
+#pragma unroll
+					for(int k = 0; k < kGradWords; k++)
+						atomicAdd(&s_grad[(uint32_t)e * kGradStride + k],
+								  (unsigned long long)__double2ll_rn((double)out[k] * scale));
+				}
 			}
 		}
 		wave_sync_lds();
-	};
-	run_pass(zero_grad, write_rows, [&](const BwdSample &s) {
-		float out[kGradWords];
-		bwd_fragment(t, rr, rg, rb, gsel[0], gsel[1], gsel[2], s, out);
-		if(gsel[0] != 0.0f || gsel[1] != 0.0f || gsel[2] != 0.0f) {
+		for(uint32_t e = lane; e < n; e += 64) {
+			const size_t row = (size_t)row0 + s0 + e;
+			if(row < g.rows_cap) {
+				g.row_tri[row] = ent.tri[s0 + e];
 #pragma unroll
-			for(int k = 0; k < kGradWords; k++)
-				atomicAdd(&s_grad[s.e * kGradWords + k], out[k]);
+				for(int k = 0; k < kGradWords; k++)
+					g.rows[row * kGradWords + k] =
+						(float)((double)(long long)s_grad[e * kGradStride + k] * inv_scale);
+			}
 		}
-	});
+		wave_sync_lds();
+	}
 }
 
-// Exclusive scan of the per-block gradient row counts (blocks the block kernel processes).
+// Exclusive scans over the blocks the block kernels process: gradient rows (one per list
+// entry) and depth ranges (start states of pass B).
 __global__ __launch_bounds__(1024) void grad_row_scan_kernel(FrameConst c, FrameBufs b, BwdBufs g) {
 	const uint32_t n = (uint32_t)c.bin_count * 16;
-	__shared__ uint32_t s_wave[16];
-	__shared__ uint32_t s_carry;
-	if(threadIdx.x == 0)
-		s_carry = 0;
+	__shared__ uint32_t s_wave[2][16];
+	__shared__ uint32_t s_carry[2];
+	if(threadIdx.x < 2)
+		s_carry[threadIdx.x] = 0;
 	__syncthreads();
 	const int wave = threadIdx.x >> 6;
 	for(uint32_t base = 0; base < n; base += 1024) {
 		const uint32_t slot = base + threadIdx.x;
-		uint32_t v = 0;
+		uint32_t v[2] = {0u, 0u};
 		if(slot < n) {
 			const uint32_t flags = b.blk_flags[slot];
 			const uint32_t level = flags & kSlotLevelMask;
-			if(level != LUCID_BIN_LEVEL_EMPTY && !(flags & (kSlotWhole | kSlotPromoted)))
-				v = b.blk_slot[slot].y;
+			const uint32_t m = b.blk_slot[slot].y;
+			if(level != LUCID_BIN_LEVEL_EMPTY && !(flags & (kSlotWhole | kSlotPromoted)) && m) {
+				v[0] = m;
+				v[1] = (flags & kSlotPresorted) ? b.sorted_pool[b.blk_sorted[slot]] : 1u;
+			}
 		}
-		const uint32_t inc = wave_inclusive_add(v);
-		if(lane_id() == 63)
-			s_wave[wave] = inc;
+		uint32_t inc[2];
+		for(int k = 0; k < 2; k++) {
+			inc[k] = wave_inclusive_add(v[k]);
+			if(lane_id() == 63)
+				s_wave[k][wave] = inc[k];
+		}
 		__syncthreads();
-		if(threadIdx.x == 0) {
-			uint32_t run = s_carry;
+		if(threadIdx.x < 2) {
+			const int k = threadIdx.x;
+			uint32_t run = s_carry[k];
 			for(int w = 0; w < 16; w++) {
-				uint32_t x = s_wave[w];
-				s_wave[w] = run;
+				uint32_t x = s_wave[k][w];
+				s_wave[k][w] = run;
 				run += x;
 			}
-			s_carry = run;
+			s_carry[k] = run;
 		}
 		__syncthreads();
-		if(slot < n)
-			g.row_base[slot] = s_wave[wave] + inc - v;
+		if(slot < n) {
+			g.row_base[slot] = s_wave[0][wave] + inc[0] - v[0];
+			g.range_base[slot] = s_wave[1][wave] + inc[1] - v[1];
+		}
 		__syncthreads();
 	}
 	if(threadIdx.x == 0) {
-		b.ctrl[kCtrlBwdRows] = s_carry;
+		b.ctrl[kCtrlBwdRows] = s_carry[0];
+		b.ctrl[kCtrlBwdRanges] = s_carry[1];
 		b.ctrl[kCtrlBwdExtra] = 0;
+		b.ctrl[kCtrlBwdGradMax] = 0;
 	}
 }
 
@@ -1803,7 +1976,7 @@ __device__ void bwd_fallback_pixel(const FrameConst &c, const FrameBufs &b, cons
 		const uint32_t t = (uint32_t)key_at(i);
 		uint4 q[kRecQuads];
 		load_shading_record(c, b, t, q);
-		return shade_grad(c, q, fx, fy, record_inv_depth(q[0], fx, fy), 0u, smp);
+		return shade_grad(c, q, fx, fy, record_inv_depth(q[0], fx, fy), smp);
 	};
 	// pass A (serial over the sorted list, 64 samples shaded at a time)
 	BwdTotals tot;
@@ -1818,7 +1991,7 @@ __device__ void bwd_fallback_pixel(const FrameConst &c, const FrameBufs &b, cons
 			s.r = __shfl(smp.r, (int)j, 64), s.g = __shfl(smp.g, (int)j, 64);
 			s.b = __shfl(smp.b, (int)j, 64), s.a = __shfl(smp.a, (int)j, 64);
 			if(__shfl((int)ok, (int)j, 64))
-				tot.add(s);
+				tot.add(s.r, s.g, s.b, s.a);
 		}
 	}
 	const float bg[3] = {c.background[0], c.background[1], c.background[2]};
@@ -1944,7 +2117,10 @@ hipError_t launch_backward_prepare(const FrameConst &c, const FrameBufs &b, cons
 // Backward, second part: per-block adjoints and the exact per-pixel adjoints.
 hipError_t launch_backward_blocks(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
 								  hipStream_t stream) {
-	hipLaunchKernelGGL(block_backward_kernel, dim3(c.bin_count * 16), dim3(64), 0, stream, c, b, g);
+	hipLaunchKernelGGL(bwd_pass_a_kernel, dim3(c.bin_count * 16), dim3(kBigWaves * 64), 0, stream,
+					   c, b, g);
+	hipLaunchKernelGGL(bwd_pass_b_kernel, dim3(c.bin_count * 16 * kBigWaves), dim3(64), 0, stream, c,
+					   b, g);
 	hipLaunchKernelGGL(bwd_fallback_kernel, dim3(1024), dim3(kFallbackThreads), 0, stream, c, b, g);
 	return hipGetLastError();
 }
