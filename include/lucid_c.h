@@ -54,6 +54,17 @@ extern "C" {
 #define LUCID_OPT_FP32_SAMPLES (1u << 16)
 
 typedef struct lucid_renderer lucid_renderer;
+typedef struct lucid_batch lucid_batch;
+
+/* Performance parameters of the block rasteriser. Coverage, bins, lists and fragment
+ * counts do not depend on them; colours only through the rounding of partial composites
+ * (blocks split into depth ranges). The bin size (32) and block size (8) are the
+ * reference's and are not tunable: they decide which row starts each scanline. */
+typedef struct lucid_tuning {
+	int slice_words;      /* entries staged in LDS per step: 64 * slice_words (1 or 2) */
+	int presort_min;      /* high-bin blocks with more entries are pre-sorted (64..256) */
+	int samples_per_wave; /* target samples per wave of a pre-sorted block (256..65536) */
+} lucid_tuning;
 
 typedef struct lucid_create_info {
 	int view_width, view_height; /* at most 2560 x 2048 (src/lucid_renderer.h:27) */
@@ -102,6 +113,27 @@ int lucid_render(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_fr
  * extension of the drop-in renderer. */
 int lucid_backward(lucid_renderer *r, const lucid_frame_inputs *in, const float *grad_image,
 				   float *grad_colors_rgb, float *grad_tri_alpha, void *stream);
+
+/* Tuning of a renderer (defaults 1, 256, 2048). No counterpart in the reference. */
+int lucid_set_tuning(lucid_renderer *r, const lucid_tuning *t);
+int lucid_get_tuning(const lucid_renderer *r, lucid_tuning *t);
+/* Times every tuning candidate on this frame (`iters` timed frames each, HIP events on
+ * `stream`), keeps the fastest in the renderer and returns it in *best (may be NULL).
+ * Synchronous. No counterpart in the reference (its parameters are shader constants). */
+int lucid_autotune(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_frame_outputs *out,
+				   void *stream, int iters, lucid_tuning *best);
+
+/* Batches: one frame of each of n renderers (distinct, same device, created without
+ * LUCID_OPT_TIMERS) recorded into one hipGraph, frames spread over forked streams, and
+ * launched with a single hipGraphLaunch on `stream`. The graph is recorded again only when
+ * a frame's constants or buffers change (new sizes, grown capacities). lucid_batch_finish
+ * waits and returns LUCID_ERR_CAPACITY if any frame overflowed (render the batch again).
+ * No counterpart in the reference (one scene per frame there). */
+int lucid_batch_create(lucid_renderer *const *renderers, int n, lucid_batch **out);
+void lucid_batch_destroy(lucid_batch *b);
+int lucid_batch_render(lucid_batch *b, const lucid_frame_inputs *ins,
+					   const lucid_frame_outputs *outs, void *stream);
+int lucid_batch_finish(lucid_batch *b);
 
 /* Waits for the last frame. Returns LUCID_ERR_CAPACITY if it overflowed an internal buffer
  * (the buffers have then been grown and the frame must be rendered again). */

@@ -34,6 +34,8 @@ EXPORTED = (
     "lucid_error_string", "lucid_create", "lucid_destroy", "lucid_render", "lucid_finish",
     "lucid_read_info", "lucid_verify_info", "lucid_read_bin_lists", "lucid_read_setup",
     "lucid_stage_times", "lucid_backward",
+    "lucid_set_tuning", "lucid_get_tuning", "lucid_autotune",
+    "lucid_batch_create", "lucid_batch_destroy", "lucid_batch_render", "lucid_batch_finish",
     "lucid_bin_count", "lucid_bin_size", "lucid_block_size", "lucid_max_visible_quads",
     "lucid_max_scene_quads",
 )
@@ -44,6 +46,15 @@ class CreateInfo(ctypes.Structure):
         ("view_width", ctypes.c_int), ("view_height", ctypes.c_int),
         ("opts", ctypes.c_uint32), ("device", ctypes.c_int), ("max_quads", ctypes.c_int64),
     ]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("slice_words", ctypes.c_int), ("presort_min", ctypes.c_int),
+                ("samples_per_wave", ctypes.c_int)]
+
+    def as_dict(self):
+        return {"slice_words": self.slice_words, "presort_min": self.presort_min,
+                "samples_per_wave": self.samples_per_wave}
 
 
 class FrameInputs(ctypes.Structure):
@@ -113,6 +124,22 @@ def load(path: str = LIB_PATH):
     lib.lucid_stage_times.argtypes = [vp, vp, i32]
     lib.lucid_backward.restype = i32
     lib.lucid_backward.argtypes = [vp, ctypes.POINTER(FrameInputs), vp, vp, vp, vp]
+    lib.lucid_set_tuning.restype = i32
+    lib.lucid_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+    lib.lucid_get_tuning.restype = i32
+    lib.lucid_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+    lib.lucid_autotune.restype = i32
+    lib.lucid_autotune.argtypes = [vp, ctypes.POINTER(FrameInputs), ctypes.POINTER(FrameOutputs),
+                                   vp, i32, ctypes.POINTER(Tuning)]
+    lib.lucid_batch_create.restype = i32
+    lib.lucid_batch_create.argtypes = [ctypes.POINTER(vp), i32, ctypes.POINTER(vp)]
+    lib.lucid_batch_destroy.restype = None
+    lib.lucid_batch_destroy.argtypes = [vp]
+    lib.lucid_batch_render.restype = i32
+    lib.lucid_batch_render.argtypes = [vp, ctypes.POINTER(FrameInputs), ctypes.POINTER(FrameOutputs),
+                                       vp]
+    lib.lucid_batch_finish.restype = i32
+    lib.lucid_batch_finish.argtypes = [vp]
     for name in ("lucid_bin_count", "lucid_bin_size", "lucid_block_size"):
         getattr(lib, name).restype = i32
         getattr(lib, name).argtypes = [vp]

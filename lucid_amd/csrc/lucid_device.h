@@ -87,6 +87,10 @@ struct FrameConst {
 	int num_bin_wgs; // work-groups of the bin counter/dispatcher
 	uint32_t opts;
 	uint32_t backface;
+	// tuning (results do not depend on it)
+	uint32_t presort_min;      // high-bin blocks with more entries are pre-sorted (<= kRegSortCap)
+	uint32_t samples_per_wave; // target samples per wave of a pre-sorted block
+	int slice_words;           // block rasteriser slice width: 1 or 2 words of 64 entries
 	int color_mode;
 	int has_normals;
 	float vp[16]; // column-major view-projection matrix

@@ -78,6 +78,7 @@ struct lucid_renderer {
 	int device = 0;
 	int width = 0, height = 0;
 	uint32_t opts = 0;
+	lucid_tuning tuning = {1, 256, 2048};
 	int bin_count_x = 0, bin_count_y = 0, bin_count = 0;
 
 	DevBuf info, ctrl;
@@ -221,6 +222,9 @@ void fill_const(const lucid_renderer *r, const lucid_config &cfg, int64_t nq, in
 	c.num_bin_wgs = kNumBinWgs;
 	c.opts = r->opts;
 	c.backface = cfg.enable_backface_culling;
+	c.presort_min = (uint32_t)r->tuning.presort_min;
+	c.samples_per_wave = (uint32_t)r->tuning.samples_per_wave;
+	c.slice_words = r->tuning.slice_words;
 	c.color_mode = color_mode;
 	c.has_normals = has_normals ? 1 : 0;
 	for(int col = 0; col < 4; col++) {
@@ -485,6 +489,7 @@ int lucid_render(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_fr
 	int pe = prepare_frame(r, in, out, c, b, nq);
 	if(pe != LUCID_OK)
 		return pe;
+	(void)hipGetLastError(); // launches below report their own status, not an older call's
 
 	uint32_t n_info = LUCID_INFO_U32_SIZE + 10u * (uint32_t)r->bin_count;
 	if(r->timers)
@@ -606,6 +611,268 @@ int lucid_backward(lucid_renderer *r, const lucid_frame_inputs *in, const float 
 								 grad_colors_rgb, grad_tri_alpha, stream));
 	LUCID_CHECK(hipStreamSynchronize(stream));
 	return LUCID_OK;
+}
+
+int lucid_set_tuning(lucid_renderer *r, const lucid_tuning *t) {
+	if(!r || !t || (t->slice_words != 1 && t->slice_words != 2) || t->presort_min < 64 ||
+	   t->presort_min > kRegSortCap || t->samples_per_wave < 256 || t->samples_per_wave > 65536)
+		return LUCID_ERR_INVALID_ARG;
+	r->tuning = *t;
+	return LUCID_OK;
+}
+
+int lucid_get_tuning(const lucid_renderer *r, lucid_tuning *t) {
+	if(!r || !t)
+		return LUCID_ERR_INVALID_ARG;
+	*t = r->tuning;
+	return LUCID_OK;
+}
+
+int lucid_autotune(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_frame_outputs *out,
+				   void *stream_ptr, int iters, lucid_tuning *best_out) {
+	if(!r || !in || !out || iters < 1)
+		return LUCID_ERR_INVALID_ARG;
+	hipStream_t stream = (hipStream_t)stream_ptr;
+	LUCID_CHECK(hipSetDevice(r->device));
+	const lucid_tuning saved = r->tuning;
+	// one frame at the current tuning settles capacities and uploads
+	for(int k = 0; k < 4; k++) {
+		int e = lucid_render(r, in, out, stream);
+		if(e != LUCID_OK)
+			return e;
+		e = lucid_finish(r);
+		if(e == LUCID_OK)
+			break;
+		if(e != LUCID_ERR_CAPACITY)
+			return e;
+	}
+	hipEvent_t e0, e1;
+	LUCID_CHECK(hipEventCreate(&e0));
+	LUCID_CHECK(hipEventCreate(&e1));
+	const int slice_words[] = {1, 2};
+	const int presort[] = {64, 128, 256};
+	const int spw[] = {1024, 2048, 4096};
+	lucid_tuning best = saved;
+	float best_ms = 1e30f;
+	int rc = LUCID_OK;
+	for(int sw : slice_words)
+		for(int pm : presort)
+			for(int sp : spw) {
+				r->tuning = {sw, pm, sp};
+				int e = LUCID_OK;
+				for(int k = 0; k <= iters && e == LUCID_OK; k++) { // first frame untimed
+					if(k == 1)
+						e = hip_status(hipEventRecord(e0, stream));
+					if(e == LUCID_OK)
+						e = lucid_render(r, in, out, stream);
+				}
+				if(e == LUCID_OK)
+					e = hip_status(hipEventRecord(e1, stream));
+				int f = lucid_finish(r);
+				if(e != LUCID_OK || f != LUCID_OK) {
+					rc = e != LUCID_OK ? e : f;
+					if(rc == LUCID_ERR_CAPACITY)
+						continue; // buffers grew: this candidate is skipped, the others still run
+					break;
+				}
+				float ms = 0.0f;
+				if(hipEventSynchronize(e1) == hipSuccess &&
+				   hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < best_ms)
+					best_ms = ms, best = r->tuning;
+			}
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	r->tuning = best_ms < 1e30f ? best : saved;
+	if(best_out)
+		*best_out = r->tuning;
+	return best_ms < 1e30f ? LUCID_OK : (rc != LUCID_OK ? rc : LUCID_ERR_HIP);
+}
+
+// ------------------------------------------------------------------ batches
+
+struct lucid_batch {
+	int device = 0;
+	std::vector<lucid_renderer *> rs;
+	std::vector<FrameConst> cs;
+	std::vector<FrameBufs> bs;
+	std::vector<hipStream_t> streams;
+	std::vector<hipEvent_t> join;
+	hipEvent_t fork = nullptr, done = nullptr;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	bool pending = false;
+	~lucid_batch() {
+		if(exec)
+			(void)hipGraphExecDestroy(exec);
+		if(graph)
+			(void)hipGraphDestroy(graph);
+		for(hipEvent_t e : join)
+			(void)hipEventDestroy(e);
+		if(fork)
+			(void)hipEventDestroy(fork);
+		if(done)
+			(void)hipEventDestroy(done);
+		for(hipStream_t s : streams)
+			(void)hipStreamDestroy(s);
+	}
+};
+
+namespace {
+constexpr int kBatchStreams = 8;
+
+// Records every frame of the batch into one graph: frames are spread round-robin over
+// kBatchStreams forked streams (default priority) so that small scenes run side by side.
+int capture_batch(lucid_batch *bt) {
+	if(bt->exec)
+		(void)hipGraphExecDestroy(bt->exec), bt->exec = nullptr;
+	if(bt->graph)
+		(void)hipGraphDestroy(bt->graph), bt->graph = nullptr;
+	hipStream_t s0 = bt->streams[0];
+	LUCID_CHECK(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+	hipError_t err = hipEventRecord(bt->fork, s0);
+	for(size_t k = 1; k < bt->streams.size() && err == hipSuccess; k++)
+		err = hipStreamWaitEvent(bt->streams[k], bt->fork, 0);
+	for(size_t i = 0; i < bt->rs.size() && err == hipSuccess; i++) {
+		lucid_renderer *r = bt->rs[i];
+		hipStream_t s = bt->streams[i % bt->streams.size()];
+		const uint32_t n_info = LUCID_INFO_U32_SIZE + 10u * (uint32_t)r->bin_count;
+		err = launch_setup(bt->cs[i], bt->bs[i], n_info, s);
+		if(err == hipSuccess)
+			err = launch_binning(bt->cs[i], bt->bs[i], s);
+		if(err == hipSuccess)
+			err = launch_raster(bt->cs[i], bt->bs[i], s, nullptr);
+		if(err == hipSuccess)
+			err = hipMemcpyAsync(r->status, r->ctrl.ptr, kCtrlWords * 4, hipMemcpyDeviceToHost, s);
+	}
+	for(size_t k = 1; k < bt->streams.size() && err == hipSuccess; k++) {
+		err = hipEventRecord(bt->join[k], bt->streams[k]);
+		if(err == hipSuccess)
+			err = hipStreamWaitEvent(s0, bt->join[k], 0);
+	}
+	hipGraph_t g = nullptr;
+	hipError_t end = hipStreamEndCapture(s0, &g);
+	if(err != hipSuccess || end != hipSuccess) {
+		if(g)
+			(void)hipGraphDestroy(g);
+		return hip_status(err != hipSuccess ? err : end);
+	}
+	bt->graph = g;
+	LUCID_CHECK(hipGraphInstantiate(&bt->exec, g, nullptr, nullptr, 0));
+	return LUCID_OK;
+}
+} // namespace
+
+int lucid_batch_create(lucid_renderer *const *renderers, int n, lucid_batch **out) {
+	if(!renderers || n <= 0 || !out)
+		return LUCID_ERR_INVALID_ARG;
+	for(int i = 0; i < n; i++)
+		if(!renderers[i] || renderers[i]->device != renderers[0]->device || renderers[i]->timers)
+			return LUCID_ERR_INVALID_ARG; // one device; stage timers are per renderer only
+	for(int i = 0; i < n; i++)
+		for(int j = 0; j < i; j++)
+			if(renderers[i] == renderers[j])
+				return LUCID_ERR_INVALID_ARG; // frames run concurrently: one renderer each
+	lucid_batch *bt = new(std::nothrow) lucid_batch;
+	if(!bt)
+		return LUCID_ERR_OUT_OF_MEMORY;
+	bt->device = renderers[0]->device;
+	bt->rs.assign(renderers, renderers + n);
+	int rc = LUCID_OK;
+	if(hipSetDevice(bt->device) != hipSuccess)
+		rc = LUCID_ERR_HIP;
+	const int ns = std::min(n, kBatchStreams);
+	for(int k = 0; k < ns && rc == LUCID_OK; k++) {
+		hipStream_t s = nullptr;
+		hipEvent_t e = nullptr;
+		if(hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+		   hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+			rc = LUCID_ERR_HIP;
+		if(s)
+			bt->streams.push_back(s);
+		if(e)
+			bt->join.push_back(e);
+	}
+	if(rc == LUCID_OK && (hipEventCreateWithFlags(&bt->fork, hipEventDisableTiming) != hipSuccess ||
+						  hipEventCreateWithFlags(&bt->done, hipEventDisableTiming) != hipSuccess))
+		rc = LUCID_ERR_HIP;
+	if(rc != LUCID_OK) {
+		delete bt;
+		return rc;
+	}
+	*out = bt;
+	return LUCID_OK;
+}
+
+void lucid_batch_destroy(lucid_batch *bt) {
+	if(!bt)
+		return;
+	(void)hipSetDevice(bt->device);
+	if(bt->pending)
+		(void)hipEventSynchronize(bt->done);
+	delete bt;
+}
+
+int lucid_batch_render(lucid_batch *bt, const lucid_frame_inputs *ins,
+					   const lucid_frame_outputs *outs, void *stream_ptr) {
+	if(!bt || !ins || !outs)
+		return LUCID_ERR_INVALID_ARG;
+	hipStream_t stream = (hipStream_t)stream_ptr;
+	LUCID_CHECK(hipSetDevice(bt->device));
+	if(bt->pending) {
+		int f = lucid_batch_finish(bt);
+		if(f != LUCID_OK && f != LUCID_ERR_CAPACITY)
+			return f;
+	}
+	const size_t n = bt->rs.size();
+	std::vector<FrameConst> cs(n);
+	std::vector<FrameBufs> bs(n);
+	for(size_t i = 0; i < n; i++) {
+		lucid_renderer *r = bt->rs[i];
+		if(r->frame_pending) {
+			int f = lucid_finish(r);
+			if(f != LUCID_OK && f != LUCID_ERR_CAPACITY)
+				return f;
+		}
+		int64_t nq = 0;
+		int e = prepare_frame(r, &ins[i], &outs[i], cs[i], bs[i], nq);
+		if(e != LUCID_OK)
+			return e;
+		r->last_num_quads = nq;
+		r->last_stream = stream;
+	}
+	// re-record the graph only when a frame's constants or buffers changed
+	const bool same = bt->exec && bt->cs.size() == n &&
+					  memcmp(bt->cs.data(), cs.data(), n * sizeof(FrameConst)) == 0 &&
+					  memcmp(bt->bs.data(), bs.data(), n * sizeof(FrameBufs)) == 0;
+	if(!same) {
+		bt->cs = cs;
+		bt->bs = bs;
+		int e = capture_batch(bt);
+		if(e != LUCID_OK)
+			return e;
+	}
+	LUCID_CHECK(hipGraphLaunch(bt->exec, stream));
+	LUCID_CHECK(hipEventRecord(bt->done, stream));
+	bt->pending = true;
+	return LUCID_OK;
+}
+
+int lucid_batch_finish(lucid_batch *bt) {
+	if(!bt)
+		return LUCID_ERR_INVALID_ARG;
+	LUCID_CHECK(hipSetDevice(bt->device));
+	if(!bt->pending)
+		return LUCID_OK;
+	LUCID_CHECK(hipEventSynchronize(bt->done));
+	bt->pending = false;
+	int rc = LUCID_OK;
+	for(lucid_renderer *r : bt->rs) {
+		int g = grow_after_overflow(r);
+		memset(r->status, 0, kCtrlWords * 4);
+		if(g != LUCID_OK && rc == LUCID_OK)
+			rc = g;
+	}
+	return rc;
 }
 
 int lucid_finish(lucid_renderer *r) {

@@ -813,13 +813,11 @@ __global__ __launch_bounds__(kRowgenThreads) void rowgen_high_kernel(FrameConst 
 
 // Entries (triangles of one block in depth order) are handed to the rasteriser in slices
 // of up to kSliceEntries: entry lane + 64 k of the slice sits in register k of that lane.
-#ifndef LUCID_SLICE_WORDS
-#define LUCID_SLICE_WORDS 1
-#endif
-constexpr int kSliceWords = LUCID_SLICE_WORDS;
+// (The block rasterisers take the slice width W = 1 or 2 words as a template parameter,
+// chosen per renderer by its tuning; the backward kernels use kSliceWords.)
+constexpr int kSliceWords = 1;
 constexpr int kSliceEntries = 64 * kSliceWords;
-constexpr uint32_t kSamplesPerWave = 2048; // target samples per wave of a pre-sorted block
-constexpr int kBigWaves = 4;               // waves of block_raster_big_kernel
+constexpr int kBigWaves = 4; // waves of block_raster_big_kernel / the backward block kernels
 
 // Pre-sorted block layout in the sorted pool (offsets in words from an even base):
 //   [0] number of waves W, [1 .. W + 1] first entry of each wave's range (then m),
@@ -853,7 +851,7 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 	const int px0 = bin_x + bx * 8, py0 = bin_y + by * 8;
 	if(threadIdx.x == 0)
 		b.blk_slot[slot] = make_uint4(sl.x, m, sl.z, nrec);
-	if(m <= (uint32_t)kRegSortCap) {
+	if(m <= c.presort_min) { // sorted in registers by the block rasteriser (<= kRegSortCap)
 		if(threadIdx.x == 0)
 			b.blk_flags[slot] = LUCID_BIN_LEVEL_HIGH;
 		return;
@@ -909,7 +907,7 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 			s_samples[t] = tot; // exclusive prefix
 			tot += v;
 		}
-		uint32_t nw = (tot + kSamplesPerWave - 1) / kSamplesPerWave;
+		uint32_t nw = (tot + c.samples_per_wave - 1) / c.samples_per_wave;
 		nw = nw < 1u ? 1u : nw > (uint32_t)kBigWaves ? (uint32_t)kBigWaves : nw;
 		s_parts = nw;
 		s_total = tot;
@@ -986,10 +984,11 @@ __device__ __forceinline__ void transpose_bits(uint64_t msk, uint32_t &lo, uint3
 // Transposes the coverage masks of a slice (64-bit pixel masks, one per entry) into
 // per-pixel entry bitmaps: after the call bit e of bm[k] in lane p is set iff entry
 // 64 k + e covers pixel p. One ballot per (word, pixel).
-__device__ __forceinline__ void transpose_masks(const uint64_t (&msk)[kSliceWords], int nwords,
-												uint64_t (&bm)[kSliceWords]) {
+template <int W>
+__device__ __forceinline__ void transpose_masks(const uint64_t (&msk)[W], int nwords,
+												uint64_t (&bm)[W]) {
 #pragma unroll
-	for(int k = 0; k < kSliceWords; k++) {
+	for(int k = 0; k < W; k++) {
 		uint32_t lo = 0, hi = 0;
 		if(k < nwords)
 			transpose_bits<0>(msk[k], lo, hi);
@@ -998,12 +997,13 @@ __device__ __forceinline__ void transpose_masks(const uint64_t (&msk)[kSliceWord
 }
 
 // Stages the shading records of a slice in LDS: lane l writes entries l + 64 k.
+template <int W>
 __device__ __forceinline__ void stage_slice(const FrameConst &c, const FrameBufs &b,
-											const uint32_t (&tri)[kSliceWords], uint32_t n,
+											const uint32_t (&tri)[W], uint32_t n,
 											uint4 *s_rec) {
 	const int lane = lane_id();
 #pragma unroll
-	for(int k = 0; k < kSliceWords; k++) {
+	for(int k = 0; k < W; k++) {
 		const uint32_t i = k * 64 + lane;
 		if(i < n) {
 			uint4 q[kRecQuads];
@@ -1017,28 +1017,28 @@ __device__ __forceinline__ void stage_slice(const FrameConst &c, const FrameBufs
 
 // Visits this lane's set bitmap bits in increasing entry order, one per step; the wave
 // steps until every lane is done. visit(e) runs for the lanes that still have entries.
-template <class Visit>
-__device__ __forceinline__ void walk_bits(uint64_t (&bm)[kSliceWords], Visit &&visit) {
+template <int W, class Visit>
+__device__ __forceinline__ void walk_bits(uint64_t (&bm)[W], Visit &&visit) {
 	while(true) {
-		bool nz[kSliceWords], any = false;
+		bool nz[W], any = false;
 #pragma unroll
-		for(int k = 0; k < kSliceWords; k++) {
+		for(int k = 0; k < W; k++) {
 			nz[k] = bm[k] != 0ull;
 			any = any || nz[k];
 		}
 		if(!ballot(any))
 			break;
 		if(any) {
-			uint64_t cur = bm[kSliceWords - 1];
-			int word = kSliceWords - 1;
+			uint64_t cur = bm[W - 1];
+			int word = W - 1;
 #pragma unroll
-			for(int k = kSliceWords - 2; k >= 0; k--) {
+			for(int k = W - 2; k >= 0; k--) {
 				cur = nz[k] ? bm[k] : cur;
 				word = nz[k] ? k : word;
 			}
 			const uint64_t rest = cur & (cur - 1ull);
 #pragma unroll
-			for(int k = 0; k < kSliceWords; k++)
+			for(int k = 0; k < W; k++)
 				bm[k] = word == k ? rest : bm[k];
 			visit(word * 64 + (int)__builtin_ctzll(cur));
 		}
@@ -1048,9 +1048,9 @@ __device__ __forceinline__ void walk_bits(uint64_t (&bm)[kSliceWords], Visit &&v
 // Walks this lane's pixel through the slice in entry order: each step takes the nearest
 // remaining entry that covers the pixel, shades the sample from the entry's LDS record and
 // feeds the ordering window.
-template <bool FP32>
+template <bool FP32, int W>
 __device__ __forceinline__ void walk_pixel(const FrameConst &c, const uint4 *s_rec,
-										   uint64_t (&bm)[kSliceWords], float px, float py,
+										   uint64_t (&bm)[W], float px, float py,
 										   PixelState<FP32> &ps, bool additive) {
 	walk_bits(bm, [&](int e) {
 		const uint4 *rec = s_rec + e * kRecQuads;
@@ -1062,10 +1062,11 @@ __device__ __forceinline__ void walk_pixel(const FrameConst &c, const uint4 *s_r
 	});
 }
 
-__device__ __forceinline__ uint32_t bitmap_count(const uint64_t (&bm)[kSliceWords]) {
+template <int W>
+__device__ __forceinline__ uint32_t bitmap_count(const uint64_t (&bm)[W]) {
 	uint32_t n = 0;
 #pragma unroll
-	for(int k = 0; k < kSliceWords; k++)
+	for(int k = 0; k < W; k++)
 		n += (uint32_t)__popcll(bm[k]);
 	return n;
 }
@@ -1091,11 +1092,11 @@ __device__ __forceinline__ void finish_pixels(const FrameConst &c, const FrameBu
 // One wave per block whose list fits the register sort (every low bin, most high blocks):
 // sorts the list by centroid depth in registers, then walks every pixel through it slice by
 // slice (each slice's shading records staged in LDS).
-template <bool FP32>
+template <bool FP32, int W>
 __global__ __launch_bounds__(64) void block_raster_kernel(FrameConst c, FrameBufs b) {
 	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
 		return;
-	__shared__ uint4 s_rec[kSliceEntries * kRecQuads];
+	__shared__ uint4 s_rec[(64 * W) * kRecQuads];
 	const int lane = lane_id();
 	const uint32_t slot = blockIdx.x;
 	const int bin = slot >> 4, blk = slot & 15, by = blk >> 2, bx = blk & 3;
@@ -1153,26 +1154,26 @@ __global__ __launch_bounds__(64) void block_raster_kernel(FrameConst c, FrameBuf
 	PixelState<FP32> ps;
 	ps.init();
 	uint32_t count = 0;
-	static_assert(4 % kSliceWords == 0, "register-sorted lists split into whole slices");
-	for(uint32_t s0 = 0; s0 < m; s0 += kSliceEntries) {
+	static_assert(4 % W == 0, "register-sorted lists split into whole slices");
+	for(uint32_t s0 = 0; s0 < m; s0 += (64 * W)) {
 		const int j0 = (int)(s0 / 64);
-		uint64_t msk[kSliceWords];
-		uint32_t tri[kSliceWords];
+		uint64_t msk[W];
+		uint32_t tri[W];
 #pragma unroll
-		for(int k = 0; k < kSliceWords; k++) {
+		for(int k = 0; k < W; k++) {
 			msk[k] = msk_all[k], tri[k] = tri_all[k];
 #pragma unroll
-			for(int j = kSliceWords; j < 4; j++)
+			for(int j = W; j < 4; j++)
 				if(j0 + k == j)
 					msk[k] = msk_all[j], tri[k] = tri_all[j];
 		}
-		const uint32_t n = min(m - s0, (uint32_t)kSliceEntries);
+		const uint32_t n = min(m - s0, (uint32_t)(64 * W));
 		stage_slice(c, b, tri, n, s_rec);
 		wave_sync_lds();
-		uint64_t bm[kSliceWords];
+		uint64_t bm[W];
 		transpose_masks(msk, (int)((n + 63) / 64), bm);
 		count += bitmap_count(bm);
-		walk_pixel<FP32>(c, s_rec, bm, (float)px, (float)py, ps, additive);
+		walk_pixel<FP32, W>(c, s_rec, bm, (float)px, (float)py, ps, additive);
 		wave_sync_lds(); // every lane is done with this slice's records
 	}
 	ps.flush(additive);
@@ -1184,7 +1185,7 @@ __global__ __launch_bounds__(64) void block_raster_kernel(FrameConst c, FrameBuf
 // by sample count), walks it slice by slice with one ordering window per pixel, and leaves
 // a partial composite; wave 0 chains the partials front to back. A pixel is exact iff each
 // range's blended keys increase and every range starts behind the previous one's last key.
-template <bool FP32>
+template <bool FP32, int W>
 __global__ __launch_bounds__(kBigWaves * 64) void block_raster_big_kernel(FrameConst c,
 																		   FrameBufs b) {
 	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
@@ -1199,7 +1200,7 @@ __global__ __launch_bounds__(kBigWaves * 64) void block_raster_big_kernel(FrameC
 	const uint32_t flags = b.blk_flags[slot];
 	if(!(flags & kSlotPresorted) || (flags & kSlotPromoted))
 		return;
-	__shared__ uint4 s_rec[kBigWaves][kSliceEntries * kRecQuads];
+	__shared__ uint4 s_rec[kBigWaves][(64 * W) * kRecQuads];
 	__shared__ float4 s_part[kBigWaves][64];
 	__shared__ uint64_t s_first[kBigWaves][64], s_last[kBigWaves][64];
 	__shared__ uint32_t s_cnt[kBigWaves][64];
@@ -1219,22 +1220,22 @@ __global__ __launch_bounds__(kBigWaves * 64) void block_raster_big_kernel(FrameC
 		PixelState<FP32> ps;
 		ps.init();
 		uint32_t count = 0;
-		for(uint32_t s0 = e0; s0 < e1; s0 += kSliceEntries) {
-			const uint32_t n = min(e1 - s0, (uint32_t)kSliceEntries);
-			uint64_t msk[kSliceWords];
-			uint32_t t[kSliceWords];
+		for(uint32_t s0 = e0; s0 < e1; s0 += (64 * W)) {
+			const uint32_t n = min(e1 - s0, (uint32_t)(64 * W));
+			uint64_t msk[W];
+			uint32_t t[W];
 #pragma unroll
-			for(int k = 0; k < kSliceWords; k++) {
+			for(int k = 0; k < W; k++) {
 				uint32_t i = k * 64 + lane;
 				msk[k] = i < n ? mask[s0 + i] : 0ull;
 				t[k] = i < n ? tri[s0 + i] : 0u;
 			}
 			stage_slice(c, b, t, n, s_rec[wave]);
 			wave_sync_lds();
-			uint64_t bm[kSliceWords];
+			uint64_t bm[W];
 			transpose_masks(msk, (int)((n + 63) / 64), bm);
 			count += bitmap_count(bm);
-			walk_pixel<FP32>(c, s_rec[wave], bm, (float)px, (float)py, ps, additive);
+			walk_pixel<FP32, W>(c, s_rec[wave], bm, (float)px, (float)py, ps, additive);
 			wave_sync_lds(); // every lane is done with this slice's records
 		}
 		ps.flush(additive);
@@ -2078,9 +2079,17 @@ static void launch_raster_t(const FrameConst &c, const FrameBufs &b, hipStream_t
 					   c, b);
 	if(ev)
 		(void)hipEventRecord(ev[1], stream);
-	hipLaunchKernelGGL(block_raster_kernel<FP32>, dim3(c.bin_count * 16), dim3(64), 0, stream, c, b);
-	hipLaunchKernelGGL(block_raster_big_kernel<FP32>, dim3(c.bin_count * 16), dim3(kBigWaves * 64), 0,
-					   stream, c, b);
+	if(c.slice_words == 2) {
+		hipLaunchKernelGGL((block_raster_kernel<FP32, 2>), dim3(c.bin_count * 16), dim3(64), 0, stream,
+						   c, b);
+		hipLaunchKernelGGL((block_raster_big_kernel<FP32, 2>), dim3(c.bin_count * 16),
+						   dim3(kBigWaves * 64), 0, stream, c, b);
+	} else {
+		hipLaunchKernelGGL((block_raster_kernel<FP32, 1>), dim3(c.bin_count * 16), dim3(64), 0, stream,
+						   c, b);
+		hipLaunchKernelGGL((block_raster_big_kernel<FP32, 1>), dim3(c.bin_count * 16),
+						   dim3(kBigWaves * 64), 0, stream, c, b);
+	}
 	if(ev)
 		(void)hipEventRecord(ev[2], stream);
 	hipLaunchKernelGGL(fallback_kernel<FP32>, dim3(1024), dim3(kFallbackThreads), 0, stream, c, b);

@@ -104,6 +104,39 @@ class LucidRenderer:
     def max_scene_quads(self) -> int:
         return self.lib.lucid_max_scene_quads(self.handle)
 
+    # tuning (performance only; see lucid_tuning in include/lucid_c.h)
+    def get_tuning(self) -> dict:
+        t = _lib.Tuning()
+        _lib.check(self.lib.lucid_get_tuning(self.handle, ctypes.byref(t)), "lucid_get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, slice_words: int = 1, presort_min: int = 256,
+                   samples_per_wave: int = 2048) -> None:
+        t = _lib.Tuning(slice_words, presort_min, samples_per_wave)
+        _lib.check(self.lib.lucid_set_tuning(self.handle, ctypes.byref(t)), "lucid_set_tuning")
+
+    def autotune(self, gscene: "GpuScene", outputs: Optional[dict] = None, iters: int = 3,
+                 stream=None) -> dict:
+        """Times every tuning candidate on this scene and keeps the fastest."""
+        torch = self._torch
+        if outputs is None:
+            outputs = self.alloc_outputs()
+        st = stream if stream is not None else torch.cuda.current_stream(self.device)
+        best = _lib.Tuning()
+        inp = gscene.inputs()
+        _lib.check(self.lib.lucid_autotune(self.handle, ctypes.byref(inp),
+                                           ctypes.byref(self._outputs(outputs)),
+                                           ctypes.c_void_p(st.cuda_stream), iters,
+                                           ctypes.byref(best)), "lucid_autotune")
+        return best.as_dict()
+
+    @staticmethod
+    def _outputs(outputs: dict) -> _lib.FrameOutputs:
+        return _lib.FrameOutputs(
+            outputs["image"].data_ptr() if outputs.get("image") is not None else None,
+            outputs["image_f"].data_ptr() if outputs.get("image_f") is not None else None,
+            outputs["frag_counts"].data_ptr() if outputs.get("frag_counts") is not None else None)
+
     def alloc_outputs(self, rgba8: bool = True, rgba32f: bool = False, counts: bool = True):
         torch = self._torch
         dev = torch.device("cuda", self.device)

@@ -246,6 +246,26 @@ def config0(seed: int = 1000, num_tris: int = 10_000, width: int = 512, height: 
                       name=f"c0_{num_tris}_{width}x{height}")
 
 
+BATCH_RESOLUTIONS = ((256, 256), (512, 512), (768, 432), (1024, 1024))
+
+
+def config3(seed: int = 1003, num_scenes: int = 64, min_tris: int = 5_000,
+            max_tris: int = 500_000):
+    """The batch workload: independent config0-generator scenes with log-normal triangle
+    counts clipped to [min_tris, max_tris] (median ~50k) at resolutions drawn from
+    {256x256, 512x512, 768x432, 1024x1024}."""
+    rng = np.random.default_rng(seed)
+    counts = np.clip(np.exp(rng.normal(np.log(50_000), 1.0, num_scenes)), min_tris, max_tris)
+    res = rng.integers(0, len(BATCH_RESOLUTIONS), num_scenes)
+    out = []
+    for i in range(num_scenes):
+        w, h = BATCH_RESOLUTIONS[res[i]]
+        sc = config0(seed=seed * 1000 + i, num_tris=int(counts[i]), width=w, height=h)
+        sc.name = f"c3_{i}_{int(counts[i])}_{w}x{h}"
+        out.append(sc)
+    return out
+
+
 def config4(seed: int = 1004, num_tris: int = 200_000, width: int = 512,
             height: int = 512) -> Scene:
     """The differentiable workload: the config0 generator at 200k triangles (mean depth
