@@ -40,7 +40,57 @@ def parse():
     p.add_argument("--no-backward", action="store_true",
                    help="skip the fwd+bwd / fwd ratio leg (200k triangles at 512x512)")
     p.add_argument("--bwd-steps", type=int, default=5)
+    p.add_argument("--no-batch", action="store_true",
+                   help="skip the batch leg (64 scenes in one hipGraph vs 64 sequential frames)")
+    p.add_argument("--batch-scenes", type=int, default=64)
     return p.parse_args()
+
+
+def batch_leg(num_scenes: int, steps: int = 3):
+    """Batch throughput: the C3 scenes (config0 generator, log-normal 5k-500k triangles,
+    resolutions {256^2, 512^2, 768x432, 1024^2}) rendered as one hipGraph launch vs the same
+    frames as sequential lucid_render + lucid_finish calls. Each resolution class is
+    autotuned once on its largest scene; the tuning is applied to the whole class."""
+    import torch
+    from lucid_amd import scenes
+    from lucid_amd.batch import LucidBatch
+    from lucid_amd.renderer import GpuScene, LucidRenderer
+    scs = scenes.config3(num_scenes=num_scenes)
+    rs = [LucidRenderer((s.width, s.height), max_quads=s.num_quads) for s in scs]
+    gs = [GpuScene(s) for s in scs]
+    outs = [r.alloc_outputs(rgba8=True, rgba32f=False, counts=False) for r in rs]
+    tuned = {}
+    for res in sorted({(s.width, s.height) for s in scs}):
+        idx = max((i for i, s in enumerate(scs) if (s.width, s.height) == res),
+                  key=lambda i: scs[i].num_quads)
+        tuned[res] = rs[idx].autotune(gs[idx], outs[idx], iters=2)
+    for s, r in zip(scs, rs):
+        r.set_tuning(**tuned[(s.width, s.height)])
+    frags = 0
+    for r, g, o in zip(rs, gs, outs):
+        r.render(g, o)
+        frags += int(r.read_info()[60])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for r, g, o in zip(rs, gs, outs):
+            r.render(g, o)
+    torch.cuda.synchronize()
+    t_seq = (time.perf_counter() - t0) / steps
+    batch = LucidBatch(rs)
+    batch.render(gs, outs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.render(gs, outs)
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / steps
+    return {"workload": f"{num_scenes} scenes (config0 generator, log-normal 5k-500k triangles, "
+                        "resolutions 256^2/512^2/768x432/1024^2, seed 1003), RGBA8",
+            "triangles": int(sum(s.num_quads for s in scs)), "fragments": frags,
+            "sequential_ms": t_seq * 1e3, "batch_ms": t_batch * 1e3,
+            "batch_fragments_per_s": frags / t_batch, "speedup": t_seq / t_batch,
+            "tuning": {f"{w}x{h}": t for (w, h), t in tuned.items()}}
 
 
 def backward_ratio(steps: int):
@@ -187,6 +237,11 @@ def main():
         print("[bench] fwd+bwd ratio (200k tris, 512x512)", file=sys.stderr, flush=True)
         bwd = backward_ratio(args.bwd_steps)
 
+    batch = None
+    if not args.no_batch:
+        print(f"[bench] batch ({args.batch_scenes} scenes)", file=sys.stderr, flush=True)
+        batch = batch_leg(args.batch_scenes)
+
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle)", file=sys.stderr, flush=True)
@@ -216,6 +271,7 @@ def main():
             "fallback_pixels": int(info[62]), "oversized_blocks": int(info[63]),
             "high_bins": int(info[9]), "low_bins": int(info[7]),
             "backward": bwd,
+            "batch": batch,
         },
         "roofline": {
             "bound": "hbm",
