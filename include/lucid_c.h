@@ -162,6 +162,16 @@ int lucid_read_setup(lucid_renderer *r, uint32_t *tri_records, size_t num_tris,
  * Returns the number of stages written, or a negative status. */
 int lucid_stage_times(lucid_renderer *r, float *ms, int max_stages);
 
+/* Quad generator (host; replaces genQuads/quadNodes/triNeighbours, src/quad_generator.cpp:8-201,
+ * as called by Scene::generateQuads, src/scene.cpp:237-247): pairs edge-adjacent triangles
+ * (u32 index triples) into quads {v0, v1, v2, v3} = triangles (v0, v1, v2) + (v0, v2, v3), in
+ * the original triangle order; unpaired triangles become (a, b, c, c). quads_out needs room
+ * for 4 * num_tris indices (or is NULL to count only); *num_quads receives the count.
+ * square_weight: the reference's squareness weight (0.5 in its app). */
+int lucid_generate_quads(const float *positions, int64_t num_verts, const uint32_t *tris,
+						 int64_t num_tris, float square_weight, uint32_t *quads_out,
+						 int64_t *num_quads);
+
 int lucid_bin_count(const lucid_renderer *r);
 int lucid_bin_size(const lucid_renderer *r);
 int lucid_block_size(const lucid_renderer *r);

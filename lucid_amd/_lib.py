@@ -36,6 +36,7 @@ EXPORTED = (
     "lucid_stage_times", "lucid_backward",
     "lucid_set_tuning", "lucid_get_tuning", "lucid_autotune",
     "lucid_batch_create", "lucid_batch_destroy", "lucid_batch_render", "lucid_batch_finish",
+    "lucid_generate_quads",
     "lucid_bin_count", "lucid_bin_size", "lucid_block_size", "lucid_max_visible_quads",
     "lucid_max_scene_quads",
 )
@@ -140,6 +141,8 @@ def load(path: str = LIB_PATH):
                                        vp]
     lib.lucid_batch_finish.restype = i32
     lib.lucid_batch_finish.argtypes = [vp]
+    lib.lucid_generate_quads.restype = i32
+    lib.lucid_generate_quads.argtypes = [vp, i64, vp, i64, ctypes.c_float, vp, ctypes.POINTER(i64)]
     for name in ("lucid_bin_count", "lucid_bin_size", "lucid_block_size"):
         getattr(lib, name).restype = i32
         getattr(lib, name).argtypes = [vp]

@@ -246,6 +246,67 @@ def config0(seed: int = 1000, num_tris: int = 10_000, width: int = 512, height: 
                       name=f"c0_{num_tris}_{width}x{height}")
 
 
+def icosphere(level: int = 3):
+    """Unit icosphere: the icosahedron with each triangle split into 4, `level` times
+    (20 * 4^level triangles; level 3 = 1280). Returns (positions (V, 3), tris (T, 3))."""
+    p = (1.0 + 5.0 ** 0.5) / 2.0
+    verts = [(-1, p, 0), (1, p, 0), (-1, -p, 0), (1, -p, 0), (0, -1, p), (0, 1, p), (0, -1, -p),
+             (0, 1, -p), (p, 0, -1), (p, 0, 1), (-p, 0, -1), (-p, 0, 1)]
+    verts = [np.array(v, np.float64) / np.linalg.norm(v) for v in verts]
+    tris = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+            (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+            (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(level):
+        mid = {}
+
+        def midpoint(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in mid:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                mid[key] = len(verts) - 1
+            return mid[key]
+        nxt = []
+        for a, b, c in tris:
+            ab, bc, ca = midpoint(a, b), midpoint(b, c), midpoint(c, a)
+            nxt += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        tris = nxt
+    return np.array(verts, np.float32), np.array(tris, np.uint32)
+
+
+def config2(seed: int = 1002, grid=(16, 16, 16), width: int = 2560, height: int = 1440,
+            level: int = 3, square_weight: float = 0.5) -> Scene:
+    """Mesh workload: grid[0]*grid[1]*grid[2] icospheres (20 * 4^level triangles each, 1280
+    at level 3; 4096 spheres by default) on a jittered 3D grid in view, alpha 0.5 for every
+    triangle, fp32 per-vertex colours. Triangles are paired into quads by the quad generator
+    (lucid_generate_quads), run once on the sphere mesh and replicated (pairing depends only
+    on the mesh's shape, which every sphere shares up to translation and scale)."""
+    from .quads import generate_quads
+    rng = np.random.default_rng(seed)
+    sp, st = icosphere(level)
+    sq = generate_quads(sp, st, square_weight)
+    gx, gy, gz = grid
+    n = gx * gy * gz
+    cam = Camera(width, height)
+    k = np.arange(n)
+    i, j, l = k % gx, (k // gx) % gy, k // (gx * gy)
+    depth = 4.0 + (l + rng.uniform(-0.3, 0.3, n)) * (40.0 / max(gz, 1))
+    ndc = np.stack([(i + 0.5 + rng.uniform(-0.3, 0.3, n)) / gx * 1.8 - 0.9,
+                    (j + 0.5 + rng.uniform(-0.3, 0.3, n)) / gy * 1.8 - 0.9], axis=1)
+    centres = cam.ndc_to_view(ndc[:, None, :], depth[:, None])[:, 0, :]
+    radius = depth * (1.8 / max(gx, gy)) * 0.45 * np.tan(np.radians(45.0))
+    pos = (centres[:, None, :] + sp[None, :, :] * radius[:, None, None]).astype(np.float32)
+    nv = len(sp)
+    quads = (sq[None, :, :].astype(np.int64) + (np.arange(n) * nv)[:, None, None]).astype(np.uint32)
+    quads = quads.reshape(-1, 4)
+    inst = split_instances(len(quads), INST_HAS_VERTEX_COLORS)
+    sc = Scene(width, height, pos.reshape(-1, 3), quads, inst, make_config(cam, len(inst)),
+               name=f"c2_{n}x{len(st)}_{width}x{height}")
+    sc.colors_rgb = rng.uniform(0.0, 1.0, (n * nv, 3)).astype(np.float32)
+    sc.tri_alpha = np.full(2 * len(quads), 0.5, np.float32)
+    return sc
+
+
 BATCH_RESOLUTIONS = ((256, 256), (512, 512), (768, 432), (1024, 1024))
 
 

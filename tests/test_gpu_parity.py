@@ -184,3 +184,18 @@ def test_determinism(oracle_mod):
     fa = a["image_f"].cpu().numpy().copy()
     b = r.render(g, r.alloc_outputs(rgba32f=True))
     assert np.array_equal(fa, b["image_f"].cpu().numpy())
+
+
+def test_c2_icospheres_reduced(oracle_mod):
+    """Mesh quads from the quad generator (two triangles per quad, shared vertices)."""
+    sc = scenes.config2(grid=(4, 4, 4), width=640, height=360)
+    r, gpu, ref = _render_both(sc, oracle_mod)
+    _check(sc, gpu, ref, oracle_mod)
+
+
+def test_c2_full_size(oracle_mod):
+    """The full mesh workload: 4096 icospheres (5.2M triangles in 2.6M generated quads) at
+    2560x1440, alpha 0.5."""
+    sc = scenes.config2()
+    r, gpu, ref = _render_both(sc, oracle_mod)
+    _check(sc, gpu, ref, oracle_mod)
