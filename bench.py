@@ -43,6 +43,8 @@ def parse():
     p.add_argument("--no-batch", action="store_true",
                    help="skip the batch leg (64 scenes in one hipGraph vs 64 sequential frames)")
     p.add_argument("--batch-scenes", type=int, default=64)
+    p.add_argument("--no-autotune", action="store_true",
+                   help="keep the default tuning instead of autotuning on the workload first")
     return p.parse_args()
 
 
@@ -187,6 +189,10 @@ def main():
     g = GpuScene(scene)
     out = r.alloc_outputs(rgba8=True, counts=True)
     torch.cuda.synchronize()
+    tuning = r.get_tuning()
+    if not args.no_autotune:
+        print("[bench] autotune", file=sys.stderr, flush=True)
+        tuning = r.autotune(g, out, iters=3)
     print("[bench] warmup", file=sys.stderr, flush=True)
     for _ in range(max(1, args.warmup)):
         r.render(g, out)
@@ -269,6 +275,7 @@ def main():
             "median_frame_ms_stage_events": float(np.median(frame_ms)),
             "stages_ms": stages,
             "fallback_pixels": int(info[62]), "oversized_blocks": int(info[63]),
+            "tuning": tuning,
             "high_bins": int(info[9]), "low_bins": int(info[7]),
             "backward": bwd,
             "batch": batch,

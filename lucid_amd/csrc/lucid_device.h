@@ -75,7 +75,6 @@ constexpr uint32_t kSlotWhole = 1u << 10;    // list too long: every pixel takes
 constexpr uint32_t kSlotPartShift = 12;      // log2 of the waves sharing a pre-sorted block
 constexpr int kRegSortCap = 256;  // lists sorted in registers by the raster wave itself
 constexpr int kBigSortCap = 8192; // lists sorted in LDS by high_finalize_kernel
-constexpr int kRowgenSplit = 8;   // work-groups per high bin in rowgen_high
 
 enum ColorMode : int { kColorNone = 0, kColorRGBA8 = 1, kColorF32 = 2 };
 

@@ -414,39 +414,58 @@ __device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
 }
 
 // Ascending bitonic sort of 64*E keys held in registers; element i = e * 64 + lane.
+// Bitonic network steps j = j_start .. 1 of merge level k on 64 * E keys held by one wave
+// (key e * 64 + lane of the block in v[e]); `base` is the block's index in the whole
+// sequence, which decides each pair's direction: ascending iff (index & k) == 0.
 template <int E>
-__device__ __forceinline__ void bitonic_regs(uint64_t (&v)[E]) {
+__device__ __forceinline__ void bitonic_regs_steps(uint64_t (&v)[E], uint32_t base, uint32_t k,
+												   int j_start) {
 	const int lane = lane_id();
 #pragma unroll
-	for(int k = 2; k <= 64 * E; k <<= 1) {
+	for(int j = 64 * E / 2; j > 0; j >>= 1) {
+		if(j > j_start)
+			continue;
+		if(j >= 64) {
+			const int jr = j / 64;
 #pragma unroll
-		for(int j = k >> 1; j > 0; j >>= 1) {
-			if(j >= 64) {
-				const int jr = j / 64;
-#pragma unroll
-				for(int e = 0; e < E; e++) {
-					if((e & jr) == 0) {
-						const int pe = e | jr;
-						bool asc = (((e * 64) & k) == 0);
-						uint64_t a = v[e], bb = v[pe];
-						bool sw = (a > bb) == asc;
-						v[e] = sw ? bb : a;
-						v[pe] = sw ? a : bb;
-					}
+			for(int e = 0; e < E; e++) {
+				if((e & jr) == 0) {
+					const int pe = e | jr;
+					const bool asc = ((base + (uint32_t)(e * 64 + lane)) & k) == 0;
+					uint64_t a = v[e], bb = v[pe];
+					bool sw = (a > bb) == asc;
+					v[e] = sw ? bb : a;
+					v[pe] = sw ? a : bb;
 				}
-			} else {
+			}
+		} else {
 #pragma unroll
-				for(int e = 0; e < E; e++) {
-					int i = e * 64 + lane;
-					uint64_t p = shfl_xor64(v[e], j);
-					bool asc = (i & k) == 0;
-					bool lower = (lane & j) == 0;
-					uint64_t mn = v[e] < p ? v[e] : p, mx = v[e] < p ? p : v[e];
-					v[e] = (asc == lower) ? mn : mx;
-				}
+			for(int e = 0; e < E; e++) {
+				const uint32_t i = base + (uint32_t)(e * 64 + lane);
+				uint64_t p = shfl_xor64(v[e], j);
+				bool asc = (i & k) == 0;
+				bool lower = (lane & j) == 0;
+				uint64_t mn = v[e] < p ? v[e] : p, mx = v[e] < p ? p : v[e];
+				v[e] = (asc == lower) ? mn : mx;
 			}
 		}
 	}
+}
+
+// Bitonic sort levels k = 2 .. 64 * E of a 64 * E block starting at index `base` of a longer
+// sequence (ascending for base = 0 alone; alternating directions per block otherwise, as the
+// network's later levels expect).
+template <int E>
+__device__ __forceinline__ void bitonic_regs_from(uint64_t (&v)[E], uint32_t base) {
+#pragma unroll
+	for(int k = 2; k <= 64 * E; k <<= 1)
+		bitonic_regs_steps<E>(v, base, (uint32_t)k, k >> 1);
+}
+
+// Ascending bitonic sort of 64 * E keys held by one wave (key e * 64 + lane in v[e]).
+template <int E>
+__device__ __forceinline__ void bitonic_regs(uint64_t (&v)[E]) {
+	bitonic_regs_from<E>(v, 0u);
 }
 
 // Ascending bitonic sort of P (power of two) keys in LDS by one wave.
@@ -714,29 +733,32 @@ __global__ __launch_bounds__(1024) void high_prep_kernel(FrameConst c, FrameBufs
 	}
 }
 
-// Block-row records of a high bin with the 4-row rule (raster_high.glsl:54-90). Each lane
-// takes one triangle of the bin and scans its rows once, from the first row of its 4-row
-// group (the start row fixes the scanline arithmetic) to its last row, emitting one record
-// per 8-row band it covers; kRowgenSplit work-groups share a bin. Record and list
-// positions come from wave-aggregated atomics.
-__global__ __launch_bounds__(kRowgenThreads) void rowgen_high_kernel(FrameConst c, FrameBufs b) {
+// Block-row records of a high bin with the 4-row rule (raster_high.glsl:54-90). One
+// work-group of kRowgenHighThreads per high bin; each lane takes one triangle of the bin and
+// scans its rows once, from the first row of its 4-row group (the start row fixes the
+// scanline arithmetic) to its last row, emitting one record per 8-row band it covers. Record
+// and list positions come from wave-aggregated LDS counters, written out at the end.
+constexpr int kRowgenHighThreads = 1024;
+__global__ __launch_bounds__(kRowgenHighThreads) void rowgen_high_kernel(FrameConst c, FrameBufs b) {
 	if(b.ctrl[kCtrlOverflow] & (kOvfBinLists | kOvfRecords | kOvfLists))
 		return;
 	const lucid_info *info = (const lucid_info *)b.info;
-	const uint32_t slot = blockIdx.x / kRowgenSplit;
-	const int split = blockIdx.x % kRowgenSplit;
+	const uint32_t slot = blockIdx.x;
 	if(slot >= (uint32_t)info->bin_level_counts[LUCID_BIN_LEVEL_HIGH])
 		return;
 	const uint32_t *counts = b.info + LUCID_INFO_U32_SIZE;
 	const int bin = (int)counts[LUCID_COUNTS_HIGH_BINS * c.bin_count + slot];
 	const BinInfo bi = load_bin(c, b, bin);
 	const uint4 *slot16 = b.blk_slot + (size_t)bin * 16;
-	uint32_t *band_n = b.band_count + (size_t)bin * 4;
-	uint32_t *blk_n = b.blk_count + (size_t)bin * 16;
+	__shared__ uint32_t s_band_n[4], s_blk_n[16];
+	if(threadIdx.x < 4)
+		s_band_n[threadIdx.x] = 0;
+	if(threadIdx.x < 16)
+		s_blk_n[threadIdx.x] = 0;
+	__syncthreads();
 	const int lane = lane_id();
-	const uint32_t stride = kRowgenThreads * kRowgenSplit;
-	const uint32_t wave_base = split * kRowgenThreads + (threadIdx.x & ~63u);
-	for(uint32_t base = wave_base; base < bi.nlevel; base += stride) {
+	const uint32_t wave_base = threadIdx.x & ~63u;
+	for(uint32_t base = wave_base; base < bi.nlevel; base += kRowgenHighThreads) {
 		const uint32_t i = base + lane;
 		const uint32_t t = i < bi.nlevel ? bin_tri_at(b, i, bi.qc, bi.qo, bi.to) : 0xffffffffu;
 		int row = 0, row_lo = 0, row_hi = -1;
@@ -787,7 +809,7 @@ __global__ __launch_bounds__(kRowgenThreads) void rowgen_high_kernel(FrameConst 
 				continue;
 			uint32_t rbase = 0;
 			if(lane == 0)
-				rbase = atomicAdd(&band_n[by], (uint32_t)__popcll(emit));
+				rbase = atomicAdd(&s_band_n[by], (uint32_t)__popcll(emit));
 			rbase = __shfl(rbase, 0, 64);
 			const uint32_t idx = rbase + (uint32_t)__popcll(emit & ((1ull << lane) - 1ull));
 			if(bxm)
@@ -801,7 +823,7 @@ __global__ __launch_bounds__(kRowgenThreads) void rowgen_high_kernel(FrameConst 
 					continue;
 				uint32_t lb = 0;
 				if(lane == 0)
-					lb = atomicAdd(&blk_n[by * 4 + bx], (uint32_t)__popcll(sel));
+					lb = atomicAdd(&s_blk_n[by * 4 + bx], (uint32_t)__popcll(sel));
 				lb = __shfl(lb, 0, 64);
 				if((bxm >> bx) & 1u)
 					b.blk_lists[slot16[by * 4 + bx].x + lb +
@@ -809,6 +831,11 @@ __global__ __launch_bounds__(kRowgenThreads) void rowgen_high_kernel(FrameConst 
 			}
 		}
 	}
+	__syncthreads();
+	if(threadIdx.x < 4)
+		b.band_count[(size_t)bin * 4 + threadIdx.x] = s_band_n[threadIdx.x];
+	if(threadIdx.x < 16)
+		b.blk_count[(size_t)bin * 16 + threadIdx.x] = s_blk_n[threadIdx.x];
 }
 
 // Entries (triangles of one block in depth order) are handed to the rasteriser in slices
@@ -873,12 +900,27 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 	__shared__ uint32_t s_base, s_parts, s_total;
 	const uint32_t *list = b.blk_lists + sl.x;
 	const uint4 *recs = b.high_records + sl.z;
-	const uint32_t P = next_pow2(m);
-	for(uint32_t j = threadIdx.x; j < P; j += kSortThreads)
-		s_key[j] = j < m ? block_key(b, recs[list[j]], bx, px0, py0, j) : ~0ull;
+	// Bitonic sort of the block keys: levels up to 256 in registers (one 256-key chunk per
+	// wave at a time), then per level the strides >= 256 in LDS and the rest in registers.
+	static_assert(kSortThreads == 256, "four waves share the chunks");
+	const uint32_t P = max(256u, next_pow2(m));
+	const uint32_t nchunks = P / 256;
+	const int wave_s = threadIdx.x >> 6, lane_s = lane_id();
+	for(uint32_t ch = wave_s; ch < nchunks; ch += kSortThreads / 64) {
+		uint64_t v[4];
+#pragma unroll
+		for(int e = 0; e < 4; e++) {
+			const uint32_t j = ch * 256 + e * 64 + lane_s;
+			v[e] = j < m ? block_key(b, recs[list[j]], bx, px0, py0, j) : ~0ull;
+		}
+		bitonic_regs_from<4>(v, ch * 256);
+#pragma unroll
+		for(int e = 0; e < 4; e++)
+			s_key[ch * 256 + e * 64 + lane_s] = v[e];
+	}
 	__syncthreads();
-	for(uint32_t k = 2; k <= P; k <<= 1) {
-		for(uint32_t j = k >> 1; j > 0; j >>= 1) {
+	for(uint32_t k = 512; k <= P; k <<= 1) {
+		for(uint32_t j = k >> 1; j >= 256; j >>= 1) {
 			for(uint32_t i = threadIdx.x; i < P / 2; i += kSortThreads) {
 				uint32_t lo = ((i & ~(j - 1)) << 1) | (i & (j - 1));
 				uint32_t hi = lo | j;
@@ -891,6 +933,17 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 			}
 			__syncthreads();
 		}
+		for(uint32_t ch = wave_s; ch < nchunks; ch += kSortThreads / 64) {
+			uint64_t v[4];
+#pragma unroll
+			for(int e = 0; e < 4; e++)
+				v[e] = s_key[ch * 256 + e * 64 + lane_s];
+			bitonic_regs_steps<4>(v, ch * 256, k, 128);
+#pragma unroll
+			for(int e = 0; e < 4; e++)
+				s_key[ch * 256 + e * 64 + lane_s] = v[e];
+		}
+		__syncthreads();
 	}
 	// s_key[i] now holds the list position of the i-th entry in depth order.
 	const uint32_t per = (m + kSortThreads - 1) / kSortThreads;
@@ -2073,8 +2126,8 @@ static void launch_raster_t(const FrameConst &c, const FrameBufs &b, hipStream_t
 	if(ev)
 		(void)hipEventRecord(ev[0], stream);
 	hipLaunchKernelGGL(high_prep_kernel, dim3(1), dim3(1024), 0, stream, c, b);
-	hipLaunchKernelGGL(rowgen_high_kernel, dim3(c.bin_count * kRowgenSplit), dim3(kRowgenThreads),
-					   0, stream, c, b);
+	hipLaunchKernelGGL(rowgen_high_kernel, dim3(c.bin_count), dim3(kRowgenHighThreads), 0, stream,
+					   c, b);
 	hipLaunchKernelGGL(high_finalize_kernel, dim3(c.bin_count * 16), dim3(kSortThreads), 0, stream,
 					   c, b);
 	if(ev)
@@ -2115,8 +2168,8 @@ hipError_t launch_backward_prepare(const FrameConst &c, const FrameBufs &b, cons
 								   hipStream_t stream) {
 	hipLaunchKernelGGL(rowgen_low_kernel, dim3(c.bin_count), dim3(kRowgenThreads), 0, stream, c, b);
 	hipLaunchKernelGGL(high_prep_kernel, dim3(1), dim3(1024), 0, stream, c, b);
-	hipLaunchKernelGGL(rowgen_high_kernel, dim3(c.bin_count * kRowgenSplit), dim3(kRowgenThreads),
-					   0, stream, c, b);
+	hipLaunchKernelGGL(rowgen_high_kernel, dim3(c.bin_count), dim3(kRowgenHighThreads), 0, stream,
+					   c, b);
 	hipLaunchKernelGGL(high_finalize_kernel, dim3(c.bin_count * 16), dim3(kSortThreads), 0, stream,
 					   c, b);
 	hipLaunchKernelGGL(grad_row_scan_kernel, dim3(1), dim3(1024), 0, stream, c, b, g);
