@@ -113,6 +113,16 @@ int lucid_render(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_fr
  * extension of the drop-in renderer. */
 int lucid_backward(lucid_renderer *r, const lucid_frame_inputs *in, const float *grad_image,
 				   float *grad_colors_rgb, float *grad_tri_alpha, void *stream);
+/* lucid_backward with flags. LUCID_BACKWARD_REUSE_FRAME: when the renderer's last frame was
+ * rendered from the same inputs (same device pointers, counts, instances, config and
+ * tuning; their contents must not have changed since) and finished without an overflow,
+ * its triangle setup, bins and per-block triangle lists are reused instead of rebuilt; the
+ * per-pixel depth-sorted fragment lists are still recomputed. Otherwise (or with flags 0)
+ * this is lucid_backward. Results are bit-identical either way.
+ * No counterpart in the reference. */
+#define LUCID_BACKWARD_REUSE_FRAME 1u
+int lucid_backward_ex(lucid_renderer *r, const lucid_frame_inputs *in, const float *grad_image,
+					  float *grad_colors_rgb, float *grad_tri_alpha, uint32_t flags, void *stream);
 
 /* Tuning of a renderer (defaults 1, 256, 2048). No counterpart in the reference. */
 int lucid_set_tuning(lucid_renderer *r, const lucid_tuning *t);

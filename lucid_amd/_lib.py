@@ -27,13 +27,15 @@ OPT_VISUALIZE_ERRORS = 1 << 6
 OPT_ALPHA_THRESHOLD = 1 << 7
 OPT_FP32_SAMPLES = 1 << 16
 
+BACKWARD_REUSE_FRAME = 1
+
 INFO_U32 = 1152
 
 # Every symbol include/lucid_c.h declares.
 EXPORTED = (
     "lucid_error_string", "lucid_create", "lucid_destroy", "lucid_render", "lucid_finish",
     "lucid_read_info", "lucid_verify_info", "lucid_read_bin_lists", "lucid_read_setup",
-    "lucid_stage_times", "lucid_backward",
+    "lucid_stage_times", "lucid_backward", "lucid_backward_ex",
     "lucid_set_tuning", "lucid_get_tuning", "lucid_autotune",
     "lucid_batch_create", "lucid_batch_destroy", "lucid_batch_render", "lucid_batch_finish",
     "lucid_generate_quads",
@@ -125,6 +127,9 @@ def load(path: str = LIB_PATH):
     lib.lucid_stage_times.argtypes = [vp, vp, i32]
     lib.lucid_backward.restype = i32
     lib.lucid_backward.argtypes = [vp, ctypes.POINTER(FrameInputs), vp, vp, vp, vp]
+    lib.lucid_backward_ex.restype = i32
+    lib.lucid_backward_ex.argtypes = [vp, ctypes.POINTER(FrameInputs), vp, vp, vp,
+                                      ctypes.c_uint32, vp]
     lib.lucid_set_tuning.restype = i32
     lib.lucid_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
     lib.lucid_get_tuning.restype = i32

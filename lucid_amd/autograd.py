@@ -1,8 +1,10 @@
 """torch.autograd binding of the differentiable renderer.
 
 Forward: ``lucid_render`` with fp32 samples (no RGBA8 quantisation of shaded samples, which
-has no useful derivative). Backward: ``lucid_backward``, the hand-written adjoint of sorted
-front-to-back compositing; it recomputes the sorted fragment lists instead of storing them.
+has no useful derivative). Backward: ``lucid_backward_ex``, the hand-written adjoint of sorted
+front-to-back compositing; it recomputes the per-pixel sorted fragment lists instead of
+storing them, and reuses the forward frame's bins and block lists when no other frame was
+rendered in between (the saved colour and alpha tensors guard against in-place changes).
 The reference renderer (src/lucid_renderer.h:21-96) has no backward pass; this is the
 differentiable extension of the same forward.
 """
@@ -43,7 +45,8 @@ class LucidRenderFunction:
                 @staticmethod
                 def backward(ctx, grad_image):
                     colors, alpha = ctx.saved_tensors
-                    g_col, g_alpha = ctx.renderer.backward(ctx.gscene, grad_image, colors, alpha)
+                    g_col, g_alpha = ctx.renderer.backward(ctx.gscene, grad_image, colors, alpha,
+                                                           reuse_frame=True)
                     return g_col, g_alpha, None, None
 
             cls._fn = _Fn

@@ -40,7 +40,8 @@ enum CtrlWord : int {
 	kCtrlBwdRows = 11,      // gradient rows of the block kernel (set by the row scan)
 	kCtrlBwdExtra = 12,     // bump allocator of per-fragment gradient rows of exact pixels
 	kCtrlBwdRanges = 13,    // depth ranges of the backward block kernels (set by the row scan)
-	kCtrlBwdGradMax = 14,   // max |dL/dpixel| (float bits) of the pixels the block kernels handle
+	kCtrlBwdGradMax = 14,   // max |dL/dpixel| (float bits) over the gradient image
+	kCtrlTopoChanged = 15,  // backward: the corner -> vertex map differs from the cached one
 	kCtrlWords = 16,
 };
 
@@ -50,10 +51,10 @@ constexpr uint32_t kOvfBinLists = 1u, kOvfRecords = 2u, kOvfFallback = 4u, kOvfS
 
 // Buffers of the backward pass (passed by value beside FrameBufs).
 constexpr int kGradWords = 10; // per row: corner 0 rgb, corner 1 rgb, corner 2 rgb, alpha
-// Start state of one (depth range, pixel) for pass B: transmittance before the range,
-// remainder out - C_before (fp64), and the masked dL/d(pixel rgb).
+// Start state of one (depth range, pixel) for pass B: transmittance before the range and
+// q = g . (out - C_before) (fp64), with g the masked dL/d(pixel rgb).
 struct BwdStart {
-	double t, rr, rg, rb;
+	double t, q;
 	float gr, gg, gb, pad;
 };
 struct BwdBufs {
@@ -64,7 +65,19 @@ struct BwdBufs {
 	uint32_t *row_tri;        // triangle id of each row
 	uint64_t rows_cap;
 	BwdStart *start;          // 64 per depth range
+	uint64_t start_cap;       // depth ranges
 	uint8_t *seq;             // 64 bytes per gradient row: recorded blend order
+	uint64_t seq_rows_cap;    // gradient rows
+};
+
+// Scratch of the gradient reduction (ntri triangle ids, nrows gradient rows).
+struct ReduceBufs {
+	uint32_t *counts, *offsets; // ntri + 1 each: rows per triangle, exclusive scan
+	uint32_t *grouped;          // nrows: row indices grouped by triangle
+	float *tri_grad;            // ntri * kGradWords
+	uint32_t *vkeys, *vsorted, *vcorners, *viota; // 3 ntri each: corners grouped by vertex
+	void *temp;
+	size_t temp_bytes;
 };
 
 // Per-block slot flags (blk_flags)
@@ -75,6 +88,7 @@ constexpr uint32_t kSlotWhole = 1u << 10;    // list too long: every pixel takes
 constexpr uint32_t kSlotPartShift = 12;      // log2 of the waves sharing a pre-sorted block
 constexpr int kRegSortCap = 256;  // lists sorted in registers by the raster wave itself
 constexpr int kBigSortCap = 8192; // lists sorted in LDS by high_finalize_kernel
+constexpr int kRowgenSplit = 8;   // work-groups per high bin in rowgen_high
 
 enum ColorMode : int { kColorNone = 0, kColorRGBA8 = 1, kColorF32 = 2 };
 
@@ -378,5 +392,35 @@ __device__ __forceinline__ uint64_t record_block_mask(uint4 rec, int bx) {
 	}
 	return m;
 }
+
+// ------------------------------------------------------------------ gradient fixed point
+
+// The backward pass sums gradients as 64-bit fixed point (integer sums are associative, so
+// a sum does not depend on the order its terms arrive in). The scale 2^k is fixed per
+// frame from max |dL/dpixel| over the image and a bound on one
+// fragment's contribution (|d sample / d colour| <= 12.92 * 2.4 * lighting, |d out / d a|
+// <= 2 per channel): one contribution stays below 2^50 units and a row of at most 64 below
+// 2^56, a resolution of ~1e-15 of the largest possible contribution.
+__device__ __forceinline__ int grad_fixed_exp(const FrameConst &c, float gmax) {
+	float lmax = 0.0f; // bound on the lighting factor (light in [0, 1])
+#pragma unroll
+	for(int k = 0; k < 3; k++)
+		lmax = fmaxf(lmax, fabsf(c.ambient[k] * c.ambient_power) +
+							   fabsf(c.sun_color[k] * c.sun_power));
+	const double bound = 64.0 * (double)gmax * fmax(32.0 * (double)lmax, 8.0);
+	int ex = 0;
+	(void)frexp(bound, &ex); // bound < 2^ex: one contribution < 2^(ex - 6)
+	int k = 56 - ex;         // one contribution < 2^50 units, a row < 2^56
+	return k < -1000 ? -1000 : k > 1000 ? 1000 : k;
+}
+
+// round(v scale) as a two's-complement 64-bit integer, for |v scale| < 2^51: added to
+// 1.5 * 2^52 in one rounding (fma), the integer lands in the low mantissa bits of a double
+// whose exponent is fixed, so the integer is the bit pattern minus that of 1.5 * 2^52.
+__device__ __forceinline__ unsigned long long to_fixed(float v, double scale) {
+	const double y = __builtin_fma((double)v, scale, 0x1.8p52);
+	return (unsigned long long)__double_as_longlong(y) - 0x4338000000000000ull;
+}
+
 
 } // namespace lucid

@@ -28,14 +28,22 @@ hipError_t launch_raster(const FrameConst &c, const FrameBufs &b, hipStream_t st
 						 hipEvent_t *events);
 hipError_t launch_backward_prepare(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
 								   hipStream_t stream);
-hipError_t launch_backward_blocks(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
-								  hipStream_t stream);
-size_t reduce_sort_temp_bytes(uint32_t n);
-hipError_t reduce_gradients(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
-							uint32_t nrows, uint32_t ntri, uint32_t nverts, uint32_t *keys_out,
-							uint32_t *vals_in, uint32_t *vals_out, uint32_t *keys_in2,
-							float *tri_grad, void *temp, size_t temp_bytes, float *grad_colors,
-							float *grad_alpha, hipStream_t stream);
+hipError_t launch_backward_reuse(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
+								 hipStream_t stream);
+hipError_t launch_backward_order(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
+								 hipStream_t stream);
+hipError_t launch_backward_rows(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
+								hipStream_t stream);
+size_t reduce_temp_bytes(uint32_t npairs, uint32_t nscan);
+hipError_t reduce_check_corners(const FrameConst &c, const FrameBufs &b, uint32_t ntri,
+								uint32_t *vkeys, uint32_t *changed, hipStream_t stream);
+hipError_t reduce_group_rows(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
+							 const ReduceBufs &r, uint32_t ntri, hipStream_t stream);
+hipError_t reduce_sum(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
+					  const ReduceBufs &r, uint32_t ntri, uint32_t nverts, bool regroup,
+					  float *grad_colors, float *grad_alpha, hipStream_t stream);
+hipError_t reduce_vertices(const ReduceBufs &r, uint32_t ntri, uint32_t nverts, bool regroup,
+						   float *grad_colors, float *grad_alpha, hipStream_t stream);
 } // namespace lucid
 
 using namespace lucid;
@@ -90,8 +98,25 @@ struct lucid_renderer {
 	DevBuf blk_slot, blk_flags, blk_sorted, blk_lists, blk_count, band_count, sorted_pool;
 	// backward
 	DevBuf grad_row_base, grad_range_base, grad_rows, grad_row_tri, grad_start, grad_seq,
-		red_keys_out, red_vals_in, red_vals_out, red_keys_in2, tri_grad, red_temp;
-	uint64_t grad_rows_cap = 0;
+		red_counts, red_offsets, red_grouped, tri_grad, red_temp;
+	uint64_t grad_rows_cap = 0, grad_seq_rows_cap = 0, grad_start_cap = 0;
+	// corners grouped by vertex (depends on the mesh only): the corner -> vertex map of the
+	// last backward pass, and the corners sorted by vertex
+	DevBuf vtx_keys, vtx_sorted, vtx_corners, vtx_iota;
+	uint64_t vtx_ntri = 0, vtx_nverts = 0;
+	bool vtx_valid = false;
+	// helper stream of the backward pass (the row grouping runs beside pass B)
+	hipStream_t aux = nullptr;
+	hipEvent_t aux_fork = nullptr, aux_join = nullptr;
+	// The setup records, bins and block lists in the buffers, and the inputs they were built
+	// from (valid once the frame that built them finished without an overflow).
+	struct Retained {
+		bool valid = false, pending = false;
+		lucid_frame_inputs in;
+		lucid_config cfg;
+		lucid_tuning tuning;
+		std::vector<lucid_instance_data> instances;
+	} retained;
 
 	int64_t quad_cap = 0;
 	uint64_t bin_quads_cap = 0, bin_tris_cap = 0, high_records_cap = 0, spill_cap = 0;
@@ -114,6 +139,12 @@ struct lucid_renderer {
 	~lucid_renderer() {
 		if(done)
 			(void)hipEventDestroy(done);
+		if(aux_fork)
+			(void)hipEventDestroy(aux_fork);
+		if(aux_join)
+			(void)hipEventDestroy(aux_join);
+		if(aux)
+			(void)hipStreamDestroy(aux);
 		for(hipEvent_t e : stage_ev)
 			if(e)
 				(void)hipEventDestroy(e);
@@ -478,6 +509,40 @@ int prepare_frame(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_f
 
 } // namespace
 
+namespace {
+
+// Records the inputs the buffers are being built from; `pending` until the frame finishes.
+void retain_frame(lucid_renderer *r, const lucid_frame_inputs *in, bool pending) {
+	lucid_renderer::Retained &k = r->retained;
+	k.in = *in;
+	k.cfg = *in->config;
+	k.tuning = r->tuning;
+	k.instances.assign(in->instances, in->instances + std::max(in->num_instances, 0));
+	k.in.instances = nullptr;
+	k.in.config = nullptr;
+	k.pending = pending;
+	k.valid = !pending;
+}
+
+// The buffers hold the setup, bins and block lists of exactly these inputs.
+bool retained_matches(const lucid_renderer *r, const lucid_frame_inputs *in) {
+	const lucid_renderer::Retained &k = r->retained;
+	if(!k.valid || k.pending || in->num_instances != k.in.num_instances ||
+	   in->instance_colors != k.in.instance_colors || in->quad_indices != k.in.quad_indices ||
+	   in->positions != k.in.positions || in->num_verts != k.in.num_verts ||
+	   in->colors_rgba8 != k.in.colors_rgba8 || in->colors_rgb != k.in.colors_rgb ||
+	   in->tri_alpha != k.in.tri_alpha || in->normals != k.in.normals)
+		return false;
+	if(memcmp(&k.cfg, in->config, sizeof(lucid_config)) != 0 ||
+	   memcmp(&k.tuning, &r->tuning, sizeof(lucid_tuning)) != 0)
+		return false;
+	return in->num_instances <= 0 ||
+		   memcmp(k.instances.data(), in->instances,
+				  sizeof(lucid_instance_data) * (size_t)in->num_instances) == 0;
+}
+
+} // namespace
+
 int lucid_render(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_frame_outputs *out,
 				 void *stream_ptr) {
 	if(!r || !in || !out)
@@ -489,6 +554,7 @@ int lucid_render(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_fr
 	int pe = prepare_frame(r, in, out, c, b, nq);
 	if(pe != LUCID_OK)
 		return pe;
+	r->retained.valid = r->retained.pending = false; // the buffers are rebuilt below
 	(void)hipGetLastError(); // launches below report their own status, not an older call's
 
 	uint32_t n_info = LUCID_INFO_U32_SIZE + 10u * (uint32_t)r->bin_count;
@@ -507,12 +573,20 @@ int lucid_render(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_fr
 	r->frame_pending = true;
 	r->last_num_quads = nq;
 	r->last_stream = stream;
+	retain_frame(r, in, true);
 	return LUCID_OK;
 }
 
 int lucid_backward(lucid_renderer *r, const lucid_frame_inputs *in, const float *grad_image,
 				   float *grad_colors_rgb, float *grad_tri_alpha, void *stream_ptr) {
-	if(!r || !in || !grad_image || (!grad_colors_rgb && !grad_tri_alpha))
+	return lucid_backward_ex(r, in, grad_image, grad_colors_rgb, grad_tri_alpha, 0u, stream_ptr);
+}
+
+int lucid_backward_ex(lucid_renderer *r, const lucid_frame_inputs *in, const float *grad_image,
+					  float *grad_colors_rgb, float *grad_tri_alpha, uint32_t flags,
+					  void *stream_ptr) {
+	if(!r || !in || !grad_image || (!grad_colors_rgb && !grad_tri_alpha) ||
+	   (flags & ~LUCID_BACKWARD_REUSE_FRAME))
 		return LUCID_ERR_INVALID_ARG;
 	// the adjoint is of the fp32-colour, fp32-sample, "over" forward
 	if(!in->colors_rgb || !in->tri_alpha || in->colors_rgba8 || (r->opts & LUCID_OPT_ADDITIVE_BLENDING))
@@ -530,6 +604,10 @@ int lucid_backward(lucid_renderer *r, const lucid_frame_inputs *in, const float 
 	int pe = prepare_frame(r, in, nullptr, c, b, nq);
 	if(pe != LUCID_OK)
 		return pe;
+	const uint64_t ntri = 2 * (uint64_t)nq;
+	const uint64_t nverts = (uint64_t)std::max<int64_t>(in->num_verts, 0);
+	if(3 * ntri >= (1ull << 32))
+		return LUCID_ERR_UNSUPPORTED;
 	const size_t slots = (size_t)r->bin_count * 16;
 	LUCID_CHECK(r->grad_row_base.reserve(slots * 4));
 	LUCID_CHECK(r->grad_range_base.reserve(slots * 4));
@@ -538,78 +616,131 @@ int lucid_backward(lucid_renderer *r, const lucid_frame_inputs *in, const float 
 		LUCID_CHECK(r->grad_rows.reserve(r->grad_rows_cap * kGradWords * 4));
 		LUCID_CHECK(r->grad_row_tri.reserve(r->grad_rows_cap * 4));
 	}
+	if(r->grad_seq_rows_cap == 0) {
+		r->grad_seq_rows_cap = 1u << 16;
+		r->grad_start_cap = 1u << 12;
+		LUCID_CHECK(r->grad_seq.reserve(r->grad_seq_rows_cap * 64));
+		LUCID_CHECK(r->grad_start.reserve(r->grad_start_cap * 64 * sizeof(BwdStart)));
+	}
+	// corner -> vertex map, checked against the one the cached vertex grouping was built from
+	const bool vtx_same_sizes = r->vtx_valid && r->vtx_ntri == ntri && r->vtx_nverts == nverts;
+	LUCID_CHECK(r->vtx_keys.reserve(std::max<uint64_t>(3 * ntri, 1) * 4));
+	LUCID_CHECK(r->vtx_sorted.reserve(std::max<uint64_t>(3 * ntri, 1) * 4));
+	LUCID_CHECK(r->vtx_corners.reserve(std::max<uint64_t>(3 * ntri, 1) * 4));
+	LUCID_CHECK(r->vtx_iota.reserve(std::max<uint64_t>(3 * ntri, 1) * 4));
 	BwdBufs g;
 	g.grad_image = (const float4 *)grad_image;
 	g.row_base = r->grad_row_base.as<uint32_t>();
 	g.range_base = r->grad_range_base.as<uint32_t>();
-	g.start = nullptr;
-	g.seq = nullptr;
 	g.rows = r->grad_rows.as<float>();
 	g.row_tri = r->grad_row_tri.as<uint32_t>();
 	g.rows_cap = r->grad_rows_cap;
+	g.start = r->grad_start.as<BwdStart>();
+	g.start_cap = r->grad_start_cap;
+	g.seq = r->grad_seq.as<uint8_t>();
+	g.seq_rows_cap = r->grad_seq_rows_cap;
 
-	// forward stages (the fragment lists are recomputed), then the gradient row layout
+	if(!r->aux) {
+		LUCID_CHECK(hipStreamCreateWithFlags(&r->aux, hipStreamNonBlocking));
+		LUCID_CHECK(hipEventCreateWithFlags(&r->aux_fork, hipEventDisableTiming));
+		LUCID_CHECK(hipEventCreateWithFlags(&r->aux_join, hipEventDisableTiming));
+	}
+	LUCID_CHECK(r->red_counts.reserve((ntri + 1) * 4));
+	LUCID_CHECK(r->red_offsets.reserve((ntri + 1) * 4));
+	LUCID_CHECK(r->red_grouped.reserve(r->grad_rows_cap * 4));
+	LUCID_CHECK(r->tri_grad.reserve(std::max<uint64_t>(ntri, 1) * kGradWords * 4));
+	const size_t temp_bytes =
+		std::max<size_t>(reduce_temp_bytes((uint32_t)(3 * ntri), (uint32_t)(ntri + 1)), 16);
+	LUCID_CHECK(r->red_temp.reserve(temp_bytes));
+	ReduceBufs rb;
+	rb.counts = r->red_counts.as<uint32_t>();
+	rb.offsets = r->red_offsets.as<uint32_t>();
+	rb.grouped = r->red_grouped.as<uint32_t>();
+	rb.tri_grad = r->tri_grad.as<float>();
+	rb.vkeys = r->vtx_keys.as<uint32_t>();
+	rb.vsorted = r->vtx_sorted.as<uint32_t>();
+	rb.vcorners = r->vtx_corners.as<uint32_t>();
+	rb.viota = r->vtx_iota.as<uint32_t>();
+	rb.temp = r->red_temp.ptr;
+	rb.temp_bytes = r->red_temp.bytes;
+
+	// Forward stages (the fragment lists are recomputed), the gradient row layout, the
+	// recorded blend order (pass A) and the exact per-pixel adjoints; then the block rows
+	// (pass B) while the helper stream groups the rows by triangle; then the sums. Buffers
+	// have the sizes of the previous pass: a frame that needs more reports an overflow, and
+	// the buffers grow for the caller's next attempt.
 	const uint32_t n_info = LUCID_INFO_U32_SIZE + 10u * (uint32_t)r->bin_count;
-	LUCID_CHECK(launch_setup(c, b, n_info, stream));
-	LUCID_CHECK(launch_binning(c, b, stream));
-	LUCID_CHECK(launch_backward_prepare(c, b, g, stream));
+	const bool regroup = !vtx_same_sizes;
+	r->vtx_valid = false; // vtx_keys is rewritten below
+	const bool reuse = (flags & LUCID_BACKWARD_REUSE_FRAME) && retained_matches(r, in);
+	if(reuse) {
+		LUCID_CHECK(launch_backward_reuse(c, b, g, stream));
+	} else {
+		r->retained.valid = r->retained.pending = false; // the buffers are rebuilt below
+		LUCID_CHECK(launch_setup(c, b, n_info, stream));
+		LUCID_CHECK(launch_binning(c, b, stream));
+		LUCID_CHECK(launch_backward_prepare(c, b, g, stream));
+	}
+	LUCID_CHECK(reduce_check_corners(c, b, (uint32_t)ntri, rb.vkeys,
+									 r->ctrl.as<uint32_t>() + kCtrlTopoChanged, stream));
+	LUCID_CHECK(launch_backward_order(c, b, g, stream));
+	LUCID_CHECK(hipEventRecord(r->aux_fork, stream));
+	LUCID_CHECK(hipStreamWaitEvent(r->aux, r->aux_fork, 0));
+	LUCID_CHECK(reduce_group_rows(c, b, g, rb, (uint32_t)ntri, r->aux));
+	LUCID_CHECK(hipEventRecord(r->aux_join, r->aux));
+	LUCID_CHECK(launch_backward_rows(c, b, g, stream));
+	LUCID_CHECK(hipStreamWaitEvent(stream, r->aux_join, 0));
+	LUCID_CHECK(reduce_sum(c, b, g, rb, (uint32_t)ntri, (uint32_t)nverts, regroup,
+						   grad_colors_rgb, grad_tri_alpha, stream));
 	LUCID_CHECK(hipMemcpyAsync(r->status, r->ctrl.ptr, kCtrlWords * 4, hipMemcpyDeviceToHost,
 							   stream));
 	LUCID_CHECK(hipStreamSynchronize(stream));
 	r->last_num_quads = nq;
 	r->last_stream = stream;
+	const uint64_t block_rows = r->status[kCtrlBwdRows], ranges = r->status[kCtrlBwdRanges];
 	if(r->status[kCtrlOverflow]) {
+		// the recorded order and start states take block_rows and ranges (sizes of this
+		// frame, known once its row scan ran); the rows add room for the exact pixels
+		if(block_rows > r->grad_seq_rows_cap) {
+			r->grad_seq_rows_cap = block_rows + block_rows / 8;
+			LUCID_CHECK(r->grad_seq.reserve(r->grad_seq_rows_cap * 64));
+		}
+		if(ranges > r->grad_start_cap) {
+			r->grad_start_cap = ranges + ranges / 8;
+			LUCID_CHECK(r->grad_start.reserve(r->grad_start_cap * 64 * sizeof(BwdStart)));
+		}
+		const uint64_t extra = r->status[kCtrlBwdExtra];
+		const uint64_t want = block_rows + std::max<uint64_t>(std::max<uint64_t>(1u << 16, block_rows / 8),
+															  extra + extra / 8);
+		if(want > r->grad_rows_cap) {
+			LUCID_CHECK(r->grad_rows.reserve(want * kGradWords * 4));
+			LUCID_CHECK(r->grad_row_tri.reserve(want * 4));
+			r->grad_rows_cap = want;
+		}
+		r->status[kCtrlOverflow] &= ~kOvfGradRows; // grown above
+		// a reused frame stays valid when only the backward's own buffers overflowed
+		const uint32_t own = kOvfFallback | kOvfSpill;
+		if(!reuse || (r->status[kCtrlOverflow] & ~own))
+			r->retained.valid = false;
 		int gr = grow_after_overflow(r);
 		memset(r->status, 0, kCtrlWords * 4);
-		return gr;
+		return gr == LUCID_OK ? LUCID_ERR_CAPACITY : gr; // this pass's results are incomplete
 	}
-	// rows of the block kernel plus room for the exact per-pixel rows
-	const uint64_t block_rows = r->status[kCtrlBwdRows];
-	const uint64_t want = block_rows + std::max<uint64_t>(1u << 16, block_rows / 8);
-	if(want > r->grad_rows_cap) {
-		LUCID_CHECK(r->grad_rows.reserve(want * kGradWords * 4));
-		LUCID_CHECK(r->grad_row_tri.reserve(want * 4));
-		r->grad_rows_cap = want;
-		g.rows = r->grad_rows.as<float>();
-		g.row_tri = r->grad_row_tri.as<uint32_t>();
-		g.rows_cap = r->grad_rows_cap;
-	}
-	// pass B start states (64 per depth range) and the recorded blend order (a byte per
-	// sample; at most 64 samples per list entry)
-	const uint64_t ranges = r->status[kCtrlBwdRanges];
-	LUCID_CHECK(r->grad_start.reserve(std::max<uint64_t>(ranges, 1) * 64 * sizeof(BwdStart)));
-	LUCID_CHECK(r->grad_seq.reserve(std::max<uint64_t>(block_rows, 1) * 64));
-	g.start = r->grad_start.as<BwdStart>();
-	g.seq = r->grad_seq.as<uint8_t>();
-	LUCID_CHECK(launch_backward_blocks(c, b, g, stream));
-	LUCID_CHECK(hipMemcpyAsync(r->status, r->ctrl.ptr, kCtrlWords * 4, hipMemcpyDeviceToHost,
-							   stream));
-	LUCID_CHECK(hipStreamSynchronize(stream));
-	if(r->status[kCtrlOverflow]) {
-		int gr = grow_after_overflow(r);
-		memset(r->status, 0, kCtrlWords * 4);
-		return gr;
-	}
-	const uint64_t nrows = block_rows + r->status[kCtrlBwdExtra];
+	if(!reuse)
+		retain_frame(r, in, false); // the buffers now hold this frame's setup and lists
+	const bool changed = r->status[kCtrlTopoChanged] != 0;
 	memset(r->status, 0, kCtrlWords * 4);
-	const uint64_t ntri = 2 * (uint64_t)nq;
-	const uint64_t nverts = (uint64_t)std::max<int64_t>(in->num_verts, 0);
-	if(nrows >= (1ull << 32) || 3 * ntri >= (1ull << 32))
-		return LUCID_ERR_UNSUPPORTED;
-	const uint64_t npairs = std::max<uint64_t>(std::max<uint64_t>(nrows, 3 * ntri), 1);
-	LUCID_CHECK(r->red_keys_out.reserve(npairs * 4));
-	LUCID_CHECK(r->red_vals_in.reserve(npairs * 4));
-	LUCID_CHECK(r->red_vals_out.reserve(npairs * 4));
-	LUCID_CHECK(r->red_keys_in2.reserve(npairs * 4));
-	LUCID_CHECK(r->tri_grad.reserve(std::max<uint64_t>(ntri, 1) * kGradWords * 4));
-	const size_t temp_bytes = std::max<size_t>(reduce_sort_temp_bytes((uint32_t)npairs), 16);
-	LUCID_CHECK(r->red_temp.reserve(temp_bytes));
-	LUCID_CHECK(reduce_gradients(c, b, g, (uint32_t)nrows, (uint32_t)ntri, (uint32_t)nverts,
-								 r->red_keys_out.as<uint32_t>(), r->red_vals_in.as<uint32_t>(),
-								 r->red_vals_out.as<uint32_t>(), r->red_keys_in2.as<uint32_t>(),
-								 r->tri_grad.as<float>(), r->red_temp.ptr, r->red_temp.bytes,
-								 grad_colors_rgb, grad_tri_alpha, stream));
-	LUCID_CHECK(hipStreamSynchronize(stream));
+	bool grouped = grad_colors_rgb != nullptr && regroup;
+	if(changed && !regroup && grad_colors_rgb) {
+		// same sizes, different mesh: the vertex sums above used the old grouping
+		LUCID_CHECK(reduce_vertices(rb, (uint32_t)ntri, (uint32_t)nverts, true, grad_colors_rgb,
+									grad_tri_alpha, stream));
+		LUCID_CHECK(hipStreamSynchronize(stream));
+		grouped = true;
+	}
+	// the grouping matches vtx_keys if it was just built, or if nothing changed
+	r->vtx_valid = grouped || (!regroup && !changed);
+	r->vtx_ntri = ntri, r->vtx_nverts = nverts;
 	return LUCID_OK;
 }
 
@@ -839,6 +970,7 @@ int lucid_batch_render(lucid_batch *bt, const lucid_frame_inputs *ins,
 			return e;
 		r->last_num_quads = nq;
 		r->last_stream = stream;
+		r->retained.valid = r->retained.pending = false;
 	}
 	// re-record the graph only when a frame's constants or buffers changed
 	const bool same = bt->exec && bt->cs.size() == n &&
@@ -885,6 +1017,10 @@ int lucid_finish(lucid_renderer *r) {
 	r->frame_pending = false;
 	int g = grow_after_overflow(r);
 	memset(r->status, 0, kCtrlWords * 4);
+	if(r->retained.pending) {
+		r->retained.pending = false;
+		r->retained.valid = g == LUCID_OK;
+	}
 	return g;
 }
 

@@ -94,16 +94,53 @@ template <> struct SampleColor<true> {
 	}
 };
 
-// Per-channel lighting factor of finalShading (funcs.glsl:261-271): ambient + sun * light,
-// light = max(0, dot(-sun_dir, normal) * 0.7 + 0.3).
-__device__ __forceinline__ void light_factors(const FrameConst &c, V3 normal, float (&f)[3]) {
-	float light = fmx(0.0f, dot3(neg3(ld3(c.sun_dir)), normal) * 0.7f + 0.3f);
-#pragma unroll
-	for(int k = 0; k < 3; k++) {
-		float amb = c.ambient[k] * c.ambient_power;
-		float dif = (c.sun_color[k] * c.sun_power) * light;
-		f[k] = amb + dif;
+// Lighting of finalShading (funcs.glsl:261-271): light = max(0, dot(-sun_dir, normal) *
+// 0.7 + 0.3), and per channel the factor ambient + sun * light.
+__device__ __forceinline__ float light_value(const FrameConst &c, V3 normal) {
+	return fmx(0.0f, dot3(neg3(ld3(c.sun_dir)), normal) * 0.7f + 0.3f);
+}
+__device__ __forceinline__ float light_factor(const FrameConst &c, int k, float light) {
+	const float amb = c.ambient[k] * c.ambient_power;
+	const float dif = (c.sun_color[k] * c.sun_power) * light;
+	return amb + dif;
+}
+
+// sRGB round trip of finalShading, s = L2S(S2L(x) * f), unclamped. fp = f^(1/2.4).
+// Above both linear segments the two powers cancel: L2S(S2L(x) f) = (x + 0.055) fp - 0.055
+// exactly (the threshold S2L(x) f >= 0.0031308 is (x + 0.055) fp >= kSrgbHiU); below both,
+// s = 12.92 ((x / 12.92) f) with the reference's own operations. Only lanes in neither
+// case (or in a NaN case) evaluate the powers, in a branch the wave skips when none is.
+constexpr float kSrgbHiU = 0.09544991f; // 1.055 * 0.0031308^(1/2.4)
+struct LitChannel {
+	float s;     // L2S(S2L(x) f) before the clamp
+	float deriv; // ds/dx
+};
+template <bool DERIV>
+__device__ __forceinline__ LitChannel lit_channel(float x, float f, float fp) {
+	const float u = (x + 0.055f) * fp;
+	const float lin_lo = (1.0f / 12.92f) * x;
+	const bool hi = x >= 0.04045f && u >= kSrgbHiU;
+	const bool lo = x < 0.04045f && lin_lo * f < 0.0031308f;
+	LitChannel o;
+	o.s = hi ? u - 0.055f : 12.92f * (lin_lo * f);
+	o.deriv = hi ? fp : f;
+	if(!hi && !lo) {
+		const float lin = srgb_to_linear(x);
+		const float y = lin * f;
+		o.s = linear_to_srgb(y);
+		if constexpr(DERIV) {
+			// S2L'(x) = 2.4 S2L(x) / (x + 0.055), L2S'(y) = (L2S(y) + 0.055) / (2.4 y) above
+			// their linear segments; no further powers
+			const float d_lin = x < 0.04045f
+									? (1.0f / 12.92f)
+									: 2.4f * lin * __builtin_amdgcn_rcpf(x + 0.055f);
+			const float d_srgb = y < 0.0031308f
+									 ? 12.92f
+									 : (o.s + 0.055f) * __builtin_amdgcn_rcpf(2.4f * y);
+			o.deriv = d_srgb * f * d_lin;
+		}
 	}
+	return o;
 }
 
 // Shading record of one triangle, 7 x uint4 (the rasteriser stages one per list entry in LDS):
@@ -112,9 +149,10 @@ __device__ __forceinline__ void light_factors(const FrameConst &c, V3 normal, fl
 //   q3  RGBA8 colours: the triangle's three vertex colours | 0
 //       fp32 colours:  c0.rgb | triangle alpha
 //   q4  c1.rgb | L0;  q5 c2.rgb | L1
-//   q6  instance colour (RGBA8; white without one) | L2 | triangle id | 0
-// L0-L2 are the lighting factors of a flat-shaded triangle, or its three packed vertex
-// normals with LUCID_INST_HAS_VERTEX_NORMALS.
+//   q6  instance colour (RGBA8; white without one) | L2 | triangle id | light
+// A flat-shaded triangle keeps its light value and, in L0-L2, fp = f^(1/2.4) of its three
+// lighting factors f; with LUCID_INST_HAS_VERTEX_NORMALS, L0-L2 are the packed vertex
+// normals (light is then per pixel).
 constexpr int kRecQuads = 7;
 
 __device__ __forceinline__ void load_shading_record(const FrameConst &c, const FrameBufs &b,
@@ -139,14 +177,16 @@ __device__ __forceinline__ void load_shading_record(const FrameConst &c, const F
 	}
 	const uint32_t icol =
 		((flags & LUCID_INST_HAS_COLOR) && b.inst_colors) ? b.inst_colors[inst] : 0xffffffffu;
-	uint32_t l0, l1, l2;
+	uint32_t l0, l1, l2, lv = 0u;
 	if(flags & LUCID_INST_HAS_VERTEX_NORMALS) {
 		uint4 nq = b.quad_normal[t >> 1];
 		l0 = nq.x, l1 = s ? nq.z : nq.y, l2 = s ? nq.w : nq.z;
 	} else {
-		float f[3];
-		light_factors(c, decode_normal(b.tri_normal[tri_rec(b, t)]), f);
-		l0 = f2bits(f[0]), l1 = f2bits(f[1]), l2 = f2bits(f[2]);
+		const float light = light_value(c, decode_normal(b.tri_normal[tri_rec(b, t)]));
+		lv = f2bits(light);
+		l0 = f2bits(glsl_pow(light_factor(c, 0, light), 1.0f / 2.4f));
+		l1 = f2bits(glsl_pow(light_factor(c, 1, light), 1.0f / 2.4f));
+		l2 = f2bits(glsl_pow(light_factor(c, 2, light), 1.0f / 2.4f));
 	}
 	q4.w = l0;
 	q5.w = l1;
@@ -156,7 +196,7 @@ __device__ __forceinline__ void load_shading_record(const FrameConst &c, const F
 	q[3] = q3;
 	q[4] = q4;
 	q[5] = q5;
-	q[6] = make_uint4(icol, l2, t, 0u);
+	q[6] = make_uint4(icol, l2, t, lv);
 }
 
 // Depth plane at a pixel: inverse distance along the pixel's ray (shading.glsl:108).
@@ -173,6 +213,7 @@ struct ShadeState {
 	float4 color; // interpolated vertex colour (white without one) times the instance colour
 	float4 inst;  // instance colour (white without one)
 	float f[3];   // lighting factors
+	float fp[3];  // f^(1/2.4)
 };
 
 __device__ __forceinline__ bool shade_prepare(const FrameConst &c, const uint4 *q, float px,
@@ -227,9 +268,18 @@ __device__ __forceinline__ bool shade_prepare(const FrameConst &c, const uint4 *
 		n2 = sub3(n2, n0);
 		V3 normal = mk3(bary0 * n1.x + (bary1 * n2.x + n0.x), bary0 * n1.y + (bary1 * n2.y + n0.y),
 						bary0 * n1.z + (bary1 * n2.z + n0.z));
-		light_factors(c, normal, st.f);
+		const float light = light_value(c, normal);
+#pragma unroll
+		for(int k = 0; k < 3; k++) {
+			st.f[k] = light_factor(c, k, light);
+			st.fp[k] = glsl_pow(st.f[k], 1.0f / 2.4f);
+		}
 	} else {
-		st.f[0] = bits2f(q[4].w), st.f[1] = bits2f(q[5].w), st.f[2] = bits2f(q6.y);
+		const float light = bits2f(q6.w);
+#pragma unroll
+		for(int k = 0; k < 3; k++)
+			st.f[k] = light_factor(c, k, light);
+		st.fp[0] = bits2f(q[4].w), st.fp[1] = bits2f(q[5].w), st.fp[2] = bits2f(q6.y);
 	}
 	return true;
 }
@@ -248,7 +298,7 @@ __device__ __forceinline__ SampleColor<FP32> shade_record(const FrameConst &c, c
 	float rgb[3] = {st.color.x, st.color.y, st.color.z};
 #pragma unroll
 	for(int k = 0; k < 3; k++)
-		rgb[k] = sat(linear_to_srgb(srgb_to_linear(rgb[k]) * st.f[k]));
+		rgb[k] = sat(lit_channel<false>(rgb[k], st.f[k], st.fp[k]).s);
 	if constexpr(FP32)
 		out.col = make_float4(rgb[0], rgb[1], rgb[2], st.color.w);
 	else
@@ -733,32 +783,29 @@ __global__ __launch_bounds__(1024) void high_prep_kernel(FrameConst c, FrameBufs
 	}
 }
 
-// Block-row records of a high bin with the 4-row rule (raster_high.glsl:54-90). One
-// work-group of kRowgenHighThreads per high bin; each lane takes one triangle of the bin and
-// scans its rows once, from the first row of its 4-row group (the start row fixes the
-// scanline arithmetic) to its last row, emitting one record per 8-row band it covers. Record
-// and list positions come from wave-aggregated LDS counters, written out at the end.
-constexpr int kRowgenHighThreads = 1024;
-__global__ __launch_bounds__(kRowgenHighThreads) void rowgen_high_kernel(FrameConst c, FrameBufs b) {
+// Block-row records of a high bin with the 4-row rule (raster_high.glsl:54-90). Each lane
+// takes one triangle of the bin and scans its rows once, from the first row of its 4-row
+// group (the start row fixes the scanline arithmetic) to its last row, emitting one record
+// per 8-row band it covers; kRowgenSplit work-groups share a bin. Record and list
+// positions come from wave-aggregated atomics.
+__global__ __launch_bounds__(kRowgenThreads) void rowgen_high_kernel(FrameConst c, FrameBufs b) {
 	if(b.ctrl[kCtrlOverflow] & (kOvfBinLists | kOvfRecords | kOvfLists))
 		return;
 	const lucid_info *info = (const lucid_info *)b.info;
-	const uint32_t slot = blockIdx.x;
+	const uint32_t slot = blockIdx.x / kRowgenSplit;
+	const int split = blockIdx.x % kRowgenSplit;
 	if(slot >= (uint32_t)info->bin_level_counts[LUCID_BIN_LEVEL_HIGH])
 		return;
 	const uint32_t *counts = b.info + LUCID_INFO_U32_SIZE;
 	const int bin = (int)counts[LUCID_COUNTS_HIGH_BINS * c.bin_count + slot];
 	const BinInfo bi = load_bin(c, b, bin);
 	const uint4 *slot16 = b.blk_slot + (size_t)bin * 16;
-	__shared__ uint32_t s_band_n[4], s_blk_n[16];
-	if(threadIdx.x < 4)
-		s_band_n[threadIdx.x] = 0;
-	if(threadIdx.x < 16)
-		s_blk_n[threadIdx.x] = 0;
-	__syncthreads();
+	uint32_t *band_n = b.band_count + (size_t)bin * 4;
+	uint32_t *blk_n = b.blk_count + (size_t)bin * 16;
 	const int lane = lane_id();
-	const uint32_t wave_base = threadIdx.x & ~63u;
-	for(uint32_t base = wave_base; base < bi.nlevel; base += kRowgenHighThreads) {
+	const uint32_t stride = kRowgenThreads * kRowgenSplit;
+	const uint32_t wave_base = split * kRowgenThreads + (threadIdx.x & ~63u);
+	for(uint32_t base = wave_base; base < bi.nlevel; base += stride) {
 		const uint32_t i = base + lane;
 		const uint32_t t = i < bi.nlevel ? bin_tri_at(b, i, bi.qc, bi.qo, bi.to) : 0xffffffffu;
 		int row = 0, row_lo = 0, row_hi = -1;
@@ -809,7 +856,7 @@ __global__ __launch_bounds__(kRowgenHighThreads) void rowgen_high_kernel(FrameCo
 				continue;
 			uint32_t rbase = 0;
 			if(lane == 0)
-				rbase = atomicAdd(&s_band_n[by], (uint32_t)__popcll(emit));
+				rbase = atomicAdd(&band_n[by], (uint32_t)__popcll(emit));
 			rbase = __shfl(rbase, 0, 64);
 			const uint32_t idx = rbase + (uint32_t)__popcll(emit & ((1ull << lane) - 1ull));
 			if(bxm)
@@ -823,7 +870,7 @@ __global__ __launch_bounds__(kRowgenHighThreads) void rowgen_high_kernel(FrameCo
 					continue;
 				uint32_t lb = 0;
 				if(lane == 0)
-					lb = atomicAdd(&s_blk_n[by * 4 + bx], (uint32_t)__popcll(sel));
+					lb = atomicAdd(&blk_n[by * 4 + bx], (uint32_t)__popcll(sel));
 				lb = __shfl(lb, 0, 64);
 				if((bxm >> bx) & 1u)
 					b.blk_lists[slot16[by * 4 + bx].x + lb +
@@ -831,11 +878,6 @@ __global__ __launch_bounds__(kRowgenHighThreads) void rowgen_high_kernel(FrameCo
 			}
 		}
 	}
-	__syncthreads();
-	if(threadIdx.x < 4)
-		b.band_count[(size_t)bin * 4 + threadIdx.x] = s_band_n[threadIdx.x];
-	if(threadIdx.x < 16)
-		b.blk_count[(size_t)bin * 16 + threadIdx.x] = s_blk_n[threadIdx.x];
 }
 
 // Entries (triangles of one block in depth order) are handed to the rasteriser in slices
@@ -1505,9 +1547,8 @@ struct BwdSample {
 	__device__ __forceinline__ void zero() { r = g = b = a = b0 = b1 = dr = dg = db = da = 0.0f; }
 };
 
-// The fp32-sample shading of shade_record with the derivatives of the lighting stage,
-// s = sat(L2S(S2L(x) f)): S2L'(x) = 2.4 S2L(x) / (x + 0.055) above the linear segment, and
-// L2S'(y) = (L2S(y) + 0.055) / (2.4 y), so no further powers are evaluated.
+// The fp32-sample shading of shade_record with the derivative of the lighting stage,
+// s = sat(L2S(S2L(x) f)) (lit_channel: ds/dx = fp above both linear segments, f below both).
 __device__ __forceinline__ bool shade_grad(const FrameConst &c, const uint4 *q, float px, float py,
 										   float inv_ray, BwdSample &o) {
 	ShadeState st;
@@ -1520,14 +1561,10 @@ __device__ __forceinline__ bool shade_grad(const FrameConst &c, const uint4 *q, 
 	float sv[3], dv[3];
 #pragma unroll
 	for(int k = 0; k < 3; k++) {
-		const float lin = srgb_to_linear(x[k]);
-		const float y = lin * st.f[k];
-		const float s = linear_to_srgb(y);
-		sv[k] = sat(s);
-		const float d_lin = x[k] < 0.04045f ? (1.0f / 12.92f) : 2.4f * lin / (x[k] + 0.055f);
-		const float d_srgb = y < 0.0031308f ? 12.92f : (s + 0.055f) / (2.4f * y);
+		const LitChannel l = lit_channel<true>(x[k], st.f[k], st.fp[k]);
+		sv[k] = sat(l.s);
 		// clamp passes the gradient on [0, 1] (as torch.clamp does)
-		dv[k] = (s >= 0.0f && s <= 1.0f) ? d_srgb * st.f[k] * d_lin * ic[k] : 0.0f;
+		dv[k] = (l.s >= 0.0f && l.s <= 1.0f) ? l.deriv * ic[k] : 0.0f;
 	}
 	o.r = sv[0], o.g = sv[1], o.b = sv[2], o.a = st.color.w;
 	o.b0 = st.bary0, o.b1 = st.bary1;
@@ -1567,18 +1604,23 @@ struct BwdTotals {
 	}
 };
 
-__device__ __forceinline__ void bwd_fragment(double &t, double &rr, double &rg, double &rb,
-											 float gr, float gg, float gb, const BwdSample &s,
-											 float (&out)[kGradWords]) {
+// Adjoint of one fragment of front-to-back "over" compositing, out = sum_i s_i a_i T_i +
+// T_end bg. State: t = T before the fragment and q = g . R, the gradient-weighted remainder
+// R = out - (composite before the fragment), both fp64 (R cancels to tiny values behind
+// deep fragments). dL/da = g . (s T - R_after / (1 - a)) = T (g . s) - q_after / (1 - a),
+// dL/ds_c = g_c a T.
+__device__ __forceinline__ void bwd_fragment(double &t, double &q, float gr, float gg, float gb,
+											 const BwdSample &s, float (&out)[kGradWords]) {
 	const double a = s.a;
-	const double cr = (double)s.r * a * t, cg = (double)s.g * a * t, cb = (double)s.b * a * t;
-	const double ar = rr - cr, ag = rg - cg, ab = rb - cb; // remainder behind this fragment
-	const double om = 1.0 - a;
-	const double inv_om = om > 0.0 ? 1.0 / om : 0.0;
-	const double ga = (double)gr * ((double)s.r * t - ar * inv_om) +
-					  (double)gg * ((double)s.g * t - ag * inv_om) +
-					  (double)gb * ((double)s.b * t - ab * inv_om);
-	const float w = (float)(a * t);
+	const double at = a * t;
+	const double gs = (double)gr * (double)s.r + (double)gg * (double)s.g + (double)gb * (double)s.b;
+	const double q_after = q - at * gs; // g . (remainder behind this fragment)
+	const double om = 1.0 - a;          // exact: a is a float in [0, 1]
+	// 1 / om: the v_rcp_f32 seed (relative error ~1e-7) refined by one Newton step
+	const double r0 = (double)__builtin_amdgcn_rcpf((float)om);
+	const double inv_om = om > 0.0 ? r0 * (2.0 - om * r0) : 0.0;
+	const double ga = t * gs - q_after * inv_om;
+	const float w = (float)at;
 	const float xr = gr * w * s.dr, xg = gg * w * s.dg, xb = gb * w * s.db;
 	const float w0 = (1.0f - s.b0) - s.b1;
 	out[0] = xr * w0, out[1] = xg * w0, out[2] = xb * w0;
@@ -1586,7 +1628,7 @@ __device__ __forceinline__ void bwd_fragment(double &t, double &rr, double &rg, 
 	out[6] = xr * s.b1, out[7] = xg * s.b1, out[8] = xb * s.b1;
 	out[9] = (float)ga * s.da;
 	t *= om;
-	rr = ar, rg = ag, rb = ab;
+	q = q_after;
 }
 
 // Entries of one block in depth order (pre-sorted pool, or a register sort kept in LDS) and
@@ -1677,10 +1719,23 @@ __device__ __forceinline__ size_t seq_index(uint32_t row0, uint32_t s0, uint32_t
 	return ((size_t)row0 + s0 + j) * 64 + (size_t)lane;
 }
 
+// The recorded order, start states and rows are sized by the host from an earlier pass; a
+// frame that needs more skips the block and per-pixel adjoints and reports kOvfGradRows
+// (the host grows the buffers and the caller runs the backward pass again).
+__device__ __forceinline__ bool bwd_fits(const FrameBufs &b, const BwdBufs &g) {
+	const uint32_t rows = b.ctrl[kCtrlBwdRows], ranges = b.ctrl[kCtrlBwdRanges];
+	return rows <= g.seq_rows_cap && rows <= g.rows_cap && ranges <= g.start_cap;
+}
+
 __global__ __launch_bounds__(kBigWaves * 64) void bwd_pass_a_kernel(FrameConst c, FrameBufs b,
 																	 BwdBufs g) {
 	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
 		return;
+	if(!bwd_fits(b, g)) {
+		if(blockIdx.x == 0 && threadIdx.x == 0)
+			atomicOr(&b.ctrl[kCtrlOverflow], kOvfGradRows);
+		return;
+	}
 	__shared__ uint4 s_rec[kBigWaves][kSliceEntries * kRecQuads];
 	__shared__ uint32_t s_tri_all[kRegSortCap];
 	__shared__ uint64_t s_mask_all[kRegSortCap];
@@ -1717,6 +1772,9 @@ __global__ __launch_bounds__(kBigWaves * 64) void bwd_pass_a_kernel(FrameConst c
 			const uint32_t n = min(ent.e1 - s0, (uint32_t)kSliceEntries);
 			uint64_t msk[kSliceWords];
 			bwd_stage(c, b, ent, s0, n, s_rec[wave], msk);
+			// the triangle of each gradient row (pass B fills the row itself)
+			for(uint32_t e = lane; e < n; e += 64)
+				g.row_tri[(size_t)row0 + s0 + e] = ent.tri[s0 + e];
 			uint64_t bm[kSliceWords];
 			transpose_masks(msk, (int)((n + 63) / 64), bm);
 			uint32_t j = 0;
@@ -1793,53 +1851,26 @@ __global__ __launch_bounds__(kBigWaves * 64) void bwd_pass_a_kernel(FrameConst c
 		for(int k = 0; k < 3; k++)
 			gsel[k] = (o32[k] >= 0.0f && o32[k] <= 1.0f) ? gv[k] : 0.0f;
 	}
-	// max |dL/dpixel| of the frame: fixes pass B's fixed-point scale
-	float gm = fmaxf(fmaxf(fabsf(gsel[0]), fabsf(gsel[1])), fabsf(gsel[2]));
-#pragma unroll
-	for(int d = 1; d < 64; d <<= 1)
-		gm = fmaxf(gm, __shfl_xor(gm, d, 64));
-	if(lane == 0 && gm > 0.0f)
-		atomicMax(&b.ctrl[kCtrlBwdGradMax], f2bits(gm));
 	BwdStart st;
 	st.t = tb4;
-	st.rr = (ct[0] + tt * (double)bg[0]) - cb4[0];
-	st.rg = (ct[1] + tt * (double)bg[1]) - cb4[1];
-	st.rb = (ct[2] + tt * (double)bg[2]) - cb4[2];
+	st.q = 0.0;
+#pragma unroll
+	for(int k = 0; k < 3; k++)
+		st.q += (double)gsel[k] * ((ct[k] + tt * (double)bg[k]) - cb4[k]);
 	st.gr = gsel[0], st.gg = gsel[1], st.gb = gsel[2];
 	st.pad = 0.0f;
 	g.start[((size_t)g.range_base[slot] + wave) * 64 + lane] = st;
 }
 
-// Pass B sums gradient rows in LDS as 64-bit fixed point (integer LDS atomics; integer sums
-// are associative, so a row's value does not depend on the order its pixels add in). The
-// scale is a power of two fixed per frame from max |dL/dpixel| and a bound on one
-// fragment's contribution (|d sample / d colour| <= 12.92 * 2.4 * lighting, |d out / d a|
-// <= 2 per channel), such that a row of at most 64 contributions cannot overflow; the
-// resolution is then ~1e-12 of the largest possible contribution.
 constexpr int kGradStride = kGradWords + 1; // odd stride against bank conflicts
-
-__device__ __forceinline__ double grad_fixed_scale(const FrameConst &c, float gmax) {
-	float lmax = 0.0f; // bound on the lighting factor (light in [0, 1])
-#pragma unroll
-	for(int k = 0; k < 3; k++)
-		lmax = fmaxf(lmax, fabsf(c.ambient[k] * c.ambient_power) +
-							   fabsf(c.sun_color[k] * c.sun_power));
-	const double bound = 64.0 * (double)gmax * fmax(32.0 * (double)lmax, 8.0);
-	int ex = 0;
-	(void)frexp(bound, &ex); // bound < 2^ex
-	int k = 61 - ex;
-	k = k < 0 ? 0 : k > 1000 ? 1000 : k;
-	return ldexp(1.0, k);
-}
 
 // One wave per (block, depth range); single-wave work-groups (no LDS is shared between
 // waves), so that more of them fit on a CU.
 __global__ __launch_bounds__(64) void bwd_pass_b_kernel(FrameConst c, FrameBufs b, BwdBufs g) {
-	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
+	if((b.ctrl[kCtrlOverflow] & kOvfBinLists) || !bwd_fits(b, g))
 		return;
 	__shared__ uint4 s_rec[kSliceEntries * kRecQuads];
 	__shared__ unsigned long long s_grad[kSliceEntries * kGradStride];
-	__shared__ uint4 s_seq[kSliceEntries * 64 / 16]; // the slice's recorded order
 	__shared__ uint32_t s_tri_all[kRegSortCap];
 	__shared__ uint64_t s_mask_all[kRegSortCap];
 	const int lane = lane_id();
@@ -1866,41 +1897,40 @@ __global__ __launch_bounds__(64) void bwd_pass_b_kernel(FrameConst c, FrameBufs 
 	const BlockEntries ent = bwd_block_entries(c, b, slot, flags, sl, wave, s_tri_all, s_mask_all);
 	const uint32_t row0 = g.row_base[slot];
 	const BwdStart st = g.start[((size_t)g.range_base[slot] + wave) * 64 + lane];
-	double t = st.t, rr = st.rr, rg = st.rg, rb = st.rb;
+	double t = st.t, q = st.q;
 	const bool live = st.gr != 0.0f || st.gg != 0.0f || st.gb != 0.0f;
-	const double scale = grad_fixed_scale(c, bits2f(b.ctrl[kCtrlBwdGradMax]));
-	const double inv_scale = 1.0 / scale;
+	const int fix_k = grad_fixed_exp(c, bits2f(b.ctrl[kCtrlBwdGradMax]));
+	const double fix_scale = ldexp(1.0, fix_k);
 	for(uint32_t s0 = ent.e0; s0 < ent.e1; s0 += kSliceEntries) {
 		const uint32_t n = min(ent.e1 - s0, (uint32_t)kSliceEntries);
 		uint64_t msk[kSliceWords];
 		bwd_stage(c, b, ent, s0, n, s_rec, msk);
 		for(int i = lane; i < kSliceEntries * kGradStride; i += 64)
 			s_grad[i] = 0ull;
-		// the slice's recorded order (n * 64 bytes, 64-byte aligned) into LDS in one copy
-		const uint4 *src = (const uint4 *)(g.seq + seq_index(row0, s0, 0, 0));
-		for(uint32_t i = lane; i < n * 4; i += 64)
-			s_seq[i] = src[i];
 		wave_sync_lds();
-		const uint8_t *seq = (const uint8_t *)s_seq;
-		// replay the order recorded by pass A
+		// replay the order recorded by pass A (byte j of this lane at seq[64 j], read one
+		// step ahead; a lane stops at its end marker and reads nothing beyond it)
+		const uint8_t *seq = g.seq + seq_index(row0, s0, 0, lane);
+		uint8_t next = seq[0];
 		bool more = true;
 		for(uint32_t j = 0; j < n; j++) {
-			const uint8_t e = more ? seq[j * 64 + lane] : kSeqEnd;
+			const uint8_t e = more ? next : kSeqEnd;
 			more = e != kSeqEnd;
 			if(!ballot(more))
 				break;
 			if(more) {
+				if(j + 1 < n)
+					next = seq[(size_t)(j + 1) * 64];
 				const uint4 *rec = s_rec + (uint32_t)e * kRecQuads;
 				const float inv = record_inv_depth(rec[0], fx, fy);
 				BwdSample smp;
 				shade_grad(c, rec, fx, fy, inv, smp);
 				float out[kGradWords];
-				bwd_fragment(t, rr, rg, rb, st.gr, st.gg, st.gb, smp, out);
+				bwd_fragment(t, q, st.gr, st.gg, st.gb, smp, out);
 				if(live) {
 #pragma unroll
 					for(int k = 0; k < kGradWords; k++)
-						atomicAdd(&s_grad[(uint32_t)e * kGradStride + k],
-								  (unsigned long long)__double2ll_rn((double)out[k] * scale));
+						atomicAdd(&s_grad[(uint32_t)e * kGradStride + k], to_fixed(out[k], fix_scale));
 				}
 			}
 		}
@@ -1908,11 +1938,10 @@ __global__ __launch_bounds__(64) void bwd_pass_b_kernel(FrameConst c, FrameBufs 
 		for(uint32_t e = lane; e < n; e += 64) {
 			const size_t row = (size_t)row0 + s0 + e;
 			if(row < g.rows_cap) {
-				g.row_tri[row] = ent.tri[s0 + e];
 #pragma unroll
 				for(int k = 0; k < kGradWords; k++)
 					g.rows[row * kGradWords + k] =
-						(float)((double)(long long)s_grad[e * kGradStride + k] * inv_scale);
+						(float)ldexp((double)(long long)s_grad[e * kGradStride + k], -fix_k);
 			}
 		}
 		wave_sync_lds();
@@ -1971,6 +2000,27 @@ __global__ __launch_bounds__(1024) void grad_row_scan_kernel(FrameConst c, Frame
 		b.ctrl[kCtrlBwdExtra] = 0;
 		b.ctrl[kCtrlBwdGradMax] = 0;
 	}
+}
+
+__global__ __launch_bounds__(64) void clear_ctrl_kernel(uint32_t *ctrl) {
+	if(threadIdx.x < (uint32_t)kCtrlWords)
+		ctrl[threadIdx.x] = 0u;
+}
+
+// max |dL/dpixel| (rgb) over the image into kCtrlBwdGradMax (after the row scan cleared it):
+// fixes the frame's gradient fixed-point scale.
+__global__ __launch_bounds__(256) void grad_max_kernel(FrameConst c, BwdBufs g, uint32_t *ctrl) {
+	const uint32_t n = (uint32_t)c.width * (uint32_t)c.height;
+	float gm = 0.0f;
+	for(uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+		const float4 v = g.grad_image[i];
+		gm = fmaxf(gm, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fabsf(v.z)));
+	}
+#pragma unroll
+	for(int d = 1; d < 64; d <<= 1)
+		gm = fmaxf(gm, __shfl_xor(gm, d, 64));
+	if(lane_id() == 0 && gm > 0.0f)
+		atomicMax(&ctrl[kCtrlBwdGradMax], f2bits(gm)); // non-negative floats order as uints
 }
 
 // Exact adjoint of one queued pixel: its fragments sorted exactly (as fallback_pixel), then
@@ -2071,8 +2121,9 @@ __device__ void bwd_fallback_pixel(const FrameConst &c, const FrameBufs &b, cons
 		return;
 	const size_t row0 = (size_t)b.ctrl[kCtrlBwdRows] + base_row;
 	double t = 1.0;
-	double rr = tot.cr + tot.t * (double)bg[0], rg = tot.cg + tot.t * (double)bg[1],
-		   rb = tot.cb + tot.t * (double)bg[2];
+	double q = (double)gsel[0] * (tot.cr + tot.t * (double)bg[0]) +
+			   (double)gsel[1] * (tot.cg + tot.t * (double)bg[1]) +
+			   (double)gsel[2] * (tot.cb + tot.t * (double)bg[2]);
 	for(uint32_t base = 0; base < n; base += 64) {
 		BwdSample smp;
 		smp.zero();
@@ -2091,7 +2142,7 @@ __device__ void bwd_fallback_pixel(const FrameConst &c, const FrameBufs &b, cons
 			s.db = __shfl(smp.db, (int)j, 64), s.da = __shfl(smp.da, (int)j, 64);
 			if(__shfl((int)ok, (int)j, 64)) {
 				float out[kGradWords];
-				bwd_fragment(t, rr, rg, rb, gsel[0], gsel[1], gsel[2], s, out);
+				bwd_fragment(t, q, gsel[0], gsel[1], gsel[2], s, out);
 				if((uint32_t)lane == j) {
 #pragma unroll
 					for(int k = 0; k < kGradWords; k++)
@@ -2112,6 +2163,8 @@ __device__ void bwd_fallback_pixel(const FrameConst &c, const FrameBufs &b, cons
 __global__ __launch_bounds__(kFallbackThreads) void bwd_fallback_kernel(FrameConst c, FrameBufs b,
 																		 BwdBufs g) {
 	__shared__ uint64_t s_keys[kFallbackThreads / 64][kFallbackCap];
+	if((b.ctrl[kCtrlOverflow] & kOvfBinLists) || !bwd_fits(b, g))
+		return;
 	const uint32_t nq = min(b.ctrl[kCtrlFallbackCount], b.fallback_cap);
 	const uint32_t waves = gridDim.x * (kFallbackThreads / 64);
 	const uint32_t w = blockIdx.x * (kFallbackThreads / 64) + (threadIdx.x >> 6);
@@ -2126,8 +2179,8 @@ static void launch_raster_t(const FrameConst &c, const FrameBufs &b, hipStream_t
 	if(ev)
 		(void)hipEventRecord(ev[0], stream);
 	hipLaunchKernelGGL(high_prep_kernel, dim3(1), dim3(1024), 0, stream, c, b);
-	hipLaunchKernelGGL(rowgen_high_kernel, dim3(c.bin_count), dim3(kRowgenHighThreads), 0, stream,
-					   c, b);
+	hipLaunchKernelGGL(rowgen_high_kernel, dim3(c.bin_count * kRowgenSplit), dim3(kRowgenThreads), 0,
+					   stream, c, b);
 	hipLaunchKernelGGL(high_finalize_kernel, dim3(c.bin_count * 16), dim3(kSortThreads), 0, stream,
 					   c, b);
 	if(ev)
@@ -2162,28 +2215,60 @@ hipError_t launch_raster(const FrameConst &c, const FrameBufs &b, hipStream_t st
 }
 
 // Backward, first part: the forward's row generation and block sorting (fragment lists are
-// recomputed, not stored), then the gradient row allocation. The row counts are then read
-// back by the host to size the row buffers.
+// recomputed, not stored), then the gradient row allocation.
 hipError_t launch_backward_prepare(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
 								   hipStream_t stream) {
 	hipLaunchKernelGGL(rowgen_low_kernel, dim3(c.bin_count), dim3(kRowgenThreads), 0, stream, c, b);
 	hipLaunchKernelGGL(high_prep_kernel, dim3(1), dim3(1024), 0, stream, c, b);
-	hipLaunchKernelGGL(rowgen_high_kernel, dim3(c.bin_count), dim3(kRowgenHighThreads), 0, stream,
-					   c, b);
+	hipLaunchKernelGGL(rowgen_high_kernel, dim3(c.bin_count * kRowgenSplit), dim3(kRowgenThreads), 0,
+					   stream, c, b);
 	hipLaunchKernelGGL(high_finalize_kernel, dim3(c.bin_count * 16), dim3(kSortThreads), 0, stream,
 					   c, b);
 	hipLaunchKernelGGL(grad_row_scan_kernel, dim3(1), dim3(1024), 0, stream, c, b, g);
+	hipLaunchKernelGGL(grad_max_kernel, dim3(256), dim3(256), 0, stream, c, g, b.ctrl);
 	return hipGetLastError();
 }
 
-// Backward, second part: per-block adjoints and the exact per-pixel adjoints.
-hipError_t launch_backward_blocks(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
-								  hipStream_t stream) {
+// Pixels of the blocks whose list is too long to pre-sort go to the exact per-pixel path
+// (as high_finalize queues them); used when a finished frame's lists are reused.
+__global__ __launch_bounds__(64) void requeue_whole_kernel(FrameConst c, FrameBufs b) {
+	const uint32_t slot = blockIdx.x;
+	if(!(b.blk_flags[slot] & kSlotWhole))
+		return;
+	const int bin = slot >> 4, blk = slot & 15;
+	const int px = (bin % c.bin_count_x) * kBinSize + (blk & 3) * 8 + (threadIdx.x & 7);
+	const int py = (bin / c.bin_count_x) * kBinSize + (blk >> 2) * 8 + (threadIdx.x >> 3);
+	enqueue_pixels(b, slot, px < c.width && py < c.height, px, py, true);
+}
+
+// Backward, first part when the last finished frame's setup, bins and block lists are
+// reused: fresh control words, the exact-pixel queue of the too-long blocks, then the
+// gradient row allocation.
+hipError_t launch_backward_reuse(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
+								 hipStream_t stream) {
+	hipLaunchKernelGGL(clear_ctrl_kernel, dim3(1), dim3(64), 0, stream, b.ctrl);
+	hipLaunchKernelGGL(requeue_whole_kernel, dim3(c.bin_count * 16), dim3(64), 0, stream, c, b);
+	hipLaunchKernelGGL(grad_row_scan_kernel, dim3(1), dim3(1024), 0, stream, c, b, g);
+	hipLaunchKernelGGL(grad_max_kernel, dim3(256), dim3(256), 0, stream, c, g, b.ctrl);
+	return hipGetLastError();
+}
+
+// Backward, second part: the recorded blend order and start states (pass A, which also
+// fills the triangle of every block gradient row), then the exact per-pixel adjoints (their
+// rows follow the block rows). After these every row's triangle is known.
+hipError_t launch_backward_order(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
+								 hipStream_t stream) {
 	hipLaunchKernelGGL(bwd_pass_a_kernel, dim3(c.bin_count * 16), dim3(kBigWaves * 64), 0, stream,
 					   c, b, g);
+	hipLaunchKernelGGL(bwd_fallback_kernel, dim3(1024), dim3(kFallbackThreads), 0, stream, c, b, g);
+	return hipGetLastError();
+}
+
+// Backward, third part: the block gradient rows (pass B replays the recorded order).
+hipError_t launch_backward_rows(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
+								hipStream_t stream) {
 	hipLaunchKernelGGL(bwd_pass_b_kernel, dim3(c.bin_count * 16 * kBigWaves), dim3(64), 0, stream, c,
 					   b, g);
-	hipLaunchKernelGGL(bwd_fallback_kernel, dim3(1024), dim3(kFallbackThreads), 0, stream, c, b, g);
 	return hipGetLastError();
 }
 

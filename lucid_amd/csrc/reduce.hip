@@ -1,11 +1,14 @@
-// Gradient reduction of the backward pass: per-(block, entry) and per-fragment gradient rows
-// are summed per triangle, then per vertex, by stable radix sorts on the triangle / vertex
-// id (rocPRIM) followed by segmented sums in a fixed order. Rows of one key are summed in
-// ascending row order, so the result does not depend on scheduling, and no gradient value
-// goes through a global atomic.
+// Gradient reduction of the backward pass: the gradient rows (one per block list entry and
+// one per fragment of an exactly sorted pixel) are grouped by triangle id with a counting
+// sort (atomics only count and place row indices), and each triangle's group is summed in
+// 64-bit fixed point, whose sums do not depend on the order of the rows. The per-vertex
+// sums then run over the triangle corners grouped by vertex (a stable rocPRIM radix sort of
+// the corner -> vertex map, kept while the mesh does not change). No gradient value goes
+// through a global atomic, and the result does not depend on scheduling.
 #include <hip/hip_runtime.h>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "lucid_device.h"
 
@@ -19,62 +22,121 @@ __global__ void iota_kernel(uint32_t *v, uint32_t n) {
 		v[i] = i;
 }
 
-__global__ void zero_kernel(float *p, size_t n) {
+__global__ void zero_kernel(uint32_t *p, size_t n) {
 	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if(i < n)
-		p[i] = 0.0f;
+		p[i] = 0u;
 }
 
-// One thread per sorted position; the first position of each key sums its run of rows.
-__global__ void tri_segment_sum_kernel(const uint32_t *keys, const uint32_t *rows_sorted,
-									   uint32_t n, const float *rows, float *tri_grad) {
+// Gradient rows of the frame (block rows, then the rows of exactly sorted pixels), read on
+// the device so that the grouping can be enqueued before the counts are known; 0 after an
+// overflow (the pass is then repeated with larger buffers).
+__device__ __forceinline__ uint32_t frame_rows(const uint32_t *ctrl, const BwdBufs &g) {
+	if(ctrl[kCtrlOverflow])
+		return 0u;
+	const uint64_t n = (uint64_t)ctrl[kCtrlBwdRows] + ctrl[kCtrlBwdExtra];
+	return (uint32_t)(n < g.rows_cap ? n : g.rows_cap);
+}
+
+__global__ void row_hist_kernel(const uint32_t *ctrl, BwdBufs g, uint32_t ntri, uint32_t *counts) {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	if(i >= n)
+	if(i < frame_rows(ctrl, g)) {
+		const uint32_t t = g.row_tri[i];
+		if(t < ntri)
+			atomicAdd(&counts[t], 1u);
+	}
+}
+
+// Places every row index in its triangle's group; counts run back down to zero.
+__global__ void row_scatter_kernel(const uint32_t *ctrl, BwdBufs g, uint32_t ntri,
+								   const uint32_t *offsets, uint32_t *counts, uint32_t *grouped) {
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i < frame_rows(ctrl, g)) {
+		const uint32_t t = g.row_tri[i];
+		if(t < ntri)
+			grouped[offsets[t] + atomicSub(&counts[t], 1u) - 1u] = i;
+	}
+}
+
+// One thread per triangle: the sum of its group of rows in fixed point at the frame's scale
+// 2^(k - 6) (a row is below 2^50 units there), lowered by one bit per doubling of the group
+// beyond 4096 rows so that the sum stays below 2^62.
+__global__ void tri_group_sum_kernel(FrameConst c, const uint32_t *ctrl, const uint32_t *offsets,
+									 uint32_t ntri, const uint32_t *grouped, const float *rows,
+									 float *tri_grad) {
+	const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+	if(t >= ntri)
 		return;
-	const uint32_t k = keys[i];
-	if(i > 0 && keys[i - 1] == k)
-		return;
-	float sum[kGradWords];
+	const uint32_t o0 = offsets[t], o1 = offsets[t + 1];
+	const uint32_t n = o1 - o0;
+	int k = grad_fixed_exp(c, __uint_as_float(ctrl[kCtrlBwdGradMax])) - 6;
+	for(uint32_t cap = 4096; cap < n; cap <<= 1)
+		k--;
+	const double scale = ldexp(1.0, k);
+	unsigned long long acc[kGradWords];
 #pragma unroll
 	for(int w = 0; w < kGradWords; w++)
-		sum[w] = 0.0f;
-	for(uint32_t j = i; j < n && keys[j] == k; j++) {
-		const float *r = rows + (size_t)rows_sorted[j] * kGradWords;
+		acc[w] = 0ull;
+	uint32_t j = o0;
+	for(; j + 2 <= o1; j += 2) { // two independent row loads in flight
+		const float *r0 = rows + (size_t)grouped[j] * kGradWords;
+		const float *r1 = rows + (size_t)grouped[j + 1] * kGradWords;
+		float a[kGradWords], b[kGradWords];
 #pragma unroll
 		for(int w = 0; w < kGradWords; w++)
-			sum[w] += r[w];
+			a[w] = r0[w], b[w] = r1[w];
+#pragma unroll
+		for(int w = 0; w < kGradWords; w++)
+			acc[w] += to_fixed(a[w], scale) + to_fixed(b[w], scale);
+	}
+	if(j < o1) {
+		const float *r0 = rows + (size_t)grouped[j] * kGradWords;
+#pragma unroll
+		for(int w = 0; w < kGradWords; w++)
+			acc[w] += to_fixed(r0[w], scale);
 	}
 #pragma unroll
 	for(int w = 0; w < kGradWords; w++)
-		tri_grad[(size_t)k * kGradWords + w] = sum[w];
+		tri_grad[(size_t)t * kGradWords + w] = (float)ldexp((double)(long long)acc[w], -k);
 }
 
 // Vertex of each triangle corner: triangle 2 q + s of quad q uses quad vertices
 // (0, 1 + s, 2 + s), quads addressed through the instances (quad_setup.glsl:136-160).
-__global__ void corner_pairs_kernel(FrameConst c, FrameBufs b, uint32_t ntri, uint32_t *vkeys,
-									uint32_t *vvals) {
+// Writes the corner -> vertex map into `vkeys` and sets *changed when it differs from what
+// `vkeys` held (the vertex grouping of the previous backward pass is then stale).
+__global__ void corner_check_kernel(FrameConst c, FrameBufs b, uint32_t ntri, uint32_t *vkeys,
+									uint32_t *changed) {
 	const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-	if(t >= ntri)
-		return;
-	const uint32_t q = t >> 1, s = t & 1u;
-	int lo = 0, hi = c.num_instances - 1;
-	while(lo < hi) {
-		const int mid = (lo + hi + 1) >> 1;
-		if((uint32_t)b.inst_prefix[mid] <= q)
-			lo = mid;
-		else
-			hi = mid - 1;
+	bool diff = false;
+	if(t < ntri) {
+		const uint32_t q = t >> 1, s = t & 1u;
+		int lo = 0, hi = c.num_instances - 1;
+		while(lo < hi) {
+			const int mid = (lo + hi + 1) >> 1;
+			if((uint32_t)b.inst_prefix[mid] <= q)
+				lo = mid;
+			else
+				hi = mid - 1;
+		}
+		const lucid_instance_data ins = b.instances[lo];
+		const uint32_t k = q - (uint32_t)b.inst_prefix[lo];
+		const uint32_t *ix = b.quad_indices + (size_t)ins.index_offset + (size_t)k * 4;
+		const uint32_t v[3] = {ix[0] + (uint32_t)ins.vertex_offset,
+							   ix[1 + s] + (uint32_t)ins.vertex_offset,
+							   ix[2 + s] + (uint32_t)ins.vertex_offset};
+#pragma unroll
+		for(int i = 0; i < 3; i++)
+			if(vkeys[3 * t + i] != v[i]) {
+				vkeys[3 * t + i] = v[i];
+				diff = true;
+			}
 	}
-	const lucid_instance_data ins = b.instances[lo];
-	const uint32_t k = q - (uint32_t)b.inst_prefix[lo];
-	const uint32_t *ix = b.quad_indices + (size_t)ins.index_offset + (size_t)k * 4;
-	const uint32_t v0 = ix[0] + (uint32_t)ins.vertex_offset;
-	const uint32_t v1 = ix[1 + s] + (uint32_t)ins.vertex_offset;
-	const uint32_t v2 = ix[2 + s] + (uint32_t)ins.vertex_offset;
-	vkeys[3 * t + 0] = v0, vkeys[3 * t + 1] = v1, vkeys[3 * t + 2] = v2;
-	vvals[3 * t + 0] = 3 * t + 0, vvals[3 * t + 1] = 3 * t + 1, vvals[3 * t + 2] = 3 * t + 2;
+	if(__ballot(diff) != 0 && (threadIdx.x & 63) == 0)
+		atomicOr(changed, 1u);
 }
 
+// One thread per sorted corner position; the first position of each vertex sums its run of
+// corners in sorted (stable, corner id) order.
 __global__ void vertex_segment_sum_kernel(const uint32_t *keys, const uint32_t *corners,
 										  uint32_t n, const float *tri_grad, uint32_t nverts,
 										  float *grad_colors) {
@@ -114,55 +176,87 @@ inline dim3 grid_for(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
 } // namespace
 
-// Temporary storage of one stable key/value radix sort of n pairs.
-size_t reduce_sort_temp_bytes(uint32_t n) {
-	size_t bytes = 0;
-	(void)rocprim::radix_sort_pairs((void *)nullptr, bytes, (const uint32_t *)nullptr,
+// Temporary storage of the reduction: one stable key/value radix sort of `npairs` pairs or
+// one exclusive scan of `nscan` counts.
+size_t reduce_temp_bytes(uint32_t npairs, uint32_t nscan) {
+	size_t sort = 0, scan = 0;
+	(void)rocprim::radix_sort_pairs((void *)nullptr, sort, (const uint32_t *)nullptr,
 									(uint32_t *)nullptr, (const uint32_t *)nullptr,
-									(uint32_t *)nullptr, n, 0, 32);
-	return bytes;
+									(uint32_t *)nullptr, npairs, 0, 32);
+	(void)rocprim::exclusive_scan((void *)nullptr, scan, (const uint32_t *)nullptr,
+								  (uint32_t *)nullptr, 0u, (size_t)nscan,
+								  rocprim::plus<uint32_t>());
+	return sort > scan ? sort : scan;
 }
 
-// Sums the gradient rows into dL/d(vertex rgb) (3 floats per vertex) and dL/d(triangle
-// alpha) (per triangle id). Scratch: keys/vals of max(nrows, 3 ntri) pairs, tri_grad of
-// ntri * kGradWords floats, `temp` of reduce_sort_temp_bytes bytes.
-hipError_t reduce_gradients(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
-							uint32_t nrows, uint32_t ntri, uint32_t nverts, uint32_t *keys_out,
-							uint32_t *vals_in, uint32_t *vals_out, uint32_t *keys_in2,
-							float *tri_grad, void *temp, size_t temp_bytes, float *grad_colors,
-							float *grad_alpha, hipStream_t stream) {
-	hipError_t e;
-	hipLaunchKernelGGL(zero_kernel, grid_for((size_t)ntri * kGradWords), dim3(256), 0, stream,
-					   tri_grad, (size_t)ntri * kGradWords);
-	if(nrows) {
-		hipLaunchKernelGGL(iota_kernel, grid_for(nrows), dim3(256), 0, stream, vals_in, nrows);
-		size_t bytes = temp_bytes;
-		e = rocprim::radix_sort_pairs(temp, bytes, (const uint32_t *)g.row_tri, keys_out,
-									  (const uint32_t *)vals_in, vals_out, nrows, 0,
-									  key_bits(ntri), stream);
-		if(e != hipSuccess)
-			return e;
-		hipLaunchKernelGGL(tri_segment_sum_kernel, grid_for(nrows), dim3(256), 0, stream,
-						   keys_out, vals_out, nrows, g.rows, tri_grad);
-	}
+// Corner -> vertex map of the frame's triangles into vkeys (3 per triangle); sets *changed
+// when it differs from the map already there.
+hipError_t reduce_check_corners(const FrameConst &c, const FrameBufs &b, uint32_t ntri,
+								uint32_t *vkeys, uint32_t *changed, hipStream_t stream) {
+	if(ntri)
+		hipLaunchKernelGGL(corner_check_kernel, grid_for(ntri), dim3(256), 0, stream, c, b, ntri,
+						   vkeys, changed);
+	return hipGetLastError();
+}
+
+hipError_t reduce_vertices(const ReduceBufs &r, uint32_t ntri, uint32_t nverts, bool regroup,
+						   float *grad_colors, float *grad_alpha, hipStream_t stream);
+
+// Groups the gradient rows by triangle (counts, offsets, grouped row indices); the row
+// count is read on the device, `rows_cap` bounds the grid.
+hipError_t reduce_group_rows(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
+							 const ReduceBufs &r, uint32_t ntri, hipStream_t stream) {
+	hipLaunchKernelGGL(zero_kernel, grid_for((size_t)ntri + 1), dim3(256), 0, stream, r.counts,
+					   (size_t)ntri + 1);
+	if(g.rows_cap)
+		hipLaunchKernelGGL(row_hist_kernel, grid_for(g.rows_cap), dim3(256), 0, stream, b.ctrl, g,
+						   ntri, r.counts);
+	size_t bytes = r.temp_bytes;
+	hipError_t e = rocprim::exclusive_scan(r.temp, bytes, (const uint32_t *)r.counts, r.offsets,
+										   0u, (size_t)ntri + 1, rocprim::plus<uint32_t>(), stream);
+	if(e != hipSuccess)
+		return e;
+	if(g.rows_cap)
+		hipLaunchKernelGGL(row_scatter_kernel, grid_for(g.rows_cap), dim3(256), 0, stream, b.ctrl,
+						   g, ntri, r.offsets, r.counts, r.grouped);
+	return hipGetLastError();
+}
+
+// Sums the grouped rows into dL/d(vertex rgb) (3 floats per vertex) and dL/d(triangle
+// alpha) (per triangle id). The corners grouped by vertex (r.vkeys sorted into r.vsorted /
+// r.vcorners) depend only on the mesh: they are regrouped only when `regroup` is set.
+hipError_t reduce_sum(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
+					  const ReduceBufs &r, uint32_t ntri, uint32_t nverts, bool regroup,
+					  float *grad_colors, float *grad_alpha, hipStream_t stream) {
+	if(ntri)
+		hipLaunchKernelGGL(tri_group_sum_kernel, grid_for(ntri), dim3(256), 0, stream, c, b.ctrl,
+						   r.offsets, ntri, r.grouped, g.rows, r.tri_grad);
+	return reduce_vertices(r, ntri, nverts, regroup, grad_colors, grad_alpha, stream);
+}
+
+// Per-vertex colour sums (and the alpha copy) from r.tri_grad.
+hipError_t reduce_vertices(const ReduceBufs &r, uint32_t ntri, uint32_t nverts, bool regroup,
+						   float *grad_colors, float *grad_alpha, hipStream_t stream) {
 	if(grad_colors) {
 		hipLaunchKernelGGL(zero_kernel, grid_for((size_t)nverts * 3), dim3(256), 0, stream,
-						   grad_colors, (size_t)nverts * 3);
+						   (uint32_t *)grad_colors, (size_t)nverts * 3);
 		const uint32_t np = 3 * ntri;
-		hipLaunchKernelGGL(corner_pairs_kernel, grid_for(ntri), dim3(256), 0, stream, c, b, ntri,
-						   keys_in2, vals_in);
-		size_t bytes = temp_bytes;
-		e = rocprim::radix_sort_pairs(temp, bytes, (const uint32_t *)keys_in2, keys_out,
-									  (const uint32_t *)vals_in, vals_out, np, 0,
-									  key_bits(nverts), stream);
-		if(e != hipSuccess)
-			return e;
-		hipLaunchKernelGGL(vertex_segment_sum_kernel, grid_for(np), dim3(256), 0, stream,
-						   keys_out, vals_out, np, tri_grad, nverts, grad_colors);
+		if(regroup && np) {
+			hipLaunchKernelGGL(iota_kernel, grid_for(np), dim3(256), 0, stream, r.viota, np);
+			size_t bytes = r.temp_bytes;
+			hipError_t e = rocprim::radix_sort_pairs(
+				r.temp, bytes, (const uint32_t *)r.vkeys, r.vsorted, (const uint32_t *)r.viota,
+				r.vcorners, np, 0, key_bits(nverts), stream);
+			if(e != hipSuccess)
+				return e;
+		}
+		if(np)
+			hipLaunchKernelGGL(vertex_segment_sum_kernel, grid_for(np), dim3(256), 0, stream,
+							   r.vsorted, r.vcorners, np, r.tri_grad, nverts, grad_colors);
 	}
-	if(grad_alpha)
-		hipLaunchKernelGGL(alpha_grad_kernel, grid_for(ntri), dim3(256), 0, stream, tri_grad, ntri,
-						   grad_alpha);
+	if(grad_alpha && ntri)
+		hipLaunchKernelGGL(alpha_grad_kernel, grid_for(ntri), dim3(256), 0, stream, r.tri_grad,
+						   ntri, grad_alpha);
 	return hipGetLastError();
 }
 

@@ -178,9 +178,12 @@ class LucidRenderer:
         raise _lib.LucidError(_lib.LUCID_ERR_CAPACITY, "render")
 
     def backward(self, gscene: GpuScene, grad_image, colors_rgb=None, tri_alpha=None,
-                 stream=None, max_retries: int = 4):
+                 stream=None, max_retries: int = 4, reuse_frame: bool = False):
         """Adjoint of the fp32-sample forward: dL/d(image_f) (H x W x 4 fp32 device tensor)
-        -> (dL/d(colors_rgb) [V, 3], dL/d(tri_alpha) [2 Q]). Recomputes the fragment lists."""
+        -> (dL/d(colors_rgb) [V, 3], dL/d(tri_alpha) [2 Q]). The per-pixel sorted fragment
+        lists are recomputed. With `reuse_frame`, the setup, bins and block lists of the
+        renderer's last finished frame are reused when it was rendered from the same inputs
+        (whose contents must be unchanged); the result is identical either way."""
         torch = self._torch
         dev = torch.device("cuda", self.device)
         st = stream if stream is not None else torch.cuda.current_stream(self.device)
@@ -191,14 +194,15 @@ class LucidRenderer:
         g_col = torch.empty((nv, 3), dtype=torch.float32, device=dev)
         g_alpha = torch.empty((2 * gscene.scene.num_quads,), dtype=torch.float32, device=dev)
         inp = gscene.inputs(colors_rgb, tri_alpha)
+        flags = _lib.BACKWARD_REUSE_FRAME if reuse_frame else 0
         for _ in range(max_retries + 1):
-            rc = self.lib.lucid_backward(self.handle, ctypes.byref(inp), g_img.data_ptr(),
-                                         g_col.data_ptr(), g_alpha.data_ptr(),
-                                         ctypes.c_void_p(st.cuda_stream))
+            rc = self.lib.lucid_backward_ex(self.handle, ctypes.byref(inp), g_img.data_ptr(),
+                                            g_col.data_ptr(), g_alpha.data_ptr(), flags,
+                                            ctypes.c_void_p(st.cuda_stream))
             if rc == _lib.LUCID_OK:
                 return g_col, g_alpha
             if rc != _lib.LUCID_ERR_CAPACITY:
-                _lib.check(rc, "lucid_backward")
+                _lib.check(rc, "lucid_backward_ex")
         raise _lib.LucidError(_lib.LUCID_ERR_CAPACITY, "lucid_backward")
 
     # ------------------------------------------------------------ readback / stats

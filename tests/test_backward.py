@@ -161,3 +161,35 @@ def test_backward_is_deterministic(oracle_mod):
     a = _gpu_grads(sc, target.numpy())
     b = _gpu_grads(sc, target.numpy())
     assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+
+
+@pytest.mark.gpu
+def test_backward_reusing_the_frame_matches_full_recompute(oracle_mod):
+    """lucid_backward_ex with LUCID_BACKWARD_REUSE_FRAME (the autograd path) gives the same
+    gradients, bit for bit, as the full recompute; it falls back to the full recompute when
+    another frame was rendered in between. Covers ordering violations (tilted planes: exact
+    per-pixel adjoints) and a block too long to pre-sort (block stack: its pixels are queued
+    again from the reused lists)."""
+    import torch
+    from lucid_amd import _lib
+    from lucid_amd.renderer import GpuScene, LucidRenderer
+    for sc in (scenes.config0(num_tris=1500, width=96, height=96, seed=35),
+               scenes.tilted_planes(40, 32, 24, seed=12),
+               scenes.block_stack(32, 32, 8300, center_px=(12, 12), size_px=4.0, seed=14)):
+        print(f"\n[case] backward reuse {sc.name} {sc.width}x{sc.height}", flush=True)
+        r = LucidRenderer((sc.width, sc.height), opts=_lib.OPT_FP32_SAMPLES)
+        gs = GpuScene(sc)
+        other = GpuScene(scenes.config0(num_tris=500, width=sc.width, height=sc.height, seed=36))
+        out = r.alloc_outputs(rgba8=False, rgba32f=True, counts=False)
+        gen = torch.Generator(device="cuda").manual_seed(7)
+        grad = torch.randn((sc.height, sc.width, 4), device="cuda", generator=gen) * 1e-2
+        r.render(gs, out)
+        full = r.backward(gs, grad)                   # rebuilds everything
+        r.render(gs, out)
+        reused = r.backward(gs, grad, reuse_frame=True)
+        again = r.backward(gs, grad, reuse_frame=True)  # the lists stay valid
+        r.render(other, out)
+        fallback = r.backward(gs, grad, reuse_frame=True)  # other inputs: full recompute
+        torch.cuda.synchronize()
+        for got in (reused, again, fallback):
+            assert torch.equal(got[0], full[0]) and torch.equal(got[1], full[1])
