@@ -28,15 +28,15 @@ constexpr int kLowRowCap = 1024;     // records per block row of a low bin (< 10
 // Internal control words (separate from LucidInfo, whose layout is the reference's).
 enum CtrlWord : int {
 	kCtrlNumLarge = 0,      // entries in the large-quad list
-	kCtrlRecordCursor = 1,  // bump allocator of the raster_high record pool (uint4 units)
+	kCtrlRecordCursor = 1,  // high-bin record pool size the frame needs (uint4 units)
 	kCtrlFallbackCount = 2, // pixels in the exact-fallback queue
 	kCtrlOverflow = 3,      // bit 0 bin lists, 1 record pool, 2 fallback queue, 3 spill pool
 	kCtrlSpillCursor = 4,   // bump allocator of the deep-pixel spill pool (u64 units)
 	kCtrlTotalBinQuads = 5,
 	kCtrlTotalBinTris = 6,
 	kCtrlFallbackPixels = 7,
-	kCtrlListCursor = 8,   // bump allocator of per-block index lists of high bins (u32 units)
-	kCtrlSortedCursor = 10, // bump allocator of pre-sorted block arrays (u32 units)
+	kCtrlListCursor = 8,    // high-bin block-list space the frame needs (u32 units)
+	kCtrlSortedCursor = 10, // pre-sorted pool size a frame needed, after an overflow (u32 units)
 	kCtrlBwdRows = 11,      // gradient rows of the block kernel (set by the row scan)
 	kCtrlBwdExtra = 12,     // bump allocator of per-fragment gradient rows of exact pixels
 	kCtrlBwdRanges = 13,    // depth ranges of the backward block kernels (set by the row scan)
@@ -44,6 +44,26 @@ enum CtrlWord : int {
 	kCtrlTopoChanged = 15,  // backward: the corner -> vertex map differs from the cached one
 	kCtrlWords = 16,
 };
+
+// Frame statistics summed per block (fragments, list entries) go to kStatStripes counters
+// per statistic, one 128-byte line each, after the control words in the same buffer: the
+// block kernels add to stripe (slot % kStatStripes), so that tens of thousands of
+// work-groups do not serialise on one address; the fallback kernel folds them into
+// LucidInfo once.
+constexpr int kStatStripes = 64;
+constexpr int kStatStride = 32; // u32 words per stripe
+constexpr int kStatStripeBase = 32;
+__device__ __forceinline__ uint32_t *stat_stripe(uint32_t *ctrl, int stat, uint32_t stripe) {
+	return ctrl + kStatStripeBase + (stat * kStatStripes + (int)stripe) * kStatStride;
+}
+// The pre-sorted block pool is split into kSortStripes equal regions with a bump cursor
+// each (block slot % kSortStripes picks one), for the same reason.
+constexpr int kSortStripes = 16;
+constexpr int kSortCursorBase = kStatStripeBase + 2 * kStatStripes * kStatStride;
+constexpr int kCtrlAllocWords = kSortCursorBase + kSortStripes * kStatStride;
+__device__ __forceinline__ uint32_t *sort_cursor(uint32_t *ctrl, uint32_t stripe) {
+	return ctrl + kSortCursorBase + (int)stripe * kStatStride;
+}
 
 // Overflow bits of kCtrlOverflow
 constexpr uint32_t kOvfBinLists = 1u, kOvfRecords = 2u, kOvfFallback = 4u, kOvfSpill = 8u,

@@ -326,7 +326,7 @@ int lucid_create(const lucid_create_info *ci, lucid_renderer **out) {
 			rc = hip_status(e);
 	};
 	chk(r->info.reserve(((size_t)LUCID_INFO_U32_SIZE + 10 * (size_t)NB) * 4));
-	chk(r->ctrl.reserve(kCtrlWords * 4));
+	chk(r->ctrl.reserve(kCtrlAllocWords * 4));
 	chk(r->wg_hist.reserve((size_t)kNumBinWgs * 2 * NB * 4));
 	chk(r->low_records.reserve((size_t)NB * 4 * kLowRowCap * 16));
 	// one 16-byte item per pixel sent to the exact sort; grows after an overflow

@@ -768,6 +768,10 @@ __global__ __launch_bounds__(1024) void high_prep_kernel(FrameConst c, FrameBufs
 			bool ok = rec + 4ull * n <= b.high_records_cap && lst + 16ull * n <= b.blk_lists_cap;
 			if(!ok)
 				atomicOr(&b.ctrl[kCtrlOverflow], kOvfRecords | kOvfLists);
+			if(i == nh - 1) { // sizes the host grows the pools to after an overflow
+				b.ctrl[kCtrlRecordCursor] = (uint32_t)min(rec + 4ull * n, 0xffffffffull);
+				b.ctrl[kCtrlListCursor] = (uint32_t)min(16ull * (ex + n), 0xffffffffull);
+			}
 			for(int blk = 0; blk < 16; blk++) {
 				int band = blk >> 2, bx = blk & 3;
 				b.blk_slot[(size_t)bin * 16 + blk] =
@@ -1009,10 +1013,16 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 		for(int w = 0; w <= kBigWaves; w++)
 			s_bound[w] = w == 0 ? 0u : m;
 		const uint32_t need = sorted_mask_offset(m) + 2 * m + 1;
-		uint32_t base = atomicAdd(&b.ctrl[kCtrlSortedCursor], need);
-		if((uint64_t)base + need > b.sorted_cap) {
+		const uint32_t stripe = slot % kSortStripes;
+		const uint64_t region = (b.sorted_cap / kSortStripes) & ~1ull;
+		const uint32_t off = atomicAdd(sort_cursor(b.ctrl, stripe), need);
+		uint32_t base = 0xffffffffu;
+		if((uint64_t)off + need <= region) {
+			base = (uint32_t)(stripe * region + off);
+		} else {
 			atomicOr(&b.ctrl[kCtrlOverflow], kOvfSorted);
-			base = 0xffffffffu;
+			const uint64_t want = ((uint64_t)off + need + 2) * kSortStripes; // + even rounding
+			atomicMax(&b.ctrl[kCtrlSortedCursor], (uint32_t)min(want, (uint64_t)0xffffffffu));
 		}
 		s_base = base == 0xffffffffu ? base : (base + 1u) & ~1u;
 	}
@@ -1177,10 +1187,9 @@ __device__ __forceinline__ void finish_pixels(const FrameConst &c, const FrameBu
 	enqueue_pixels(b, slot, bad && on_screen, px, py, false);
 	const uint32_t frags = __shfl(wave_inclusive_add(on_screen ? count : 0u), 63, 64);
 	if(lane == 0) {
-		lucid_info *info = (lucid_info *)b.info;
-		atomicAdd(&info->stats[0], frags);
+		atomicAdd(stat_stripe(b.ctrl, 0, slot % kStatStripes), frags);
 		if(m)
-			atomicAdd(&info->stats[1], m);
+			atomicAdd(stat_stripe(b.ctrl, 1, slot % kStatStripes), m);
 	}
 }
 
@@ -1511,6 +1520,18 @@ __device__ void fallback_pixel(const FrameConst &c, const FrameBufs &b, uint4 it
 template <bool FP32>
 __global__ __launch_bounds__(kFallbackThreads) void fallback_kernel(FrameConst c, FrameBufs b) {
 	__shared__ uint64_t s_keys[kFallbackThreads / 64][kFallbackCap];
+	if(blockIdx.x == 0 && threadIdx.x < 64) {
+		// the block kernels' statistics stripes into LucidInfo (stats[0] fragments, [1]
+		// block-list entries); the block kernels have finished
+		static_assert(kStatStripes == 64, "one stripe per lane");
+		uint32_t v[2] = {*stat_stripe(b.ctrl, 0, threadIdx.x), *stat_stripe(b.ctrl, 1, threadIdx.x)};
+#pragma unroll
+		for(int k = 0; k < 2; k++) {
+			v[k] = __shfl(wave_inclusive_add(v[k]), 63, 64);
+			if(threadIdx.x == 0 && v[k])
+				atomicAdd(&((lucid_info *)b.info)->stats[k], v[k]);
+		}
+	}
 	const uint32_t nq = min(b.ctrl[kCtrlFallbackCount], b.fallback_cap);
 	const uint32_t waves = gridDim.x * (kFallbackThreads / 64);
 	const uint32_t w = blockIdx.x * (kFallbackThreads / 64) + (threadIdx.x >> 6);

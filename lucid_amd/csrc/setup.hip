@@ -318,13 +318,13 @@ __global__ __launch_bounds__(256) void clear_kernel(uint32_t *info, uint32_t n_i
 	uint32_t i = blockIdx.x * 256u + threadIdx.x;
 	if(i < n_info)
 		info[i] = i == 0 ? num_quads : 0u;
-	if(i < (uint32_t)kCtrlWords)
-		ctrl[i] = 0u;
+	if(i < (uint32_t)kCtrlAllocWords)
+		ctrl[i] = 0u; // control words and statistics stripes
 }
 
 hipError_t launch_setup(const FrameConst &c, const FrameBufs &b, uint32_t n_info,
 						hipStream_t stream) {
-	uint32_t n = n_info > (uint32_t)kCtrlWords ? n_info : (uint32_t)kCtrlWords;
+	uint32_t n = n_info > (uint32_t)kCtrlAllocWords ? n_info : (uint32_t)kCtrlAllocWords;
 	hipLaunchKernelGGL(clear_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, b.info, n_info,
 					   b.ctrl, (uint32_t)c.num_quads);
 	if(c.num_quads > 0)
