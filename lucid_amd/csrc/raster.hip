@@ -858,10 +858,24 @@ __global__ __launch_bounds__(kRowgenThreads) void rowgen_high_kernel(FrameConst 
 			const uint64_t emit = ballot(bxm != 0);
 			if(!emit)
 				continue;
-			uint32_t rbase = 0;
+			// the band's record counter and its four block-list counters: lanes 0-4 reserve
+			// all five ranges with one atomic instruction (one round trip, not five)
+			uint64_t sel[4];
+#pragma unroll
+			for(int bx = 0; bx < 4; bx++)
+				sel[bx] = ballot((bxm >> bx) & 1u);
+			uint32_t add = 0;
+			uint32_t *ctr = band_n + by;
+#pragma unroll
+			for(int bx = 0; bx < 4; bx++)
+				if(lane == 1 + bx)
+					add = (uint32_t)__popcll(sel[bx]), ctr = blk_n + by * 4 + bx;
 			if(lane == 0)
-				rbase = atomicAdd(&band_n[by], (uint32_t)__popcll(emit));
-			rbase = __shfl(rbase, 0, 64);
+				add = (uint32_t)__popcll(emit);
+			uint32_t got = 0;
+			if(lane < 5 && add)
+				got = atomicAdd(ctr, add);
+			const uint32_t rbase = __shfl(got, 0, 64);
 			const uint32_t idx = rbase + (uint32_t)__popcll(emit & ((1ull << lane) - 1ull));
 			if(bxm)
 				b.high_records[slot16[by * 4].z + idx] =
@@ -869,16 +883,10 @@ __global__ __launch_bounds__(kRowgenThreads) void rowgen_high_kernel(FrameConst 
 							   hi | (bxm << 20));
 #pragma unroll
 			for(int bx = 0; bx < 4; bx++) {
-				const uint64_t sel = ballot((bxm >> bx) & 1u);
-				if(!sel)
-					continue;
-				uint32_t lb = 0;
-				if(lane == 0)
-					lb = atomicAdd(&blk_n[by * 4 + bx], (uint32_t)__popcll(sel));
-				lb = __shfl(lb, 0, 64);
+				const uint32_t lb = __shfl(got, 1 + bx, 64);
 				if((bxm >> bx) & 1u)
 					b.blk_lists[slot16[by * 4 + bx].x + lb +
-								(uint32_t)__popcll(sel & ((1ull << lane) - 1ull))] = idx;
+								(uint32_t)__popcll(sel[bx] & ((1ull << lane) - 1ull))] = idx;
 			}
 		}
 	}
@@ -957,7 +965,14 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 #pragma unroll
 		for(int e = 0; e < 4; e++) {
 			const uint32_t j = ch * 256 + e * 64 + lane_s;
-			v[e] = j < m ? block_key(b, recs[list[j]], bx, px0, py0, j) : ~0ull;
+			v[e] = ~0ull;
+			if(j < m) {
+				// low word: list position (13 bits) | samples in the block << 13; equal depths
+				// order by samples, then position (any fixed order is exact per pixel)
+				const uint4 rec = recs[list[j]];
+				const uint32_t smp = (uint32_t)__popcll(record_block_mask(rec, bx));
+				v[e] = block_key(b, rec, bx, px0, py0, j | (smp << 13));
+			}
 		}
 		bitonic_regs_from<4>(v, ch * 256);
 #pragma unroll
@@ -991,21 +1006,27 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 		}
 		__syncthreads();
 	}
-	// s_key[i] now holds the list position of the i-th entry in depth order.
+	// s_key[i] now holds (samples << 13 | list position) of the i-th entry in depth order.
+	static_assert(kBigSortCap <= (1 << 13), "list positions fit 13 bits of the key");
 	const uint32_t per = (m + kSortThreads - 1) / kSortThreads;
 	const uint32_t i0 = min(m, threadIdx.x * per), i1 = min(m, i0 + per);
 	uint32_t samples = 0;
 	for(uint32_t i = i0; i < i1; i++)
-		samples += (uint32_t)__popcll(record_block_mask(recs[list[(uint32_t)s_key[i]]], bx));
-	s_samples[threadIdx.x] = samples;
+		samples += ((uint32_t)s_key[i] >> 13) & 0x7fu;
+	// exclusive prefix over the threads: wave scans, then the wave totals
+	const uint32_t inc = wave_inclusive_add(samples);
+	if(lane_s == 63)
+		s_samples[wave_s] = inc;
 	__syncthreads();
+	uint32_t before = 0, tot = 0;
+#pragma unroll
+	for(int w = 0; w < kSortThreads / 64; w++) {
+		const uint32_t t = s_samples[w];
+		before += w < wave_s ? t : 0u;
+		tot += t;
+	}
+	const uint32_t run0 = before + inc - samples;
 	if(threadIdx.x == 0) {
-		uint32_t tot = 0;
-		for(int t = 0; t < kSortThreads; t++) {
-			uint32_t v = s_samples[t];
-			s_samples[t] = tot; // exclusive prefix
-			tot += v;
-		}
 		uint32_t nw = (tot + c.samples_per_wave - 1) / c.samples_per_wave;
 		nw = nw < 1u ? 1u : nw > (uint32_t)kBigWaves ? (uint32_t)kBigWaves : nw;
 		s_parts = nw;
@@ -1037,9 +1058,9 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 	uint32_t *tri_out = hdr + kSortedHeader;
 	uint64_t *mask_out = (uint64_t *)(hdr + sorted_mask_offset(m));
 	// wave w starts at the first entry with at least w * total / nw samples before it
-	uint32_t run = s_samples[threadIdx.x], found = 0;
+	uint32_t run = run0, found = 0;
 	for(uint32_t i = i0; i < i1; i++) {
-		uint4 rec = recs[list[(uint32_t)s_key[i]]];
+		uint4 rec = recs[list[(uint32_t)s_key[i] & 0x1fffu]];
 		uint64_t msk = record_block_mask(rec, bx);
 		for(uint32_t w = 1; w < nw; w++) {
 			uint32_t target = (uint32_t)(((uint64_t)w * total) / nw);
