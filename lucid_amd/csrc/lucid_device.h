@@ -155,11 +155,7 @@ struct FrameBufs {
 	uint32_t *quad_aabb;  // per quad
 	uint4 *tri_scan;      // 2 per tri
 	uint4 *tri_depth;     // 1 per tri
-	uint4 *tri_bary;      // 2 per tri
-	uint32_t *tri_normal; // 1 per tri
-	uint4 *quad_color;    // 1 per quad (RGBA8 vertex colours)
-	uint4 *quad_normal;   // 1 per quad (vertex normals)
-	float4 *tri_color;    // 3 per tri (fp32 vertex colours: rgb0+alpha, rgb1, rgb2)
+	uint4 *tri_shade;     // kRecStride per tri: the shading record the rasterisers stage
 	uint32_t *large_list; // quads whose bin AABB covers more than 4 bins
 	// binning
 	uint32_t *wg_hist; // num_bin_wgs * 2 * bin_count
@@ -411,6 +407,88 @@ __device__ __forceinline__ uint64_t record_block_mask(uint4 rec, int bx) {
 		}
 	}
 	return m;
+}
+
+// ------------------------------------------------------------------ shading records
+
+// GLSL specifies pow(x, y) with the precision of exp2(y * log2(x)); the hardware log2/exp2
+// evaluate exactly that.
+__device__ __forceinline__ float glsl_pow(float x, float y) {
+	return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
+// Lighting of finalShading (funcs.glsl:261-271): light = max(0, dot(-sun_dir, normal) *
+// 0.7 + 0.3), and per channel the factor ambient + sun * light.
+__device__ __forceinline__ float light_value(const FrameConst &c, V3 normal) {
+	return fmx(0.0f, dot3(neg3(ld3(c.sun_dir)), normal) * 0.7f + 0.3f);
+}
+__device__ __forceinline__ float light_factor(const FrameConst &c, int k, float light) {
+	const float amb = c.ambient[k] * c.ambient_power;
+	const float dif = (c.sun_color[k] * c.sun_power) * light;
+	return amb + dif;
+}
+
+// Shading record of one triangle, 7 x uint4 (the rasteriser stages one per list entry in LDS):
+//   q0  depth plane x, y, z | flags | instance << 16
+//   q1  barycentric plane 0 x, y, z | offset;  q2 the same for plane 1
+//   q3  RGBA8 colours: the triangle's three vertex colours | 0
+//       fp32 colours:  c0.rgb | triangle alpha
+//   q4  c1.rgb | L0;  q5 c2.rgb | L1
+//   q6  instance colour (RGBA8; white without one) | L2 | triangle id | light
+// A flat-shaded triangle keeps its light value and, in L0-L2, fp = f^(1/2.4) of its three
+// lighting factors f; with LUCID_INST_HAS_VERTEX_NORMALS, L0-L2 are the packed vertex
+// normals (light is then per pixel).
+constexpr int kRecQuads = 7;
+constexpr int kRecStride = 8; // uint4 per record in tri_shade: one 128-byte line
+
+// Composes the record of triangle t = 2 q + s from what setup computed for it: its depth
+// record d, barycentric planes, packed face normal, and the quad's vertex ids vq (the
+// triangle uses vq[0], vq[1 + s], vq[2 + s]). Setup stores it in tri_shade once per frame.
+__device__ __forceinline__ void make_shading_record(const FrameConst &c, const FrameBufs &b,
+													uint32_t t, uint4 d, uint4 bary0, uint4 bary1,
+													uint32_t packed_normal,
+													const uint32_t (&vq)[4],
+													uint4 (&q)[kRecQuads]) {
+	// every slot is computed into a named local and stored once, so that no part of q
+	// is written under a branch (which would push q to scratch memory)
+	const uint32_t flags = d.w & 0xffffu, inst = d.w >> 16;
+	const bool s = t & 1u;
+	const uint32_t v0 = vq[0], v1 = s ? vq[2] : vq[1], v2 = s ? vq[3] : vq[2];
+	const bool vcol = (flags & LUCID_INST_HAS_VERTEX_COLORS) != 0;
+	uint4 q3 = make_uint4(0u, 0u, 0u, 0u), q4 = q3, q5 = q3;
+	if(vcol && c.color_mode == kColorRGBA8)
+		q3 = make_uint4(b.colors_rgba8[v0], b.colors_rgba8[v1], b.colors_rgba8[v2], 0u);
+	if(vcol && c.color_mode == kColorF32) {
+		const float *c0 = b.colors_rgb + (size_t)v0 * 3, *c1 = b.colors_rgb + (size_t)v1 * 3,
+					*c2 = b.colors_rgb + (size_t)v2 * 3;
+		const float alpha = b.tri_alpha ? b.tri_alpha[t] : 1.0f;
+		q3 = make_uint4(f2bits(c0[0]), f2bits(c0[1]), f2bits(c0[2]), f2bits(alpha));
+		q4 = make_uint4(f2bits(c1[0]), f2bits(c1[1]), f2bits(c1[2]), 0u);
+		q5 = make_uint4(f2bits(c2[0]), f2bits(c2[1]), f2bits(c2[2]), 0u);
+	}
+	const uint32_t icol =
+		((flags & LUCID_INST_HAS_COLOR) && b.inst_colors) ? b.inst_colors[inst] : 0xffffffffu;
+	uint32_t l0, l1, l2, lv = 0u;
+	if(flags & LUCID_INST_HAS_VERTEX_NORMALS) {
+		const bool have = b.normals != nullptr;
+		l0 = have ? b.normals[v0] : 0u, l1 = have ? b.normals[v1] : 0u;
+		l2 = have ? b.normals[v2] : 0u;
+	} else {
+		// the packed normal, as the reference stores and shades it
+		const float light = light_value(c, decode_normal(packed_normal));
+		lv = f2bits(light);
+		l0 = f2bits(glsl_pow(light_factor(c, 0, light), 1.0f / 2.4f));
+		l1 = f2bits(glsl_pow(light_factor(c, 1, light), 1.0f / 2.4f));
+		l2 = f2bits(glsl_pow(light_factor(c, 2, light), 1.0f / 2.4f));
+	}
+	q4.w = l0;
+	q5.w = l1;
+	q[0] = d;
+	q[1] = bary0;
+	q[2] = bary1;
+	q[3] = q3;
+	q[4] = q4;
+	q[5] = q5;
+	q[6] = make_uint4(icol, l2, t, lv);
 }
 
 // ------------------------------------------------------------------ gradient fixed point

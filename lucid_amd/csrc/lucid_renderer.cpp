@@ -91,8 +91,7 @@ struct lucid_renderer {
 
 	DevBuf info, ctrl;
 	DevBuf instances, inst_prefix, inst_colors;
-	DevBuf quad_aabb, tri_scan, tri_depth, tri_bary, tri_normal, quad_color, quad_normal,
-		tri_color, large_list;
+	DevBuf quad_aabb, tri_scan, tri_depth, tri_shade, large_list;
 	DevBuf wg_hist, bin_quads, bin_tris;
 	DevBuf low_records, high_records, fallback_queue, spill;
 	DevBuf blk_slot, blk_flags, blk_sorted, blk_lists, blk_count, band_count, sorted_pool;
@@ -178,11 +177,7 @@ int ensure_quad_capacity(lucid_renderer *r, int64_t nq) {
 	LUCID_CHECK(r->quad_aabb.reserve(cap * 4));
 	LUCID_CHECK(r->tri_scan.reserve(tris * 32));
 	LUCID_CHECK(r->tri_depth.reserve(tris * 16));
-	LUCID_CHECK(r->tri_bary.reserve(tris * 32));
-	LUCID_CHECK(r->tri_normal.reserve(tris * 4));
-	LUCID_CHECK(r->quad_color.reserve(cap * 16));
-	LUCID_CHECK(r->quad_normal.reserve(cap * 16));
-	LUCID_CHECK(r->tri_color.reserve(tris * 48));
+	LUCID_CHECK(r->tri_shade.reserve(tris * kRecStride * 16));
 	LUCID_CHECK(r->large_list.reserve(cap * 4));
 	// small quads cover at most 4 bins: 4 entries per quad is an exact bound
 	uint64_t bq = (uint64_t)cap * 4;
@@ -473,11 +468,7 @@ int prepare_frame(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_f
 	b.quad_aabb = r->quad_aabb.as<uint32_t>();
 	b.tri_scan = r->tri_scan.as<uint4>();
 	b.tri_depth = r->tri_depth.as<uint4>();
-	b.tri_bary = r->tri_bary.as<uint4>();
-	b.tri_normal = r->tri_normal.as<uint32_t>();
-	b.quad_color = r->quad_color.as<uint4>();
-	b.quad_normal = r->quad_normal.as<uint4>();
-	b.tri_color = r->tri_color.as<float4>();
+	b.tri_shade = r->tri_shade.as<uint4>();
 	b.large_list = r->large_list.as<uint32_t>();
 	b.wg_hist = r->wg_hist.as<uint32_t>();
 	b.bin_quads = r->bin_quads.as<uint32_t>();
@@ -1100,20 +1091,23 @@ int lucid_read_setup(lucid_renderer *r, uint32_t *tri_records, size_t num_tris,
 	if(tri_records && q) {
 		// record arrays hold the first triangles of all quads, then the second ones
 		const size_t half = (size_t)r->quad_cap;
-		std::vector<uint32_t> scan(2 * q * 8), depth(2 * q * 4), bary(2 * q * 8);
+		// barycentric planes are quads 1-2 of each shading record (kRecStride uint4)
+		const size_t rec_words = (size_t)kRecStride * 4;
+		std::vector<uint32_t> scan(2 * q * 8), depth(2 * q * 4), shade(2 * q * rec_words);
 		for(size_t h = 0; h < 2; h++) {
 			const size_t src = h * half, dst = h * q;
 			LUCID_CHECK(hipMemcpy(&scan[dst * 8], (char *)r->tri_scan.ptr + src * 32, q * 32,
 								  hipMemcpyDeviceToHost));
 			LUCID_CHECK(hipMemcpy(&depth[dst * 4], (char *)r->tri_depth.ptr + src * 16, q * 16,
 								  hipMemcpyDeviceToHost));
-			LUCID_CHECK(hipMemcpy(&bary[dst * 8], (char *)r->tri_bary.ptr + src * 32, q * 32,
-								  hipMemcpyDeviceToHost));
+			LUCID_CHECK(hipMemcpy(&shade[dst * rec_words],
+								  (char *)r->tri_shade.ptr + src * rec_words * 4,
+								  q * rec_words * 4, hipMemcpyDeviceToHost));
 		}
 		for(size_t i = 0; i < 2 * q; i++) {
 			const size_t k = (i >> 1) + (i & 1) * q;
 			uint32_t *o = tri_records + i * 20;
-			memcpy(o, &bary[k * 8], 32);
+			memcpy(o, &shade[k * rec_words + 4], 32);
 			memcpy(o + 8, &scan[k * 8], 32);
 			memcpy(o + 16, &depth[k * 4], 16);
 		}

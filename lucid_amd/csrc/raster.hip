@@ -35,11 +35,6 @@ constexpr int kFallbackCap = 512; // fragments per pixel sorted in LDS by the fa
 
 // ------------------------------------------------------------------ shading
 
-// GLSL specifies pow(x, y) with the precision of exp2(y * log2(x)); the hardware log2/exp2
-// evaluate exactly that.
-__device__ __forceinline__ float glsl_pow(float x, float y) {
-	return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
-}
 __device__ __forceinline__ float srgb_to_linear(float x) {
 	return x < 0.04045f ? (1.0f / 12.92f) * x : glsl_pow((x + 0.055f) * (1.0f / 1.055f), 2.4f);
 }
@@ -94,17 +89,6 @@ template <> struct SampleColor<true> {
 	}
 };
 
-// Lighting of finalShading (funcs.glsl:261-271): light = max(0, dot(-sun_dir, normal) *
-// 0.7 + 0.3), and per channel the factor ambient + sun * light.
-__device__ __forceinline__ float light_value(const FrameConst &c, V3 normal) {
-	return fmx(0.0f, dot3(neg3(ld3(c.sun_dir)), normal) * 0.7f + 0.3f);
-}
-__device__ __forceinline__ float light_factor(const FrameConst &c, int k, float light) {
-	const float amb = c.ambient[k] * c.ambient_power;
-	const float dif = (c.sun_color[k] * c.sun_power) * light;
-	return amb + dif;
-}
-
 // sRGB round trip of finalShading, s = L2S(S2L(x) * f), unclamped. fp = f^(1/2.4).
 // Above both linear segments the two powers cancel: L2S(S2L(x) f) = (x + 0.055) fp - 0.055
 // exactly (the threshold S2L(x) f >= 0.0031308 is (x + 0.055) fp >= kSrgbHiU); below both,
@@ -143,60 +127,13 @@ __device__ __forceinline__ LitChannel lit_channel(float x, float f, float fp) {
 	return o;
 }
 
-// Shading record of one triangle, 7 x uint4 (the rasteriser stages one per list entry in LDS):
-//   q0  depth plane x, y, z | flags | instance << 16
-//   q1  barycentric plane 0 x, y, z | offset;  q2 the same for plane 1
-//   q3  RGBA8 colours: the triangle's three vertex colours | 0
-//       fp32 colours:  c0.rgb | triangle alpha
-//   q4  c1.rgb | L0;  q5 c2.rgb | L1
-//   q6  instance colour (RGBA8; white without one) | L2 | triangle id | light
-// A flat-shaded triangle keeps its light value and, in L0-L2, fp = f^(1/2.4) of its three
-// lighting factors f; with LUCID_INST_HAS_VERTEX_NORMALS, L0-L2 are the packed vertex
-// normals (light is then per pixel).
-constexpr int kRecQuads = 7;
-
-__device__ __forceinline__ void load_shading_record(const FrameConst &c, const FrameBufs &b,
-													uint32_t t, uint4 (&q)[kRecQuads]) {
-	// every slot is computed into a named local and stored once, so that no part of q
-	// is written under a branch (which would push q to scratch memory)
-	const uint4 d = b.tri_depth[tri_rec(b, t)];
-	const uint32_t flags = d.w & 0xffffu, inst = d.w >> 16;
-	const bool s = t & 1u;
-	const bool vcol = (flags & LUCID_INST_HAS_VERTEX_COLORS) != 0;
-	uint4 q3 = make_uint4(0u, 0u, 0u, 0u), q4 = q3, q5 = q3;
-	if(vcol && c.color_mode == kColorRGBA8) {
-		uint4 qc = b.quad_color[t >> 1];
-		q3 = make_uint4(qc.x, s ? qc.z : qc.y, s ? qc.w : qc.z, 0u);
-	}
-	if(vcol && c.color_mode == kColorF32) {
-		float4 c0 = b.tri_color[3 * (size_t)tri_rec(b, t)], c1 = b.tri_color[3 * (size_t)tri_rec(b, t) + 1],
-			   c2 = b.tri_color[3 * (size_t)tri_rec(b, t) + 2];
-		q3 = make_uint4(f2bits(c0.x), f2bits(c0.y), f2bits(c0.z), f2bits(c0.w));
-		q4 = make_uint4(f2bits(c1.x), f2bits(c1.y), f2bits(c1.z), 0u);
-		q5 = make_uint4(f2bits(c2.x), f2bits(c2.y), f2bits(c2.z), 0u);
-	}
-	const uint32_t icol =
-		((flags & LUCID_INST_HAS_COLOR) && b.inst_colors) ? b.inst_colors[inst] : 0xffffffffu;
-	uint32_t l0, l1, l2, lv = 0u;
-	if(flags & LUCID_INST_HAS_VERTEX_NORMALS) {
-		uint4 nq = b.quad_normal[t >> 1];
-		l0 = nq.x, l1 = s ? nq.z : nq.y, l2 = s ? nq.w : nq.z;
-	} else {
-		const float light = light_value(c, decode_normal(b.tri_normal[tri_rec(b, t)]));
-		lv = f2bits(light);
-		l0 = f2bits(glsl_pow(light_factor(c, 0, light), 1.0f / 2.4f));
-		l1 = f2bits(glsl_pow(light_factor(c, 1, light), 1.0f / 2.4f));
-		l2 = f2bits(glsl_pow(light_factor(c, 2, light), 1.0f / 2.4f));
-	}
-	q4.w = l0;
-	q5.w = l1;
-	q[0] = d;
-	q[1] = b.tri_bary[2 * (size_t)tri_rec(b, t)];
-	q[2] = b.tri_bary[2 * (size_t)tri_rec(b, t) + 1];
-	q[3] = q3;
-	q[4] = q4;
-	q[5] = q5;
-	q[6] = make_uint4(icol, l2, t, lv);
+// The shading record of triangle t, built by setup (one 128-byte line per triangle).
+__device__ __forceinline__ void load_shading_record(const FrameBufs &b, uint32_t t,
+													uint4 (&q)[kRecQuads]) {
+	const uint4 *src = b.tri_shade + (size_t)kRecStride * tri_rec(b, t);
+#pragma unroll
+	for(int j = 0; j < kRecQuads; j++)
+		q[j] = src[j];
 }
 
 // Depth plane at a pixel: inverse distance along the pixel's ray (shading.glsl:108).
@@ -312,7 +249,7 @@ __device__ __forceinline__ SampleColor<FP32> shade_sample(const FrameConst &c, c
 														  uint32_t t, float px, float py,
 														  float inv_ray) {
 	uint4 q[kRecQuads];
-	load_shading_record(c, b, t, q);
+	load_shading_record(b, t, q);
 	return shade_record<FP32>(c, q, px, py, inv_ray);
 }
 
@@ -1133,7 +1070,7 @@ __device__ __forceinline__ void stage_slice(const FrameConst &c, const FrameBufs
 		const uint32_t i = k * 64 + lane;
 		if(i < n) {
 			uint4 q[kRecQuads];
-			load_shading_record(c, b, tri[k], q);
+			load_shading_record(b, tri[k], q);
 #pragma unroll
 			for(int j = 0; j < kRecQuads; j++)
 				s_rec[i * kRecQuads + j] = q[j];
@@ -2121,7 +2058,7 @@ __device__ void bwd_fallback_pixel(const FrameConst &c, const FrameBufs &b, cons
 	auto shade_at = [&](uint32_t i, BwdSample &smp) -> bool {
 		const uint32_t t = (uint32_t)key_at(i);
 		uint4 q[kRecQuads];
-		load_shading_record(c, b, t, q);
+		load_shading_record(b, t, q);
 		return shade_grad(c, q, fx, fy, record_inv_depth(q[0], fx, fy), smp);
 	};
 	// pass A (serial over the sorted list, 64 samples shaded at a time)

@@ -82,15 +82,17 @@ __device__ __forceinline__ V3 load_pos(const float *pos, uint32_t v) {
 	return mk3(p[0], p[1], p[2]);
 }
 
-// storeTri (quad_setup.glsl:274-340)
+// storeTri (quad_setup.glsl:274-340). The depth record and scan parameters are stored for
+// the binning and row kernels; the depth record, barycentric planes and packed normal are
+// also returned for the triangle's shading record.
 __device__ void store_tri(const FrameConst &c, const FrameBufs &b, uint32_t t, uint32_t flags_id,
-						  V3 tri0, V3 tri1, V3 tri2, uint32_t y_aabb) {
+						  V3 tri0, V3 tri1, V3 tri2, uint32_t y_aabb, uint4 &depth_out,
+						  uint4 &bary0_out, uint4 &bary1_out, uint32_t &normal_out) {
 	V3 dirx = ld3(c.dirx), diry = ld3(c.diry), dir0 = ld3(c.dir0), rd0 = ld3(c.ray_dir0);
 	V3 normal = cross3(sub3(tri0, tri2), sub3(tri1, tri0));
 	float mult = 1.0f / sqrtf(dot3(normal, normal));
 	normal = mul3(normal, mult);
-	if((flags_id & LUCID_INST_HAS_VERTEX_NORMALS) == 0)
-		b.tri_normal[tri_rec(b, t)] = encode_normal(normal);
+	normal_out = encode_normal(normal);
 
 	V3 edge0 = mul3(sub3(tri0, tri2), mult);
 	V3 edge1 = mul3(sub3(tri1, tri0), mult);
@@ -105,11 +107,10 @@ __device__ void store_tri(const FrameConst &c, const FrameBufs &b, uint32_t t, u
 	V3 pn = mul3(normal, 1.0f / plane_dist);
 	V3 deq = mk3(dot3(pn, dirx), dot3(pn, diry), dot3(pn, rd0));
 
-	b.tri_depth[tri_rec(b, t)] = make_uint4(f2bits(deq.x), f2bits(deq.y), f2bits(deq.z), flags_id);
-	b.tri_bary[2 * (size_t)tri_rec(b, t) + 0] =
-		make_uint4(f2bits(edge0.x), f2bits(edge0.y), f2bits(edge0.z), f2bits(param0));
-	b.tri_bary[2 * (size_t)tri_rec(b, t) + 1] =
-		make_uint4(f2bits(edge1.x), f2bits(edge1.y), f2bits(edge1.z), f2bits(param1));
+	depth_out = make_uint4(f2bits(deq.x), f2bits(deq.y), f2bits(deq.z), flags_id);
+	b.tri_depth[tri_rec(b, t)] = depth_out;
+	bary0_out = make_uint4(f2bits(edge0.x), f2bits(edge0.y), f2bits(edge0.z), f2bits(param0));
+	bary1_out = make_uint4(f2bits(edge1.x), f2bits(edge1.y), f2bits(edge1.z), f2bits(param1));
 
 	V3 n0 = cross3(tri2, sub3(tri1, tri2));
 	V3 n1 = cross3(tri0, sub3(tri2, tri0));
@@ -224,36 +225,24 @@ __device__ int process_quad(const FrameConst &c, const FrameBufs &b, uint32_t q,
 
 	uint32_t flags_id = flags | (inst << 16);
 	V3 o = ld3(c.origin);
+	const uint32_t vq[4] = {vi[0], vi[1], vi[2], vi[3]};
 #pragma unroll
 	for(int s = 0; s < 2; s++) {
 		if((cull_flags >> s) & 1u)
-			continue;
-		store_tri(c, b, 2 * q + s, flags_id, sub3(p[0], o), sub3(p[1 + s], o), sub3(p[2 + s], o),
-				  y_aabb[s]);
-	}
-	// storeQuad (quad_setup.glsl:256-272)
-	if((flags & LUCID_INST_HAS_VERTEX_NORMALS) && b.normals)
-		b.quad_normal[q] = make_uint4(b.normals[vi[0]], b.normals[vi[1]], b.normals[vi[2]],
-									  b.normals[vi[3]]);
-	if(flags & LUCID_INST_HAS_VERTEX_COLORS) {
-		if(c.color_mode == kColorRGBA8) {
-			b.quad_color[q] = make_uint4(b.colors_rgba8[vi[0]], b.colors_rgba8[vi[1]],
-										 b.colors_rgba8[vi[2]], b.colors_rgba8[vi[3]]);
-		} else if(c.color_mode == kColorF32) {
+			continue; // no record is read for a culled triangle
+		const uint32_t t = 2 * q + s;
+		uint4 d, b0, b1;
+		uint32_t nrm;
+		store_tri(c, b, t, flags_id, sub3(p[0], o), sub3(p[1 + s], o), sub3(p[2 + s], o),
+				  y_aabb[s], d, b0, b1, nrm);
+		// shading record: depth and barycentric planes, colours (storeQuad,
+		// quad_setup.glsl:256-272) and lighting, one 128-byte line
+		uint4 rec[kRecQuads];
+		make_shading_record(c, b, t, d, b0, b1, nrm, vq, rec);
+		uint4 *dst = b.tri_shade + (size_t)kRecStride * tri_rec(b, t);
 #pragma unroll
-			for(int s = 0; s < 2; s++) {
-				if((cull_flags >> s) & 1u)
-					continue; // no record is read for a culled triangle
-				uint32_t t = 2 * q + s;
-				const float *c0 = b.colors_rgb + (size_t)vi[0] * 3;
-				const float *c1 = b.colors_rgb + (size_t)vi[1 + s] * 3;
-				const float *c2 = b.colors_rgb + (size_t)vi[2 + s] * 3;
-				float alpha = b.tri_alpha ? b.tri_alpha[t] : 1.0f;
-				b.tri_color[3 * (size_t)tri_rec(b, t) + 0] = make_float4(c0[0], c0[1], c0[2], alpha);
-				b.tri_color[3 * (size_t)tri_rec(b, t) + 1] = make_float4(c1[0], c1[1], c1[2], 0.0f);
-				b.tri_color[3 * (size_t)tri_rec(b, t) + 2] = make_float4(c2[0], c2[1], c2[2], 0.0f);
-			}
-		}
+		for(int j = 0; j < kRecQuads; j++)
+			dst[j] = rec[j];
 	}
 	return area <= 4u ? 0 : 1;
 }
