@@ -57,6 +57,8 @@ __device__ __forceinline__ bool small_quad(uint32_t a, int &bsx, int &bsy, int &
 
 struct Slice {
 	uint32_t q0, q1, l0, l1;
+	uint32_t lstart[kLargeStripes + 1]; // flat index of each large sub-list's first entry
+	uint32_t lcap;
 };
 
 __device__ __forceinline__ Slice wg_slice(const FrameConst &c, const FrameBufs &b, uint32_t wg) {
@@ -64,10 +66,26 @@ __device__ __forceinline__ Slice wg_slice(const FrameConst &c, const FrameBufs &
 	uint32_t nq = (uint32_t)c.num_quads, nw = (uint32_t)c.num_bin_wgs;
 	uint32_t cq = (nq + nw - 1) / nw;
 	s.q0 = min(nq, wg * cq), s.q1 = min(nq, s.q0 + cq);
-	uint32_t nl = b.ctrl[kCtrlNumLarge];
+	uint32_t nl = 0;
+#pragma unroll
+	for(int k = 0; k < kLargeStripes; k++) {
+		s.lstart[k] = nl;
+		nl += *large_cursor(b.ctrl, k);
+	}
+	s.lstart[kLargeStripes] = nl;
+	s.lcap = large_stripe_cap(nq);
 	uint32_t cl = (nl + nw - 1) / nw;
 	s.l0 = min(nl, wg * cl), s.l1 = min(nl, s.l0 + cl);
 	return s;
+}
+
+// Quad of flat large-list entry i (< the total of the sub-lists).
+__device__ __forceinline__ uint32_t large_at(const FrameBufs &b, const Slice &s, uint32_t i) {
+	int k = 0;
+#pragma unroll
+	for(int j = 1; j < kLargeStripes; j++)
+		k += i >= s.lstart[j] ? 1 : 0;
+	return b.large_list[(size_t)k * s.lcap + (i - s.lstart[k])];
 }
 
 // bin_counter.glsl: per-work-group histograms of small quads and large tris.
@@ -92,7 +110,7 @@ __global__ __launch_bounds__(kBinThreads) void bin_count_kernel(FrameConst c, Fr
 	}
 	// large tris: per bin row +1/-1 marks (bin_counter.glsl:112-134)
 	for(uint32_t i = sl.l0 * 2 + threadIdx.x; i < sl.l1 * 2; i += kBinThreads) {
-		uint32_t q = b.large_list[i >> 1], s = i & 1u;
+		uint32_t q = large_at(b, sl, i >> 1), s = i & 1u;
 		uint32_t a = b.quad_aabb[q];
 		if((a >> (30 + s)) & 1u)
 			continue;
@@ -278,7 +296,7 @@ __global__ __launch_bounds__(kBinThreads) void bin_dispatch_kernel(FrameConst c,
 				b.bin_quads[atomicAdd(&pq[by * NBX + bx], 1u)] = entry;
 	}
 	for(uint32_t i = sl.l0 * 2 + threadIdx.x; i < sl.l1 * 2; i += kBinThreads) {
-		uint32_t q = b.large_list[i >> 1], s = i & 1u;
+		uint32_t q = large_at(b, sl, i >> 1), s = i & 1u;
 		uint32_t a = b.quad_aabb[q];
 		if((a >> (30 + s)) & 1u)
 			continue;

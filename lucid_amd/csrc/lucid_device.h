@@ -27,7 +27,7 @@ constexpr int kLowRowCap = 1024;     // records per block row of a low bin (< 10
 
 // Internal control words (separate from LucidInfo, whose layout is the reference's).
 enum CtrlWord : int {
-	kCtrlNumLarge = 0,      // entries in the large-quad list
+	kCtrlNumLarge = 0,      // unused (the large-quad sub-lists keep their own counters)
 	kCtrlRecordCursor = 1,  // high-bin record pool size the frame needs (uint4 units)
 	kCtrlFallbackCount = 2, // pixels in the exact-fallback queue
 	kCtrlOverflow = 3,      // bit 0 bin lists, 1 record pool, 2 fallback queue, 3 spill pool
@@ -60,9 +60,22 @@ __device__ __forceinline__ uint32_t *stat_stripe(uint32_t *ctrl, int stat, uint3
 // each (block slot % kSortStripes picks one), for the same reason.
 constexpr int kSortStripes = 16;
 constexpr int kSortCursorBase = kStatStripeBase + 2 * kStatStripes * kStatStride;
-constexpr int kCtrlAllocWords = kSortCursorBase + kSortStripes * kStatStride;
 __device__ __forceinline__ uint32_t *sort_cursor(uint32_t *ctrl, uint32_t stripe) {
 	return ctrl + kSortCursorBase + (int)stripe * kStatStride;
+}
+// The large-quad list likewise: setup work-group w appends to sub-list w % kLargeStripes,
+// whose capacity (large_stripe_cap) holds every quad of those work-groups; the binning
+// kernels index the sub-lists as one list, in stripe order.
+constexpr int kLargeStripes = 16;
+constexpr int kSetupThreads = 256;
+constexpr int kLargeCursorBase = kSortCursorBase + kSortStripes * kStatStride;
+constexpr int kCtrlAllocWords = kLargeCursorBase + kLargeStripes * kStatStride;
+__device__ __forceinline__ uint32_t *large_cursor(uint32_t *ctrl, uint32_t stripe) {
+	return ctrl + kLargeCursorBase + (int)stripe * kStatStride;
+}
+__host__ __device__ __forceinline__ uint32_t large_stripe_cap(uint32_t num_quads) {
+	const uint32_t wgs = (num_quads + kSetupThreads - 1) / kSetupThreads;
+	return (wgs + kLargeStripes - 1) / kLargeStripes * kSetupThreads;
 }
 
 // Overflow bits of kCtrlOverflow
@@ -156,7 +169,7 @@ struct FrameBufs {
 	uint4 *tri_scan;      // 2 per tri
 	uint4 *tri_depth;     // 1 per tri
 	uint4 *tri_shade;     // kRecStride per tri: the shading record the rasterisers stage
-	uint32_t *large_list; // quads whose bin AABB covers more than 4 bins
+	uint32_t *large_list; // quads whose bin AABB covers more than 4 bins (kLargeStripes sub-lists)
 	// binning
 	uint32_t *wg_hist; // num_bin_wgs * 2 * bin_count
 	uint32_t *bin_quads, *bin_tris;

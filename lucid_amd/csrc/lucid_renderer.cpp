@@ -178,7 +178,7 @@ int ensure_quad_capacity(lucid_renderer *r, int64_t nq) {
 	LUCID_CHECK(r->tri_scan.reserve(tris * 32));
 	LUCID_CHECK(r->tri_depth.reserve(tris * 16));
 	LUCID_CHECK(r->tri_shade.reserve(tris * kRecStride * 16));
-	LUCID_CHECK(r->large_list.reserve(cap * 4));
+	LUCID_CHECK(r->large_list.reserve((size_t)kLargeStripes * large_stripe_cap((uint32_t)cap) * 4));
 	// small quads cover at most 4 bins: 4 entries per quad is an exact bound
 	uint64_t bq = (uint64_t)cap * 4;
 	LUCID_CHECK(r->bin_quads.reserve(bq * 4));

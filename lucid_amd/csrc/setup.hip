@@ -247,13 +247,13 @@ __device__ int process_quad(const FrameConst &c, const FrameBufs &b, uint32_t q,
 	return area <= 4u ? 0 : 1;
 }
 
-__global__ __launch_bounds__(256) void setup_kernel(FrameConst c, FrameBufs b) {
+__global__ __launch_bounds__(kSetupThreads) void setup_kernel(FrameConst c, FrameBufs b) {
 	__shared__ uint32_t s_counts[6]; // small, large, rejected x4
 	if(threadIdx.x < 6)
 		s_counts[threadIdx.x] = 0;
 	__syncthreads();
 
-	uint32_t q = blockIdx.x * 256u + threadIdx.x;
+	uint32_t q = blockIdx.x * (uint32_t)kSetupThreads + threadIdx.x;
 	int kind = -1;
 	if(q < (uint32_t)c.num_quads) {
 		int lo = 0, hi = c.num_instances - 1;
@@ -275,15 +275,16 @@ __global__ __launch_bounds__(256) void setup_kernel(FrameConst c, FrameBufs b) {
 		if(kind >= 2)
 			b.quad_aabb[q] = kInvalidAabb;
 	}
-	// wave-aggregated counters and large-quad list
+	// wave-aggregated counters and this work-group's large-quad sub-list
 	uint64_t large = ballot(kind == 1);
+	const uint32_t stripe = blockIdx.x % kLargeStripes;
 	uint32_t base = 0;
 	if(lane_id() == 0 && large)
-		base = atomicAdd(&b.ctrl[kCtrlNumLarge], (uint32_t)__popcll(large));
+		base = atomicAdd(large_cursor(b.ctrl, stripe), (uint32_t)__popcll(large));
 	base = __shfl(base, 0, 64);
 	if(kind == 1) {
 		uint32_t rank = (uint32_t)__popcll(large & ((1ull << lane_id()) - 1ull));
-		b.large_list[base + rank] = q;
+		b.large_list[(size_t)stripe * large_stripe_cap((uint32_t)c.num_quads) + base + rank] = q;
 	}
 #pragma unroll
 	for(int k = 0; k < 6; k++) {
@@ -317,7 +318,8 @@ hipError_t launch_setup(const FrameConst &c, const FrameBufs &b, uint32_t n_info
 	hipLaunchKernelGGL(clear_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, b.info, n_info,
 					   b.ctrl, (uint32_t)c.num_quads);
 	if(c.num_quads > 0)
-		hipLaunchKernelGGL(setup_kernel, dim3((c.num_quads + 255) / 256), dim3(256), 0, stream, c, b);
+		hipLaunchKernelGGL(setup_kernel, dim3((c.num_quads + kSetupThreads - 1) / kSetupThreads),
+						   dim3(kSetupThreads), 0, stream, c, b);
 	return hipGetLastError();
 }
 
