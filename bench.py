@@ -25,6 +25,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ms/frame and fragments blended/s @1080p 1M tris; fwd+bwd vs plain fwd ratio"
 HBM_PEAK_GBS = 8000.0
+DEFAULT_TRIS = 1_000_000
+RASTER_TRAFFIC_BYTES = 644.9e6  # block_raster + block_raster_big, one C1 frame (rocprofv3)
 
 
 def parse():
@@ -36,13 +38,15 @@ def parse():
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--copy-calibration", action="store_true",
                    help="also run a 256 MiB device copy (reference kernel for rocprof counters)")
-    p.add_argument("--tris", type=int, default=1_000_000)
+    p.add_argument("--tris", type=int, default=DEFAULT_TRIS)
     p.add_argument("--no-backward", action="store_true",
                    help="skip the fwd+bwd / fwd ratio leg (200k triangles at 512x512)")
     p.add_argument("--bwd-steps", type=int, default=5)
     p.add_argument("--no-batch", action="store_true",
                    help="skip the batch leg (64 scenes in one hipGraph vs 64 sequential frames)")
     p.add_argument("--batch-scenes", type=int, default=64)
+    p.add_argument("--only-backward", action="store_true",
+                   help="profiling aid: run only the forward+backward leg and print its JSON")
     p.add_argument("--no-autotune", action="store_true",
                    help="keep the default tuning instead of autotuning on the workload first")
     return p.parse_args()
@@ -190,6 +194,9 @@ def main():
     from lucid_amd import _lib, scenes
     from lucid_amd.renderer import GpuScene, LucidRenderer
 
+    if args.only_backward:
+        print(json.dumps({"backward": backward_ratio(args.bwd_steps)}), flush=True)
+        return
     print(f"[bench] generating workload ({args.tris} tris)", file=sys.stderr, flush=True)
     scene = scenes.config1(num_tris=args.tris)
     r = LucidRenderer((scene.width, scene.height), opts=_lib.OPT_TIMERS,
@@ -295,7 +302,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            # HBM bytes of the block-raster stage (both kernels) per frame, measured on this
+            # workload with rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a 256 MiB copy
+            # (profiles/c1_forward_kernels.md); not re-measured here (no counters in-process)
+            "traffic": RASTER_TRAFFIC_BYTES if args.tris == DEFAULT_TRIS else None,
+            "traffic_source": "rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated",
             "algorithmic_bytes": dom_bytes,
             "frame_algorithmic_bytes": ab["frame"],
             "frame_achieved": frame_gbs,
