@@ -400,12 +400,15 @@ __device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
 	return ((uint64_t)hi << 32) | lo;
 }
 
+__device__ __forceinline__ uint64_t shfl_xor_key(uint64_t v, int m) { return shfl_xor64(v, m); }
+__device__ __forceinline__ uint32_t shfl_xor_key(uint32_t v, int m) { return __shfl_xor(v, m, 64); }
+
 // Ascending bitonic sort of 64*E keys held in registers; element i = e * 64 + lane.
-// Bitonic network steps j = j_start .. 1 of merge level k on 64 * E keys held by one wave
-// (key e * 64 + lane of the block in v[e]); `base` is the block's index in the whole
-// sequence, which decides each pair's direction: ascending iff (index & k) == 0.
-template <int E>
-__device__ __forceinline__ void bitonic_regs_steps(uint64_t (&v)[E], uint32_t base, uint32_t k,
+// Bitonic network steps j = j_start .. 1 of merge level k on 64 * E keys (K: 32- or 64-bit)
+// held by one wave (key e * 64 + lane of the block in v[e]); `base` is the block's index in
+// the whole sequence, which decides each pair's direction: ascending iff (index & k) == 0.
+template <int E, class K>
+__device__ __forceinline__ void bitonic_regs_steps(K (&v)[E], uint32_t base, uint32_t k,
 												   int j_start) {
 	const int lane = lane_id();
 #pragma unroll
@@ -419,7 +422,7 @@ __device__ __forceinline__ void bitonic_regs_steps(uint64_t (&v)[E], uint32_t ba
 				if((e & jr) == 0) {
 					const int pe = e | jr;
 					const bool asc = ((base + (uint32_t)(e * 64 + lane)) & k) == 0;
-					uint64_t a = v[e], bb = v[pe];
+					K a = v[e], bb = v[pe];
 					bool sw = (a > bb) == asc;
 					v[e] = sw ? bb : a;
 					v[pe] = sw ? a : bb;
@@ -429,10 +432,10 @@ __device__ __forceinline__ void bitonic_regs_steps(uint64_t (&v)[E], uint32_t ba
 #pragma unroll
 			for(int e = 0; e < E; e++) {
 				const uint32_t i = base + (uint32_t)(e * 64 + lane);
-				uint64_t p = shfl_xor64(v[e], j);
+				K p = shfl_xor_key(v[e], j);
 				bool asc = (i & k) == 0;
 				bool lower = (lane & j) == 0;
-				uint64_t mn = v[e] < p ? v[e] : p, mx = v[e] < p ? p : v[e];
+				K mn = v[e] < p ? v[e] : p, mx = v[e] < p ? p : v[e];
 				v[e] = (asc == lower) ? mn : mx;
 			}
 		}
@@ -442,16 +445,16 @@ __device__ __forceinline__ void bitonic_regs_steps(uint64_t (&v)[E], uint32_t ba
 // Bitonic sort levels k = 2 .. 64 * E of a 64 * E block starting at index `base` of a longer
 // sequence (ascending for base = 0 alone; alternating directions per block otherwise, as the
 // network's later levels expect).
-template <int E>
-__device__ __forceinline__ void bitonic_regs_from(uint64_t (&v)[E], uint32_t base) {
+template <int E, class K>
+__device__ __forceinline__ void bitonic_regs_from(K (&v)[E], uint32_t base) {
 #pragma unroll
 	for(int k = 2; k <= 64 * E; k <<= 1)
 		bitonic_regs_steps<E>(v, base, (uint32_t)k, k >> 1);
 }
 
 // Ascending bitonic sort of 64 * E keys held by one wave (key e * 64 + lane in v[e]).
-template <int E>
-__device__ __forceinline__ void bitonic_regs(uint64_t (&v)[E]) {
+template <int E, class K>
+__device__ __forceinline__ void bitonic_regs(K (&v)[E]) {
 	bitonic_regs_from<E>(v, 0u);
 }
 
@@ -508,10 +511,10 @@ __device__ __forceinline__ uint4 make_record(uint32_t t, const uint32_t sp[8], u
 					  sp[6] | (sp[7] << 10) | (bxm << 20));
 }
 
-// Sort key of a record inside block column bx: depth at the covered-pixel centroid (nearest
-// first), then the list position (ties are resolved exactly per pixel later).
-__device__ __forceinline__ uint64_t block_key(const FrameBufs &b, uint4 rec, int bx, int px0,
-											  int py0, uint32_t idx) {
+// Inverse depth of a record inside block column bx at the centroid of its covered pixels:
+// the block-level sort key (ties and centroid error are resolved exactly per pixel later).
+__device__ __forceinline__ float block_inv_depth(const FrameBufs &b, uint4 rec, int bx, int px0,
+												 int py0) {
 	int n = 0, sx = 0, sy = 0;
 	int x0 = bx * 8;
 #pragma unroll
@@ -528,8 +531,79 @@ __device__ __forceinline__ uint64_t block_key(const FrameBufs &b, uint4 rec, int
 	float inv_n = 1.0f / (float)(n > 0 ? n : 1);
 	float cx = (float)px0 + (float)sx * 0.5f * inv_n;
 	float cy = (float)py0 + (float)sy * inv_n;
-	float inv = sample_inv_depth(b, rec.x, cx, cy);
-	return ((uint64_t)(~ord_f32(inv)) << 32) | idx;
+	return sample_inv_depth(b, rec.x, cx, cy);
+}
+
+// Depth rank of a block entry from its depth key ~ord(inverse depth) (ascending = nearest
+// first): the key minus the block's smallest, shifted right just enough that the block's key
+// range fits BITS bits. Monotone in the key, and float keys keep relative precision (about
+// BITS - log2(binades spanned) mantissa bits), so ties are rare and are resolved per pixel.
+template <int BITS>
+struct DepthGrid {
+	uint32_t lo;
+	int shift;
+	__device__ __forceinline__ DepthGrid(uint32_t lo_, uint32_t hi_) : lo(lo_) {
+		const uint32_t span = hi_ - lo_;
+		const int need = span ? 32 - __clz(span) : 0;
+		shift = need > BITS ? need - BITS : 0;
+	}
+	__device__ __forceinline__ uint32_t rank(uint32_t key) const { return (key - lo) >> shift; }
+};
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+	for(int m = 1; m < 64; m <<= 1)
+		v = min(v, (uint32_t)__shfl_xor(v, m, 64));
+	return v;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+	for(int m = 1; m < 64; m <<= 1)
+		v = max(v, (uint32_t)__shfl_xor(v, m, 64));
+	return v;
+}
+
+// Depth order of a block list of m <= 256 entries, sorted by one wave in registers: sorted
+// entry 64 k + lane is list position pos[k]. Keys are (24-bit depth rank << 8 | position).
+__device__ __forceinline__ void sort_block_positions(const FrameBufs &b, const uint4 *recs,
+													 const uint32_t *list, uint32_t m, int bx,
+													 int px0, int py0, uint32_t (&pos)[4]) {
+	static_assert(kRegSortCap <= 256, "list positions fit 8 bits of the key");
+	const int lane = lane_id();
+	uint32_t dk[4];
+	uint32_t lo = ~0u, hi = 0u;
+#pragma unroll
+	for(int k = 0; k < 4; k++) {
+		const uint32_t j = k * 64 + lane;
+		dk[k] = 0u;
+		if(j < m) {
+			dk[k] = ~ord_f32(block_inv_depth(b, recs[list[j]], bx, px0, py0));
+			lo = min(lo, dk[k]);
+			hi = max(hi, dk[k]);
+		}
+	}
+	const DepthGrid<24> grid(wave_min_u32(lo), wave_max_u32(hi));
+	// a real key never equals the padding: position 255 exists only when nothing is padded
+	auto key = [&](int k) -> uint32_t {
+		const uint32_t j = k * 64 + lane;
+		return j < m ? (grid.rank(dk[k]) << 8) | j : ~0u;
+	};
+	if(m <= 64) {
+		uint32_t w1[1] = {key(0)};
+		bitonic_regs<1>(w1);
+		pos[0] = w1[0] & 0xffu, pos[1] = pos[2] = pos[3] = 0u;
+	} else if(m <= 128) {
+		uint32_t w2[2] = {key(0), key(1)};
+		bitonic_regs<2>(w2);
+		pos[0] = w2[0] & 0xffu, pos[1] = w2[1] & 0xffu, pos[2] = pos[3] = 0u;
+	} else {
+		uint32_t w4[4] = {key(0), key(1), key(2), key(3)};
+		bitonic_regs<4>(w4);
+#pragma unroll
+		for(int k = 0; k < 4; k++)
+			pos[k] = w4[k] & 0xffu;
+	}
 }
 
 __device__ __forceinline__ uint32_t bin_tri_at(const FrameBufs &b, uint32_t i, uint32_t qc,
@@ -885,84 +959,47 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 		}
 		return;
 	}
-	__shared__ uint64_t s_key[kBigSortCap];
-	__shared__ uint32_t s_samples[kSortThreads];
+	__shared__ uint32_t s_key[kBigSortCap];
+	__shared__ uint32_t s_lo[kSortThreads / 64], s_hi[kSortThreads / 64];
+	__shared__ uint32_t s_wsum[2][kSortThreads / 64];
 	__shared__ uint32_t s_bound[kBigWaves + 1];
 	__shared__ uint32_t s_base, s_parts, s_total;
+	static_assert(kBigSortCap <= (1 << 13), "list positions fit 13 bits of the key");
+	static_assert(kSortThreads == 256, "four waves share the chunks");
 	const uint32_t *list = b.blk_lists + sl.x;
 	const uint4 *recs = b.high_records + sl.z;
-	// Bitonic sort of the block keys: levels up to 256 in registers (one 256-key chunk per
-	// wave at a time), then per level the strides >= 256 in LDS and the rest in registers.
-	static_assert(kSortThreads == 256, "four waves share the chunks");
+	const int wave_s = threadIdx.x >> 6, lane_s = lane_id();
 	const uint32_t P = max(256u, next_pow2(m));
 	const uint32_t nchunks = P / 256;
-	const int wave_s = threadIdx.x >> 6, lane_s = lane_id();
+	// Keys are (19-bit depth rank << 13 | list position). Pass 1 gathers the records once:
+	// depth keys (in s_key), the total of the sample counts and the block's key range.
+	uint32_t lo = ~0u, hi = 0u;
+	uint32_t mine = 0;
 	for(uint32_t ch = wave_s; ch < nchunks; ch += kSortThreads / 64) {
-		uint64_t v[4];
 #pragma unroll
 		for(int e = 0; e < 4; e++) {
 			const uint32_t j = ch * 256 + e * 64 + lane_s;
-			v[e] = ~0ull;
 			if(j < m) {
-				// low word: list position (13 bits) | samples in the block << 13; equal depths
-				// order by samples, then position (any fixed order is exact per pixel)
 				const uint4 rec = recs[list[j]];
 				const uint32_t smp = (uint32_t)__popcll(record_block_mask(rec, bx));
-				v[e] = block_key(b, rec, bx, px0, py0, j | (smp << 13));
+				mine += smp;
+				const uint32_t dk = ~ord_f32(block_inv_depth(b, rec, bx, px0, py0));
+				s_key[j] = dk;
+				lo = min(lo, dk);
+				hi = max(hi, dk);
 			}
 		}
-		bitonic_regs_from<4>(v, ch * 256);
-#pragma unroll
-		for(int e = 0; e < 4; e++)
-			s_key[ch * 256 + e * 64 + lane_s] = v[e];
 	}
+	lo = wave_min_u32(lo);
+	hi = wave_max_u32(hi);
+	mine = __shfl(wave_inclusive_add(mine), 63, 64);
+	if(lane_s == 0)
+		s_lo[wave_s] = lo, s_hi[wave_s] = hi, s_wsum[1][wave_s] = mine;
 	__syncthreads();
-	for(uint32_t k = 512; k <= P; k <<= 1) {
-		for(uint32_t j = k >> 1; j >= 256; j >>= 1) {
-			for(uint32_t i = threadIdx.x; i < P / 2; i += kSortThreads) {
-				uint32_t lo = ((i & ~(j - 1)) << 1) | (i & (j - 1));
-				uint32_t hi = lo | j;
-				bool asc = (lo & k) == 0;
-				uint64_t a = s_key[lo], bb = s_key[hi];
-				if((a > bb) == asc) {
-					s_key[lo] = bb;
-					s_key[hi] = a;
-				}
-			}
-			__syncthreads();
-		}
-		for(uint32_t ch = wave_s; ch < nchunks; ch += kSortThreads / 64) {
-			uint64_t v[4];
+	uint32_t tot = 0;
 #pragma unroll
-			for(int e = 0; e < 4; e++)
-				v[e] = s_key[ch * 256 + e * 64 + lane_s];
-			bitonic_regs_steps<4>(v, ch * 256, k, 128);
-#pragma unroll
-			for(int e = 0; e < 4; e++)
-				s_key[ch * 256 + e * 64 + lane_s] = v[e];
-		}
-		__syncthreads();
-	}
-	// s_key[i] now holds (samples << 13 | list position) of the i-th entry in depth order.
-	static_assert(kBigSortCap <= (1 << 13), "list positions fit 13 bits of the key");
-	const uint32_t per = (m + kSortThreads - 1) / kSortThreads;
-	const uint32_t i0 = min(m, threadIdx.x * per), i1 = min(m, i0 + per);
-	uint32_t samples = 0;
-	for(uint32_t i = i0; i < i1; i++)
-		samples += ((uint32_t)s_key[i] >> 13) & 0x7fu;
-	// exclusive prefix over the threads: wave scans, then the wave totals
-	const uint32_t inc = wave_inclusive_add(samples);
-	if(lane_s == 63)
-		s_samples[wave_s] = inc;
-	__syncthreads();
-	uint32_t before = 0, tot = 0;
-#pragma unroll
-	for(int w = 0; w < kSortThreads / 64; w++) {
-		const uint32_t t = s_samples[w];
-		before += w < wave_s ? t : 0u;
-		tot += t;
-	}
-	const uint32_t run0 = before + inc - samples;
+	for(int w = 0; w < kSortThreads / 64; w++)
+		lo = min(lo, s_lo[w]), hi = max(hi, s_hi[w]), tot += s_wsum[1][w];
 	if(threadIdx.x == 0) {
 		uint32_t nw = (tot + c.samples_per_wave - 1) / c.samples_per_wave;
 		nw = nw < 1u ? 1u : nw > (uint32_t)kBigWaves ? (uint32_t)kBigWaves : nw;
@@ -990,25 +1027,96 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 			b.blk_flags[slot] = kSlotPromoted; // nothing rendered; the frame reports overflow
 		return;
 	}
+	const DepthGrid<19> grid(lo, hi);
+	// Bitonic sort: levels up to 256 in registers (one 256-key chunk per wave at a time; each
+	// lane reads back only what it wrote in pass 1), then per level the strides >= 256 in LDS
+	// and the rest in registers. A real key never equals the padding ~0u: position 8191
+	// exists only when nothing is padded.
+	for(uint32_t ch = wave_s; ch < nchunks; ch += kSortThreads / 64) {
+		uint32_t v[4];
+#pragma unroll
+		for(int e = 0; e < 4; e++) {
+			const uint32_t j = ch * 256 + e * 64 + lane_s;
+			v[e] = j < m ? (grid.rank(s_key[j]) << 13) | j : ~0u;
+		}
+		bitonic_regs_from<4>(v, ch * 256);
+#pragma unroll
+		for(int e = 0; e < 4; e++)
+			s_key[ch * 256 + e * 64 + lane_s] = v[e];
+	}
+	__syncthreads();
+	for(uint32_t k = 512; k <= P; k <<= 1) {
+		for(uint32_t j = k >> 1; j >= 256; j >>= 1) {
+			for(uint32_t i = threadIdx.x; i < P / 2; i += kSortThreads) {
+				uint32_t lo_i = ((i & ~(j - 1)) << 1) | (i & (j - 1));
+				uint32_t hi_i = lo_i | j;
+				bool asc = (lo_i & k) == 0;
+				uint32_t a = s_key[lo_i], bb = s_key[hi_i];
+				if((a > bb) == asc) {
+					s_key[lo_i] = bb;
+					s_key[hi_i] = a;
+				}
+			}
+			__syncthreads();
+		}
+		for(uint32_t ch = wave_s; ch < nchunks; ch += kSortThreads / 64) {
+			uint32_t v[4];
+#pragma unroll
+			for(int e = 0; e < 4; e++)
+				v[e] = s_key[ch * 256 + e * 64 + lane_s];
+			bitonic_regs_steps<4>(v, ch * 256, k, 128);
+#pragma unroll
+			for(int e = 0; e < 4; e++)
+				s_key[ch * 256 + e * 64 + lane_s] = v[e];
+		}
+		__syncthreads();
+	}
+	// s_key[i] & 0x1fff is the list position of the i-th entry in depth order. Entries are
+	// visited in rounds of kSortThreads consecutive positions (coalesced LDS and output).
+	const uint32_t rounds = (m + kSortThreads - 1) / kSortThreads;
 	const uint32_t nw = s_parts, total = s_total;
 	uint32_t *hdr = b.sorted_pool + s_base;
 	uint32_t *tri_out = hdr + kSortedHeader;
 	uint64_t *mask_out = (uint64_t *)(hdr + sorted_mask_offset(m));
-	// wave w starts at the first entry with at least w * total / nw samples before it
-	uint32_t run = run0, found = 0;
-	for(uint32_t i = i0; i < i1; i++) {
-		uint4 rec = recs[list[(uint32_t)s_key[i] & 0x1fffu]];
-		uint64_t msk = record_block_mask(rec, bx);
-		for(uint32_t w = 1; w < nw; w++) {
-			uint32_t target = (uint32_t)(((uint64_t)w * total) / nw);
-			if(!((found >> w) & 1u) && run >= target) {
-				atomicMin(&s_bound[w], i);
-				found |= 1u << w;
-			}
+	// wave w starts at the first entry with at least w * total / nw samples before it whose
+	// depth rank differs from its predecessor's: the waves' ranges are chained per pixel
+	// and must not split a group of equal ranks (whose order is only the list order). The
+	// running prefix is a block scan per round (wave scans, then the wave totals).
+	uint32_t carry = 0, found = 0;
+	for(uint32_t r = 0; r < rounds; r++) {
+		const uint32_t i = r * kSortThreads + threadIdx.x;
+		uint4 rec = make_uint4(0u, 0u, 0u, 0u);
+		uint64_t msk = 0;
+		if(i < m) {
+			rec = recs[list[s_key[i] & 0x1fffu]];
+			msk = record_block_mask(rec, bx);
 		}
-		tri_out[i] = rec.x;
-		mask_out[i] = msk;
-		run += (uint32_t)__popcll(msk);
+		const uint32_t smp = (uint32_t)__popcll(msk);
+		const uint32_t inc = wave_inclusive_add(smp);
+		if(lane_s == 63)
+			s_wsum[r & 1][wave_s] = inc;
+		__syncthreads();
+		uint32_t before = carry, round_tot = 0;
+#pragma unroll
+		for(int w = 0; w < kSortThreads / 64; w++) {
+			const uint32_t t = s_wsum[r & 1][w];
+			before += w < wave_s ? t : 0u;
+			round_tot += t;
+		}
+		carry += round_tot;
+		if(i < m) {
+			const uint32_t run = before + inc - smp; // samples of the entries before i
+			const bool cut = i == 0 || (s_key[i] >> 13) != (s_key[i - 1] >> 13);
+			for(uint32_t w = 1; w < nw; w++) {
+				const uint32_t target = (uint32_t)(((uint64_t)w * total) / nw);
+				if(!((found >> w) & 1u) && run >= target && cut) {
+					atomicMin(&s_bound[w], i);
+					found |= 1u << w;
+				}
+			}
+			tri_out[i] = rec.x;
+			mask_out[i] = msk;
+		}
 	}
 	__syncthreads();
 	if(threadIdx.x == 0) {
@@ -1183,31 +1291,15 @@ __global__ __launch_bounds__(64) void block_raster_kernel(FrameConst c, FrameBuf
 	}
 	const uint4 *recs = (level == LUCID_BIN_LEVEL_LOW ? b.low_records : b.high_records) + sl.z;
 	const uint32_t *list = b.blk_lists + sl.x;
-	auto load = [&](uint32_t j) -> uint64_t {
-		return j < m ? block_key(b, recs[list[j]], bx, px0, py0, j) : ~0ull;
-	};
-	uint64_t v[4];
-	uint32_t P = next_pow2(m < 64 ? 64 : m);
-	if(P <= 64) {
-		uint64_t w1[1] = {load(lane)};
-		bitonic_regs<1>(w1);
-		v[0] = w1[0], v[1] = v[2] = v[3] = ~0ull;
-	} else if(P <= 128) {
-		uint64_t w2[2] = {load(lane), load(64 + lane)};
-		bitonic_regs<2>(w2);
-		v[0] = w2[0], v[1] = w2[1], v[2] = v[3] = ~0ull;
-	} else {
-		v[0] = load(lane), v[1] = load(64 + lane), v[2] = load(128 + lane), v[3] = load(192 + lane);
-		bitonic_regs<4>(v);
-	}
-	// sorted entry 64 j + lane: list position (uint32_t)v[j]
+	uint32_t pos[4];
+	sort_block_positions(b, recs, list, m, bx, px0, py0, pos);
 	uint64_t msk_all[4];
 	uint32_t tri_all[4];
 #pragma unroll
 	for(int j = 0; j < 4; j++) {
 		msk_all[j] = 0, tri_all[j] = 0;
 		if(j * 64 + lane < (int)m) {
-			uint4 rec = recs[list[(uint32_t)v[j]]];
+			uint4 rec = recs[list[pos[j]]];
 			msk_all[j] = record_block_mask(rec, bx);
 			tri_all[j] = rec.x;
 		}
@@ -1642,27 +1734,12 @@ __device__ __forceinline__ BlockEntries bwd_block_entries(const FrameConst &c, c
 	const uint32_t level = flags & kSlotLevelMask;
 	const uint4 *recs = (level == LUCID_BIN_LEVEL_LOW ? b.low_records : b.high_records) + sl.z;
 	const uint32_t *list = b.blk_lists + sl.x;
-	auto load = [&](uint32_t j) -> uint64_t {
-		return j < m ? block_key(b, recs[list[j]], bx, px0, py0, j) : ~0ull;
-	};
-	uint64_t v[4];
-	uint32_t P = next_pow2(m < 64 ? 64 : m);
-	if(P <= 64) {
-		uint64_t w1[1] = {load(lane)};
-		bitonic_regs<1>(w1);
-		v[0] = w1[0], v[1] = v[2] = v[3] = ~0ull;
-	} else if(P <= 128) {
-		uint64_t w2[2] = {load(lane), load(64 + lane)};
-		bitonic_regs<2>(w2);
-		v[0] = w2[0], v[1] = w2[1], v[2] = v[3] = ~0ull;
-	} else {
-		v[0] = load(lane), v[1] = load(64 + lane), v[2] = load(128 + lane), v[3] = load(192 + lane);
-		bitonic_regs<4>(v);
-	}
+	uint32_t pos[4];
+	sort_block_positions(b, recs, list, m, bx, px0, py0, pos);
 #pragma unroll
 	for(int j = 0; j < 4; j++)
 		if(j * 64 + lane < (int)m) {
-			uint4 rec = recs[list[(uint32_t)v[j]]];
+			uint4 rec = recs[list[pos[j]]];
 			s_mask_all[j * 64 + lane] = record_block_mask(rec, bx);
 			s_tri_all[j * 64 + lane] = rec.x;
 		}
