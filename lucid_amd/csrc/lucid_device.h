@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/lucid_c.h"
 #include "../../include/lucid_structs.h"
 
 namespace lucid {

@@ -909,6 +909,7 @@ __global__ __launch_bounds__(kRowgenThreads) void rowgen_high_kernel(FrameConst 
 // chosen per renderer by its tuning; the backward kernels use kSliceWords.)
 constexpr int kSliceWords = 1;
 constexpr int kSliceEntries = 64 * kSliceWords;
+constexpr uint32_t kTieRun = 32; // longest run of equal block ranks put in exact order
 constexpr int kBigWaves = 4; // waves of block_raster_big_kernel / the backward block kernels
 
 // Pre-sorted block layout in the sorted pool (offsets in words from an even base):
@@ -962,6 +963,7 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 	__shared__ uint32_t s_key[kBigSortCap];
 	__shared__ uint32_t s_lo[kSortThreads / 64], s_hi[kSortThreads / 64];
 	__shared__ uint32_t s_wsum[2][kSortThreads / 64];
+	__shared__ uint32_t s_tdk[2][kSortThreads]; // full depth keys of tied entries, per round
 	__shared__ uint32_t s_bound[kBigWaves + 1];
 	__shared__ uint32_t s_base, s_parts, s_total;
 	static_assert(kBigSortCap <= (1 << 13), "list positions fit 13 bits of the key");
@@ -1082,15 +1084,30 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 	// depth rank differs from its predecessor's: the waves' ranges are chained per pixel
 	// and must not split a group of equal ranks (whose order is only the list order). The
 	// running prefix is a block scan per round (wave scans, then the wave totals).
+	// Equal ranks are ordered by list position only. The forward's ordering window absorbs
+	// that, but the backward's pass A flushes its window at every slice, so in frames with
+	// fp32 samples (the differentiable ones) each run of equal ranks that lies within one
+	// round (and is at most kTieRun long) is put in exact depth-key order here: its members
+	// recompute their full depth key and take their place among the run.
+	const bool exact_runs = (c.opts & LUCID_OPT_FP32_SAMPLES) != 0;
 	uint32_t carry = 0, found = 0;
 	for(uint32_t r = 0; r < rounds; r++) {
 		const uint32_t i = r * kSortThreads + threadIdx.x;
+		const uint32_t r0 = r * kSortThreads, r1 = min(m, r0 + kSortThreads);
 		uint4 rec = make_uint4(0u, 0u, 0u, 0u);
 		uint64_t msk = 0;
+		uint32_t key = 0, dk = 0;
+		bool tie = false;
 		if(i < m) {
-			rec = recs[list[s_key[i] & 0x1fffu]];
+			key = s_key[i];
+			rec = recs[list[key & 0x1fffu]];
 			msk = record_block_mask(rec, bx);
+			tie = exact_runs && ((i > 0 && (s_key[i - 1] >> 13) == (key >> 13)) ||
+								 (i + 1 < m && (s_key[i + 1] >> 13) == (key >> 13)));
+			if(tie)
+				dk = ~ord_f32(block_inv_depth(b, rec, bx, px0, py0));
 		}
+		s_tdk[r & 1][threadIdx.x] = dk;
 		const uint32_t smp = (uint32_t)__popcll(msk);
 		const uint32_t inc = wave_inclusive_add(smp);
 		if(lane_s == 63)
@@ -1114,8 +1131,29 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 					found |= 1u << w;
 				}
 			}
-			tri_out[i] = rec.x;
-			mask_out[i] = msk;
+			uint32_t out = i;
+			if(tie) {
+				const uint32_t rk = key >> 13;
+				uint32_t gs = i, ge = i;
+				while(gs > r0 && i - gs < kTieRun && (s_key[gs - 1] >> 13) == rk)
+					gs--;
+				while(ge + 1 < r1 && ge - i < kTieRun && (s_key[ge + 1] >> 13) == rk)
+					ge++;
+				// the whole run is [gs, ge]: it starts and ends within this round
+				const bool inside = (gs == 0 || (s_key[gs - 1] >> 13) != rk) &&
+									(ge + 1 >= m || (s_key[ge + 1] >> 13) != rk) && ge - gs < kTieRun;
+				if(inside) {
+					uint32_t ahead = 0;
+					for(uint32_t k = gs; k <= ge; k++) {
+						const uint32_t dk_k = s_tdk[r & 1][k - r0];
+						const uint32_t j_k = s_key[k] & 0x1fffu;
+						ahead += (dk_k < dk || (dk_k == dk && j_k < (key & 0x1fffu))) ? 1u : 0u;
+					}
+					out = gs + ahead;
+				}
+			}
+			tri_out[out] = rec.x;
+			mask_out[out] = msk;
 		}
 	}
 	__syncthreads();
@@ -2263,7 +2301,7 @@ static void launch_raster_t(const FrameConst &c, const FrameBufs &b, hipStream_t
 // after block rasterisation, after the fallback kernel.
 hipError_t launch_raster(const FrameConst &c, const FrameBufs &b, hipStream_t stream,
 						 hipEvent_t *events) {
-	if(c.opts & (1u << 16))
+	if(c.opts & LUCID_OPT_FP32_SAMPLES)
 		launch_raster_t<true>(c, b, stream, events);
 	else
 		launch_raster_t<false>(c, b, stream, events);
