@@ -102,9 +102,10 @@ def batch_leg(num_scenes: int, steps: int = 3):
 def backward_ratio(steps: int):
     """fwd+bwd vs plain fwd on the differentiable workload: 200k soup triangles at 512x512
     (the config0 generator, ~160 fragments/pixel), fp32 samples. A fwd+bwd step is a forward
-    frame followed by the backward pass for a dL/dimage, as the autograd binding runs it (the
-    per-pixel sorted fragment lists are recomputed; the frame's bins and block lists are
-    reused); the full-recompute backward (lucid_backward) is timed beside it."""
+    frame followed by the backward pass for a dL/dimage, as the autograd binding runs them:
+    the forward records the backward's first pass (blend order and partial composites) and
+    the backward reuses the frame's bins, block lists and that record. The full-recompute
+    backward (lucid_backward after a plain forward) is timed beside it."""
     import torch
     from lucid_amd import _lib, scenes
     from lucid_amd.renderer import GpuScene, LucidRenderer
@@ -116,14 +117,15 @@ def backward_ratio(steps: int):
     r.render(g, out)
     frags = int(r.read_info()[60])
     r.backward(g, grad)
-    r.render(g, out)
-    r.backward(g, grad, reuse_frame=True)
+    for _ in range(2):  # the first recording frame may size the backward buffers instead
+        r.render(g, out, record_backward=True)
+        r.backward(g, grad, reuse_frame=True)
     torch.cuda.synchronize()
 
     def timed(reuse=None):
         t0 = time.perf_counter()
         for _ in range(steps):
-            r.render(g, out)
+            r.render(g, out, record_backward=reuse is True)
             if reuse is not None:
                 r.backward(g, grad, reuse_frame=reuse)
         torch.cuda.synchronize()

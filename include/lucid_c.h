@@ -102,6 +102,23 @@ void lucid_destroy(lucid_renderer *r);
 
 int lucid_render(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_frame_outputs *out,
 				 void *stream);
+/* lucid_render with flags. LUCID_RENDER_RECORD_BACKWARD: the frame's rasterisation also
+ * records what the backward's first pass computes (each pixel's blend order per depth range,
+ * the ranges' fp64 partial composites and the gradient row layout), so that a following
+ * lucid_backward_ex(LUCID_BACKWARD_REUSE_FRAME) on the same inputs starts from the record
+ * instead of walking the frame again. Needs LUCID_OPT_FP32_SAMPLES, fp32 colours and "over"
+ * blending (else LUCID_ERR_UNSUPPORTED). Fragment counts equal lucid_render's; colours too,
+ * up to the rounding of pixels the recording pass leaves to the exact per-pixel sort. When
+ * the record does not fit the renderer's backward buffers (sized by earlier frames), the
+ * frame is rendered without it, the buffers grow, and the backward walks the frame itself.
+ * Flags 0: lucid_render. No counterpart in the reference (the differentiable extension). */
+#define LUCID_RENDER_RECORD_BACKWARD 1u
+int lucid_render_ex(lucid_renderer *r, const lucid_frame_inputs *in,
+					const lucid_frame_outputs *out, uint32_t flags, void *stream);
+/* 1 if the renderer's last finished frame holds a backward record (rendered with
+ * LUCID_RENDER_RECORD_BACKWARD, the record fitted, and no backward buffer has grown since),
+ * else 0. No counterpart in the reference. */
+int lucid_frame_recorded(const lucid_renderer *r);
 /* Adjoint of the fp32-sample forward ("over" blending, fp32 vertex colours colors_rgb and
  * per-triangle alpha tri_alpha): given dL/d(image_rgba32f) (device, width*height float4;
  * the alpha channel is ignored), writes dL/d(colors_rgb) (device, num_verts float3) and

@@ -28,12 +28,14 @@ OPT_ALPHA_THRESHOLD = 1 << 7
 OPT_FP32_SAMPLES = 1 << 16
 
 BACKWARD_REUSE_FRAME = 1
+RENDER_RECORD_BACKWARD = 1
 
 INFO_U32 = 1152
 
 # Every symbol include/lucid_c.h declares.
 EXPORTED = (
-    "lucid_error_string", "lucid_create", "lucid_destroy", "lucid_render", "lucid_finish",
+    "lucid_error_string", "lucid_create", "lucid_destroy", "lucid_render", "lucid_render_ex",
+    "lucid_frame_recorded", "lucid_finish",
     "lucid_read_info", "lucid_verify_info", "lucid_read_bin_lists", "lucid_read_setup",
     "lucid_stage_times", "lucid_backward", "lucid_backward_ex",
     "lucid_set_tuning", "lucid_get_tuning", "lucid_autotune",
@@ -112,6 +114,11 @@ def load(path: str = LIB_PATH):
     lib.lucid_destroy.argtypes = [vp]
     lib.lucid_render.restype = i32
     lib.lucid_render.argtypes = [vp, ctypes.POINTER(FrameInputs), ctypes.POINTER(FrameOutputs), vp]
+    lib.lucid_render_ex.restype = i32
+    lib.lucid_render_ex.argtypes = [vp, ctypes.POINTER(FrameInputs), ctypes.POINTER(FrameOutputs),
+                                    ctypes.c_uint32, vp]
+    lib.lucid_frame_recorded.restype = i32
+    lib.lucid_frame_recorded.argtypes = [vp]
     lib.lucid_finish.restype = i32
     lib.lucid_finish.argtypes = [vp]
     lib.lucid_read_info.restype = i32

@@ -37,7 +37,11 @@ class LucidRenderFunction:
                     colors = colors_rgb.detach().contiguous().to(torch.float32)
                     alpha = tri_alpha.detach().contiguous().to(torch.float32)
                     out = renderer.alloc_outputs(rgba8=False, rgba32f=True, counts=False)
-                    renderer.render(gscene, out, colors_rgb=colors, tri_alpha=alpha)
+                    # the frame records the backward's first pass only when a backward can
+                    # follow (autograd recording this op)
+                    renderer.render(gscene, out, colors_rgb=colors, tri_alpha=alpha,
+                                    record_backward=ctx.needs_input_grad[0]
+                                    or ctx.needs_input_grad[1])
                     ctx.renderer, ctx.gscene = renderer, gscene
                     ctx.save_for_backward(colors, alpha)
                     return out["image_f"]

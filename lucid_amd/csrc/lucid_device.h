@@ -28,7 +28,7 @@ constexpr int kLowRowCap = 1024;     // records per block row of a low bin (< 10
 
 // Internal control words (separate from LucidInfo, whose layout is the reference's).
 enum CtrlWord : int {
-	kCtrlNumLarge = 0,      // unused (the large-quad sub-lists keep their own counters)
+	kCtrlRecorded = 0,      // 1: this frame's rasterisation records the backward's pass A
 	kCtrlRecordCursor = 1,  // high-bin record pool size the frame needs (uint4 units)
 	kCtrlFallbackCount = 2, // pixels in the exact-fallback queue
 	kCtrlOverflow = 3,      // bit 0 bin lists, 1 record pool, 2 fallback queue, 3 spill pool
@@ -91,6 +91,14 @@ struct BwdStart {
 	double t, q;
 	float gr, gg, gb, pad;
 };
+
+// What the backward's pass A leaves for one (depth range, pixel) when a forward frame
+// records it: T before the range and, per channel, D = (out - composite before the range)
+// in fp64, so that the start state is q = sum_c g_c D_c once dL/dpixel is known.
+struct BwdRecord {
+	double t, d[3];
+};
+
 struct BwdBufs {
 	const float4 *grad_image; // dL/d(rgba32f image); alpha ignored
 	uint32_t *row_base;       // bin_count * 16: first gradient row of each block
@@ -102,6 +110,8 @@ struct BwdBufs {
 	uint64_t start_cap;       // depth ranges
 	uint8_t *seq;             // 64 bytes per gradient row: recorded blend order
 	uint64_t seq_rows_cap;    // gradient rows
+	BwdRecord *rec;           // 64 per depth range (start_cap): pass A recorded by a forward
+	uint8_t *pix_flags;       // width * height, with rec: clamp bits 0-2, bit 3 = exact pixel
 };
 
 // Scratch of the gradient reduction (ntri triangle ids, nrows gradient rows).

@@ -34,6 +34,12 @@ hipError_t launch_backward_order(const FrameConst &c, const FrameBufs &b, const 
 								 hipStream_t stream);
 hipError_t launch_backward_rows(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
 								hipStream_t stream);
+hipError_t launch_raster_recording(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
+								   hipStream_t stream, hipEvent_t *events);
+hipError_t launch_backward_recorded(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
+									hipStream_t stream);
+hipError_t launch_backward_recorded_order(const FrameConst &c, const FrameBufs &b,
+										  const BwdBufs &g, hipStream_t stream);
 size_t reduce_temp_bytes(uint32_t npairs, uint32_t nscan);
 hipError_t reduce_check_corners(const FrameConst &c, const FrameBufs &b, uint32_t ntri,
 								uint32_t *vkeys, uint32_t *changed, hipStream_t stream);
@@ -98,6 +104,7 @@ struct lucid_renderer {
 	// backward
 	DevBuf grad_row_base, grad_range_base, grad_rows, grad_row_tri, grad_start, grad_seq,
 		red_counts, red_offsets, red_grouped, tri_grad, red_temp;
+	DevBuf grad_rec, grad_pix_flags; // pass A as recorded by a forward frame
 	uint64_t grad_rows_cap = 0, grad_seq_rows_cap = 0, grad_start_cap = 0;
 	// corners grouped by vertex (depends on the mesh only): the corner -> vertex map of the
 	// last backward pass, and the corners sorted by vertex
@@ -111,6 +118,9 @@ struct lucid_renderer {
 	// from (valid once the frame that built them finished without an overflow).
 	struct Retained {
 		bool valid = false, pending = false;
+		// record: the frame ran the recording pass A; recorded: its gate admitted it, so the
+		// recorded order, start records and gradient row layout belong to this frame
+		bool record = false, recorded = false;
 		lucid_frame_inputs in;
 		lucid_config cfg;
 		lucid_tuning tuning;
@@ -502,6 +512,65 @@ int prepare_frame(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_f
 
 namespace {
 
+// Backward buffers at their current capacities (first use: optimistic sizes; a frame that
+// needs more reports it and they grow), described in g.
+int backward_buffers(lucid_renderer *r, const float *grad_image, BwdBufs &g) {
+	const size_t slots = (size_t)r->bin_count * 16;
+	LUCID_CHECK(r->grad_row_base.reserve(slots * 4));
+	LUCID_CHECK(r->grad_range_base.reserve(slots * 4));
+	if(r->grad_rows_cap == 0) {
+		r->grad_rows_cap = 1u << 20;
+		LUCID_CHECK(r->grad_rows.reserve(r->grad_rows_cap * kGradWords * 4));
+		LUCID_CHECK(r->grad_row_tri.reserve(r->grad_rows_cap * 4));
+	}
+	if(r->grad_seq_rows_cap == 0) {
+		r->grad_seq_rows_cap = 1u << 16;
+		r->grad_start_cap = 1u << 12;
+		LUCID_CHECK(r->grad_seq.reserve(r->grad_seq_rows_cap * 64));
+		LUCID_CHECK(r->grad_start.reserve(r->grad_start_cap * 64 * sizeof(BwdStart)));
+	}
+	LUCID_CHECK(r->grad_rec.reserve(r->grad_start_cap * 64 * sizeof(BwdRecord)));
+	LUCID_CHECK(r->grad_pix_flags.reserve((size_t)r->width * r->height));
+	g.grad_image = (const float4 *)grad_image;
+	g.row_base = r->grad_row_base.as<uint32_t>();
+	g.range_base = r->grad_range_base.as<uint32_t>();
+	g.rows = r->grad_rows.as<float>();
+	g.row_tri = r->grad_row_tri.as<uint32_t>();
+	g.rows_cap = r->grad_rows_cap;
+	g.start = r->grad_start.as<BwdStart>();
+	g.start_cap = r->grad_start_cap;
+	g.seq = r->grad_seq.as<uint8_t>();
+	g.seq_rows_cap = r->grad_seq_rows_cap;
+	g.rec = r->grad_rec.as<BwdRecord>();
+	g.pix_flags = r->grad_pix_flags.as<uint8_t>();
+	return LUCID_OK;
+}
+
+// Grows the backward buffers to a frame's needs: block_rows gradient rows of the block
+// kernels (each with its 64-byte recorded order), `ranges` depth ranges (start states and
+// records) and `extra` rows of exactly sorted pixels. Contents are not kept, so a recorded
+// pass A no longer stands after a growth.
+int grow_backward(lucid_renderer *r, uint64_t block_rows, uint64_t ranges, uint64_t extra) {
+	if(block_rows > r->grad_seq_rows_cap) {
+		r->grad_seq_rows_cap = block_rows + block_rows / 8;
+		LUCID_CHECK(r->grad_seq.reserve(r->grad_seq_rows_cap * 64));
+	}
+	if(ranges > r->grad_start_cap) {
+		r->grad_start_cap = ranges + ranges / 8;
+		LUCID_CHECK(r->grad_start.reserve(r->grad_start_cap * 64 * sizeof(BwdStart)));
+		LUCID_CHECK(r->grad_rec.reserve(r->grad_start_cap * 64 * sizeof(BwdRecord)));
+	}
+	const uint64_t want = block_rows + std::max<uint64_t>(std::max<uint64_t>(1u << 16, block_rows / 8),
+														  extra + extra / 8);
+	if(want > r->grad_rows_cap) {
+		LUCID_CHECK(r->grad_rows.reserve(want * kGradWords * 4));
+		LUCID_CHECK(r->grad_row_tri.reserve(want * 4));
+		r->grad_rows_cap = want;
+	}
+	r->retained.recorded = false;
+	return LUCID_OK;
+}
+
 // Records the inputs the buffers are being built from; `pending` until the frame finishes.
 void retain_frame(lucid_renderer *r, const lucid_frame_inputs *in, bool pending) {
 	lucid_renderer::Retained &k = r->retained;
@@ -513,6 +582,7 @@ void retain_frame(lucid_renderer *r, const lucid_frame_inputs *in, bool pending)
 	k.in.config = nullptr;
 	k.pending = pending;
 	k.valid = !pending;
+	k.record = k.recorded = false;
 }
 
 // The buffers hold the setup, bins and block lists of exactly these inputs.
@@ -536,8 +606,18 @@ bool retained_matches(const lucid_renderer *r, const lucid_frame_inputs *in) {
 
 int lucid_render(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_frame_outputs *out,
 				 void *stream_ptr) {
-	if(!r || !in || !out)
+	return lucid_render_ex(r, in, out, 0u, stream_ptr);
+}
+
+int lucid_render_ex(lucid_renderer *r, const lucid_frame_inputs *in,
+					const lucid_frame_outputs *out, uint32_t flags, void *stream_ptr) {
+	if(!r || !in || !out || (flags & ~LUCID_RENDER_RECORD_BACKWARD))
 		return LUCID_ERR_INVALID_ARG;
+	const bool record = (flags & LUCID_RENDER_RECORD_BACKWARD) != 0;
+	// the recorded pass is the backward's: fp32 samples of fp32 colours, "over" blending
+	if(record && (!(r->opts & LUCID_OPT_FP32_SAMPLES) || !in->colors_rgb || !in->tri_alpha ||
+				  in->colors_rgba8 || (r->opts & LUCID_OPT_ADDITIVE_BLENDING)))
+		return LUCID_ERR_UNSUPPORTED;
 	hipStream_t stream = (hipStream_t)stream_ptr;
 	FrameConst c;
 	FrameBufs b;
@@ -545,6 +625,12 @@ int lucid_render(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_fr
 	int pe = prepare_frame(r, in, out, c, b, nq);
 	if(pe != LUCID_OK)
 		return pe;
+	BwdBufs g;
+	if(record) {
+		pe = backward_buffers(r, nullptr, g);
+		if(pe != LUCID_OK)
+			return pe;
+	}
 	r->retained.valid = r->retained.pending = false; // the buffers are rebuilt below
 	(void)hipGetLastError(); // launches below report their own status, not an older call's
 
@@ -557,7 +643,10 @@ int lucid_render(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_fr
 	LUCID_CHECK(launch_binning(c, b, stream));
 	if(r->timers)
 		LUCID_CHECK(hipEventRecord(r->stage_ev[2], stream));
-	LUCID_CHECK(launch_raster(c, b, stream, r->timers ? r->stage_ev + 3 : nullptr));
+	if(record)
+		LUCID_CHECK(launch_raster_recording(c, b, g, stream, r->timers ? r->stage_ev + 3 : nullptr));
+	else
+		LUCID_CHECK(launch_raster(c, b, stream, r->timers ? r->stage_ev + 3 : nullptr));
 	LUCID_CHECK(hipMemcpyAsync(r->status, r->ctrl.ptr, kCtrlWords * 4, hipMemcpyDeviceToHost,
 							   stream));
 	LUCID_CHECK(hipEventRecord(r->done, stream));
@@ -565,7 +654,12 @@ int lucid_render(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_fr
 	r->last_num_quads = nq;
 	r->last_stream = stream;
 	retain_frame(r, in, true);
+	r->retained.record = record;
 	return LUCID_OK;
+}
+
+int lucid_frame_recorded(const lucid_renderer *r) {
+	return r && r->retained.valid && !r->retained.pending && r->retained.recorded ? 1 : 0;
 }
 
 int lucid_backward(lucid_renderer *r, const lucid_frame_inputs *in, const float *grad_image,
@@ -599,20 +693,6 @@ int lucid_backward_ex(lucid_renderer *r, const lucid_frame_inputs *in, const flo
 	const uint64_t nverts = (uint64_t)std::max<int64_t>(in->num_verts, 0);
 	if(3 * ntri >= (1ull << 32))
 		return LUCID_ERR_UNSUPPORTED;
-	const size_t slots = (size_t)r->bin_count * 16;
-	LUCID_CHECK(r->grad_row_base.reserve(slots * 4));
-	LUCID_CHECK(r->grad_range_base.reserve(slots * 4));
-	if(r->grad_rows_cap == 0) {
-		r->grad_rows_cap = 1u << 20;
-		LUCID_CHECK(r->grad_rows.reserve(r->grad_rows_cap * kGradWords * 4));
-		LUCID_CHECK(r->grad_row_tri.reserve(r->grad_rows_cap * 4));
-	}
-	if(r->grad_seq_rows_cap == 0) {
-		r->grad_seq_rows_cap = 1u << 16;
-		r->grad_start_cap = 1u << 12;
-		LUCID_CHECK(r->grad_seq.reserve(r->grad_seq_rows_cap * 64));
-		LUCID_CHECK(r->grad_start.reserve(r->grad_start_cap * 64 * sizeof(BwdStart)));
-	}
 	// corner -> vertex map, checked against the one the cached vertex grouping was built from
 	const bool vtx_same_sizes = r->vtx_valid && r->vtx_ntri == ntri && r->vtx_nverts == nverts;
 	LUCID_CHECK(r->vtx_keys.reserve(std::max<uint64_t>(3 * ntri, 1) * 4));
@@ -620,16 +700,9 @@ int lucid_backward_ex(lucid_renderer *r, const lucid_frame_inputs *in, const flo
 	LUCID_CHECK(r->vtx_corners.reserve(std::max<uint64_t>(3 * ntri, 1) * 4));
 	LUCID_CHECK(r->vtx_iota.reserve(std::max<uint64_t>(3 * ntri, 1) * 4));
 	BwdBufs g;
-	g.grad_image = (const float4 *)grad_image;
-	g.row_base = r->grad_row_base.as<uint32_t>();
-	g.range_base = r->grad_range_base.as<uint32_t>();
-	g.rows = r->grad_rows.as<float>();
-	g.row_tri = r->grad_row_tri.as<uint32_t>();
-	g.rows_cap = r->grad_rows_cap;
-	g.start = r->grad_start.as<BwdStart>();
-	g.start_cap = r->grad_start_cap;
-	g.seq = r->grad_seq.as<uint8_t>();
-	g.seq_rows_cap = r->grad_seq_rows_cap;
+	pe = backward_buffers(r, grad_image, g);
+	if(pe != LUCID_OK)
+		return pe;
 
 	if(!r->aux) {
 		LUCID_CHECK(hipStreamCreateWithFlags(&r->aux, hipStreamNonBlocking));
@@ -664,7 +737,11 @@ int lucid_backward_ex(lucid_renderer *r, const lucid_frame_inputs *in, const flo
 	const bool regroup = !vtx_same_sizes;
 	r->vtx_valid = false; // vtx_keys is rewritten below
 	const bool reuse = (flags & LUCID_BACKWARD_REUSE_FRAME) && retained_matches(r, in);
-	if(reuse) {
+	// a frame rendered with LUCID_RENDER_RECORD_BACKWARD already holds pass A's results
+	const bool recorded = reuse && r->retained.recorded;
+	if(recorded) {
+		LUCID_CHECK(launch_backward_recorded(c, b, g, stream));
+	} else if(reuse) {
 		LUCID_CHECK(launch_backward_reuse(c, b, g, stream));
 	} else {
 		r->retained.valid = r->retained.pending = false; // the buffers are rebuilt below
@@ -674,7 +751,10 @@ int lucid_backward_ex(lucid_renderer *r, const lucid_frame_inputs *in, const flo
 	}
 	LUCID_CHECK(reduce_check_corners(c, b, (uint32_t)ntri, rb.vkeys,
 									 r->ctrl.as<uint32_t>() + kCtrlTopoChanged, stream));
-	LUCID_CHECK(launch_backward_order(c, b, g, stream));
+	if(recorded)
+		LUCID_CHECK(launch_backward_recorded_order(c, b, g, stream));
+	else
+		LUCID_CHECK(launch_backward_order(c, b, g, stream));
 	LUCID_CHECK(hipEventRecord(r->aux_fork, stream));
 	LUCID_CHECK(hipStreamWaitEvent(r->aux, r->aux_fork, 0));
 	LUCID_CHECK(reduce_group_rows(c, b, g, rb, (uint32_t)ntri, r->aux));
@@ -692,22 +772,9 @@ int lucid_backward_ex(lucid_renderer *r, const lucid_frame_inputs *in, const flo
 	if(r->status[kCtrlOverflow]) {
 		// the recorded order and start states take block_rows and ranges (sizes of this
 		// frame, known once its row scan ran); the rows add room for the exact pixels
-		if(block_rows > r->grad_seq_rows_cap) {
-			r->grad_seq_rows_cap = block_rows + block_rows / 8;
-			LUCID_CHECK(r->grad_seq.reserve(r->grad_seq_rows_cap * 64));
-		}
-		if(ranges > r->grad_start_cap) {
-			r->grad_start_cap = ranges + ranges / 8;
-			LUCID_CHECK(r->grad_start.reserve(r->grad_start_cap * 64 * sizeof(BwdStart)));
-		}
-		const uint64_t extra = r->status[kCtrlBwdExtra];
-		const uint64_t want = block_rows + std::max<uint64_t>(std::max<uint64_t>(1u << 16, block_rows / 8),
-															  extra + extra / 8);
-		if(want > r->grad_rows_cap) {
-			LUCID_CHECK(r->grad_rows.reserve(want * kGradWords * 4));
-			LUCID_CHECK(r->grad_row_tri.reserve(want * 4));
-			r->grad_rows_cap = want;
-		}
+		pe = grow_backward(r, block_rows, ranges, r->status[kCtrlBwdExtra]);
+		if(pe != LUCID_OK)
+			return pe;
 		r->status[kCtrlOverflow] &= ~kOvfGradRows; // grown above
 		// a reused frame stays valid when only the backward's own buffers overflowed
 		const uint32_t own = kOvfFallback | kOvfSpill;
@@ -1006,11 +1073,20 @@ int lucid_finish(lucid_renderer *r) {
 		return LUCID_OK;
 	LUCID_CHECK(hipEventSynchronize(r->done));
 	r->frame_pending = false;
+	const bool recorded = r->status[kCtrlRecorded] != 0;
+	const uint64_t block_rows = r->status[kCtrlBwdRows], ranges = r->status[kCtrlBwdRanges];
 	int g = grow_after_overflow(r);
 	memset(r->status, 0, kCtrlWords * 4);
 	if(r->retained.pending) {
 		r->retained.pending = false;
 		r->retained.valid = g == LUCID_OK;
+		if(g == LUCID_OK && r->retained.record && !recorded) {
+			// rendered without the record: size the backward buffers for this frame
+			int e = grow_backward(r, block_rows, ranges, 0);
+			if(e != LUCID_OK)
+				return e;
+		}
+		r->retained.recorded = r->retained.valid && r->retained.record && recorded;
 	}
 	return g;
 }

@@ -1327,6 +1327,8 @@ __global__ __launch_bounds__(64) void block_raster_kernel(FrameConst c, FrameBuf
 		write_pixel(c, b, px, py, bg, 0u);
 		return;
 	}
+	if(b.ctrl[kCtrlRecorded])
+		return; // the recording pass A renders this block
 	const uint4 *recs = (level == LUCID_BIN_LEVEL_LOW ? b.low_records : b.high_records) + sl.z;
 	const uint32_t *list = b.blk_lists + sl.x;
 	uint32_t pos[4];
@@ -1390,8 +1392,8 @@ __global__ __launch_bounds__(kBigWaves * 64) void block_raster_big_kernel(FrameC
 	const int bin = (int)b.info[LUCID_INFO_U32_SIZE + LUCID_COUNTS_HIGH_BINS * c.bin_count + hslot];
 	const uint32_t slot = (uint32_t)bin * 16 + blk;
 	const uint32_t flags = b.blk_flags[slot];
-	if(!(flags & kSlotPresorted) || (flags & kSlotPromoted))
-		return;
+	if(!(flags & kSlotPresorted) || (flags & kSlotPromoted) || b.ctrl[kCtrlRecorded])
+		return; // (a recording pass A renders the block)
 	__shared__ uint4 s_rec[kBigWaves][(64 * W) * kRecQuads];
 	__shared__ float4 s_part[kBigWaves][64];
 	__shared__ uint64_t s_first[kBigWaves][64], s_last[kBigWaves][64];
@@ -1821,11 +1823,20 @@ __device__ __forceinline__ bool bwd_fits(const FrameBufs &b, const BwdBufs &g) {
 	return rows <= g.seq_rows_cap && rows <= g.rows_cap && ranges <= g.start_cap;
 }
 
+// RECORD: run by a forward frame (when the record gate admitted it) instead of the block
+// rasterisers of the blocks it covers. It writes their pixels as the forward does (image,
+// counts, statistics, exact-fallback queue) and, instead of the start states (which need
+// dL/dpixel), the BwdRecord of each (range, pixel) and the pixel's clamp/exact flags;
+// bwd_start_kernel turns them into start states when the backward runs.
+template <bool RECORD>
 __global__ __launch_bounds__(kBigWaves * 64) void bwd_pass_a_kernel(FrameConst c, FrameBufs b,
 																	 BwdBufs g) {
 	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
 		return;
-	if(!bwd_fits(b, g)) {
+	if constexpr(RECORD) {
+		if(!b.ctrl[kCtrlRecorded])
+			return;
+	} else if(!bwd_fits(b, g)) {
 		if(blockIdx.x == 0 && threadIdx.x == 0)
 			atomicOr(&b.ctrl[kCtrlOverflow], kOvfGradRows);
 		return;
@@ -1837,6 +1848,7 @@ __global__ __launch_bounds__(kBigWaves * 64) void bwd_pass_a_kernel(FrameConst c
 	__shared__ float4 s_p32[kBigWaves][64];   // fp32 partial composite
 	__shared__ uint64_t s_first[kBigWaves][64], s_last[kBigWaves][64];
 	__shared__ uint32_t s_bad[kBigWaves][64];
+	__shared__ uint32_t s_count[kBigWaves][64]; // RECORD: fragments per range and pixel
 	const int lane = lane_id(), wave = threadIdx.x >> 6;
 	const uint32_t slot = blockIdx.x;
 	const uint32_t flags = b.blk_flags[slot];
@@ -1862,6 +1874,7 @@ __global__ __launch_bounds__(kBigWaves * 64) void bwd_pass_a_kernel(FrameConst c
 		w.init();
 		BwdTotals part;
 		part.init();
+		uint32_t count = 0;
 		for(uint32_t s0 = ent.e0; s0 < ent.e1; s0 += kSliceEntries) {
 			const uint32_t n = min(ent.e1 - s0, (uint32_t)kSliceEntries);
 			uint64_t msk[kSliceWords];
@@ -1871,6 +1884,8 @@ __global__ __launch_bounds__(kBigWaves * 64) void bwd_pass_a_kernel(FrameConst c
 				g.row_tri[(size_t)row0 + s0 + e] = ent.tri[s0 + e];
 			uint64_t bm[kSliceWords];
 			transpose_masks(msk, (int)((n + 63) / 64), bm);
+			if constexpr(RECORD)
+				count += bitmap_count(bm);
 			uint32_t j = 0;
 			auto emit = [&](const BwdColor &s) {
 				part.add(s.r, s.g, s.b, s.a);
@@ -1899,6 +1914,7 @@ __global__ __launch_bounds__(kBigWaves * 64) void bwd_pass_a_kernel(FrameConst c
 		s_p32[wave][lane] = make_float4(part.acc.cr, part.acc.cg, part.acc.cb, part.acc.tr);
 		s_first[wave][lane] = w.first, s_last[wave][lane] = w.last;
 		s_bad[wave][lane] = w.bad ? 1u : 0u;
+		s_count[wave][lane] = count;
 	}
 	if(presorted)
 		__syncthreads();
@@ -1933,10 +1949,32 @@ __global__ __launch_bounds__(kBigWaves * 64) void bwd_pass_a_kernel(FrameConst c
 		a32.cb = a32.cb + a32.tr * p.z;
 		a32.tr = a32.tr * p.w;
 	}
-	if(wave == 0)
-		enqueue_pixels(b, slot, on_screen && bad, px, py, false);
 	const float o32[3] = {a32.cr + a32.tr * bg[0], a32.cg + a32.tr * bg[1],
 						  a32.cb + a32.tr * bg[2]};
+	if constexpr(RECORD) {
+		if(wave == 0) {
+			uint32_t count = 0;
+			for(uint32_t v = 0; v < ent.nw; v++)
+				count += s_count[v][lane];
+			finish_pixels(c, b, a32, bad, count, px, py, slot, sl.y);
+			if(on_screen) {
+				uint32_t pf = bad ? 0u : 8u;
+#pragma unroll
+				for(int k = 0; k < 3; k++)
+					pf |= (o32[k] >= 0.0f && o32[k] <= 1.0f) ? 1u << k : 0u;
+				g.pix_flags[(size_t)py * c.width + px] = (uint8_t)pf;
+			}
+		}
+		BwdRecord rc;
+		rc.t = tb4;
+#pragma unroll
+		for(int k = 0; k < 3; k++)
+			rc.d[k] = (ct[k] + tt * (double)bg[k]) - cb4[k];
+		g.rec[((size_t)g.range_base[slot] + wave) * 64 + lane] = rc;
+		return;
+	}
+	if(wave == 0)
+		enqueue_pixels(b, slot, on_screen && bad, px, py, false);
 	float gsel[3] = {0.0f, 0.0f, 0.0f};
 	if(on_screen && !bad) {
 		const float4 gi = g.grad_image[(size_t)py * c.width + px];
@@ -1954,6 +1992,48 @@ __global__ __launch_bounds__(kBigWaves * 64) void bwd_pass_a_kernel(FrameConst c
 	st.gr = gsel[0], st.gg = gsel[1], st.gb = gsel[2];
 	st.pad = 0.0f;
 	g.start[((size_t)g.range_base[slot] + wave) * 64 + lane] = st;
+}
+
+// Start states of a recorded frame (what pass A computes at its end once dL/dpixel is
+// known): one wave per (block, depth range), as pass B.
+__global__ __launch_bounds__(kBigWaves * 64) void bwd_start_kernel(FrameConst c, FrameBufs b,
+																	BwdBufs g) {
+	if((b.ctrl[kCtrlOverflow] & kOvfBinLists) || !bwd_fits(b, g))
+		return;
+	const int lane = lane_id(), wave = threadIdx.x >> 6;
+	const uint32_t slot = blockIdx.x;
+	const uint32_t flags = b.blk_flags[slot];
+	const uint32_t level = flags & kSlotLevelMask;
+	if(level == LUCID_BIN_LEVEL_EMPTY || (flags & (kSlotWhole | kSlotPromoted)))
+		return;
+	if(b.blk_slot[slot].y == 0)
+		return;
+	const uint32_t nw = (flags & kSlotPresorted) ? b.sorted_pool[b.blk_sorted[slot]] : 1u;
+	if((uint32_t)wave >= nw)
+		return;
+	const int bin = slot >> 4, blk = slot & 15, by = blk >> 2, bx = blk & 3;
+	const int px = (bin % c.bin_count_x) * kBinSize + bx * 8 + (lane & 7);
+	const int py = (bin / c.bin_count_x) * kBinSize + by * 8 + (lane >> 3);
+	const size_t r = ((size_t)g.range_base[slot] + wave) * 64 + lane;
+	const BwdRecord rc = g.rec[r];
+	const uint32_t pf = (px < c.width && py < c.height) ? g.pix_flags[(size_t)py * c.width + px] : 0u;
+	float gsel[3] = {0.0f, 0.0f, 0.0f};
+	if(pf & 8u) {
+		const float4 gi = g.grad_image[(size_t)py * c.width + px];
+		const float gv[3] = {gi.x, gi.y, gi.z};
+#pragma unroll
+		for(int k = 0; k < 3; k++)
+			gsel[k] = ((pf >> k) & 1u) ? gv[k] : 0.0f;
+	}
+	BwdStart st;
+	st.t = rc.t;
+	st.q = 0.0;
+#pragma unroll
+	for(int k = 0; k < 3; k++)
+		st.q += (double)gsel[k] * rc.d[k];
+	st.gr = gsel[0], st.gg = gsel[1], st.gb = gsel[2];
+	st.pad = 0.0f;
+	g.start[r] = st;
 }
 
 constexpr int kGradStride = kGradWords + 1; // odd stride against bank conflicts
@@ -2323,6 +2403,80 @@ hipError_t launch_backward_prepare(const FrameConst &c, const FrameBufs &b, cons
 	return hipGetLastError();
 }
 
+// A frame recording the backward's pass A does so only if its gradient rows, recorded
+// order and start states fit the buffers (sized by an earlier frame); otherwise the block
+// rasterisers render it and the host grows the buffers from the row scan's counts.
+__global__ __launch_bounds__(64) void record_gate_kernel(FrameBufs b, BwdBufs g) {
+	if(threadIdx.x == 0)
+		b.ctrl[kCtrlRecorded] = (!(b.ctrl[kCtrlOverflow] & kOvfBinLists) && bwd_fits(b, g)) ? 1u : 0u;
+}
+
+// Forward frame that also records pass A (fp32 samples): the raster stage of launch_raster
+// with the gradient row scan, the gate and the recording pass before the block kernels
+// (which then render only what the recording pass does not).
+hipError_t launch_raster_recording(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
+								   hipStream_t stream, hipEvent_t *ev) {
+	hipLaunchKernelGGL(rowgen_low_kernel, dim3(c.bin_count), dim3(kRowgenThreads), 0, stream, c, b);
+	if(ev)
+		(void)hipEventRecord(ev[0], stream);
+	hipLaunchKernelGGL(high_prep_kernel, dim3(1), dim3(1024), 0, stream, c, b);
+	hipLaunchKernelGGL(rowgen_high_kernel, dim3(c.bin_count * kRowgenSplit), dim3(kRowgenThreads), 0,
+					   stream, c, b);
+	hipLaunchKernelGGL(high_finalize_kernel, dim3(c.bin_count * 16), dim3(kSortThreads), 0, stream,
+					   c, b);
+	hipLaunchKernelGGL(grad_row_scan_kernel, dim3(1), dim3(1024), 0, stream, c, b, g);
+	hipLaunchKernelGGL(record_gate_kernel, dim3(1), dim3(64), 0, stream, b, g);
+	if(ev)
+		(void)hipEventRecord(ev[1], stream);
+	hipLaunchKernelGGL(bwd_pass_a_kernel<true>, dim3(c.bin_count * 16), dim3(kBigWaves * 64), 0,
+					   stream, c, b, g);
+	if(c.slice_words == 2) {
+		hipLaunchKernelGGL((block_raster_kernel<true, 2>), dim3(c.bin_count * 16), dim3(64), 0, stream,
+						   c, b);
+		hipLaunchKernelGGL((block_raster_big_kernel<true, 2>), dim3(c.bin_count * 16),
+						   dim3(kBigWaves * 64), 0, stream, c, b);
+	} else {
+		hipLaunchKernelGGL((block_raster_kernel<true, 1>), dim3(c.bin_count * 16), dim3(64), 0, stream,
+						   c, b);
+		hipLaunchKernelGGL((block_raster_big_kernel<true, 1>), dim3(c.bin_count * 16),
+						   dim3(kBigWaves * 64), 0, stream, c, b);
+	}
+	if(ev)
+		(void)hipEventRecord(ev[2], stream);
+	hipLaunchKernelGGL(fallback_kernel<true>, dim3(1024), dim3(kFallbackThreads), 0, stream, c, b);
+	if(ev)
+		(void)hipEventRecord(ev[3], stream);
+	return hipGetLastError();
+}
+
+// Backward of a frame that recorded pass A: fresh backward control words (the frame's row
+// scan results and exact-fallback queue stay), the gradient scale, the start states and the
+// exact per-pixel adjoints.
+__global__ __launch_bounds__(64) void recorded_begin_kernel(uint32_t *ctrl) {
+	if(threadIdx.x == 0) {
+		ctrl[kCtrlOverflow] = 0;
+		ctrl[kCtrlSpillCursor] = 0;
+		ctrl[kCtrlBwdExtra] = 0;
+		ctrl[kCtrlBwdGradMax] = 0;
+		ctrl[kCtrlTopoChanged] = 0;
+	}
+}
+
+hipError_t launch_backward_recorded(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
+									hipStream_t stream) {
+	hipLaunchKernelGGL(recorded_begin_kernel, dim3(1), dim3(64), 0, stream, b.ctrl);
+	hipLaunchKernelGGL(grad_max_kernel, dim3(256), dim3(256), 0, stream, c, g, b.ctrl);
+	return hipGetLastError();
+}
+
+hipError_t launch_backward_recorded_order(const FrameConst &c, const FrameBufs &b,
+										  const BwdBufs &g, hipStream_t stream) {
+	hipLaunchKernelGGL(bwd_start_kernel, dim3(c.bin_count * 16), dim3(kBigWaves * 64), 0, stream, c,
+					   b, g);
+	hipLaunchKernelGGL(bwd_fallback_kernel, dim3(1024), dim3(kFallbackThreads), 0, stream, c, b, g);
+	return hipGetLastError();
+}
+
 // Pixels of the blocks whose list is too long to pre-sort go to the exact per-pixel path
 // (as high_finalize queues them); used when a finished frame's lists are reused.
 __global__ __launch_bounds__(64) void requeue_whole_kernel(FrameConst c, FrameBufs b) {
@@ -2352,8 +2506,8 @@ hipError_t launch_backward_reuse(const FrameConst &c, const FrameBufs &b, const 
 // rows follow the block rows). After these every row's triangle is known.
 hipError_t launch_backward_order(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
 								 hipStream_t stream) {
-	hipLaunchKernelGGL(bwd_pass_a_kernel, dim3(c.bin_count * 16), dim3(kBigWaves * 64), 0, stream,
-					   c, b, g);
+	hipLaunchKernelGGL(bwd_pass_a_kernel<false>, dim3(c.bin_count * 16), dim3(kBigWaves * 64), 0,
+					   stream, c, b, g);
 	hipLaunchKernelGGL(bwd_fallback_kernel, dim3(1024), dim3(kFallbackThreads), 0, stream, c, b, g);
 	return hipGetLastError();
 }

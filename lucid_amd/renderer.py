@@ -148,8 +148,10 @@ class LucidRenderer:
         }
 
     def render_async(self, gscene: GpuScene, outputs: dict, stream=None, colors_rgb=None,
-                     tri_alpha=None) -> None:
-        """Enqueues one frame on `stream` (torch stream or None = current stream)."""
+                     tri_alpha=None, record_backward: bool = False) -> None:
+        """Enqueues one frame on `stream` (torch stream or None = current stream). With
+        `record_backward` (fp32-sample renderers) the frame also records the backward's
+        first pass for a following backward(..., reuse_frame=True)."""
         torch = self._torch
         st = stream if stream is not None else torch.cuda.current_stream(self.device)
         o = _lib.FrameOutputs(
@@ -157,19 +159,26 @@ class LucidRenderer:
             outputs["image_f"].data_ptr() if outputs.get("image_f") is not None else None,
             outputs["frag_counts"].data_ptr() if outputs.get("frag_counts") is not None else None)
         inp = gscene.inputs(colors_rgb, tri_alpha)
-        _lib.check(self.lib.lucid_render(self.handle, ctypes.byref(inp), ctypes.byref(o),
-                                         ctypes.c_void_p(st.cuda_stream)), "lucid_render")
+        flags = _lib.RENDER_RECORD_BACKWARD if record_backward else 0
+        _lib.check(self.lib.lucid_render_ex(self.handle, ctypes.byref(inp), ctypes.byref(o), flags,
+                                            ctypes.c_void_p(st.cuda_stream)), "lucid_render_ex")
 
     def finish(self) -> int:
         return self.lib.lucid_finish(self.handle)
 
+    def frame_recorded(self) -> bool:
+        """True if the last finished frame holds a backward record (render(...,
+        record_backward=True) whose record fitted the backward buffers)."""
+        return bool(self.lib.lucid_frame_recorded(self.handle))
+
     def render(self, gscene: GpuScene, outputs: Optional[dict] = None, stream=None,
-               max_retries: int = 4, colors_rgb=None, tri_alpha=None) -> dict:
+               max_retries: int = 4, colors_rgb=None, tri_alpha=None,
+               record_backward: bool = False) -> dict:
         """Renders one frame and waits for it; re-renders after an internal buffer grew."""
         if outputs is None:
             outputs = self.alloc_outputs()
         for _ in range(max_retries + 1):
-            self.render_async(gscene, outputs, stream, colors_rgb, tri_alpha)
+            self.render_async(gscene, outputs, stream, colors_rgb, tri_alpha, record_backward)
             rc = self.finish()
             if rc == _lib.LUCID_OK:
                 return outputs
@@ -183,7 +192,8 @@ class LucidRenderer:
         -> (dL/d(colors_rgb) [V, 3], dL/d(tri_alpha) [2 Q]). The per-pixel sorted fragment
         lists are recomputed. With `reuse_frame`, the setup, bins and block lists of the
         renderer's last finished frame are reused when it was rendered from the same inputs
-        (whose contents must be unchanged); the result is identical either way."""
+        (whose contents must be unchanged), and so is the first pass if that frame recorded
+        it (render(..., record_backward=True)); the result is identical either way."""
         torch = self._torch
         dev = torch.device("cuda", self.device)
         st = stream if stream is not None else torch.cuda.current_stream(self.device)

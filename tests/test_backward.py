@@ -193,3 +193,68 @@ def test_backward_reusing_the_frame_matches_full_recompute(oracle_mod):
         torch.cuda.synchronize()
         for got in (reused, again, fallback):
             assert torch.equal(got[0], full[0]) and torch.equal(got[1], full[1])
+
+
+@pytest.mark.gpu
+def test_backward_from_a_recording_forward_matches_full_recompute():
+    """A frame rendered with LUCID_RENDER_RECORD_BACKWARD (what the autograd binding renders)
+    has the plain frame's fragment counts and colours (within fp32 rounding: pixels the
+    recording pass leaves to the exact sort), and the backward that starts from its record
+    gives the full recompute's gradients bit for bit, also when run twice on one record.
+    Covers ordering violations (tilted planes) and a block too long to pre-sort."""
+    import torch
+    from lucid_amd import _lib
+    from lucid_amd.renderer import GpuScene, LucidRenderer
+    for sc in (scenes.config0(num_tris=1500, width=96, height=96, seed=35),
+               scenes.tilted_planes(40, 32, 24, seed=12),
+               scenes.block_stack(32, 32, 8300, center_px=(12, 12), size_px=4.0, seed=14)):
+        print(f"\n[case] recording forward {sc.name} {sc.width}x{sc.height}", flush=True)
+        r = LucidRenderer((sc.width, sc.height), opts=_lib.OPT_FP32_SAMPLES)
+        gs = GpuScene(sc)
+        plain = r.alloc_outputs(rgba8=False, rgba32f=True, counts=True)
+        rec = r.alloc_outputs(rgba8=False, rgba32f=True, counts=True)
+        gen = torch.Generator(device="cuda").manual_seed(8)
+        grad = torch.randn((sc.height, sc.width, 4), device="cuda", generator=gen) * 1e-2
+        r.render(gs, plain)
+        full = r.backward(gs, grad)
+        r.render(gs, rec, record_backward=True)
+        assert r.frame_recorded(), "the full backward sized the buffers: the record must fit"
+        assert torch.equal(rec["frag_counts"], plain["frag_counts"])
+        assert (rec["image_f"] - plain["image_f"]).abs().max().item() <= 1e-6
+        first = r.backward(gs, grad, reuse_frame=True)
+        again = r.backward(gs, grad, reuse_frame=True)
+        torch.cuda.synchronize()
+        for got in (first, again):
+            assert torch.equal(got[0], full[0]) and torch.equal(got[1], full[1])
+
+
+@pytest.mark.gpu
+def test_recording_forward_beyond_the_backward_buffers_renders_without_record():
+    """On a fresh renderer the backward buffers have their initial sizes (65536 recorded
+    rows); a 60k-triangle frame needs more, so it renders without the record (same image),
+    the buffers grow, the backward walks the frame itself (same gradients), and the next
+    recording frame records."""
+    import torch
+    from lucid_amd import _lib
+    from lucid_amd.renderer import GpuScene, LucidRenderer
+    sc = scenes.config0(num_tris=60000, width=256, height=256, seed=37)
+    print(f"\n[case] recording forward beyond the buffers {sc.name}", flush=True)
+    gs = GpuScene(sc)
+    gen = torch.Generator(device="cuda").manual_seed(9)
+    grad = torch.randn((sc.height, sc.width, 4), device="cuda", generator=gen) * 1e-2
+    ref = LucidRenderer((sc.width, sc.height), opts=_lib.OPT_FP32_SAMPLES)
+    plain = ref.render(gs, ref.alloc_outputs(rgba8=False, rgba32f=True, counts=True))
+    full = ref.backward(gs, grad)
+    r = LucidRenderer((sc.width, sc.height), opts=_lib.OPT_FP32_SAMPLES)
+    out = r.render(gs, r.alloc_outputs(rgba8=False, rgba32f=True, counts=True),
+                   record_backward=True)
+    assert not r.frame_recorded()
+    assert torch.equal(out["frag_counts"], plain["frag_counts"])
+    assert torch.equal(out["image_f"], plain["image_f"])
+    got = r.backward(gs, grad, reuse_frame=True)
+    assert torch.equal(got[0], full[0]) and torch.equal(got[1], full[1])
+    r.render(gs, out, record_backward=True)
+    assert r.frame_recorded()
+    got = r.backward(gs, grad, reuse_frame=True)
+    torch.cuda.synchronize()
+    assert torch.equal(got[0], full[0]) and torch.equal(got[1], full[1])
