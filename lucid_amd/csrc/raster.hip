@@ -1224,8 +1224,9 @@ __device__ __forceinline__ void stage_slice(const FrameConst &c, const FrameBufs
 	}
 }
 
-// Visits this lane's set bitmap bits in increasing entry order, one per step; the wave
-// steps until every lane is done. visit(e) runs for the lanes that still have entries.
+// Visits this lane's set bitmap bits in increasing entry order, one per step. The loop is
+// divergent: a lane leaves when its bits run out (the wave steps until every lane is done),
+// so the lane's state needs no copies for lanes that sit out a step.
 template <int W, class Visit>
 __device__ __forceinline__ void walk_bits(uint64_t (&bm)[W], Visit &&visit) {
 	while(true) {
@@ -1235,28 +1236,23 @@ __device__ __forceinline__ void walk_bits(uint64_t (&bm)[W], Visit &&visit) {
 			nz[k] = bm[k] != 0ull;
 			any = any || nz[k];
 		}
-		if(!ballot(any))
+		if(!any)
 			break;
-		if(any) {
-			uint64_t cur = bm[W - 1];
-			int word = W - 1;
+		uint64_t cur = bm[W - 1];
+		int word = W - 1;
 #pragma unroll
-			for(int k = W - 2; k >= 0; k--) {
-				cur = nz[k] ? bm[k] : cur;
-				word = nz[k] ? k : word;
-			}
-			const uint64_t rest = cur & (cur - 1ull);
-#pragma unroll
-			for(int k = 0; k < W; k++)
-				bm[k] = word == k ? rest : bm[k];
-			visit(word * 64 + (int)__builtin_ctzll(cur));
+		for(int k = W - 2; k >= 0; k--) {
+			cur = nz[k] ? bm[k] : cur;
+			word = nz[k] ? k : word;
 		}
+		const uint64_t rest = cur & (cur - 1ull);
+#pragma unroll
+		for(int k = 0; k < W; k++)
+			bm[k] = word == k ? rest : bm[k];
+		visit(word * 64 + (int)__builtin_ctzll(cur));
 	}
 }
 
-// Walks this lane's pixel through the slice in entry order: each step takes the nearest
-// remaining entry that covers the pixel, shades the sample from the entry's LDS record and
-// feeds the ordering window.
 template <bool FP32, int W>
 __device__ __forceinline__ void walk_pixel(const FrameConst &c, const uint4 *s_rec,
 										   uint64_t (&bm)[W], float px, float py,
