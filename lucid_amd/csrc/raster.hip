@@ -2079,29 +2079,23 @@ __global__ __launch_bounds__(64) void bwd_pass_b_kernel(FrameConst c, FrameBufs 
 			s_grad[i] = 0ull;
 		wave_sync_lds();
 		// replay the order recorded by pass A (byte j of this lane at seq[64 j], read one
-		// step ahead; a lane stops at its end marker and reads nothing beyond it)
+		// step ahead; a lane stops at its end marker and reads nothing beyond it). The loop
+		// is divergent: each lane leaves at its own end, so its state (t, q) needs no copies.
 		const uint8_t *seq = g.seq + seq_index(row0, s0, 0, lane);
 		uint8_t next = seq[0];
-		bool more = true;
-		for(uint32_t j = 0; j < n; j++) {
-			const uint8_t e = more ? next : kSeqEnd;
-			more = e != kSeqEnd;
-			if(!ballot(more))
-				break;
-			if(more) {
-				if(j + 1 < n)
-					next = seq[(size_t)(j + 1) * 64];
-				const uint4 *rec = s_rec + (uint32_t)e * kRecQuads;
-				const float inv = record_inv_depth(rec[0], fx, fy);
-				BwdSample smp;
-				shade_grad(c, rec, fx, fy, inv, smp);
-				float out[kGradWords];
-				bwd_fragment(t, q, st.gr, st.gg, st.gb, smp, out);
-				if(live) {
+		for(uint32_t j = 0; j < n && next != kSeqEnd; j++) {
+			const uint8_t e = next;
+			next = j + 1 < n ? seq[(size_t)(j + 1) * 64] : kSeqEnd;
+			const uint4 *rec = s_rec + (uint32_t)e * kRecQuads;
+			const float inv = record_inv_depth(rec[0], fx, fy);
+			BwdSample smp;
+			shade_grad(c, rec, fx, fy, inv, smp);
+			float out[kGradWords];
+			bwd_fragment(t, q, st.gr, st.gg, st.gb, smp, out);
+			if(live) {
 #pragma unroll
-					for(int k = 0; k < kGradWords; k++)
-						atomicAdd(&s_grad[(uint32_t)e * kGradStride + k], to_fixed(out[k], fix_scale));
-				}
+				for(int k = 0; k < kGradWords; k++)
+					atomicAdd(&s_grad[(uint32_t)e * kGradStride + k], to_fixed(out[k], fix_scale));
 			}
 		}
 		wave_sync_lds();
