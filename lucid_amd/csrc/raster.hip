@@ -1169,40 +1169,47 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 
 // ------------------------------------------------------------------ block_raster
 
-// v_writelane_b32 x2: lane L of (lo, hi) takes the wave-uniform pair (x_lo, x_hi). The pair
-// usually comes straight from a VALU compare; a VALU read of an SGPR written by a VALU needs
-// two wait states, which the s_nop provides (the compiler does not look inside the asm).
-template <int L>
-__device__ __forceinline__ void write_lane2(uint32_t &lo, uint32_t &hi, uint32_t x_lo,
-											uint32_t x_hi) {
-	asm volatile("s_nop 1\n\tv_writelane_b32 %0, %2, %4\n\tv_writelane_b32 %1, %3, %4"
-				 : "+v"(lo), "+v"(hi)
-				 : "s"(x_lo), "s"(x_hi), "n"(L));
-}
-
-// Pixels P.. of one transpose word: ballot of mask bit P over the entry lanes lands in lane P.
-template <int P>
-__device__ __forceinline__ void transpose_bits(uint64_t msk, uint32_t &lo, uint32_t &hi) {
-	if constexpr(P < 64) {
-		const uint64_t w = ballot((msk >> P) & 1ull);
-		write_lane2<P>(lo, hi, (uint32_t)w, (uint32_t)(w >> 32));
-		transpose_bits<P + 1>(msk, lo, hi);
+// 64 x 64 bit-matrix transpose across the wave: lane i holds row i (bit j = column j) and
+// receives row i of the transpose. Six butterfly stages; at stage s lanes i and i ^ s swap
+// the off-diagonal s x s blocks: the lower lane keeps its columns with (j & s) == 0 and
+// takes the partner's as columns j + s, the upper lane keeps its columns with (j & s) != 0
+// and takes the partner's, shifted down by s (Hacker's Delight, 7-3, one row per lane).
+__device__ __forceinline__ uint64_t transpose64(uint64_t row) {
+	const int lane = lane_id();
+	{ // s = 32: whole 32-bit words
+		uint32_t lo = (uint32_t)row, hi = (uint32_t)(row >> 32);
+		const bool upper = (lane & 32) != 0;
+		const uint32_t got = __shfl_xor(upper ? lo : hi, 32, 64);
+		lo = upper ? got : lo;
+		hi = upper ? hi : got;
+		row = ((uint64_t)hi << 32) | lo;
 	}
+	constexpr uint64_t kM0[5] = {0x0000ffff0000ffffull, 0x00ff00ff00ff00ffull,
+								 0x0f0f0f0f0f0f0f0full, 0x3333333333333333ull,
+								 0x5555555555555555ull};
+	// Selects only (the lane condition is a wave-constant lane mask; the column masks and
+	// shifts are constants), so nothing lane-dependent stays live between calls.
+#pragma unroll
+	for(int st = 0; st < 5; st++) {
+		const int sh = 16 >> st;
+		const bool upper = (lane & sh) != 0;
+		const uint64_t lo_cols = row & kM0[st], hi_cols = row & ~kM0[st];
+		const uint64_t got = shfl_xor64(upper ? lo_cols : hi_cols, sh);
+		const uint64_t moved = upper ? got >> sh : got << sh;
+		row = (upper ? hi_cols : lo_cols) | moved;
+	}
+	return row;
 }
 
 // Transposes the coverage masks of a slice (64-bit pixel masks, one per entry) into
 // per-pixel entry bitmaps: after the call bit e of bm[k] in lane p is set iff entry
-// 64 k + e covers pixel p. One ballot per (word, pixel).
+// 64 k + e covers pixel p.
 template <int W>
 __device__ __forceinline__ void transpose_masks(const uint64_t (&msk)[W], int nwords,
 												uint64_t (&bm)[W]) {
 #pragma unroll
-	for(int k = 0; k < W; k++) {
-		uint32_t lo = 0, hi = 0;
-		if(k < nwords)
-			transpose_bits<0>(msk[k], lo, hi);
-		bm[k] = (uint64_t)lo | ((uint64_t)hi << 32);
-	}
+	for(int k = 0; k < W; k++)
+		bm[k] = k < nwords ? transpose64(msk[k]) : 0ull;
 }
 
 // Stages the shading records of a slice in LDS: lane l writes entries l + 64 k.
