@@ -226,12 +226,12 @@ __device__ __forceinline__ int f2i(float x) {
 	return (int)x;
 }
 
+// Saturating float -> uint32 (truncating): NaN and x <= 0 give 0, x >= 2^32 gives 2^32 - 1.
+// Branch-free: the clamp keeps the conversion in range (no float lies strictly between
+// 2^32 - 256 and 2^32, so the upper clamp changes no x below 2^32).
 __device__ __forceinline__ uint32_t f2u(float x) {
-	if(!(x > 0.0f))
-		return 0u;
-	if(x >= 4294967296.0f)
-		return 0xffffffffu;
-	return (uint32_t)x;
+	const uint32_t u = (uint32_t)fminf(fmaxf(x, 0.0f), 4294967040.0f);
+	return x >= 4294967296.0f ? 0xffffffffu : u;
 }
 
 __device__ __forceinline__ float bits2f(uint32_t u) { return __uint_as_float(u); }
