@@ -101,11 +101,13 @@ def batch_leg(num_scenes: int, steps: int = 3):
 
 def backward_ratio(steps: int):
     """fwd+bwd vs plain fwd on the differentiable workload: 200k soup triangles at 512x512
-    (the config0 generator, ~160 fragments/pixel), fp32 samples. A fwd+bwd step is a forward
-    frame followed by the backward pass for a dL/dimage, as the autograd binding runs them:
-    the forward records the backward's first pass (blend order and partial composites) and
-    the backward reuses the frame's bins, block lists and that record. The full-recompute
-    backward (lucid_backward after a plain forward) is timed beside it."""
+    (the config0 generator, ~160 fragments/pixel), fp32 samples. A fwd+bwd step is a plain
+    forward frame followed by the backward pass for a dL/dimage, as the autograd binding runs
+    them by default: the backward recomputes every pixel's depth-sorted fragment list (it
+    stores none from the forward) and reuses only the frame's bins and per-block triangle
+    lists. Timed beside it: the full recompute (lucid_backward rebuilds bins and lists too),
+    and the opt-in recording forward (the forward stores each pixel's blend order for the
+    backward, which then skips its first pass)."""
     import torch
     from lucid_amd import _lib, scenes
     from lucid_amd.renderer import GpuScene, LucidRenderer
@@ -117,29 +119,37 @@ def backward_ratio(steps: int):
     r.render(g, out)
     frags = int(r.read_info()[60])
     r.backward(g, grad)
+    r.render(g, out)
+    r.backward(g, grad, reuse_frame=True)
     for _ in range(2):  # the first recording frame may size the backward buffers instead
         r.render(g, out, record_backward=True)
         r.backward(g, grad, reuse_frame=True)
     torch.cuda.synchronize()
 
-    def timed(reuse=None):
+    def timed(backward=None, record=False):
         t0 = time.perf_counter()
         for _ in range(steps):
-            r.render(g, out, record_backward=reuse is True)
-            if reuse is not None:
-                r.backward(g, grad, reuse_frame=reuse)
+            r.render(g, out, record_backward=record)
+            if backward == "reuse":
+                r.backward(g, grad, reuse_frame=True)
+            elif backward == "full":
+                r.backward(g, grad)
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps
 
     t_fwd = timed()
-    t_both = timed(reuse=True)
-    t_full = timed(reuse=False)
+    t_both = timed("reuse")
+    t_full = timed("full")
+    t_rec = timed("reuse", record=True)
     return {"workload": "200k soup triangles at 512x512 (config0 generator, seed 1004), fp32 "
-                        "samples; host-timed synchronous steps",
+                        "samples; host-timed synchronous steps; per-pixel sorted fragment "
+                        "lists recomputed in the backward",
             "fragments": frags, "fwd_ms": t_fwd * 1e3, "fwd_bwd_ms": t_both * 1e3,
             "fwd_bwd_ratio": t_both / t_fwd,
             "fwd_bwd_full_recompute_ms": t_full * 1e3,
-            "fwd_bwd_full_recompute_ratio": t_full / t_fwd}
+            "fwd_bwd_full_recompute_ratio": t_full / t_fwd,
+            "fwd_bwd_recording_forward_ms": t_rec * 1e3,
+            "fwd_bwd_recording_forward_ratio": t_rec / t_fwd}
 
 
 def algorithmic_bytes(scene, r, info):

@@ -37,11 +37,13 @@ class LucidRenderFunction:
                     colors = colors_rgb.detach().contiguous().to(torch.float32)
                     alpha = tri_alpha.detach().contiguous().to(torch.float32)
                     out = renderer.alloc_outputs(rgba8=False, rgba32f=True, counts=False)
-                    # the frame records the backward's first pass only when a backward can
-                    # follow (autograd recording this op)
+                    # by default the backward recomputes the per-pixel sorted fragment lists;
+                    # with renderer.record_backward the frame stores each pixel's blend order
+                    # for it instead (when autograd records this op)
+                    record = bool(getattr(renderer, "record_backward", False)) and (
+                        ctx.needs_input_grad[0] or ctx.needs_input_grad[1])
                     renderer.render(gscene, out, colors_rgb=colors, tri_alpha=alpha,
-                                    record_backward=ctx.needs_input_grad[0]
-                                    or ctx.needs_input_grad[1])
+                                    record_backward=record)
                     ctx.renderer, ctx.gscene = renderer, gscene
                     ctx.save_for_backward(colors, alpha)
                     return out["image_f"]

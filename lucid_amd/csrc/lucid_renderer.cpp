@@ -34,6 +34,8 @@ hipError_t launch_backward_order(const FrameConst &c, const FrameBufs &b, const 
 								 hipStream_t stream);
 hipError_t launch_backward_rows(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
 								hipStream_t stream);
+hipError_t launch_grad_max(const FrameConst &c, const BwdBufs &g, uint32_t *ctrl,
+						   hipStream_t stream);
 hipError_t launch_raster_recording(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
 								   hipStream_t stream, hipEvent_t *events);
 hipError_t launch_backward_recorded(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
@@ -113,7 +115,7 @@ struct lucid_renderer {
 	bool vtx_valid = false;
 	// helper stream of the backward pass (the row grouping runs beside pass B)
 	hipStream_t aux = nullptr;
-	hipEvent_t aux_fork = nullptr, aux_join = nullptr;
+	hipEvent_t aux_fork = nullptr, aux_join = nullptr, aux_scale = nullptr, aux_order = nullptr;
 	// The setup records, bins and block lists in the buffers, and the inputs they were built
 	// from (valid once the frame that built them finished without an overflow).
 	struct Retained {
@@ -152,6 +154,10 @@ struct lucid_renderer {
 			(void)hipEventDestroy(aux_fork);
 		if(aux_join)
 			(void)hipEventDestroy(aux_join);
+		if(aux_scale)
+			(void)hipEventDestroy(aux_scale);
+		if(aux_order)
+			(void)hipEventDestroy(aux_order);
 		if(aux)
 			(void)hipStreamDestroy(aux);
 		for(hipEvent_t e : stage_ev)
@@ -708,6 +714,8 @@ int lucid_backward_ex(lucid_renderer *r, const lucid_frame_inputs *in, const flo
 		LUCID_CHECK(hipStreamCreateWithFlags(&r->aux, hipStreamNonBlocking));
 		LUCID_CHECK(hipEventCreateWithFlags(&r->aux_fork, hipEventDisableTiming));
 		LUCID_CHECK(hipEventCreateWithFlags(&r->aux_join, hipEventDisableTiming));
+		LUCID_CHECK(hipEventCreateWithFlags(&r->aux_scale, hipEventDisableTiming));
+		LUCID_CHECK(hipEventCreateWithFlags(&r->aux_order, hipEventDisableTiming));
 	}
 	LUCID_CHECK(r->red_counts.reserve((ntri + 1) * 4));
 	LUCID_CHECK(r->red_offsets.reserve((ntri + 1) * 4));
@@ -749,16 +757,23 @@ int lucid_backward_ex(lucid_renderer *r, const lucid_frame_inputs *in, const flo
 		LUCID_CHECK(launch_binning(c, b, stream));
 		LUCID_CHECK(launch_backward_prepare(c, b, g, stream));
 	}
+	// The helper stream takes the gradient scale and the corner check beside pass A, then the
+	// row grouping beside pass B (which needs the scale).
+	LUCID_CHECK(hipEventRecord(r->aux_fork, stream));
+	LUCID_CHECK(hipStreamWaitEvent(r->aux, r->aux_fork, 0));
+	LUCID_CHECK(launch_grad_max(c, g, b.ctrl, r->aux));
 	LUCID_CHECK(reduce_check_corners(c, b, (uint32_t)ntri, rb.vkeys,
-									 r->ctrl.as<uint32_t>() + kCtrlTopoChanged, stream));
+									 r->ctrl.as<uint32_t>() + kCtrlTopoChanged, r->aux));
+	LUCID_CHECK(hipEventRecord(r->aux_scale, r->aux));
 	if(recorded)
 		LUCID_CHECK(launch_backward_recorded_order(c, b, g, stream));
 	else
 		LUCID_CHECK(launch_backward_order(c, b, g, stream));
-	LUCID_CHECK(hipEventRecord(r->aux_fork, stream));
-	LUCID_CHECK(hipStreamWaitEvent(r->aux, r->aux_fork, 0));
+	LUCID_CHECK(hipEventRecord(r->aux_order, stream));
+	LUCID_CHECK(hipStreamWaitEvent(r->aux, r->aux_order, 0));
 	LUCID_CHECK(reduce_group_rows(c, b, g, rb, (uint32_t)ntri, r->aux));
 	LUCID_CHECK(hipEventRecord(r->aux_join, r->aux));
+	LUCID_CHECK(hipStreamWaitEvent(stream, r->aux_scale, 0));
 	LUCID_CHECK(launch_backward_rows(c, b, g, stream));
 	LUCID_CHECK(hipStreamWaitEvent(stream, r->aux_join, 0));
 	LUCID_CHECK(reduce_sum(c, b, g, rb, (uint32_t)ntri, (uint32_t)nverts, regroup,

@@ -2121,10 +2121,38 @@ __global__ __launch_bounds__(64) void bwd_pass_b_kernel(FrameConst c, FrameBufs 
 
 // Exclusive scans over the blocks the block kernels process: gradient rows (one per list
 // entry) and depth ranges (start states of pass B).
+// One thread queues every on-screen pixel of block `slot` for the exact per-pixel path (a
+// list too long to pre-sort), as high_finalize does for a new frame.
+__device__ void enqueue_block_whole(const FrameConst &c, const FrameBufs &b, uint32_t slot) {
+	const int bin = (int)(slot >> 4), blk = (int)(slot & 15);
+	const int x0 = (bin % c.bin_count_x) * kBinSize + (blk & 3) * 8;
+	const int y0 = (bin / c.bin_count_x) * kBinSize + (blk >> 2) * 8;
+	const int w = min(8, c.width - x0), h = min(8, c.height - y0);
+	if(w <= 0 || h <= 0)
+		return;
+	const uint32_t base = atomicAdd(&b.ctrl[kCtrlFallbackCount], (uint32_t)(w * h));
+	for(int i = 0; i < w * h; i++) {
+		if(base + (uint32_t)i >= b.fallback_cap) {
+			atomicOr(&b.ctrl[kCtrlOverflow], kOvfFallback);
+			break;
+		}
+		b.fallback_queue[base + i] =
+			make_uint4(slot, (uint32_t)(x0 + i % w), (uint32_t)(y0 + i / w), 1u);
+	}
+}
+
+// REUSE (a finished frame's lists, reused by the backward): the control words start afresh
+// and the exact-fallback queue is rebuilt: the blocks too long to pre-sort are queued here,
+// pass A adds the mis-ordered pixels.
+template <bool REUSE>
 __global__ __launch_bounds__(1024) void grad_row_scan_kernel(FrameConst c, FrameBufs b, BwdBufs g) {
 	const uint32_t n = (uint32_t)c.bin_count * 16;
 	__shared__ uint32_t s_wave[2][16];
 	__shared__ uint32_t s_carry[2];
+	if constexpr(REUSE) {
+		if(threadIdx.x < (uint32_t)kCtrlWords)
+			b.ctrl[threadIdx.x] = 0u;
+	}
 	if(threadIdx.x < 2)
 		s_carry[threadIdx.x] = 0;
 	__syncthreads();
@@ -2140,6 +2168,8 @@ __global__ __launch_bounds__(1024) void grad_row_scan_kernel(FrameConst c, Frame
 				v[0] = m;
 				v[1] = (flags & kSlotPresorted) ? b.sorted_pool[b.blk_sorted[slot]] : 1u;
 			}
+			if(REUSE && (flags & kSlotWhole))
+				enqueue_block_whole(c, b, slot);
 		}
 		uint32_t inc[2];
 		for(int k = 0; k < 2; k++) {
@@ -2171,11 +2201,6 @@ __global__ __launch_bounds__(1024) void grad_row_scan_kernel(FrameConst c, Frame
 		b.ctrl[kCtrlBwdExtra] = 0;
 		b.ctrl[kCtrlBwdGradMax] = 0;
 	}
-}
-
-__global__ __launch_bounds__(64) void clear_ctrl_kernel(uint32_t *ctrl) {
-	if(threadIdx.x < (uint32_t)kCtrlWords)
-		ctrl[threadIdx.x] = 0u;
 }
 
 // max |dL/dpixel| (rgb) over the image into kCtrlBwdGradMax (after the row scan cleared it):
@@ -2395,8 +2420,15 @@ hipError_t launch_backward_prepare(const FrameConst &c, const FrameBufs &b, cons
 					   stream, c, b);
 	hipLaunchKernelGGL(high_finalize_kernel, dim3(c.bin_count * 16), dim3(kSortThreads), 0, stream,
 					   c, b);
-	hipLaunchKernelGGL(grad_row_scan_kernel, dim3(1), dim3(1024), 0, stream, c, b, g);
-	hipLaunchKernelGGL(grad_max_kernel, dim3(256), dim3(256), 0, stream, c, g, b.ctrl);
+	hipLaunchKernelGGL(grad_row_scan_kernel<false>, dim3(1), dim3(1024), 0, stream, c, b, g);
+	return hipGetLastError();
+}
+
+// max |dL/dpixel| for the gradient scale (pass B and the triangle sums read it); it may run
+// on a helper stream once the row scan (which clears it) has run.
+hipError_t launch_grad_max(const FrameConst &c, const BwdBufs &g, uint32_t *ctrl,
+						   hipStream_t stream) {
+	hipLaunchKernelGGL(grad_max_kernel, dim3(256), dim3(256), 0, stream, c, g, ctrl);
 	return hipGetLastError();
 }
 
@@ -2421,7 +2453,7 @@ hipError_t launch_raster_recording(const FrameConst &c, const FrameBufs &b, cons
 					   stream, c, b);
 	hipLaunchKernelGGL(high_finalize_kernel, dim3(c.bin_count * 16), dim3(kSortThreads), 0, stream,
 					   c, b);
-	hipLaunchKernelGGL(grad_row_scan_kernel, dim3(1), dim3(1024), 0, stream, c, b, g);
+	hipLaunchKernelGGL(grad_row_scan_kernel<false>, dim3(1), dim3(1024), 0, stream, c, b, g);
 	hipLaunchKernelGGL(record_gate_kernel, dim3(1), dim3(64), 0, stream, b, g);
 	if(ev)
 		(void)hipEventRecord(ev[1], stream);
@@ -2447,8 +2479,8 @@ hipError_t launch_raster_recording(const FrameConst &c, const FrameBufs &b, cons
 }
 
 // Backward of a frame that recorded pass A: fresh backward control words (the frame's row
-// scan results and exact-fallback queue stay), the gradient scale, the start states and the
-// exact per-pixel adjoints.
+// scan results and exact-fallback queue stay), then the start states and the exact
+// per-pixel adjoints.
 __global__ __launch_bounds__(64) void recorded_begin_kernel(uint32_t *ctrl) {
 	if(threadIdx.x == 0) {
 		ctrl[kCtrlOverflow] = 0;
@@ -2462,7 +2494,6 @@ __global__ __launch_bounds__(64) void recorded_begin_kernel(uint32_t *ctrl) {
 hipError_t launch_backward_recorded(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
 									hipStream_t stream) {
 	hipLaunchKernelGGL(recorded_begin_kernel, dim3(1), dim3(64), 0, stream, b.ctrl);
-	hipLaunchKernelGGL(grad_max_kernel, dim3(256), dim3(256), 0, stream, c, g, b.ctrl);
 	return hipGetLastError();
 }
 
@@ -2474,27 +2505,12 @@ hipError_t launch_backward_recorded_order(const FrameConst &c, const FrameBufs &
 	return hipGetLastError();
 }
 
-// Pixels of the blocks whose list is too long to pre-sort go to the exact per-pixel path
-// (as high_finalize queues them); used when a finished frame's lists are reused.
-__global__ __launch_bounds__(64) void requeue_whole_kernel(FrameConst c, FrameBufs b) {
-	const uint32_t slot = blockIdx.x;
-	if(!(b.blk_flags[slot] & kSlotWhole))
-		return;
-	const int bin = slot >> 4, blk = slot & 15;
-	const int px = (bin % c.bin_count_x) * kBinSize + (blk & 3) * 8 + (threadIdx.x & 7);
-	const int py = (bin / c.bin_count_x) * kBinSize + (blk >> 2) * 8 + (threadIdx.x >> 3);
-	enqueue_pixels(b, slot, px < c.width && py < c.height, px, py, true);
-}
-
 // Backward, first part when the last finished frame's setup, bins and block lists are
-// reused: fresh control words, the exact-pixel queue of the too-long blocks, then the
-// gradient row allocation.
+// reused: fresh control words, the exact-pixel queue of the too-long blocks and the
+// gradient row allocation, in one kernel.
 hipError_t launch_backward_reuse(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
 								 hipStream_t stream) {
-	hipLaunchKernelGGL(clear_ctrl_kernel, dim3(1), dim3(64), 0, stream, b.ctrl);
-	hipLaunchKernelGGL(requeue_whole_kernel, dim3(c.bin_count * 16), dim3(64), 0, stream, c, b);
-	hipLaunchKernelGGL(grad_row_scan_kernel, dim3(1), dim3(1024), 0, stream, c, b, g);
-	hipLaunchKernelGGL(grad_max_kernel, dim3(256), dim3(256), 0, stream, c, g, b.ctrl);
+	hipLaunchKernelGGL(grad_row_scan_kernel<true>, dim3(1), dim3(1024), 0, stream, c, b, g);
 	return hipGetLastError();
 }
 

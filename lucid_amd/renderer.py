@@ -78,6 +78,10 @@ class LucidRenderer:
         self.width, self.height = int(view_size[0]), int(view_size[1])
         self.opts = opts
         self.device = device
+        # autograd (lucid_amd.autograd): False = the backward recomputes every pixel's sorted
+        # fragment list; True = differentiable frames store each pixel's blend order for the
+        # backward (about one byte per fragment), which then skips that walk
+        self.record_backward = False
         torch.cuda.set_device(device)
         handle = ctypes.c_void_p()
         ci = _lib.CreateInfo(self.width, self.height, opts, device, max_quads)

@@ -85,13 +85,15 @@ def test_dense_reference_gradient_matches_finite_differences(oracle_mod):
             assert abs(g - fd) <= 1e-4 * abs(fd) + 1e-9, (name, i, g, fd)
 
 
-def _gpu_grads(sc, target):
+def _gpu_grads(sc, target, record=False):
     import torch
     from lucid_amd import _lib
     from lucid_amd.renderer import GpuScene, LucidRenderer
     from lucid_amd.autograd import render_image
-    print(f"\n[case] backward {sc.name} {sc.width}x{sc.height} quads={sc.num_quads}", flush=True)
+    print(f"\n[case] backward {sc.name} {sc.width}x{sc.height} quads={sc.num_quads} "
+          f"record={record}", flush=True)
     r = LucidRenderer((sc.width, sc.height), opts=_lib.OPT_FP32_SAMPLES)
+    r.record_backward = record
     gs = GpuScene(sc)
     col = torch.tensor(sc.colors_rgb, device="cuda", requires_grad=True)
     alpha = torch.tensor(sc.tri_alpha, device="cuda", requires_grad=True)
@@ -258,3 +260,15 @@ def test_recording_forward_beyond_the_backward_buffers_renders_without_record():
     got = r.backward(gs, grad, reuse_frame=True)
     torch.cuda.synchronize()
     assert torch.equal(got[0], full[0]) and torch.equal(got[1], full[1])
+
+
+@pytest.mark.gpu
+def test_autograd_with_recording_forward_matches_default(oracle_mod):
+    """renderer.record_backward (the forward stores each pixel's blend order) gives the same
+    image and gradients, bit for bit, as the default (the backward recomputes them)."""
+    sc = scenes.config0(num_tris=3000, width=128, height=128, seed=32)
+    target = np.random.default_rng(3).uniform(0, 1, (sc.height, sc.width, 3))
+    a = _gpu_grads(sc, target)
+    b = _gpu_grads(sc, target, record=True)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
