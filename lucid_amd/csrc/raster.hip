@@ -1844,14 +1844,20 @@ __global__ __launch_bounds__(kBigWaves * 64) void bwd_pass_a_kernel(FrameConst c
 			atomicOr(&b.ctrl[kCtrlOverflow], kOvfGradRows);
 		return;
 	}
+	// Each wave's slice staging area holds, once its walk is done, what the chaining of the
+	// ranges reads (the range's composites and ordering keys per pixel): 31 KB of LDS per
+	// work-group instead of 49, so that four work-groups fit on a CU.
+	struct RangeEnd {
+		double pc[4][64]; // C rgb, T (fp64)
+		float4 p32[64];   // fp32 partial composite
+		uint64_t first[64], last[64];
+		uint32_t bad[64], count[64]; // count: RECORD, fragments per pixel
+	};
 	__shared__ uint4 s_rec[kBigWaves][kSliceEntries * kRecQuads];
+	static_assert(sizeof(RangeEnd) <= sizeof(s_rec[0]), "a range's end state fits its staging");
 	__shared__ uint32_t s_tri_all[kRegSortCap];
 	__shared__ uint64_t s_mask_all[kRegSortCap];
-	__shared__ double s_pc[kBigWaves][4][64]; // C rgb, T per range and pixel
-	__shared__ float4 s_p32[kBigWaves][64];   // fp32 partial composite
-	__shared__ uint64_t s_first[kBigWaves][64], s_last[kBigWaves][64];
-	__shared__ uint32_t s_bad[kBigWaves][64];
-	__shared__ uint32_t s_count[kBigWaves][64]; // RECORD: fragments per range and pixel
+	auto range_end = [&](int w) -> RangeEnd & { return *reinterpret_cast<RangeEnd *>(&s_rec[w][0]); };
 	const int lane = lane_id(), wave = threadIdx.x >> 6;
 	const uint32_t slot = blockIdx.x;
 	const uint32_t flags = b.blk_flags[slot];
@@ -1912,12 +1918,13 @@ __global__ __launch_bounds__(kBigWaves * 64) void bwd_pass_a_kernel(FrameConst c
 				g.seq[seq_index(row0, s0, j, lane)] = kSeqEnd;
 			wave_sync_lds();
 		}
-		s_pc[wave][0][lane] = part.cr, s_pc[wave][1][lane] = part.cg;
-		s_pc[wave][2][lane] = part.cb, s_pc[wave][3][lane] = part.t;
-		s_p32[wave][lane] = make_float4(part.acc.cr, part.acc.cg, part.acc.cb, part.acc.tr);
-		s_first[wave][lane] = w.first, s_last[wave][lane] = w.last;
-		s_bad[wave][lane] = w.bad ? 1u : 0u;
-		s_count[wave][lane] = count;
+		RangeEnd &re = range_end(wave); // every lane is done with the last slice's records
+		re.pc[0][lane] = part.cr, re.pc[1][lane] = part.cg;
+		re.pc[2][lane] = part.cb, re.pc[3][lane] = part.t;
+		re.p32[lane] = make_float4(part.acc.cr, part.acc.cg, part.acc.cb, part.acc.tr);
+		re.first[lane] = w.first, re.last[lane] = w.last;
+		re.bad[lane] = w.bad ? 1u : 0u;
+		re.count[lane] = count;
 	}
 	if(presorted)
 		__syncthreads();
@@ -1937,16 +1944,17 @@ __global__ __launch_bounds__(kBigWaves * 64) void bwd_pass_a_kernel(FrameConst c
 	for(uint32_t v = 0; v < ent.nw; v++) {
 		if(v == (uint32_t)wave)
 			cb4[0] = ct[0], cb4[1] = ct[1], cb4[2] = ct[2], tb4 = tt;
-		bad |= s_bad[v][lane] != 0;
-		const uint64_t f = s_first[v][lane];
+		const RangeEnd &re = range_end((int)v);
+		bad |= re.bad[lane] != 0;
+		const uint64_t f = re.first[lane];
 		if(f != kEmptyKey) {
 			bad |= f <= last;
-			last = s_last[v][lane];
+			last = re.last[lane];
 		}
 		for(int k = 0; k < 3; k++)
-			ct[k] = ct[k] + tt * s_pc[v][k][lane];
-		tt = tt * s_pc[v][3][lane];
-		const float4 p = s_p32[v][lane];
+			ct[k] = ct[k] + tt * re.pc[k][lane];
+		tt = tt * re.pc[3][lane];
+		const float4 p = re.p32[lane];
 		a32.cr = a32.cr + a32.tr * p.x;
 		a32.cg = a32.cg + a32.tr * p.y;
 		a32.cb = a32.cb + a32.tr * p.z;
@@ -1958,7 +1966,7 @@ __global__ __launch_bounds__(kBigWaves * 64) void bwd_pass_a_kernel(FrameConst c
 		if(wave == 0) {
 			uint32_t count = 0;
 			for(uint32_t v = 0; v < ent.nw; v++)
-				count += s_count[v][lane];
+				count += range_end((int)v).count[lane];
 			finish_pixels(c, b, a32, bad, count, px, py, slot, sl.y);
 			if(on_screen) {
 				uint32_t pf = bad ? 0u : 8u;
