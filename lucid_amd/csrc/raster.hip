@@ -2108,9 +2108,18 @@ __global__ __launch_bounds__(64) void bwd_pass_b_kernel(FrameConst c, FrameBufs 
 			float out[kGradWords];
 			bwd_fragment(t, q, st.gr, st.gg, st.gb, smp, out);
 			if(live) {
+				// lanes add the ten components in four rotated orders (pixel column mod 4), so
+				// that pixels at the same entry in the same step seldom hit one address (integer
+				// sums: the order changes no bit)
+				const int grp = lane & 3;
 #pragma unroll
-				for(int k = 0; k < kGradWords; k++)
-					atomicAdd(&s_grad[(uint32_t)e * kGradStride + k], to_fixed(out[k], fix_scale));
+				for(int k = 0; k < kGradWords; k++) {
+					const int k1 = (k + 3) % kGradWords, k2 = (k + 5) % kGradWords;
+					const int k3 = (k + 8) % kGradWords;
+					const int kk = grp == 0 ? k : grp == 1 ? k1 : grp == 2 ? k2 : k3;
+					const float v = grp == 0 ? out[k] : grp == 1 ? out[k1] : grp == 2 ? out[k2] : out[k3];
+					atomicAdd(&s_grad[(uint32_t)e * kGradStride + kk], to_fixed(v, fix_scale));
+				}
 			}
 		}
 		wave_sync_lds();
