@@ -216,14 +216,10 @@ __device__ __forceinline__ uint32_t tri_rec(const FrameBufs &b, uint32_t t) {
 // ------------------------------------------------------------------ conversions
 
 __device__ __forceinline__ int f2i(float x) {
-	// v_cvt_i32_f32 semantics: truncate, saturate, NaN -> 0
-	if(!(x == x))
-		return 0;
-	if(x >= 2147483648.0f)
-		return 2147483647;
-	if(x <= -2147483648.0f)
-		return (int)0x80000000u;
-	return (int)x;
+	// v_cvt_i32_f32 semantics: truncate, saturate, NaN -> 0. Branch-free: the clamp keeps the
+	// conversion in range (2^31 - 128 is the largest float below 2^31).
+	const int v = (int)fminf(fmaxf(x, -2147483648.0f), 2147483520.0f);
+	return x != x ? 0 : x >= 2147483648.0f ? 2147483647 : v;
 }
 
 // Saturating float -> uint32 (truncating): NaN and x <= 0 give 0, x >= 2^32 gives 2^32 - 1.
