@@ -40,7 +40,11 @@ extern "C" {
 #define LUCID_ERR_UNSUPPORTED (-6)
 
 /* LucidRenderOpt bits (src/lucid_renderer.h:10-11). Debug/timer options are accepted
- * and ignored; rocprof replaces them. */
+ * and ignored; rocprof replaces them. LUCID_OPT_ALPHA_THRESHOLD ("over" blending only):
+ * once every pixel of an 8x8 block (or of one depth range of a long block) has
+ * transmittance below 1/128, its later samples are not shaded (shading.glsl:62,282-284;
+ * fragment counts stay exact; colours differ from the exact composite by at most 1/128
+ * per channel); the backward rejects it. */
 #define LUCID_OPT_DEBUG_QUAD_SETUP (1u << 0)
 #define LUCID_OPT_DEBUG_BIN_COUNTER (1u << 1)
 #define LUCID_OPT_DEBUG_BIN_DISPATCHER (1u << 2)
@@ -125,7 +129,7 @@ int lucid_frame_recorded(const lucid_renderer *r);
  * dL/d(tri_alpha) (device, one float per triangle id, 2 per quad); either output may be NULL.
  * The frame's fragment lists are recomputed, not stored. Synchronous on `stream`. Returns
  * LUCID_ERR_CAPACITY after growing an internal buffer (call again), LUCID_ERR_UNSUPPORTED
- * for RGBA8 colours or additive blending.
+ * for RGBA8 colours, additive blending or the alpha threshold.
  * No counterpart in the reference (which has no backward pass): this is the differentiable
  * extension of the drop-in renderer. */
 int lucid_backward(lucid_renderer *r, const lucid_frame_inputs *in, const float *grad_image,

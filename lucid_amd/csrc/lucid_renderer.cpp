@@ -622,7 +622,8 @@ int lucid_render_ex(lucid_renderer *r, const lucid_frame_inputs *in,
 	const bool record = (flags & LUCID_RENDER_RECORD_BACKWARD) != 0;
 	// the recorded pass is the backward's: fp32 samples of fp32 colours, "over" blending
 	if(record && (!(r->opts & LUCID_OPT_FP32_SAMPLES) || !in->colors_rgb || !in->tri_alpha ||
-				  in->colors_rgba8 || (r->opts & LUCID_OPT_ADDITIVE_BLENDING)))
+				  in->colors_rgba8 ||
+				  (r->opts & (LUCID_OPT_ADDITIVE_BLENDING | LUCID_OPT_ALPHA_THRESHOLD))))
 		return LUCID_ERR_UNSUPPORTED;
 	hipStream_t stream = (hipStream_t)stream_ptr;
 	FrameConst c;
@@ -679,8 +680,9 @@ int lucid_backward_ex(lucid_renderer *r, const lucid_frame_inputs *in, const flo
 	if(!r || !in || !grad_image || (!grad_colors_rgb && !grad_tri_alpha) ||
 	   (flags & ~LUCID_BACKWARD_REUSE_FRAME))
 		return LUCID_ERR_INVALID_ARG;
-	// the adjoint is of the fp32-colour, fp32-sample, "over" forward
-	if(!in->colors_rgb || !in->tri_alpha || in->colors_rgba8 || (r->opts & LUCID_OPT_ADDITIVE_BLENDING))
+	// the adjoint is of the fp32-colour, fp32-sample, exact "over" forward
+	if(!in->colors_rgb || !in->tri_alpha || in->colors_rgba8 ||
+	   (r->opts & (LUCID_OPT_ADDITIVE_BLENDING | LUCID_OPT_ALPHA_THRESHOLD)))
 		return LUCID_ERR_UNSUPPORTED;
 	hipStream_t stream = (hipStream_t)stream_ptr;
 	LUCID_CHECK(hipSetDevice(r->device));

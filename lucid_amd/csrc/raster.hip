@@ -1231,6 +1231,17 @@ __device__ __forceinline__ void stage_slice(const FrameConst &c, const FrameBufs
 	}
 }
 
+// LUCID_OPT_ALPHA_THRESHOLD (shading.glsl:62,282-284; raster_low.glsl:258-260): once every
+// pixel of a wave has transmittance below 1/128, the wave's later samples are not shaded
+// (their fragments are still counted). "Over" blending only.
+constexpr float kAlphaThreshold = 1.0f / 128.0f;
+__device__ __forceinline__ bool alpha_threshold_on(const FrameConst &c) {
+	return (c.opts & LUCID_OPT_ALPHA_THRESHOLD) && !(c.opts & LUCID_OPT_ADDITIVE_BLENDING);
+}
+__device__ __forceinline__ bool all_below_alpha_threshold(float trans) {
+	return ballot(!(trans < kAlphaThreshold)) == 0ull;
+}
+
 // Visits this lane's set bitmap bits in increasing entry order, one per step. The loop is
 // divergent: a lane leaves when its bits run out (the wave steps until every lane is done),
 // so the lane's state needs no copies for lanes that sit out a step.
@@ -1348,9 +1359,11 @@ __global__ __launch_bounds__(64) void block_raster_kernel(FrameConst c, FrameBuf
 		}
 	}
 	const bool additive = (c.opts & (1u << 5)) != 0;
+	const bool threshold = alpha_threshold_on(c);
 	PixelState<FP32> ps;
 	ps.init();
 	uint32_t count = 0;
+	bool opaque = false; // every pixel of the block is nearly opaque (alpha threshold)
 	static_assert(4 % W == 0, "register-sorted lists split into whole slices");
 	for(uint32_t s0 = 0; s0 < m; s0 += (64 * W)) {
 		const int j0 = (int)(s0 / 64);
@@ -1365,13 +1378,17 @@ __global__ __launch_bounds__(64) void block_raster_kernel(FrameConst c, FrameBuf
 					msk[k] = msk_all[j], tri[k] = tri_all[j];
 		}
 		const uint32_t n = min(m - s0, (uint32_t)(64 * W));
-		stage_slice(c, b, tri, n, s_rec);
-		wave_sync_lds();
+		if(!opaque)
+			stage_slice(c, b, tri, n, s_rec); // loads in flight during the transpose
 		uint64_t bm[W];
 		transpose_masks(msk, (int)((n + 63) / 64), bm);
 		count += bitmap_count(bm);
+		if(opaque)
+			continue; // counted, not shaded
+		wave_sync_lds();
 		walk_pixel<FP32, W>(c, s_rec, bm, (float)px, (float)py, ps, additive);
 		wave_sync_lds(); // every lane is done with this slice's records
+		opaque = threshold && all_below_alpha_threshold(ps.acc.tr);
 	}
 	ps.flush(additive);
 	finish_pixels(c, b, ps.acc, ps.w.bad, count, px, py, slot, m);
@@ -1417,6 +1434,8 @@ __global__ __launch_bounds__(kBigWaves * 64) void block_raster_big_kernel(FrameC
 		PixelState<FP32> ps;
 		ps.init();
 		uint32_t count = 0;
+		const bool threshold = alpha_threshold_on(c);
+		bool opaque = false; // every pixel is nearly opaque within this range (alpha threshold)
 		for(uint32_t s0 = e0; s0 < e1; s0 += (64 * W)) {
 			const uint32_t n = min(e1 - s0, (uint32_t)(64 * W));
 			uint64_t msk[W];
@@ -1425,15 +1444,20 @@ __global__ __launch_bounds__(kBigWaves * 64) void block_raster_big_kernel(FrameC
 			for(int k = 0; k < W; k++) {
 				uint32_t i = k * 64 + lane;
 				msk[k] = i < n ? mask[s0 + i] : 0ull;
-				t[k] = i < n ? tri[s0 + i] : 0u;
+				t[k] = i < n && !opaque ? tri[s0 + i] : 0u;
 			}
-			stage_slice(c, b, t, n, s_rec[wave]);
-			wave_sync_lds();
+			if(!opaque)
+				stage_slice(c, b, t, n, s_rec[wave]); // loads in flight during the transpose
 			uint64_t bm[W];
 			transpose_masks(msk, (int)((n + 63) / 64), bm);
 			count += bitmap_count(bm);
+			if(opaque)
+				continue; // counted, not shaded
+			wave_sync_lds();
 			walk_pixel<FP32, W>(c, s_rec[wave], bm, (float)px, (float)py, ps, additive);
 			wave_sync_lds(); // every lane is done with this slice's records
+			// the range's own transmittance bounds the pixel's (earlier ranges only lower it)
+			opaque = threshold && all_below_alpha_threshold(ps.acc.tr);
 		}
 		ps.flush(additive);
 		s_part[wave][lane] = make_float4(ps.acc.cr, ps.acc.cg, ps.acc.cb, ps.acc.tr);
