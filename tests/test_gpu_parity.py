@@ -211,3 +211,38 @@ def test_c2_full_size(oracle_mod):
     sc = scenes.config2()
     r, gpu, ref = _render_both(sc, oracle_mod)
     _check(sc, gpu, ref, oracle_mod)
+
+
+def _lit_scene(seed=43):
+    """config0 with the shading features the benchmark scenes leave off: per-vertex normals
+    (INST_HAS_VERTEX_NORMALS, packed 10:10:10), instance colours (INST_HAS_COLOR) and a sun
+    with a tinted ambient, so finalShading takes its power path."""
+    sc = scenes.config0(num_tris=3000, width=128, height=96, seed=seed)
+    rng = np.random.default_rng(seed)
+    n = rng.normal(size=(sc.num_verts, 3))
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    q = np.floor(512.0 + n * 511.0).astype(np.uint32) & 0x3FF
+    sc.normals = (q[:, 0] | (q[:, 1] << 10) | (q[:, 2] << 20)).astype(np.uint32)
+    sc.instances["flags"] |= scenes.INST_HAS_VERTEX_NORMALS | scenes.INST_HAS_COLOR
+    c = rng.integers(64, 256, size=(len(sc.instances), 4)).astype(np.uint32)
+    c[:, 3] = 255
+    sc.instance_colors = (c[:, 0] | (c[:, 1] << 8) | (c[:, 2] << 16) | (c[:, 3] << 24)).astype(
+        np.uint32)
+    sun = np.array([0.3, -0.5, -0.8])
+    sun /= np.linalg.norm(sun)
+    sc.config = scenes.make_config(scenes.Camera(sc.width, sc.height), len(sc.instances),
+                                   ambient=(0.3, 0.35, 0.4), ambient_power=0.6,
+                                   sun_color=(1.0, 0.9, 0.7), sun_dir=tuple(sun),
+                                   sun_power=1.2)
+    sc.name = f"lit_{sc.name}"
+    return sc
+
+
+def test_normals_instance_colours_and_sun_lighting(oracle_mod):
+    """Per-vertex normals, instance colours and sun lighting (shading.glsl:107-184,
+    funcs.glsl:261-271) against the oracle: bins and counts bit-exact, colours within 1/255,
+    in RGBA8 and fp32 samples."""
+    sc = _lit_scene()
+    for opts in (0, _lib.OPT_FP32_SAMPLES):
+        r, gpu, ref = _render_both(sc, oracle_mod, opts=opts)
+        _check(sc, gpu, ref, oracle_mod)

@@ -194,6 +194,54 @@ def test_planes_closed_form(oracle_mod):
     assert np.abs(c2 - c).max() > 0.1
 
 
+def _srgb_to_linear(x):
+    return np.where(x <= 0.04045, x / 12.92, ((x + 0.055) / 1.055) ** 2.4)
+
+
+def _linear_to_srgb(y):
+    return np.where(y <= 0.0031308, 12.92 * y, 1.055 * np.abs(y) ** (1.0 / 2.4) - 0.055)
+
+
+def test_lit_planes_closed_form(oracle_mod):
+    """The oracle's shading with instance colours and sun lighting (shading.glsl:107-184,
+    funcs.glsl:261-271) against a float64 closed form on two screen-aligned planes: flat
+    normal of each quad (quad_setup.glsl, cross(v0 - v2, v1 - v0), through the 10:10:10
+    packing), light = max(0, dot(-sun, n) * 0.7 + 0.3), per-channel factor ambient *
+    ambient_power + sun_colour * sun_power * light, colour = sRGB(linear(c * inst) * factor),
+    then "over" front to back. A dropped term, a wrong normal sign or a swapped power would
+    move the colour far beyond the tolerance."""
+    layers = [(4.0, (0.8, 0.5, 0.2), 0.5), (6.0, (0.3, 0.6, 0.9), 0.6)]
+    sc = scenes.planes_scene(64, 48, layers)
+    inst_rgba = np.array([200, 150, 255, 255], np.uint32)
+    sc.instances["flags"] |= scenes.INST_HAS_COLOR
+    sc.instance_colors = np.full(len(sc.instances), inst_rgba[0] | (inst_rgba[1] << 8) |
+                                 (inst_rgba[2] << 16) | (inst_rgba[3] << 24), np.uint32)
+    sun = np.array([0.4, -0.3, -0.866])
+    sun /= np.linalg.norm(sun)
+    amb, amb_pow, sun_col, sun_pow = np.array([0.3, 0.35, 0.4]), 0.6, np.array([1.0, 0.9, 0.7]), 1.2
+    sc.config = scenes.make_config(scenes.Camera(sc.width, sc.height), len(sc.instances),
+                                   ambient=tuple(amb), ambient_power=amb_pow,
+                                   sun_color=tuple(sun_col), sun_dir=tuple(sun),
+                                   sun_power=sun_pow)
+    res = oracle_mod.render(sc, opts=oracle_mod.OPT_FP32_SAMPLES)
+    inst = inst_rgba[:3].astype(np.float64) / 255.0
+    c, t = np.zeros(3), 1.0
+    for i, (_, col, a) in sorted(enumerate(layers), key=lambda x: x[1][0]):
+        p = sc.positions[sc.quad_indices[i, :3]].astype(np.float32)
+        n = np.cross(p[0] - p[2], p[1] - p[0]).astype(np.float64)
+        n /= np.linalg.norm(n)
+        q = np.floor(512.0 + n * 511.0).astype(np.int64) & 0x3FF
+        n = (q - 512) * np.float64(np.float32(1.0 / 511.0))
+        light = max(0.0, float(np.dot(-sun, n)) * 0.7 + 0.3)
+        factor = amb * amb_pow + (sun_col * sun_pow) * light
+        lit = np.clip(_linear_to_srgb(_srgb_to_linear(np.array(col) * inst) * factor), 0, 1)
+        c += lit * a * t
+        t *= 1 - a
+    img = res["image_f"][16:32, 24:40, :3].reshape(-1, 3)
+    assert np.abs(img - c).max() < 5e-5, (img[0], c)
+    assert light != 0.3  # the sun term is live (not clamped away)
+
+
 def _brute_force_image(scene):
     """float64 per-pixel sort + front-to-back blend with unquantised colours."""
     cam = scenes.Camera(scene.width, scene.height)
