@@ -973,8 +973,39 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 	const int wave_s = threadIdx.x >> 6, lane_s = lane_id();
 	const uint32_t P = max(256u, next_pow2(m));
 	const uint32_t nchunks = P / 256;
+	// The block's pool space (allocated first): the sorted entries (header, triangle ids,
+	// masks) and a scratch copy of the same in list order, which pass 1 fills so that the
+	// sorted output is gathered from the block's own (cache-resident) scratch instead of
+	// the scattered records.
+	const uint32_t sorted_words = sorted_mask_offset(m) + 2 * m;
+	const uint32_t scratch_mask = sorted_words + ((m + 1u) & ~1u);
+	if(threadIdx.x == 0) {
+		const uint32_t need = scratch_mask + 2 * m + 1; // + 1: even rounding of the base
+		const uint32_t stripe = slot % kSortStripes;
+		const uint64_t region = (b.sorted_cap / kSortStripes) & ~1ull;
+		const uint32_t off = atomicAdd(sort_cursor(b.ctrl, stripe), need);
+		uint32_t base = 0xffffffffu;
+		if((uint64_t)off + need <= region) {
+			base = (uint32_t)(stripe * region + off);
+		} else {
+			atomicOr(&b.ctrl[kCtrlOverflow], kOvfSorted);
+			const uint64_t want = ((uint64_t)off + need + 2) * kSortStripes; // + even rounding
+			atomicMax(&b.ctrl[kCtrlSortedCursor], (uint32_t)min(want, (uint64_t)0xffffffffu));
+		}
+		s_base = base == 0xffffffffu ? base : (base + 1u) & ~1u;
+	}
+	__syncthreads();
+	if(s_base == 0xffffffffu) {
+		if(threadIdx.x == 0)
+			b.blk_flags[slot] = kSlotPromoted; // nothing rendered; the frame reports overflow
+		return;
+	}
+	uint32_t *hdr = b.sorted_pool + s_base;
+	uint32_t *scratch_tri = hdr + sorted_words;
+	uint64_t *scratch_msk = (uint64_t *)(hdr + scratch_mask);
 	// Keys are (19-bit depth rank << 13 | list position). Pass 1 gathers the records once:
-	// depth keys (in s_key), the total of the sample counts and the block's key range.
+	// depth keys (in s_key), the scratch copy, the total of the sample counts and the
+	// block's key range.
 	uint32_t lo = ~0u, hi = 0u;
 	uint32_t mine = 0;
 	for(uint32_t ch = wave_s; ch < nchunks; ch += kSortThreads / 64) {
@@ -983,8 +1014,10 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 			const uint32_t j = ch * 256 + e * 64 + lane_s;
 			if(j < m) {
 				const uint4 rec = recs[list[j]];
-				const uint32_t smp = (uint32_t)__popcll(record_block_mask(rec, bx));
-				mine += smp;
+				const uint64_t msk = record_block_mask(rec, bx);
+				scratch_tri[j] = rec.x;
+				scratch_msk[j] = msk;
+				mine += (uint32_t)__popcll(msk);
 				const uint32_t dk = ~ord_f32(block_inv_depth(b, rec, bx, px0, py0));
 				s_key[j] = dk;
 				lo = min(lo, dk);
@@ -1009,25 +1042,6 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 		s_total = tot;
 		for(int w = 0; w <= kBigWaves; w++)
 			s_bound[w] = w == 0 ? 0u : m;
-		const uint32_t need = sorted_mask_offset(m) + 2 * m + 1;
-		const uint32_t stripe = slot % kSortStripes;
-		const uint64_t region = (b.sorted_cap / kSortStripes) & ~1ull;
-		const uint32_t off = atomicAdd(sort_cursor(b.ctrl, stripe), need);
-		uint32_t base = 0xffffffffu;
-		if((uint64_t)off + need <= region) {
-			base = (uint32_t)(stripe * region + off);
-		} else {
-			atomicOr(&b.ctrl[kCtrlOverflow], kOvfSorted);
-			const uint64_t want = ((uint64_t)off + need + 2) * kSortStripes; // + even rounding
-			atomicMax(&b.ctrl[kCtrlSortedCursor], (uint32_t)min(want, (uint64_t)0xffffffffu));
-		}
-		s_base = base == 0xffffffffu ? base : (base + 1u) & ~1u;
-	}
-	__syncthreads();
-	if(s_base == 0xffffffffu) {
-		if(threadIdx.x == 0)
-			b.blk_flags[slot] = kSlotPromoted; // nothing rendered; the frame reports overflow
-		return;
 	}
 	const DepthGrid<19> grid(lo, hi);
 	// Bitonic sort: levels up to 256 in registers (one 256-key chunk per wave at a time; each
@@ -1077,7 +1091,6 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 	// visited in rounds of kSortThreads consecutive positions (coalesced LDS and output).
 	const uint32_t rounds = (m + kSortThreads - 1) / kSortThreads;
 	const uint32_t nw = s_parts, total = s_total;
-	uint32_t *hdr = b.sorted_pool + s_base;
 	uint32_t *tri_out = hdr + kSortedHeader;
 	uint64_t *mask_out = (uint64_t *)(hdr + sorted_mask_offset(m));
 	// wave w starts at the first entry with at least w * total / nw samples before it whose
@@ -1094,18 +1107,17 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 	for(uint32_t r = 0; r < rounds; r++) {
 		const uint32_t i = r * kSortThreads + threadIdx.x;
 		const uint32_t r0 = r * kSortThreads, r1 = min(m, r0 + kSortThreads);
-		uint4 rec = make_uint4(0u, 0u, 0u, 0u);
+		uint32_t tri = 0, key = 0, dk = 0;
 		uint64_t msk = 0;
-		uint32_t key = 0, dk = 0;
 		bool tie = false;
 		if(i < m) {
 			key = s_key[i];
-			rec = recs[list[key & 0x1fffu]];
-			msk = record_block_mask(rec, bx);
+			tri = scratch_tri[key & 0x1fffu];
+			msk = scratch_msk[key & 0x1fffu];
 			tie = exact_runs && ((i > 0 && (s_key[i - 1] >> 13) == (key >> 13)) ||
 								 (i + 1 < m && (s_key[i + 1] >> 13) == (key >> 13)));
 			if(tie)
-				dk = ~ord_f32(block_inv_depth(b, rec, bx, px0, py0));
+				dk = ~ord_f32(block_inv_depth(b, recs[list[key & 0x1fffu]], bx, px0, py0));
 		}
 		s_tdk[r & 1][threadIdx.x] = dk;
 		const uint32_t smp = (uint32_t)__popcll(msk);
@@ -1152,7 +1164,7 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 					out = gs + ahead;
 				}
 			}
-			tri_out[out] = rec.x;
+			tri_out[out] = tri;
 			mask_out[out] = msk;
 		}
 	}
