@@ -42,6 +42,8 @@ def parse():
     p.add_argument("--no-backward", action="store_true",
                    help="skip the fwd+bwd / fwd ratio leg (200k triangles at 512x512)")
     p.add_argument("--bwd-steps", type=int, default=5)
+    p.add_argument("--no-unsorted", action="store_true",
+                   help="skip the unsorted-blend baseline (the same frame without depth order)")
     p.add_argument("--no-batch", action="store_true",
                    help="skip the batch leg (64 scenes in one hipGraph vs 64 sequential frames)")
     p.add_argument("--batch-scenes", type=int, default=64)
@@ -97,6 +99,33 @@ def batch_leg(num_scenes: int, steps: int = 3):
             "sequential_ms": t_seq * 1e3, "batch_ms": t_batch * 1e3,
             "batch_fragments_per_s": frags / t_batch, "speedup": t_seq / t_batch,
             "tuning": {f"{w}x{h}": t for (w, h), t in tuned.items()}}
+
+
+def unsorted_baseline(scene, tuning, steps, exact_ms):
+    """The same frame with LUCID_OPT_UNSORTED: samples blended in block-list order, as
+    hardware alpha blending would (the reference's SimpleRenderer), timed like the exact
+    frame (back-to-back frames between HIP events). The ratio is what exact order costs."""
+    import torch
+    from lucid_amd import _lib
+    from lucid_amd.renderer import GpuScene, LucidRenderer
+    r = LucidRenderer((scene.width, scene.height), opts=_lib.OPT_UNSORTED,
+                      max_quads=scene.num_quads)
+    r.set_tuning(**tuning)
+    g = GpuScene(scene)
+    out = r.alloc_outputs(rgba8=True, counts=True)
+    for _ in range(3):
+        r.render(g, out)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(steps):
+        r.render_async(g, out)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    _lib.check(r.finish(), "unsorted frames")
+    ms = e0.elapsed_time(e1) / steps
+    return {"ms_per_frame": ms, "exact_over_unsorted": exact_ms / ms}
 
 
 def backward_ratio(steps: int):
@@ -265,6 +294,11 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
 
+    unsorted = None
+    if not args.no_unsorted:
+        print("[bench] unsorted baseline", file=sys.stderr, flush=True)
+        unsorted = unsorted_baseline(scene, tuning, args.steps, ms)
+
     bwd = None
     if not args.no_backward:
         print("[bench] fwd+bwd ratio (200k tris, 512x512)", file=sys.stderr, flush=True)
@@ -306,6 +340,7 @@ def main():
             "high_bins": int(info[9]), "low_bins": int(info[7]),
             "backward": bwd,
             "batch": batch,
+            "unsorted_baseline": unsorted,
         },
         "roofline": {
             "bound": "hbm",

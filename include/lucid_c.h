@@ -56,6 +56,13 @@ extern "C" {
 /* Extension: shaded samples stay fp32 instead of the reference's RGBA8 truncation
  * (encodeRGBA8, shared/funcs.glsl:118-121). Used by the differentiable path. */
 #define LUCID_OPT_FP32_SAMPLES (1u << 16)
+/* Extension: the unsorted baseline. Samples are blended in block-list order with no depth
+ * order (no block sort, no ordering window, no exact per-pixel sort except for blocks too
+ * long to pre-sort), as hardware alpha blending would (SimpleRenderer,
+ * src/simple_renderer.cpp:84-90,124). Bins, lists and fragment counts are the exact
+ * frame's; colours are not (unless blending is additive or nothing overlaps). For
+ * measuring what the exact order costs; the backward rejects it. */
+#define LUCID_OPT_UNSORTED (1u << 17)
 
 typedef struct lucid_renderer lucid_renderer;
 typedef struct lucid_batch lucid_batch;

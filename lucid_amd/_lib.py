@@ -26,6 +26,7 @@ OPT_ADDITIVE_BLENDING = 1 << 5
 OPT_VISUALIZE_ERRORS = 1 << 6
 OPT_ALPHA_THRESHOLD = 1 << 7
 OPT_FP32_SAMPLES = 1 << 16
+OPT_UNSORTED = 1 << 17
 
 BACKWARD_REUSE_FRAME = 1
 RENDER_RECORD_BACKWARD = 1

@@ -623,7 +623,8 @@ int lucid_render_ex(lucid_renderer *r, const lucid_frame_inputs *in,
 	// the recorded pass is the backward's: fp32 samples of fp32 colours, "over" blending
 	if(record && (!(r->opts & LUCID_OPT_FP32_SAMPLES) || !in->colors_rgb || !in->tri_alpha ||
 				  in->colors_rgba8 ||
-				  (r->opts & (LUCID_OPT_ADDITIVE_BLENDING | LUCID_OPT_ALPHA_THRESHOLD))))
+				  (r->opts & (LUCID_OPT_ADDITIVE_BLENDING | LUCID_OPT_ALPHA_THRESHOLD |
+							  LUCID_OPT_UNSORTED))))
 		return LUCID_ERR_UNSUPPORTED;
 	hipStream_t stream = (hipStream_t)stream_ptr;
 	FrameConst c;
@@ -682,7 +683,7 @@ int lucid_backward_ex(lucid_renderer *r, const lucid_frame_inputs *in, const flo
 		return LUCID_ERR_INVALID_ARG;
 	// the adjoint is of the fp32-colour, fp32-sample, exact "over" forward
 	if(!in->colors_rgb || !in->tri_alpha || in->colors_rgba8 ||
-	   (r->opts & (LUCID_OPT_ADDITIVE_BLENDING | LUCID_OPT_ALPHA_THRESHOLD)))
+	   (r->opts & (LUCID_OPT_ADDITIVE_BLENDING | LUCID_OPT_ALPHA_THRESHOLD | LUCID_OPT_UNSORTED)))
 		return LUCID_ERR_UNSUPPORTED;
 	hipStream_t stream = (hipStream_t)stream_ptr;
 	LUCID_CHECK(hipSetDevice(r->device));

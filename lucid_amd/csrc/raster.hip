@@ -973,6 +973,7 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 	const int wave_s = threadIdx.x >> 6, lane_s = lane_id();
 	const uint32_t P = max(256u, next_pow2(m));
 	const uint32_t nchunks = P / 256;
+	const bool unsorted = (c.opts & LUCID_OPT_UNSORTED) != 0; // the unsorted baseline
 	// The block's pool space (allocated first): the sorted entries (header, triangle ids,
 	// masks) and a scratch copy of the same in list order, which pass 1 fills so that the
 	// sorted output is gathered from the block's own (cache-resident) scratch instead of
@@ -1018,10 +1019,14 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 				scratch_tri[j] = rec.x;
 				scratch_msk[j] = msk;
 				mine += (uint32_t)__popcll(msk);
-				const uint32_t dk = ~ord_f32(block_inv_depth(b, rec, bx, px0, py0));
-				s_key[j] = dk;
-				lo = min(lo, dk);
-				hi = max(hi, dk);
+				if(unsorted) {
+					s_key[j] = j; // list order: rank 0, the position alone
+				} else {
+					const uint32_t dk = ~ord_f32(block_inv_depth(b, rec, bx, px0, py0));
+					s_key[j] = dk;
+					lo = min(lo, dk);
+					hi = max(hi, dk);
+				}
 			}
 		}
 	}
@@ -1048,7 +1053,7 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 	// lane reads back only what it wrote in pass 1), then per level the strides >= 256 in LDS
 	// and the rest in registers. A real key never equals the padding ~0u: position 8191
 	// exists only when nothing is padded.
-	for(uint32_t ch = wave_s; ch < nchunks; ch += kSortThreads / 64) {
+	for(uint32_t ch = wave_s; ch < nchunks && !unsorted; ch += kSortThreads / 64) {
 		uint32_t v[4];
 #pragma unroll
 		for(int e = 0; e < 4; e++) {
@@ -1061,7 +1066,7 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 			s_key[ch * 256 + e * 64 + lane_s] = v[e];
 	}
 	__syncthreads();
-	for(uint32_t k = 512; k <= P; k <<= 1) {
+	for(uint32_t k = 512; k <= P && !unsorted; k <<= 1) {
 		for(uint32_t j = k >> 1; j >= 256; j >>= 1) {
 			for(uint32_t i = threadIdx.x; i < P / 2; i += kSortThreads) {
 				uint32_t lo_i = ((i & ~(j - 1)) << 1) | (i & (j - 1));
@@ -1102,7 +1107,7 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 	// fp32 samples (the differentiable ones) each run of equal ranks that lies within one
 	// round (and is at most kTieRun long) is put in exact depth-key order here: its members
 	// recompute their full depth key and take their place among the run.
-	const bool exact_runs = (c.opts & LUCID_OPT_FP32_SAMPLES) != 0;
+	const bool exact_runs = (c.opts & LUCID_OPT_FP32_SAMPLES) != 0 && !unsorted;
 	uint32_t carry = 0, found = 0;
 	for(uint32_t r = 0; r < rounds; r++) {
 		const uint32_t i = r * kSortThreads + threadIdx.x;
@@ -1135,7 +1140,7 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 		carry += round_tot;
 		if(i < m) {
 			const uint32_t run = before + inc - smp; // samples of the entries before i
-			const bool cut = i == 0 || (s_key[i] >> 13) != (s_key[i - 1] >> 13);
+			const bool cut = unsorted || i == 0 || (s_key[i] >> 13) != (s_key[i - 1] >> 13);
 			for(uint32_t w = 1; w < nw; w++) {
 				const uint32_t target = (uint32_t)(((uint64_t)w * total) / nw);
 				if(!((found >> w) & 1u) && run >= target && cut) {
@@ -1287,13 +1292,18 @@ template <bool FP32, int W>
 __device__ __forceinline__ void walk_pixel(const FrameConst &c, const uint4 *s_rec,
 										   uint64_t (&bm)[W], float px, float py,
 										   PixelState<FP32> &ps, bool additive) {
+	const bool unsorted = (c.opts & LUCID_OPT_UNSORTED) != 0;
 	walk_bits(bm, [&](int e) {
 		const uint4 *rec = s_rec + e * kRecQuads;
 		const uint4 q0 = rec[0];
 		const float inv = record_inv_depth(q0, px, py);
 		const SampleColor<FP32> col = shade_record<FP32>(c, rec, px, py, inv);
-		if(col.contributes())
-			ps.insert(pixel_key(inv, rec[6].z), col, additive);
+		if(col.contributes()) {
+			if(unsorted)
+				ps.acc.blend(col.rgba(), additive); // list order, no depth order
+			else
+				ps.insert(pixel_key(inv, rec[6].z), col, additive);
+		}
 	});
 }
 
@@ -1358,7 +1368,13 @@ __global__ __launch_bounds__(64) void block_raster_kernel(FrameConst c, FrameBuf
 	const uint4 *recs = (level == LUCID_BIN_LEVEL_LOW ? b.low_records : b.high_records) + sl.z;
 	const uint32_t *list = b.blk_lists + sl.x;
 	uint32_t pos[4];
-	sort_block_positions(b, recs, list, m, bx, px0, py0, pos);
+	if(c.opts & LUCID_OPT_UNSORTED) {
+#pragma unroll
+		for(int k = 0; k < 4; k++)
+			pos[k] = k * 64 + lane; // list order
+	} else {
+		sort_block_positions(b, recs, list, m, bx, px0, py0, pos);
+	}
 	uint64_t msk_all[4];
 	uint32_t tri_all[4];
 #pragma unroll
