@@ -35,6 +35,7 @@ import numpy as np
 # definitions.glsl:84-94
 INST_HAS_VERTEX_COLORS = 0x001
 INST_HAS_VERTEX_NORMALS = 0x004
+INST_IS_OPAQUE = 0x008
 INST_HAS_COLOR = 0x200
 
 MAX_INSTANCE_QUADS = 1024  # src/lucid_renderer.h:29
