@@ -123,6 +123,13 @@ __device__ void store_tri(const FrameConst &c, const FrameBufs &b, uint32_t t, u
 	float base[3], step[3], scan[3];
 #pragma unroll
 	for(int i = 0; i < 3; i++) {
+		// An edge parallel to the pixel rows has e.x == +-0, and its x-intercept form would
+		// be inf - inf = NaN, which bounds nothing (rows past the edge would fill with the
+		// other two edges' spans). Tilted to e.x = max(|e.y| 2^-40, 2^-100) > 0, the
+		// intercept runs from about -2^40 to +2^40 pixels across the edge's row position
+		// and still empties the rows beyond it.
+		if(e[i].x == 0.0f)
+			e[i].x = fmaxf(fabsf(e[i].y) * 0x1p-40f, 0x1p-100f);
 		float inv_ex = 1.0f / e[i].x;
 		base[i] = -(e[i].z * inv_ex);
 		step[i] = -(e[i].y * inv_ex);

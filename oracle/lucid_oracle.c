@@ -300,6 +300,13 @@ static void store_tri(frame *f, uint32_t tri_idx, uint32_t flags_id, v3 tri0, v3
 	for(int i = 0; i < 3; i++)
 		edges[i] = v3_make(v3_dot(nrms[i], f->dirx), v3_dot(nrms[i], f->diry),
 						   v3_dot(nrms[i], f->dir0));
+	/* Deviation from the reference: an edge parallel to the pixel rows (edges.x == +-0)
+	 * makes the reference's x-intercept form inf - inf = NaN, whose min/max in
+	 * rasterBinStep is undefined in GLSL (and drops the edge where NaN is ignored).
+	 * It is tilted to edges.x = max(|edges.y| 2^-40, 2^-100) so it keeps bounding rows. */
+	for(int i = 0; i < 3; i++)
+		if(edges[i].x == 0.0f)
+			edges[i].x = fmaxf(fabsf(edges[i].y) * 0x1p-40f, 0x1p-100f);
 	float inv_ex[3] = {1.0f / edges[0].x, 1.0f / edges[1].x, 1.0f / edges[2].x};
 	float scan_base[3], scan_step[3], scan[3];
 	for(int i = 0; i < 3; i++) {

@@ -265,3 +265,21 @@ def test_obj_scene_gpu_parity(oracle_mod, tmp_path):
     for opts in (0, _lib.OPT_FP32_SAMPLES):
         r, gpu, ref = _render_both(sc, oracle_mod, opts=opts)
         _check(sc, gpu, ref, oracle_mod)
+
+
+def test_row_parallel_edges_cover_pixel_centres(oracle_mod, tmp_path):
+    """The OBJ sphere grid seen by an unrolled camera has edges exactly parallel to the pixel
+    rows (mirror-symmetric vertices, so the edge's x coefficient is +-0). The oracle's
+    fragment counts equal a float64 pixel-centre test away from edges; without the tilt of
+    such edges their rows past the edge fill with the other two edges' spans."""
+    from test_oracle import _pixel_centre_coverage
+    obj, mtl = sphere_grid_obj()
+    (tmp_path / "grid.obj").write_text(obj)
+    (tmp_path / "grid.mtl").write_text(mtl)
+    sc, _ = scene_io.load_obj_scene(str(tmp_path / "grid.obj"), 160, 120)
+    res = oracle_mod.render(sc, mode=oracle_mod.MODE_COUNTS)
+    count, margin = _pixel_centre_coverage(sc)
+    away = margin > 1e-3
+    assert away.mean() > 0.9
+    diff = res["frag_counts"].astype(np.int64) - count
+    assert (diff[away] == 0).all(), f"{int((diff[away] != 0).sum())} pixels differ"
