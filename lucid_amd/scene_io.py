@@ -225,6 +225,39 @@ def load_obj(path: str) -> ObjMesh:
         return parse_obj(f.read(), load_mtl)
 
 
+def merge_vertices(mesh: ObjMesh, decimal_places: int = 5) -> ObjMesh:
+    """Scene::mergeVertices (src/scene.cpp:169-236): vertices whose positions round to the
+    same multiple of 10^-decimal_places become one, at the mean of their positions (and
+    normals, when there are any), in order of first appearance; triangles are re-indexed.
+    Sums run in float32 in vertex order, then are scaled by float32(1 / count). The
+    reference merges only meshes without texture coordinates (src/scene_convert.cpp:359)."""
+    if mesh.tex_coords is not None:
+        raise ObjError("merge_vertices needs a mesh without texture coordinates")
+    pos = mesh.positions.astype(np.float32)
+    scale = np.float32(10.0 ** decimal_places)
+    q = np.round(pos * scale)  # round half to even; the reference's rounding is libfwk's
+    _, first, inverse = np.unique(q, axis=0, return_index=True, return_inverse=True)
+    inverse = inverse.reshape(-1)
+    order = np.argsort(first, kind="stable")  # unique rows -> first-appearance order
+    rank = np.empty_like(order)
+    rank[order] = np.arange(len(order))
+    new_idx = rank[inverse]
+    n = len(order)
+    if n == len(pos):
+        return mesh
+    count = np.bincount(new_idx, minlength=n).astype(np.float32)
+    inv = (np.float32(1.0) / count)[:, None]
+
+    def mean(a):
+        acc = np.zeros((n, 3), np.float32)
+        np.add.at(acc, new_idx, a.astype(np.float32))
+        return (acc * inv).astype(np.float32)
+
+    normals = None if mesh.normals is None else mean(mesh.normals)
+    return dataclasses.replace(mesh, positions=mean(pos), normals=normals,
+                               tris=new_idx[mesh.tris].astype(np.int64))
+
+
 def encode_normals(normals: np.ndarray) -> np.ndarray:
     """encodeNormalUint (src/scene.cpp:338-343): each component to uint(512 + n * 511)
     in double precision (truncated), 10 bits each, x lowest."""
@@ -336,22 +369,83 @@ def obj_scene(
     return sc, draws
 
 
-def load_obj_scene(path: str, width: int, height: int, **kwargs) -> Tuple[scenes.Scene, List[MeshDraw]]:
-    """load_obj then obj_scene; the scene is named after the file."""
+def load_obj_scene(path: str, width: int, height: int, merge_verts: bool = False,
+                   **kwargs) -> Tuple[scenes.Scene, List[MeshDraw]]:
+    """load_obj (then merge_vertices when asked and the file has no texture coordinates,
+    as convertScene does) then obj_scene; the scene is named after the file."""
     kwargs.setdefault("name", os.path.splitext(os.path.basename(path))[0])
-    return obj_scene(load_obj(path), width, height, **kwargs)
+    mesh = load_obj(path)
+    if merge_verts and mesh.tex_coords is None:
+        mesh = merge_vertices(mesh)
+    return obj_scene(mesh, width, height, **kwargs)
+
+
+@dataclasses.dataclass
+class InputScene:
+    """One <scene> of an input-scenes XML file (InputScene, src/scene_convert.h:8-19,
+    src/scene_convert.cpp:34-63)."""
+
+    name: str
+    path: str
+    quad_squareness: float = 1.0
+    merge_verts: bool = False
+    flip_uv: bool = False
+    flip_yz: bool = False
+    pbr: bool = False
+    env_map: str = ""
+
+    def load(self, width: int, height: int, **kwargs) -> Tuple[scenes.Scene, List[MeshDraw]]:
+        """The scene with this entry's conversion parameters (flip_uv, pbr and env_map have
+        no effect here: textures are not sampled)."""
+        kwargs.setdefault("name", self.name)
+        return load_obj_scene(self.path, width, height, merge_verts=self.merge_verts,
+                              squareness=self.quad_squareness, flip_yz=self.flip_yz, **kwargs)
+
+
+def load_input_scenes(xml_path: str, root_path: Optional[str] = None) -> List[InputScene]:
+    """Parses an input-scenes XML file (the reference's input_scenes.xml format): a <scenes>
+    root with an optional root_path, one <scene path=... /> per scene with optional name,
+    quad_squareness, merge_verts, flip_uv, flip_yz, pbr and env_map. `root_path` overrides
+    the file's (its scenes may live elsewhere on this machine)."""
+    import xml.etree.ElementTree as ET
+
+    root = ET.parse(xml_path).getroot()
+    if root.tag != "scenes":
+        raise ObjError(f"{xml_path}: root element is <{root.tag}>, not <scenes>")
+    base = root_path if root_path is not None else root.get("root_path", "")
+
+    def flag(node, key):
+        v = node.get(key, "false").strip().lower()
+        if v not in ("true", "false", "1", "0"):
+            raise ObjError(f"{xml_path}: {key}={v!r} is not a bool")
+        return v in ("true", "1")
+
+    out = []
+    for node in root.findall("scene"):
+        path = node.get("path")
+        if not path:
+            raise ObjError(f"{xml_path}: <scene> without a path")
+        name = node.get("name") or os.path.splitext(os.path.basename(path))[0]
+        out.append(InputScene(name, os.path.join(base, path),
+                              float(node.get("quad_squareness", "1.0")), flag(node, "merge_verts"),
+                              flag(node, "flip_uv"), flag(node, "flip_yz"), flag(node, "pbr"),
+                              node.get("env_map", "")))
+    return out
 
 
 def main(argv=None) -> int:
     """python -m lucid_amd.scene_io scene.obj [--size 1920x1080] [--frames 20]: renders an
     OBJ scene on the GPU and prints one JSON line (quads, fragments, ms per frame, stage
-    times); --ppm writes the frame."""
+    times); --ppm writes the frame. With --input-scenes list.xml the first argument names
+    a scene of that list, converted with its parameters."""
     import argparse
     import json
     import sys
 
     p = argparse.ArgumentParser(prog="python -m lucid_amd.scene_io")
-    p.add_argument("obj")
+    p.add_argument("obj", help="an OBJ file, or a scene name with --input-scenes")
+    p.add_argument("--input-scenes", default="", help="input-scenes XML file")
+    p.add_argument("--root", default=None, help="overrides the XML file's root_path")
     p.add_argument("--size", default="1920x1080")
     p.add_argument("--frames", type=int, default=20)
     p.add_argument("--opacity", type=float, default=0.5, help="scene opacity factor")
@@ -367,8 +461,16 @@ def main(argv=None) -> int:
     from . import _lib
     from .renderer import GpuScene, LucidRenderer
 
-    sc, draws = load_obj_scene(a.obj, w, h, scene_opacity=a.opacity, squareness=a.squareness,
-                               flip_yz=a.flip_yz)
+    if a.input_scenes:
+        entries = {e.name: e for e in load_input_scenes(a.input_scenes, a.root)}
+        if a.obj not in entries:
+            print(f"no scene {a.obj!r} in {a.input_scenes} (has {sorted(entries)})",
+                  file=sys.stderr)
+            return 2
+        sc, draws = entries[a.obj].load(w, h, scene_opacity=a.opacity)
+    else:
+        sc, draws = load_obj_scene(a.obj, w, h, scene_opacity=a.opacity,
+                                   squareness=a.squareness, flip_yz=a.flip_yz)
     print(f"[scene_io] {sc.name}: {len(draws)} meshes, {sc.num_quads} quads, "
           f"{sc.num_verts} vertices", file=sys.stderr, flush=True)
     r = LucidRenderer((w, h), opts=_lib.OPT_TIMERS, max_quads=sc.num_quads)

@@ -283,3 +283,54 @@ def test_row_parallel_edges_cover_pixel_centres(oracle_mod, tmp_path):
     assert away.mean() > 0.9
     diff = res["frag_counts"].astype(np.int64) - count
     assert (diff[away] == 0).all(), f"{int((diff[away] != 0).sum())} pixels differ"
+
+
+def test_merge_vertices():
+    """Scene::mergeVertices (src/scene.cpp:169-236): an icosphere written as a triangle soup
+    (every triangle with its own three vertices, positions jittered below 10^-6) merges
+    back to the sphere's vertex count, in first-appearance order, at the mean position,
+    with averaged normals and the sphere's triangles; a mesh with texture coordinates is
+    refused, a mesh with nothing to merge comes back unchanged."""
+    sp, st = scenes.icosphere(2)
+    sp = (np.round(sp.astype(np.float64) * 1e5) / 1e5).astype(np.float32)  # quantum centres
+    rng = np.random.default_rng(3)
+    soup = sp[st.reshape(-1)].astype(np.float32)
+    soup = soup + rng.uniform(-4e-7, 4e-7, soup.shape).astype(np.float32)
+    nrm = rng.normal(size=soup.shape).astype(np.float32)
+    m = scene_io.ObjMesh(soup, nrm, None, np.arange(len(soup)).reshape(-1, 3),
+                         [scene_io.ObjMaterial("default")], [(0, 0, len(st))])
+    mm = scene_io.merge_vertices(m)
+    assert len(mm.positions) == len(sp)
+    first = {}
+    for i, v in enumerate(st.reshape(-1).tolist()):
+        first.setdefault(v, i)
+    order = sorted(first, key=first.get)  # sphere vertex ids by first appearance
+    assert mm.tris.tolist() == [[order.index(v) for v in t] for t in st.tolist()]
+    for new_id, old_id in enumerate(order[:20]):
+        members = np.flatnonzero(st.reshape(-1) == old_id)
+        assert np.allclose(mm.positions[new_id], soup[members].mean(0), atol=1e-7)
+        assert np.allclose(mm.normals[new_id], nrm[members].mean(0), atol=1e-6)
+    assert scene_io.merge_vertices(mm) is mm
+    with pytest.raises(scene_io.ObjError):
+        scene_io.merge_vertices(_parse()[0])
+
+
+def test_input_scenes_xml(tmp_path):
+    """The input-scenes list format (src/scene_convert.cpp:34-63): root_path joined to each
+    path, the name from the file stem unless given, defaults for missing parameters."""
+    (tmp_path / "list.xml").write_text(
+        '<scenes root_path="/data/scenes/">\n'
+        '  <scene path="dragon2.obj" quad_squareness="1.5" merge_verts="true" />\n'
+        '  <scene path="trees/tree01.obj" name="tree" flip_yz="true" />\n'
+        '</scenes>\n')
+    a, b = scene_io.load_input_scenes(str(tmp_path / "list.xml"))
+    assert (a.name, a.path, a.quad_squareness, a.merge_verts, a.flip_yz) == \
+        ("dragon2", "/data/scenes/dragon2.obj", 1.5, True, False)
+    assert (b.name, b.path, b.quad_squareness, b.merge_verts, b.flip_yz) == \
+        ("tree", "/data/scenes/trees/tree01.obj", 1.0, False, True)
+    (tmp_path / "grid.obj").write_text(sphere_grid_obj(n=2, level=1)[0])
+    (tmp_path / "grid.mtl").write_text(sphere_grid_obj(n=2, level=1)[1])
+    (tmp_path / "local.xml").write_text('<scenes><scene path="grid.obj" merge_verts="1" /></scenes>')
+    (e,) = scene_io.load_input_scenes(str(tmp_path / "local.xml"), root_path=str(tmp_path))
+    sc, draws = e.load(64, 48)
+    assert sc.name == "grid" and len(draws) == 5 and sc.num_quads > 0
