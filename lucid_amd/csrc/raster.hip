@@ -2138,6 +2138,8 @@ __global__ __launch_bounds__(64) void bwd_pass_b_kernel(FrameConst c, FrameBufs 
 	const bool live = st.gr != 0.0f || st.gg != 0.0f || st.gb != 0.0f;
 	const int fix_k = grad_fixed_exp(c, bits2f(b.ctrl[kCtrlBwdGradMax]));
 	const double fix_scale = ldexp(1.0, fix_k);
+	const bool rot_b0 = (lane & 1) != 0, rot_b1 = (lane & 2) != 0;
+	const int rot = rot_b1 ? (rot_b0 ? 8 : 5) : (rot_b0 ? 3 : 0);
 	for(uint32_t s0 = ent.e0; s0 < ent.e1; s0 += kSliceEntries) {
 		const uint32_t n = min(ent.e1 - s0, (uint32_t)kSliceEntries);
 		uint64_t msk[kSliceWords];
@@ -2160,16 +2162,18 @@ __global__ __launch_bounds__(64) void bwd_pass_b_kernel(FrameConst c, FrameBufs 
 			float out[kGradWords];
 			bwd_fragment(t, q, st.gr, st.gg, st.gb, smp, out);
 			if(live) {
-				// lanes add the ten components in four rotated orders (pixel column mod 4), so
-				// that pixels at the same entry in the same step seldom hit one address (integer
-				// sums: the order changes no bit)
-				const int grp = lane & 3;
+				// lanes add the ten components in four rotated orders (pixel column mod 4: by
+				// 0, 3, 5 or 8), so that pixels at the same entry in the same step seldom hit
+				// one address (integer sums: the order changes no bit). The order is picked with
+				// selects on two lane bits: an equality chain on lane & 3 would compile to a
+				// branchy switch per component.
 #pragma unroll
 				for(int k = 0; k < kGradWords; k++) {
 					const int k1 = (k + 3) % kGradWords, k2 = (k + 5) % kGradWords;
 					const int k3 = (k + 8) % kGradWords;
-					const int kk = grp == 0 ? k : grp == 1 ? k1 : grp == 2 ? k2 : k3;
-					const float v = grp == 0 ? out[k] : grp == 1 ? out[k1] : grp == 2 ? out[k2] : out[k3];
+					const float v = rot_b1 ? (rot_b0 ? out[k3] : out[k2]) : (rot_b0 ? out[k1] : out[k]);
+					const int kk0 = k + rot;
+					const int kk = kk0 >= kGradWords ? kk0 - kGradWords : kk0;
 					atomicAdd(&s_grad[(uint32_t)e * kGradStride + kk], to_fixed(v, fix_scale));
 				}
 			}
