@@ -589,21 +589,23 @@ __device__ __forceinline__ void sort_block_positions(const FrameBufs &b, const u
 		const uint32_t j = k * 64 + lane;
 		return j < m ? (grid.rank(dk[k]) << 8) | j : ~0u;
 	};
+	// One key array for the three sort widths (copied in and out of the narrower sorts), so
+	// that it stays in registers: positions of empty rows are never read.
+	uint32_t w[4] = {key(0), key(1), key(2), key(3)};
 	if(m <= 64) {
-		uint32_t w1[1] = {key(0)};
+		uint32_t w1[1] = {w[0]};
 		bitonic_regs<1>(w1);
-		pos[0] = w1[0] & 0xffu, pos[1] = pos[2] = pos[3] = 0u;
+		w[0] = w1[0];
 	} else if(m <= 128) {
-		uint32_t w2[2] = {key(0), key(1)};
+		uint32_t w2[2] = {w[0], w[1]};
 		bitonic_regs<2>(w2);
-		pos[0] = w2[0] & 0xffu, pos[1] = w2[1] & 0xffu, pos[2] = pos[3] = 0u;
+		w[0] = w2[0], w[1] = w2[1];
 	} else {
-		uint32_t w4[4] = {key(0), key(1), key(2), key(3)};
-		bitonic_regs<4>(w4);
-#pragma unroll
-		for(int k = 0; k < 4; k++)
-			pos[k] = w4[k] & 0xffu;
+		bitonic_regs<4>(w);
 	}
+#pragma unroll
+	for(int k = 0; k < 4; k++)
+		pos[k] = w[k] & 0xffu;
 }
 
 __device__ __forceinline__ uint32_t bin_tri_at(const FrameBufs &b, uint32_t i, uint32_t qc,
