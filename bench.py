@@ -26,7 +26,7 @@ sys.path.insert(0, ROOT)
 METRIC = "ms/frame and fragments blended/s @1080p 1M tris; fwd+bwd vs plain fwd ratio"
 HBM_PEAK_GBS = 8000.0
 DEFAULT_TRIS = 1_000_000
-RASTER_TRAFFIC_BYTES = 647.3e6  # block_raster + block_raster_big, one C1 frame (rocprofv3)
+RASTER_TRAFFIC_BYTES = 626.6e6  # block_raster + block_raster_big, one C1 frame (rocprofv3)
 
 
 def parse():
