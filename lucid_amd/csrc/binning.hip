@@ -81,11 +81,18 @@ __device__ __forceinline__ Slice wg_slice(const FrameConst &c, const FrameBufs &
 
 // Quad of flat large-list entry i (< the total of the sub-lists).
 __device__ __forceinline__ uint32_t large_at(const FrameBufs &b, const Slice &s, uint32_t i) {
+	// the starts do not decrease, so the stripes that start at or before i are a prefix:
+	// k counts them and base ends at the last one's start (static indices only, so the
+	// starts stay in registers)
 	int k = 0;
+	uint32_t base = 0;
 #pragma unroll
-	for(int j = 1; j < kLargeStripes; j++)
-		k += i >= s.lstart[j] ? 1 : 0;
-	return b.large_list[(size_t)k * s.lcap + (i - s.lstart[k])];
+	for(int j = 1; j < kLargeStripes; j++) {
+		const bool ge = i >= s.lstart[j];
+		k += ge ? 1 : 0;
+		base = ge ? s.lstart[j] : base;
+	}
+	return b.large_list[(size_t)k * s.lcap + (i - base)];
 }
 
 // bin_counter.glsl: per-work-group histograms of small quads and large tris.
