@@ -37,6 +37,7 @@ enum CtrlWord : int {
 	kCtrlTotalBinTris = 6,
 	kCtrlFallbackPixels = 7,
 	kCtrlListCursor = 8,    // high-bin block-list space the frame needs (u32 units)
+	kCtrlOverflowCarry = 9, // overflow bits of earlier frames not yet seen by the host
 	kCtrlSortedCursor = 10, // pre-sorted pool size a frame needed, after an overflow (u32 units)
 	kCtrlBwdRows = 11,      // gradient rows of the block kernel (set by the row scan)
 	kCtrlBwdExtra = 12,     // bump allocator of per-fragment gradient rows of exact pixels
@@ -150,6 +151,10 @@ struct FrameConst {
 	int slice_words;           // block rasteriser slice width: 1 or 2 words of 64 entries
 	int color_mode;
 	int has_normals;
+	// 1 when an earlier frame of the renderer may still be running: the clear kernel then
+	// moves its overflow bits to kCtrlOverflowCarry instead of dropping them, so that they
+	// are reported with this frame's status
+	uint32_t keep_overflow;
 	float vp[16]; // column-major view-projection matrix
 	float origin[3], dir0[3], dirx[3], diry[3], ray_dir0[3];
 	float ambient[3], sun_color[3], sun_dir[3];

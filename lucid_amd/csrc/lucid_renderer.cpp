@@ -144,6 +144,8 @@ struct lucid_renderer {
 	hipEvent_t stage_ev[kStages + 1] = {};
 	bool timers = false;
 	bool frame_pending = false;
+	// a frame that was never waited for overflowed; lucid_finish reports it
+	bool unreported_overflow = false;
 	int64_t last_num_quads = 0;
 	hipStream_t last_stream = nullptr;
 
@@ -207,7 +209,8 @@ int ensure_quad_capacity(lucid_renderer *r, int64_t nq) {
 }
 
 int grow_after_overflow(lucid_renderer *r) {
-	uint32_t flags = r->status[kCtrlOverflow];
+	// this frame's overflow bits and those of earlier frames it was queued behind
+	uint32_t flags = r->status[kCtrlOverflow] | r->status[kCtrlOverflowCarry];
 	if(!flags)
 		return LUCID_OK;
 	if(flags & kOvfBinLists) {
@@ -254,19 +257,23 @@ int grow_after_overflow(lucid_renderer *r) {
 	return LUCID_ERR_CAPACITY;
 }
 
-void fill_const(const lucid_renderer *r, const lucid_config &cfg, int64_t nq, int ninst,
-				int color_mode, bool has_normals, FrameConst &c) {
+// Per-frame constants of a width x height frame (bin grid, options, tuning, LucidConfig).
+void fill_const(int width, int height, uint32_t opts, const lucid_tuning &tuning,
+				const lucid_config &cfg, int64_t nq, int ninst, int color_mode, bool has_normals,
+				FrameConst &c) {
 	memset(&c, 0, sizeof(c));
-	c.width = r->width, c.height = r->height;
-	c.bin_count_x = r->bin_count_x, c.bin_count_y = r->bin_count_y, c.bin_count = r->bin_count;
+	c.width = width, c.height = height;
+	c.bin_count_x = (width + kBinSize - 1) / kBinSize;
+	c.bin_count_y = (height + kBinSize - 1) / kBinSize;
+	c.bin_count = c.bin_count_x * c.bin_count_y;
 	c.num_quads = (int)nq;
 	c.num_instances = ninst;
 	c.num_bin_wgs = kNumBinWgs;
-	c.opts = r->opts;
+	c.opts = opts;
 	c.backface = cfg.enable_backface_culling;
-	c.presort_min = (uint32_t)r->tuning.presort_min;
-	c.samples_per_wave = (uint32_t)r->tuning.samples_per_wave;
-	c.slice_words = r->tuning.slice_words;
+	c.presort_min = (uint32_t)tuning.presort_min;
+	c.samples_per_wave = (uint32_t)tuning.samples_per_wave;
+	c.slice_words = tuning.slice_words;
 	c.color_mode = color_mode;
 	c.has_normals = has_normals ? 1 : 0;
 	for(int col = 0; col < 4; col++) {
@@ -387,18 +394,14 @@ void lucid_destroy(lucid_renderer *r) {
 
 namespace {
 
-// Validates the frame inputs, uploads instance data when it changed, grows capacities and
-// fills the per-frame constants and buffer table (shared by the forward and backward).
-int prepare_frame(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_frame_outputs *out,
-				  FrameConst &c, FrameBufs &b, int64_t &nq_out) {
+// Checks the instance table and input arrays of a frame (uploadInstances,
+// src/lucid_renderer.cpp:352-429): writes the exclusive prefix of the instances' quad counts
+// (num_instances + 1 entries), the frame's quad count and its colour mode.
+int validate_inputs(const lucid_frame_inputs *in, int *prefix, int64_t &nq, int &color_mode) {
 	if(!in->config || in->num_instances < 0 || in->num_instances > kMaxInstances ||
 	   (in->num_instances > 0 && !in->instances))
 		return LUCID_ERR_INVALID_ARG;
-	LUCID_CHECK(hipSetDevice(r->device));
-
-	// --- instances (uploadInstances, src/lucid_renderer.cpp:352-429)
-	std::vector<int> prefix(in->num_instances + 1, 0);
-	int64_t nq = 0;
+	nq = 0;
 	bool has_colors_flag = false;
 	for(int i = 0; i < in->num_instances; i++) {
 		const lucid_instance_data &ins = in->instances[i];
@@ -416,25 +419,32 @@ int prepare_frame(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_f
 	prefix[in->num_instances] = (int)nq;
 	if(nq > 0 && (!in->quad_indices || !in->positions))
 		return LUCID_ERR_INVALID_ARG;
-	int color_mode = kColorNone;
+	color_mode = kColorNone;
 	if(in->colors_rgba8)
 		color_mode = kColorRGBA8;
 	else if(in->colors_rgb)
 		color_mode = kColorF32;
 	if(has_colors_flag && color_mode == kColorNone)
 		return LUCID_ERR_INVALID_ARG;
+	return LUCID_OK;
+}
 
-	// A finished earlier frame may have reported an overflow: grow before this one.
-	if(r->frame_pending && hipEventQuery(r->done) == hipSuccess) {
-		r->frame_pending = false;
-		int g = grow_after_overflow(r);
-		if(g != LUCID_OK && g != LUCID_ERR_CAPACITY)
-			return g;
-		memset(r->status, 0, kCtrlWords * 4);
-	}
-	int e = ensure_quad_capacity(r, nq);
-	if(e != LUCID_OK)
-		return e;
+// Validates the frame inputs, uploads instance data when it changed, grows capacities and
+// fills the per-frame constants and buffer table (shared by the forward and backward).
+int prepare_frame(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_frame_outputs *out,
+				  hipStream_t stream, FrameConst &c, FrameBufs &b, int64_t &nq_out) {
+	if(!in->config || in->num_instances < 0 || in->num_instances > kMaxInstances ||
+	   (in->num_instances > 0 && !in->instances))
+		return LUCID_ERR_INVALID_ARG;
+	LUCID_CHECK(hipSetDevice(r->device));
+
+	// --- instances (uploadInstances, src/lucid_renderer.cpp:352-429)
+	std::vector<int> prefix(in->num_instances + 1, 0);
+	int64_t nq = 0;
+	int color_mode = kColorNone;
+	int ve = validate_inputs(in, prefix.data(), nq, color_mode);
+	if(ve != LUCID_OK)
+		return ve;
 
 	bool inst_changed = r->host_instances.size() != (size_t)in->num_instances ||
 						(in->num_instances &&
@@ -446,6 +456,30 @@ int prepare_frame(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_f
 						   (r->host_inst_colors.size() != (size_t)in->num_instances ||
 							memcmp(r->host_inst_colors.data(), in->instance_colors,
 								   4 * (size_t)in->num_instances) != 0));
+	// An earlier frame that has not been waited for shares every buffer with this one. When
+	// it has finished (or must finish first, because it reads the instance table replaced or
+	// the buffers reallocated below), its overflow is taken now: the buffers grow before this
+	// frame, and the next lucid_finish reports it. While it still runs, this frame is queued
+	// behind it on `stream`, and its overflow bits reach the host with this frame's status.
+	if(r->frame_pending) {
+		if(inst_changed || colors_changed || nq > r->quad_cap)
+			LUCID_CHECK(hipEventSynchronize(r->done));
+		if(hipEventQuery(r->done) == hipSuccess) {
+			r->frame_pending = false;
+			int g = grow_after_overflow(r);
+			if(g != LUCID_OK && g != LUCID_ERR_CAPACITY)
+				return g;
+			if(g == LUCID_ERR_CAPACITY)
+				r->unreported_overflow = true;
+			memset(r->status, 0, kCtrlWords * 4);
+		} else if(stream != r->last_stream) {
+			LUCID_CHECK(hipStreamWaitEvent(stream, r->done, 0));
+		}
+	}
+	int e = ensure_quad_capacity(r, nq);
+	if(e != LUCID_OK)
+		return e;
+
 	if(inst_changed || colors_changed) {
 		size_t n = std::max(1, in->num_instances);
 		LUCID_CHECK(r->instances.reserve(n * sizeof(lucid_instance_data)));
@@ -467,7 +501,9 @@ int prepare_frame(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_f
 			r->host_inst_colors.assign(in->instance_colors, in->instance_colors + in->num_instances);
 	}
 
-	fill_const(r, *in->config, nq, in->num_instances, color_mode, in->normals != nullptr, c);
+	fill_const(r->width, r->height, r->opts, r->tuning, *in->config, nq, in->num_instances,
+			   color_mode, in->normals != nullptr, c);
+	c.keep_overflow = r->frame_pending ? 1u : 0u;
 	memset(&b, 0, sizeof(b));
 	b.instances = r->instances.as<lucid_instance_data>();
 	b.inst_prefix = r->inst_prefix.as<int>();
@@ -630,7 +666,7 @@ int lucid_render_ex(lucid_renderer *r, const lucid_frame_inputs *in,
 	FrameConst c;
 	FrameBufs b;
 	int64_t nq = 0;
-	int pe = prepare_frame(r, in, out, c, b, nq);
+	int pe = prepare_frame(r, in, out, stream, c, b, nq);
 	if(pe != LUCID_OK)
 		return pe;
 	BwdBufs g;
@@ -695,7 +731,7 @@ int lucid_backward_ex(lucid_renderer *r, const lucid_frame_inputs *in, const flo
 	FrameConst c;
 	FrameBufs b;
 	int64_t nq = 0;
-	int pe = prepare_frame(r, in, nullptr, c, b, nq);
+	int pe = prepare_frame(r, in, nullptr, stream, c, b, nq);
 	if(pe != LUCID_OK)
 		return pe;
 	const uint64_t ntri = 2 * (uint64_t)nq;
@@ -866,20 +902,28 @@ int lucid_autotune(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_
 		for(int pm : presort)
 			for(int sp : spw) {
 				r->tuning = {sw, pm, sp};
-				int e = LUCID_OK;
-				for(int k = 0; k <= iters && e == LUCID_OK; k++) { // first frame untimed
-					if(k == 1)
-						e = hip_status(hipEventRecord(e0, stream));
-					if(e == LUCID_OK)
-						e = lucid_render(r, in, out, stream);
+				int e = LUCID_OK, f = LUCID_OK;
+				// A candidate that overflows a buffer (e.g. a larger pre-sorted pool) grows it
+				// on its untimed first frame and is timed again with the grown buffers.
+				for(int attempt = 0; attempt < 3; attempt++) {
+					e = lucid_render(r, in, out, stream); // untimed: settles this candidate's sizes
+					f = e == LUCID_OK ? lucid_finish(r) : LUCID_OK;
+					if(e != LUCID_OK || f != LUCID_ERR_CAPACITY)
+						break;
 				}
-				if(e == LUCID_OK)
+				if(e == LUCID_OK && f == LUCID_OK)
+					e = hip_status(hipEventRecord(e0, stream));
+				for(int k = 0; k < iters && e == LUCID_OK && f == LUCID_OK; k++)
+					e = lucid_render(r, in, out, stream);
+				if(e == LUCID_OK && f == LUCID_OK)
 					e = hip_status(hipEventRecord(e1, stream));
-				int f = lucid_finish(r);
+				int f2 = lucid_finish(r);
+				if(f == LUCID_OK)
+					f = f2;
 				if(e != LUCID_OK || f != LUCID_OK) {
 					rc = e != LUCID_OK ? e : f;
 					if(rc == LUCID_ERR_CAPACITY)
-						continue; // buffers grew: this candidate is skipped, the others still run
+						continue; // still overflowing after growth: skipped, the others still run
 					break;
 				}
 				float ms = 0.0f;
@@ -1041,7 +1085,7 @@ int lucid_batch_render(lucid_batch *bt, const lucid_frame_inputs *ins,
 				return f;
 		}
 		int64_t nq = 0;
-		int e = prepare_frame(r, &ins[i], &outs[i], cs[i], bs[i], nq);
+		int e = prepare_frame(r, &ins[i], &outs[i], stream, cs[i], bs[i], nq);
 		if(e != LUCID_OK)
 			return e;
 		r->last_num_quads = nq;
@@ -1094,6 +1138,9 @@ int lucid_finish(lucid_renderer *r) {
 	const bool recorded = r->status[kCtrlRecorded] != 0;
 	const uint64_t block_rows = r->status[kCtrlBwdRows], ranges = r->status[kCtrlBwdRanges];
 	int g = grow_after_overflow(r);
+	if(r->unreported_overflow && g == LUCID_OK)
+		g = LUCID_ERR_CAPACITY;
+	r->unreported_overflow = false;
 	memset(r->status, 0, kCtrlWords * 4);
 	if(r->retained.pending) {
 		r->retained.pending = false;

@@ -311,19 +311,26 @@ __global__ __launch_bounds__(kSetupThreads) void setup_kernel(FrameConst c, Fram
 // Per-frame reset of LucidInfo + per-bin counters and the control words
 // (src/lucid_renderer.cpp:437 clears the same region with a fill).
 __global__ __launch_bounds__(256) void clear_kernel(uint32_t *info, uint32_t n_info, uint32_t *ctrl,
-													uint32_t num_quads) {
+													uint32_t num_quads, uint32_t keep_overflow) {
 	uint32_t i = blockIdx.x * 256u + threadIdx.x;
 	if(i < n_info)
 		info[i] = i == 0 ? num_quads : 0u;
-	if(i < (uint32_t)kCtrlAllocWords)
-		ctrl[i] = 0u; // control words and statistics stripes
+	// control words and statistics stripes; the overflow bits of an unfinished earlier
+	// frame move to the carry word (which no kernel reads), so that they reach the host
+	// with this frame's status instead of being lost when its status copy replaces theirs
+	if(i == (uint32_t)kCtrlOverflow) {
+		ctrl[kCtrlOverflowCarry] = keep_overflow ? ctrl[kCtrlOverflowCarry] | ctrl[kCtrlOverflow] : 0u;
+		ctrl[kCtrlOverflow] = 0u;
+	} else if(i < (uint32_t)kCtrlAllocWords && i != (uint32_t)kCtrlOverflowCarry) {
+		ctrl[i] = 0u;
+	}
 }
 
 hipError_t launch_setup(const FrameConst &c, const FrameBufs &b, uint32_t n_info,
 						hipStream_t stream) {
 	uint32_t n = n_info > (uint32_t)kCtrlAllocWords ? n_info : (uint32_t)kCtrlAllocWords;
 	hipLaunchKernelGGL(clear_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, b.info, n_info,
-					   b.ctrl, (uint32_t)c.num_quads);
+					   b.ctrl, (uint32_t)c.num_quads, c.keep_overflow);
 	if(c.num_quads > 0)
 		hipLaunchKernelGGL(setup_kernel, dim3((c.num_quads + kSetupThreads - 1) / kSetupThreads),
 						   dim3(kSetupThreads), 0, stream, c, b);

@@ -2,17 +2,22 @@
 
 The reference builds ``LucidConfig`` from libfwk's Camera/Frustum
 (src/shading.cpp:46-74, src/lucid_renderer.cpp:431-451); libfwk is not
-available, so the camera model is fixed here and the same config bytes feed
+available, so the camera model is restated here and the same config bytes feed
 both the HIP path and the oracle:
 
-* world space == view space: the camera sits at the origin, looks down -Z,
-  +Y is up;
-* symmetric perspective with 90 degree vertical FOV, depth range
-  [1/16, 1024] (src/lucid_app.cpp:95), GL-style clip z in [-w, w] and a
-  flipped y row so that image row 0 is NDC y = -1 (top of the image);
-* corner rays are unnormalised with z = -1, so a plane's ``inv_ray_pos``
-  equals 1 / view depth; ``ws_dirx``/``ws_diry`` step one pixel
-  (src/shading.cpp:59-60);
+* a camera is a pose (world position, and a rotation whose columns are the
+  view x, y and z axes in world space; it looks down its -z axis, +y is up)
+  and a symmetric perspective projection; the view matrix is the inverse pose
+  and ``view_proj_matrix`` = projection x view (src/lucid_renderer.cpp:440-441);
+* the frustum's corner rays start at the camera position (``ws_origins`` and
+  ``ws_origin0``) and point along the rotated view-space corner directions,
+  unnormalised with view z = -1, so a plane's ``inv_ray_pos`` equals 1 / view
+  depth; ``ws_dirx``/``ws_diry`` step one pixel (src/shading.cpp:46-61, with
+  ``inverseOrZero(viewMatrix())`` applied as mulPoint/mulNormal);
+* the default camera sits at the origin looking down -Z (world == view space),
+  90 degree vertical FOV, depth range [1/16, 1024] (src/lucid_app.cpp:95),
+  GL-style clip z in [-w, w] and a flipped y row so that image row 0 is NDC
+  y = -1 (top of the image);
 * identity lighting (ambient 1, sun power 0) so finalShading reduces to an
   sRGB round trip (shared/funcs.glsl:261-271).
 
@@ -55,6 +60,9 @@ class Camera:
     fov_y_deg: float = 90.0
     near: float = NEAR
     far: float = FAR
+    # pose: world position, and world-from-view rotation (columns: view x, y, z axes)
+    position: tuple = (0.0, 0.0, 0.0)
+    rotation: Optional[np.ndarray] = None
 
     @property
     def aspect(self) -> float:
@@ -64,27 +72,55 @@ class Camera:
     def focal(self) -> float:
         return 1.0 / math.tan(math.radians(self.fov_y_deg) * 0.5)
 
-    def view_proj(self) -> np.ndarray:
-        """Column-major 4x4 (cols[c][r]) projection with flipped y."""
+    @property
+    def rot(self) -> np.ndarray:
+        """World-from-view rotation (float64 3x3)."""
+        return np.eye(3) if self.rotation is None else np.asarray(self.rotation, np.float64)
+
+    @property
+    def eye(self) -> np.ndarray:
+        return np.asarray(self.position, np.float64)
+
+    def projection(self) -> np.ndarray:
+        """Row-major float64 4x4 perspective projection with flipped y (view -> clip)."""
         f, a, n, fa = self.focal, self.aspect, self.near, self.far
-        m = np.zeros((4, 4), np.float32)  # m[col][row]
-        m[0][0] = f / a
-        m[1][1] = -f
-        m[2][2] = (fa + n) / (n - fa)
-        m[2][3] = -1.0
-        m[3][2] = 2.0 * fa * n / (n - fa)
+        m = np.zeros((4, 4))
+        m[0, 0] = f / a
+        m[1, 1] = -f
+        m[2, 2] = (fa + n) / (n - fa)
+        m[2, 3] = 2.0 * fa * n / (n - fa)
+        m[3, 2] = -1.0
         return m
 
+    def view_matrix(self) -> np.ndarray:
+        """Row-major float64 4x4 world -> view transform (the inverse pose)."""
+        m = np.eye(4)
+        rt = self.rot.T
+        m[:3, :3] = rt
+        m[:3, 3] = -rt @ self.eye
+        return m
+
+    def view_proj(self) -> np.ndarray:
+        """Column-major 4x4 float32 (cols[c][r]) of projection x view, as LucidConfig holds
+        view_proj_matrix."""
+        return (self.projection() @ self.view_matrix()).T.astype(np.float32)
+
+    def world_to_view(self, pts: np.ndarray) -> np.ndarray:
+        return (np.asarray(pts, np.float64) - self.eye) @ self.rot
+
+    def view_to_world(self, pts: np.ndarray) -> np.ndarray:
+        return np.asarray(pts, np.float64) @ self.rot.T + self.eye
+
     def ndc_to_view(self, ndc_xy: np.ndarray, depth: np.ndarray) -> np.ndarray:
-        """Points at view depth `depth` (positive) that project to NDC `ndc_xy`."""
+        """View-space points at view depth `depth` (positive) that project to NDC `ndc_xy`."""
         f, a = self.focal, self.aspect
         d = np.asarray(depth, np.float64)[..., None]
         x = ndc_xy[..., 0:1] * (a / f) * d
         y = -ndc_xy[..., 1:2] / f * d
         return np.concatenate([x, y, -d], axis=-1)
 
-    def corner_dirs(self) -> np.ndarray:
-        """Ray directions at screen corners (x, y) = (0,0), (0,H), (W,H), (W,0)."""
+    def view_corner_dirs(self) -> np.ndarray:
+        """View-space ray directions at screen corners (x, y) = (0,0), (0,H), (W,H), (W,0)."""
         f, a = np.float32(self.focal), np.float32(self.aspect)
         af = np.float32(a / f)
         inv_f = np.float32(1.0) / f
@@ -92,6 +128,32 @@ class Camera:
             [[-af, inv_f, -1.0], [-af, -inv_f, -1.0], [af, -inv_f, -1.0], [af, inv_f, -1.0]],
             np.float32,
         )
+
+    def corner_dirs(self) -> np.ndarray:
+        """World-space corner ray directions (float32): the view-space ones rotated by the
+        pose (mulNormal of the inverse view matrix, src/shading.cpp:53-56)."""
+        d = self.view_corner_dirs()
+        if self.rotation is None:
+            return d
+        return (d.astype(np.float64) @ self.rot.T).astype(np.float32)
+
+    def pixel_rays(self) -> np.ndarray:
+        """float64 view-space directions (view z = -1) through every pixel centre, (H, W, 3)."""
+        h, w = self.height, self.width
+        ys, xs = np.mgrid[0:h, 0:w]
+        ndc_x = (xs + 0.5) * (2.0 / w) - 1.0
+        ndc_y = (ys + 0.5) * (2.0 / h) - 1.0
+        return self.ndc_to_view(np.stack([ndc_x, ndc_y], -1), np.ones((h, w)))
+
+
+def look_at(eye, target, up=(0.0, 1.0, 0.0)) -> np.ndarray:
+    """World-from-view rotation of a camera at `eye` looking at `target`."""
+    eye, target, up = (np.asarray(v, np.float64) for v in (eye, target, up))
+    back = eye - target
+    back /= np.linalg.norm(back)
+    right = np.cross(up, back)
+    right /= np.linalg.norm(right)
+    return np.stack([right, np.cross(back, right), back], axis=1)
 
 
 def make_config(
@@ -112,11 +174,12 @@ def make_config(
     u32 = buf.view(np.uint32)
     i32 = buf.view(np.int32)
     dirs = camera.corner_dirs()
+    eye = camera.eye.astype(np.float32)
     # Frustum: ws_origins[4] at 0, ws_dirs[4] at 16, origin0 at 32, dir0 at 36, dirx 40, diry 44
     for i in range(4):
-        buf[4 * i : 4 * i + 4] = (0.0, 0.0, 0.0, 0.0)
+        buf[4 * i : 4 * i + 4] = (*eye, 0.0)
         buf[16 + 4 * i : 16 + 4 * i + 3] = dirs[i]
-    buf[32:36] = (0.0, 0.0, 0.0, 1.0)
+    buf[32:36] = (*eye, 1.0)
     buf[36:39] = dirs[0]
     dirx = (dirs[3] - dirs[0]) * (np.float32(1.0) / np.float32(camera.width))
     diry = (dirs[1] - dirs[0]) * (np.float32(1.0) / np.float32(camera.height))
@@ -152,6 +215,7 @@ class Scene:
     normals: Optional[np.ndarray] = None  # (V,) uint32 packed 10:10:10
     instance_colors: Optional[np.ndarray] = None  # (N,) uint32
     name: str = ""
+    camera: Optional[Camera] = None  # the camera `config` was made from (None: the default)
 
     @property
     def num_quads(self) -> int:
@@ -177,6 +241,24 @@ class Scene:
                 h.update(np.ascontiguousarray(arr).tobytes())
         h.update(self.config)
         return h.hexdigest()
+
+
+def scene_camera(scene: "Scene") -> Camera:
+    return scene.camera if scene.camera is not None else Camera(scene.width, scene.height)
+
+
+def posed(scene: Scene, position, rotation, *, backface_culling: bool = False,
+          name: Optional[str] = None) -> Scene:
+    """The scene moved rigidly so that a camera at `position` with world-from-view
+    `rotation` sees what the default camera saw (positions were view-space points of the
+    default camera), with that camera's LucidConfig. Only float rounding tells the two
+    frames apart."""
+    cam = Camera(scene.width, scene.height, position=tuple(float(v) for v in position),
+                 rotation=np.asarray(rotation, np.float64))
+    pos = cam.view_to_world(scene.positions.astype(np.float64)).astype(np.float32)
+    cfg = make_config(cam, len(scene.instances), backface_culling=backface_culling)
+    return dataclasses.replace(scene, positions=pos, config=cfg, camera=cam,
+                               name=name or f"posed_{scene.name}")
 
 
 def split_instances(num_quads: int, flags: int, quad_offset: int = 0,

@@ -1085,7 +1085,9 @@ static int raster_bin(const frame *f, oracle_outputs *out, int bin, int level, b
 				continue;
 			frag_t *lst = sorted + start[p];
 			qsort(lst, counts[p], sizeof(frag_t), frag_cmp);
-			if(out->frags) {
+			const int dump_all = out->dump_x1 <= out->dump_x0;
+			if(out->frags && (dump_all || (px >= out->dump_x0 && px < out->dump_x1 &&
+										   py >= out->dump_y0 && py < out->dump_y1))) {
 				size_t at;
 #ifdef _OPENMP
 #pragma omp atomic capture

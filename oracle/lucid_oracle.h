@@ -73,6 +73,9 @@ typedef struct oracle_outputs {
 	oracle_fragment *frags;  /* exact mode: every sample, may be NULL */
 	size_t frags_cap;
 	size_t num_frags_dumped; /* set by oracle_render (may exceed frags_cap) */
+	/* pixels whose samples frags receives: x in [dump_x0, dump_x1), y in [dump_y0, dump_y1);
+	 * all pixels when dump_x1 <= dump_x0 */
+	int dump_x0, dump_y0, dump_x1, dump_y1;
 } oracle_outputs;
 
 /* Runs the whole frame. Returns 0 on success, negative on invalid input or capacity. */

@@ -47,6 +47,8 @@ class _Outputs(ctypes.Structure):
         ("num_fragments", ctypes.c_uint64), ("num_ref_errors", ctypes.c_uint32),
         ("frags", ctypes.c_void_p), ("frags_cap", ctypes.c_size_t),
         ("num_frags_dumped", ctypes.c_size_t),
+        ("dump_x0", ctypes.c_int), ("dump_y0", ctypes.c_int),
+        ("dump_x1", ctypes.c_int), ("dump_y1", ctypes.c_int),
     ]
 
 
@@ -96,10 +98,12 @@ def bin_count(width: int, height: int) -> int:
 
 
 def render(scene, mode: int = MODE_EXACT, opts: int = 0, bin_capacity: Optional[int] = None,
-           want_records: bool = True, want_fragments: bool = False) -> dict:
+           want_records: bool = True, want_fragments: bool = False,
+           fragment_rect: Optional[tuple] = None) -> dict:
     """Runs the oracle on a lucid_amd.scenes.Scene; returns numpy outputs. With
     want_fragments (exact mode), res["fragments"] lists every sample (FRAGMENT_DTYPE),
-    sorted by pixel (row-major) and blend order."""
+    sorted by pixel (row-major) and blend order; fragment_rect = (x0, y0, x1, y1) keeps only
+    the samples of pixels in [x0, x1) x [y0, y1)."""
     L = lib()
     w, h = scene.width, scene.height
     nb = bin_count(w, h)
@@ -147,14 +151,18 @@ def render(scene, mode: int = MODE_EXACT, opts: int = 0, bin_capacity: Optional[
         out_np["bin_quads"].ctypes.data, out_np["bin_tris"].ctypes.data, bin_capacity,
         out_np["image"].ctypes.data, out_np["image_f"].ctypes.data,
         out_np["frag_counts"].ctypes.data, out_np["bin_levels"].ctypes.data, 0, 0,
-        None, 0, 0,
+        None, 0, 0, *(fragment_rect if fragment_rect is not None else (0, 0, 0, 0)),
     )
     rc = L.oracle_render(ctypes.byref(inp), ctypes.byref(out))
     if rc != 0:
         raise RuntimeError(f"oracle_render failed: {rc}")
     if want_fragments:
-        # second run with a buffer sized from the first run's fragment count
-        n = int(out.num_fragments)
+        # second run with a buffer sized from the first run's fragment counts
+        if fragment_rect is None:
+            n = int(out.num_fragments)
+        else:
+            x0, y0, x1, y1 = fragment_rect
+            n = int(out_np["frag_counts"][y0:y1, x0:x1].sum())
         frags = np.zeros(max(n, 1), FRAGMENT_DTYPE)
         out.frags, out.frags_cap, out.num_frags_dumped = frags.ctypes.data, n, 0
         rc = L.oracle_render(ctypes.byref(inp), ctypes.byref(out))

@@ -80,8 +80,8 @@ def test_conversions_saturate(oracle_mod):
 def _pixel_centre_coverage(scene):
     """float64 brute force: pixel centres (x + 0.5, y + 0.5) inside each visible triangle,
     plus the distance of the centre to the nearest edge (in pixels)."""
-    cam = scenes.Camera(scene.width, scene.height)
-    vp = cam.view_proj().astype(np.float64)
+    cam = scenes.scene_camera(scene)
+    vp = cam.projection() @ cam.view_matrix()  # float64, row-major
     P = scene.positions.astype(np.float64)
     q = scene.quad_indices
     h, w = scene.height, scene.width
@@ -98,7 +98,7 @@ def _pixel_centre_coverage(scene):
         pts = []
         for v in t:
             p = np.append(P[v], 1.0)
-            clip = vp.T @ p  # vp is column-major: vp[col][row]
+            clip = vp @ p
             ndc = clip[:2] / clip[3]
             pts.append(((ndc[0] + 1) * w / 2, (ndc[1] + 1) * h / 2))
         (x0, y0), (x1, y1), (x2, y2) = pts
@@ -244,11 +244,11 @@ def test_lit_planes_closed_form(oracle_mod):
 
 def _brute_force_image(scene):
     """float64 per-pixel sort + front-to-back blend with unquantised colours."""
-    cam = scenes.Camera(scene.width, scene.height)
+    cam = scenes.scene_camera(scene)
     h, w = scene.height, scene.width
     ys, xs = np.mgrid[0:h, 0:w]
     cx, cy = xs + 0.5, ys + 0.5
-    P = scene.positions.astype(np.float64)
+    P = cam.world_to_view(scene.positions.astype(np.float64))
     f, a = cam.focal, cam.aspect
     frags = [[] for _ in range(h * w)]
     for qi, quad in enumerate(scene.quad_indices):
