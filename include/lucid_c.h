@@ -37,7 +37,7 @@ extern "C" {
 #define LUCID_ERR_HIP (-3)
 #define LUCID_ERR_NO_DEVICE (-4)
 #define LUCID_ERR_CAPACITY (-5)  /* a frame exceeded an internal buffer; buffers were grown */
-#define LUCID_ERR_UNSUPPORTED (-6)
+#define LUCID_ERR_UNSUPPORTED (-6) /* e.g. the backward of a textured frame */
 
 /* LucidRenderOpt bits (src/lucid_renderer.h:10-11). Debug/timer options are accepted
  * and ignored; rocprof replaces them. LUCID_OPT_ALPHA_THRESHOLD ("over" blending only):
@@ -84,6 +84,17 @@ typedef struct lucid_create_info {
 	int64_t max_quads;           /* initial quad capacity (0: 1M); grows on demand */
 } lucid_create_info;
 
+/* An RGBA8 texture atlas with its mip chain (the reference's opaque and transparent atlases,
+ * Scene::textureAtlasPair, src/scene.cpp:514, bound at shading.glsl:27-28). Level 0 is
+ * width x height texels, row-major, row 0 = v 0; level l (l < num_levels <= 16) follows the
+ * previous one and is max(1, width >> l) x max(1, height >> l). Sampled as the reference's
+ * sampler is set up (src/lucid_base.h:30: linear magnification, minification and mip
+ * filtering, repeat addressing) with isotropic LOD selection. */
+typedef struct lucid_texture {
+	const uint32_t *texels; /* device; NULL = no texture */
+	int width, height, num_levels;
+} lucid_texture;
+
 typedef struct lucid_frame_inputs {
 	/* host arrays (uploaded when they change) */
 	const lucid_instance_data *instances; /* InstanceData[num_instances] */
@@ -98,6 +109,14 @@ typedef struct lucid_frame_inputs {
 	const float *colors_rgb;      /* float3 per vertex, or NULL (used when colors_rgba8 is NULL) */
 	const float *tri_alpha;       /* alpha per triangle id (2 per quad), with colors_rgb */
 	const uint32_t *normals;      /* packed 10:10:10 normal per vertex, or NULL */
+	/* albedo textures (shading.glsl:125-160), for instances with LUCID_INST_HAS_ALBEDO_TEXTURE:
+	 * per-vertex texture coordinates, per-instance uv rectangles (min.xy, size.xy within the
+	 * atlas; used with LUCID_INST_HAS_UV_RECT, uploadInstances, src/lucid_renderer.cpp:385)
+	 * and the atlases (LUCID_INST_TEX_OPAQUE picks the opaque one, whose alpha reads as 1) */
+	const float *tex_coords;              /* device, float2 per vertex, or NULL */
+	const lucid_vec4 *instance_uv_rects;  /* host, one per instance, or NULL */
+	lucid_texture opaque_texture;
+	lucid_texture transparent_texture;
 } lucid_frame_inputs;
 
 typedef struct lucid_frame_outputs {

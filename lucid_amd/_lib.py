@@ -63,6 +63,11 @@ class Tuning(ctypes.Structure):
                 "samples_per_wave": self.samples_per_wave}
 
 
+class Texture(ctypes.Structure):
+    _fields_ = [("texels", ctypes.c_void_p), ("width", ctypes.c_int), ("height", ctypes.c_int),
+                ("num_levels", ctypes.c_int)]
+
+
 class FrameInputs(ctypes.Structure):
     _fields_ = [
         ("instances", ctypes.c_void_p), ("num_instances", ctypes.c_int),
@@ -71,6 +76,8 @@ class FrameInputs(ctypes.Structure):
         ("num_verts", ctypes.c_int64),
         ("colors_rgba8", ctypes.c_void_p), ("colors_rgb", ctypes.c_void_p),
         ("tri_alpha", ctypes.c_void_p), ("normals", ctypes.c_void_p),
+        ("tex_coords", ctypes.c_void_p), ("instance_uv_rects", ctypes.c_void_p),
+        ("opaque_texture", Texture), ("transparent_texture", Texture),
     ]
 
 

@@ -155,11 +155,21 @@ struct FrameConst {
 	// moves its overflow bits to kCtrlOverflowCarry instead of dropping them, so that they
 	// are reported with this frame's status
 	uint32_t keep_overflow;
+	int has_textures; // some instance samples an albedo texture (setup stores texture coordinates)
 	float vp[16]; // column-major view-projection matrix
 	float origin[3], dir0[3], dirx[3], diry[3], ray_dir0[3];
 	float ambient[3], sun_color[3], sun_dir[3];
 	float sun_power, ambient_power;
 	float background[4];
+};
+
+// An RGBA8 texture atlas and its mip chain (lucid_texture): texels of level l start at
+// offset[l] and form a max(1, width >> l) x max(1, height >> l) row-major image.
+constexpr int kMaxTexLevels = 16;
+struct TexDesc {
+	const uint32_t *texels;
+	int width, height, levels;
+	uint32_t offset[kMaxTexLevels];
 };
 
 // Device buffers of one frame (passed by value).
@@ -174,6 +184,9 @@ struct FrameBufs {
 	const float *colors_rgb;
 	const float *tri_alpha;
 	const uint32_t *normals;
+	const float *tex_coords;    // float2 per vertex (textured instances)
+	const float4 *inst_uv_rects; // per instance: min.xy, size.xy of the texture in its atlas
+	TexDesc tex[2];              // 0: opaque atlas, 1: transparent atlas
 	// info + control
 	uint32_t *info;   // LucidInfo followed by 10 * bin_count counters
 	uint32_t *ctrl;   // kCtrlWords
@@ -185,6 +198,7 @@ struct FrameBufs {
 	uint4 *tri_scan;      // 2 per tri
 	uint4 *tri_depth;     // 1 per tri
 	uint4 *tri_shade;     // kRecStride per tri: the shading record the rasterisers stage
+	uint4 *tri_tex;       // 2 per tri of a textured instance: tex0.xy, tex1 - tex0 | tex2 - tex0
 	uint32_t *large_list; // quads whose bin AABB covers more than 4 bins (kLargeStripes sub-lists)
 	// binning
 	uint32_t *wg_hist; // num_bin_wgs * 2 * bin_count

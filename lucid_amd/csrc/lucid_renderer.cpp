@@ -98,8 +98,10 @@ struct lucid_renderer {
 	int bin_count_x = 0, bin_count_y = 0, bin_count = 0;
 
 	DevBuf info, ctrl;
-	DevBuf instances, inst_prefix, inst_colors;
+	DevBuf instances, inst_prefix, inst_colors, inst_uv_rects;
 	DevBuf quad_aabb, tri_scan, tri_depth, tri_shade, large_list;
+	DevBuf tri_tex;            // texture coordinates per triangle, once a frame is textured
+	bool tex_enabled = false;
 	DevBuf wg_hist, bin_quads, bin_tris;
 	DevBuf low_records, high_records, fallback_queue, spill;
 	DevBuf blk_slot, blk_flags, blk_sorted, blk_lists, blk_count, band_count, sorted_pool;
@@ -137,6 +139,8 @@ struct lucid_renderer {
 	std::vector<lucid_instance_data> host_instances;
 	std::vector<uint32_t> host_inst_colors;
 	bool inst_colors_null = true;
+	std::vector<lucid_vec4> host_uv_rects;
+	bool uv_rects_null = true;
 
 	uint32_t *status = nullptr; // pinned copy of the control words of the last frame
 	hipEvent_t done = nullptr;
@@ -196,6 +200,8 @@ int ensure_quad_capacity(lucid_renderer *r, int64_t nq) {
 	LUCID_CHECK(r->tri_scan.reserve(tris * 32));
 	LUCID_CHECK(r->tri_depth.reserve(tris * 16));
 	LUCID_CHECK(r->tri_shade.reserve(tris * kRecStride * 16));
+	if(r->tex_enabled)
+		LUCID_CHECK(r->tri_tex.reserve(tris * 32));
 	LUCID_CHECK(r->large_list.reserve((size_t)kLargeStripes * large_stripe_cap((uint32_t)cap) * 4));
 	// small quads cover at most 4 bins: 4 entries per quad is an exact bound
 	uint64_t bq = (uint64_t)cap * 4;
@@ -312,7 +318,7 @@ const char *lucid_error_string(int status) {
 	case LUCID_ERR_HIP: return "HIP runtime error";
 	case LUCID_ERR_NO_DEVICE: return "no HIP device";
 	case LUCID_ERR_CAPACITY: return "internal capacity exceeded; buffers grown, render again";
-	case LUCID_ERR_UNSUPPORTED: return "unsupported input (albedo textures are not implemented)";
+	case LUCID_ERR_UNSUPPORTED: return "unsupported input for this entry point";
 	default: return "unknown error";
 	}
 }
@@ -397,18 +403,62 @@ namespace {
 // Checks the instance table and input arrays of a frame (uploadInstances,
 // src/lucid_renderer.cpp:352-429): writes the exclusive prefix of the instances' quad counts
 // (num_instances + 1 entries), the frame's quad count and its colour mode.
-int validate_inputs(const lucid_frame_inputs *in, int *prefix, int64_t &nq, int &color_mode) {
+// A texture atlas descriptor is usable: texels, a positive size, 1 to 16 levels, no level
+// below 1 x 1 before the last.
+bool texture_valid(const lucid_texture &t) {
+	if(!t.texels || t.width <= 0 || t.height <= 0 || t.num_levels < 1 ||
+	   t.num_levels > kMaxTexLevels)
+		return false;
+	const int top = std::max(t.width, t.height);
+	int levels = 1;
+	while(levels < kMaxTexLevels && (top >> levels) > 0)
+		levels++;
+	return t.num_levels <= levels;
+}
+
+// Some instance of the frame samples an albedo texture.
+bool any_textured(const lucid_frame_inputs *in) {
+	for(int i = 0; i < in->num_instances && in->instances; i++)
+		if(in->instances[i].flags & LUCID_INST_HAS_ALBEDO_TEXTURE)
+			return true;
+	return false;
+}
+
+// Descriptor of an atlas for the kernels: the first texel of each mip level.
+TexDesc tex_desc(const lucid_texture &t) {
+	TexDesc d;
+	memset(&d, 0, sizeof(d));
+	if(!texture_valid(t))
+		return d;
+	d.texels = t.texels, d.width = t.width, d.height = t.height, d.levels = t.num_levels;
+	uint32_t off = 0;
+	for(int l = 0; l < t.num_levels; l++) {
+		d.offset[l] = off;
+		off += (uint32_t)std::max(1, t.width >> l) * (uint32_t)std::max(1, t.height >> l);
+	}
+	return d;
+}
+
+int validate_inputs(const lucid_frame_inputs *in, int *prefix, int64_t &nq, int &color_mode,
+					bool &textured) {
 	if(!in->config || in->num_instances < 0 || in->num_instances > kMaxInstances ||
 	   (in->num_instances > 0 && !in->instances))
 		return LUCID_ERR_INVALID_ARG;
 	nq = 0;
+	textured = false;
 	bool has_colors_flag = false;
 	for(int i = 0; i < in->num_instances; i++) {
 		const lucid_instance_data &ins = in->instances[i];
 		if(ins.num_quads < 0 || ins.index_offset < 0)
 			return LUCID_ERR_INVALID_ARG;
-		if(ins.flags & LUCID_INST_HAS_ALBEDO_TEXTURE)
-			return LUCID_ERR_UNSUPPORTED;
+		if(ins.flags & LUCID_INST_HAS_ALBEDO_TEXTURE) {
+			const lucid_texture &t = (ins.flags & LUCID_INST_TEX_OPAQUE) ? in->opaque_texture
+																		  : in->transparent_texture;
+			if(!in->tex_coords || !texture_valid(t) ||
+			   ((ins.flags & LUCID_INST_HAS_UV_RECT) && !in->instance_uv_rects))
+				return LUCID_ERR_INVALID_ARG;
+			textured = true;
+		}
 		if(ins.flags & LUCID_INST_HAS_VERTEX_COLORS)
 			has_colors_flag = true;
 		prefix[i] = (int)nq;
@@ -442,7 +492,8 @@ int prepare_frame(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_f
 	std::vector<int> prefix(in->num_instances + 1, 0);
 	int64_t nq = 0;
 	int color_mode = kColorNone;
-	int ve = validate_inputs(in, prefix.data(), nq, color_mode);
+	bool textured = false;
+	int ve = validate_inputs(in, prefix.data(), nq, color_mode, textured);
 	if(ve != LUCID_OK)
 		return ve;
 
@@ -456,13 +507,20 @@ int prepare_frame(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_f
 						   (r->host_inst_colors.size() != (size_t)in->num_instances ||
 							memcmp(r->host_inst_colors.data(), in->instance_colors,
 								   4 * (size_t)in->num_instances) != 0));
+	const bool rects_null = in->instance_uv_rects == nullptr;
+	const bool rects_changed = rects_null != r->uv_rects_null ||
+							   (!rects_null &&
+								(r->host_uv_rects.size() != (size_t)in->num_instances ||
+								 memcmp(r->host_uv_rects.data(), in->instance_uv_rects,
+										sizeof(lucid_vec4) * (size_t)in->num_instances) != 0));
+	const bool tex_alloc = textured && !r->tex_enabled;
 	// An earlier frame that has not been waited for shares every buffer with this one. When
 	// it has finished (or must finish first, because it reads the instance table replaced or
 	// the buffers reallocated below), its overflow is taken now: the buffers grow before this
 	// frame, and the next lucid_finish reports it. While it still runs, this frame is queued
 	// behind it on `stream`, and its overflow bits reach the host with this frame's status.
 	if(r->frame_pending) {
-		if(inst_changed || colors_changed || nq > r->quad_cap)
+		if(inst_changed || colors_changed || rects_changed || tex_alloc || nq > r->quad_cap)
 			LUCID_CHECK(hipEventSynchronize(r->done));
 		if(hipEventQuery(r->done) == hipSuccess) {
 			r->frame_pending = false;
@@ -479,6 +537,23 @@ int prepare_frame(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_f
 	int e = ensure_quad_capacity(r, nq);
 	if(e != LUCID_OK)
 		return e;
+	if(tex_alloc) {
+		r->tex_enabled = true;
+		LUCID_CHECK(r->tri_tex.reserve((size_t)r->quad_cap * 2 * 32));
+	}
+	if(rects_changed) {
+		if(!rects_null && in->num_instances) {
+			LUCID_CHECK(r->inst_uv_rects.reserve(sizeof(lucid_vec4) * (size_t)in->num_instances));
+			LUCID_CHECK(hipMemcpy(r->inst_uv_rects.ptr, in->instance_uv_rects,
+								  sizeof(lucid_vec4) * (size_t)in->num_instances,
+								  hipMemcpyHostToDevice));
+			r->host_uv_rects.assign(in->instance_uv_rects,
+									in->instance_uv_rects + in->num_instances);
+		} else {
+			r->host_uv_rects.clear();
+		}
+		r->uv_rects_null = rects_null;
+	}
 
 	if(inst_changed || colors_changed) {
 		size_t n = std::max(1, in->num_instances);
@@ -504,6 +579,7 @@ int prepare_frame(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_f
 	fill_const(r->width, r->height, r->opts, r->tuning, *in->config, nq, in->num_instances,
 			   color_mode, in->normals != nullptr, c);
 	c.keep_overflow = r->frame_pending ? 1u : 0u;
+	c.has_textures = textured ? 1 : 0;
 	memset(&b, 0, sizeof(b));
 	b.instances = r->instances.as<lucid_instance_data>();
 	b.inst_prefix = r->inst_prefix.as<int>();
@@ -514,6 +590,11 @@ int prepare_frame(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_f
 	b.colors_rgb = in->colors_rgb;
 	b.tri_alpha = in->tri_alpha;
 	b.normals = in->normals;
+	b.tex_coords = in->tex_coords;
+	b.inst_uv_rects = rects_null ? nullptr : r->inst_uv_rects.as<float4>();
+	b.tex[0] = tex_desc(in->opaque_texture);
+	b.tex[1] = tex_desc(in->transparent_texture);
+	b.tri_tex = r->tri_tex.as<uint4>();
 	b.info = r->info.as<uint32_t>();
 	b.ctrl = r->ctrl.as<uint32_t>();
 	b.tri_half = (uint32_t)r->quad_cap;
@@ -660,7 +741,8 @@ int lucid_render_ex(lucid_renderer *r, const lucid_frame_inputs *in,
 	if(record && (!(r->opts & LUCID_OPT_FP32_SAMPLES) || !in->colors_rgb || !in->tri_alpha ||
 				  in->colors_rgba8 ||
 				  (r->opts & (LUCID_OPT_ADDITIVE_BLENDING | LUCID_OPT_ALPHA_THRESHOLD |
-							  LUCID_OPT_UNSORTED))))
+							  LUCID_OPT_UNSORTED)) ||
+				  any_textured(in)))
 		return LUCID_ERR_UNSUPPORTED;
 	hipStream_t stream = (hipStream_t)stream_ptr;
 	FrameConst c;
@@ -717,9 +799,10 @@ int lucid_backward_ex(lucid_renderer *r, const lucid_frame_inputs *in, const flo
 	if(!r || !in || !grad_image || (!grad_colors_rgb && !grad_tri_alpha) ||
 	   (flags & ~LUCID_BACKWARD_REUSE_FRAME))
 		return LUCID_ERR_INVALID_ARG;
-	// the adjoint is of the fp32-colour, fp32-sample, exact "over" forward
+	// the adjoint is of the fp32-colour, fp32-sample, exact "over" forward, untextured
 	if(!in->colors_rgb || !in->tri_alpha || in->colors_rgba8 ||
-	   (r->opts & (LUCID_OPT_ADDITIVE_BLENDING | LUCID_OPT_ALPHA_THRESHOLD | LUCID_OPT_UNSORTED)))
+	   (r->opts & (LUCID_OPT_ADDITIVE_BLENDING | LUCID_OPT_ALPHA_THRESHOLD | LUCID_OPT_UNSORTED)) ||
+	   any_textured(in))
 		return LUCID_ERR_UNSUPPORTED;
 	hipStream_t stream = (hipStream_t)stream_ptr;
 	LUCID_CHECK(hipSetDevice(r->device));

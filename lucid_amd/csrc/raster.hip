@@ -127,6 +127,110 @@ __device__ __forceinline__ LitChannel lit_channel(float x, float f, float fp) {
 	return o;
 }
 
+// ------------------------------------------------------------------ albedo textures
+//
+// textureGrad (shading.glsl:145-150) on an RGBA8 atlas with the reference's sampler
+// (src/lucid_base.h:30: linear magnification, minification and mip filter, repeat
+// addressing), restated from the Vulkan sampling rules: the scale factors are the texture
+// coordinate derivatives times the level-0 size, lambda = log2(max(|d/dx|, |d/dy|))
+// (isotropic), the level is clamped to [0, levels - 1] and the two nearest levels are
+// blended by its fraction; within a level, bilinear weights at u = s * size - 0.5 with the
+// texel indices wrapped (repeat). s is reduced to [0, 1] first (repeat is periodic).
+
+// First texel of `level`: the offsets are kernel arguments, picked with selects (no dynamic
+// indexing of an argument array).
+__device__ __forceinline__ uint32_t tex_level_offset(const TexDesc &t, int level) {
+	uint32_t off = 0;
+#pragma unroll
+	for(int i = 1; i < kMaxTexLevels; i++)
+		off = level == i ? t.offset[i] : off;
+	return off;
+}
+
+__device__ __forceinline__ float4 tex_bilinear(const TexDesc &t, int level, float s, float v) {
+	const int w = max(1, t.width >> level), h = max(1, t.height >> level);
+	const uint32_t *texels = t.texels + tex_level_offset(t, level);
+	const float su = s - floorf(s), sv = v - floorf(v);
+	const float u = su * (float)w - 0.5f, uv = sv * (float)h - 0.5f;
+	const float fu = floorf(u), fv = floorf(uv);
+	const float au = u - fu, av = uv - fv;
+	int x0 = (int)fu, y0 = (int)fv; // in [-1, size - 1]
+	x0 = x0 < 0 ? w - 1 : x0;
+	y0 = y0 < 0 ? h - 1 : y0;
+	const int x1 = x0 + 1 == w ? 0 : x0 + 1, y1 = y0 + 1 == h ? 0 : y0 + 1;
+	const float4 c00 = decode_rgba8(texels[(uint32_t)y0 * (uint32_t)w + (uint32_t)x0]);
+	const float4 c10 = decode_rgba8(texels[(uint32_t)y0 * (uint32_t)w + (uint32_t)x1]);
+	const float4 c01 = decode_rgba8(texels[(uint32_t)y1 * (uint32_t)w + (uint32_t)x0]);
+	const float4 c11 = decode_rgba8(texels[(uint32_t)y1 * (uint32_t)w + (uint32_t)x1]);
+	const float bu = 1.0f - au, bv = 1.0f - av;
+	float4 r;
+	r.x = (c00.x * bu + c10.x * au) * bv + (c01.x * bu + c11.x * au) * av;
+	r.y = (c00.y * bu + c10.y * au) * bv + (c01.y * bu + c11.y * au) * av;
+	r.z = (c00.z * bu + c10.z * au) * bv + (c01.z * bu + c11.z * au) * av;
+	r.w = (c00.w * bu + c10.w * au) * bv + (c01.w * bu + c11.w * au) * av;
+	return r;
+}
+
+__device__ float4 texture_grad(const TexDesc &t, float s, float v, float dsx, float dvx, float dsy,
+							   float dvy) {
+	const float fw = (float)t.width, fh = (float)t.height;
+	const float ux = dsx * fw, vx = dvx * fh, uy = dsy * fw, vy = dvy * fh;
+	const float rho = fmx(sqrtf(ux * ux + vx * vx), sqrtf(uy * uy + vy * vy));
+	const float lambda = log2f(rho);
+	if(!(lambda > 0.0f)) // magnification (or a degenerate footprint): level 0
+		return tex_bilinear(t, 0, s, v);
+	const float d = fmn(lambda, (float)(t.levels - 1));
+	const float fl = floorf(d);
+	const int lo = (int)fl;
+	const float delta = d - fl;
+	float4 r = tex_bilinear(t, lo, s, v);
+	if(delta > 0.0f) {
+		const float4 h = tex_bilinear(t, min(lo + 1, t.levels - 1), s, v);
+		const float nd = 1.0f - delta;
+		r.x = r.x * nd + h.x * delta;
+		r.y = r.y * nd + h.y * delta;
+		r.z = r.z * nd + h.z * delta;
+		r.w = r.w * nd + h.w * delta;
+	}
+	return r;
+}
+
+// Albedo of a textured triangle at a pixel (shading.glsl:111-122,131-159): barycentric
+// derivatives from the planes one pixel right and one pixel down, the texture coordinate
+// and its derivatives from the triangle's corner coordinates, the instance's uv rectangle,
+// then the opaque (alpha 1) or transparent atlas. bary0/1: before the offsets are removed.
+__device__ __forceinline__ float4 albedo_sample(const FrameBufs &b, const uint4 *q, float inv_ray,
+												float e0, float e1, float bary0, float bary1,
+												float bp0, float bp1) {
+	const uint32_t flags = q[0].w & 0xffffu, inst = q[0].w >> 16, t = q[6].z;
+	const float dex = bits2f(q[0].x), dey = bits2f(q[0].y);
+	const float ray_posx = __builtin_amdgcn_rcpf(inv_ray + dex);
+	const float ray_posy = __builtin_amdgcn_rcpf(inv_ray + dey);
+	const float bdx0 = (e0 + bits2f(q[1].x)) * ray_posx - bary0;
+	const float bdx1 = (e1 + bits2f(q[2].x)) * ray_posx - bary1;
+	const float bdy0 = (e0 + bits2f(q[1].y)) * ray_posy - bary0;
+	const float bdy1 = (e1 + bits2f(q[2].y)) * ray_posy - bary1;
+	const float b0 = bary0 - bp0, b1 = bary1 - bp1;
+	const uint4 *tt = b.tri_tex + 2 * (size_t)tri_rec(b, t);
+	const uint4 ta = tt[0], tb = tt[1];
+	const float t0x = bits2f(ta.x), t0y = bits2f(ta.y), t1x = bits2f(ta.z), t1y = bits2f(ta.w);
+	const float t2x = bits2f(tb.x), t2y = bits2f(tb.y);
+	float s = b0 * t1x + (b1 * t2x + t0x), v = b0 * t1y + (b1 * t2y + t0y);
+	float dsx = bdx0 * t1x + bdx1 * t2x, dvx = bdx0 * t1y + bdx1 * t2y;
+	float dsy = bdy0 * t1x + bdy1 * t2x, dvy = bdy0 * t1y + bdy1 * t2y;
+	if(flags & LUCID_INST_HAS_UV_RECT) {
+		const float4 rc = b.inst_uv_rects[inst];
+		s = rc.z * (s - floorf(s)) + rc.x;
+		v = rc.w * (v - floorf(v)) + rc.y;
+		dsx = dsx * rc.z, dsy = dsy * rc.z;
+		dvx = dvx * rc.w, dvy = dvy * rc.w;
+	}
+	const bool opaque = (flags & LUCID_INST_TEX_OPAQUE) != 0;
+	float4 col = texture_grad(opaque ? b.tex[0] : b.tex[1], s, v, dsx, dvx, dsy, dvy);
+	col.w = opaque ? 1.0f : col.w;
+	return col;
+}
+
 // The shading record of triangle t, built by setup (one 128-byte line per triangle).
 __device__ __forceinline__ void load_shading_record(const FrameBufs &b, uint32_t t,
 													uint4 (&q)[kRecQuads]) {
@@ -153,8 +257,10 @@ struct ShadeState {
 	float fp[3];  // f^(1/2.4)
 };
 
-__device__ __forceinline__ bool shade_prepare(const FrameConst &c, const uint4 *q, float px,
-											  float py, float inv_ray, ShadeState &st) {
+template <bool TEX>
+__device__ __forceinline__ bool shade_prepare(const FrameConst &c, const FrameBufs &b,
+											  const uint4 *q, float px, float py, float inv_ray,
+											  ShadeState &st) {
 	const uint32_t flags = q[0].w & 0xffffu;
 	const uint4 b0 = q[1], b1 = q[2];
 	// GLSL's division is specified to 2.5 ulp; v_rcp_f32 (1 ulp) meets that
@@ -162,6 +268,14 @@ __device__ __forceinline__ bool shade_prepare(const FrameConst &c, const uint4 *
 	float e0 = bits2f(b0.x) * px + (bits2f(b0.y) * py + bits2f(b0.z));
 	float e1 = bits2f(b1.x) * px + (bits2f(b1.y) * py + bits2f(b1.z));
 	float bary0 = e0 * ray_pos, bary1 = e1 * ray_pos;
+	float4 tex = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+	bool textured = false;
+	if constexpr(TEX) {
+		if(flags & LUCID_INST_HAS_ALBEDO_TEXTURE) {
+			textured = true;
+			tex = albedo_sample(b, q, inv_ray, e0, e1, bary0, bary1, bits2f(b0.w), bits2f(b1.w));
+		}
+	}
 	bary0 = bary0 - bits2f(b0.w);
 	bary1 = bary1 - bits2f(b1.w);
 	st.bary0 = bary0, st.bary1 = bary1;
@@ -188,7 +302,19 @@ __device__ __forceinline__ bool shade_prepare(const FrameConst &c, const uint4 *
 		}
 	}
 	st.inst = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-	if(flags & LUCID_INST_HAS_COLOR) {
+	if(textured) {
+		// shading.glsl: colour = instance colour, times the texture, times the vertex colour
+		float4 f = tex;
+		if(flags & LUCID_INST_HAS_COLOR) {
+			const float4 ic = decode_rgba8(q6.x);
+			st.inst = ic;
+			f = make_float4(ic.x * tex.x, ic.y * tex.y, ic.z * tex.z, ic.w * tex.w);
+		}
+		color.x = f.x * color.x;
+		color.y = f.y * color.y;
+		color.z = f.z * color.z;
+		color.w = f.w * color.w;
+	} else if(flags & LUCID_INST_HAS_COLOR) {
 		const float4 ic = decode_rgba8(q6.x);
 		st.inst = ic;
 		color.x = ic.x * color.x;
@@ -223,12 +349,13 @@ __device__ __forceinline__ bool shade_prepare(const FrameConst &c, const uint4 *
 
 // shadeSample (shading.glsl:107-184) without textures: lighting through the sRGB round trip
 // of finalShading (funcs.glsl:261-271), then RGBA8 quantisation unless FP32.
-template <bool FP32>
-__device__ __forceinline__ SampleColor<FP32> shade_record(const FrameConst &c, const uint4 *q,
-														  float px, float py, float inv_ray) {
+template <bool FP32, bool TEX>
+__device__ __forceinline__ SampleColor<FP32> shade_record(const FrameConst &c, const FrameBufs &b,
+														  const uint4 *q, float px, float py,
+														  float inv_ray) {
 	ShadeState st;
 	SampleColor<FP32> out;
-	if(!shade_prepare(c, q, px, py, inv_ray, st)) {
+	if(!shade_prepare<TEX>(c, b, q, px, py, inv_ray, st)) {
 		out.zero();
 		return out;
 	}
@@ -244,13 +371,13 @@ __device__ __forceinline__ SampleColor<FP32> shade_record(const FrameConst &c, c
 }
 
 // Shades triangle t at a pixel straight from the setup records (exact fallback path).
-template <bool FP32>
+template <bool FP32, bool TEX>
 __device__ __forceinline__ SampleColor<FP32> shade_sample(const FrameConst &c, const FrameBufs &b,
 														  uint32_t t, float px, float py,
 														  float inv_ray) {
 	uint4 q[kRecQuads];
 	load_shading_record(b, t, q);
-	return shade_record<FP32>(c, q, px, py, inv_ray);
+	return shade_record<FP32, TEX>(c, b, q, px, py, inv_ray);
 }
 
 __device__ __forceinline__ uint64_t pixel_key(float inv_ray, uint32_t t) {
@@ -1290,16 +1417,16 @@ __device__ __forceinline__ void walk_bits(uint64_t (&bm)[W], Visit &&visit) {
 	}
 }
 
-template <bool FP32, int W>
-__device__ __forceinline__ void walk_pixel(const FrameConst &c, const uint4 *s_rec,
-										   uint64_t (&bm)[W], float px, float py,
-										   PixelState<FP32> &ps, bool additive) {
+template <bool FP32, int W, bool TEX>
+__device__ __forceinline__ void walk_pixel(const FrameConst &c, const FrameBufs &b,
+										   const uint4 *s_rec, uint64_t (&bm)[W], float px,
+										   float py, PixelState<FP32> &ps, bool additive) {
 	const bool unsorted = (c.opts & LUCID_OPT_UNSORTED) != 0;
 	walk_bits(bm, [&](int e) {
 		const uint4 *rec = s_rec + e * kRecQuads;
 		const uint4 q0 = rec[0];
 		const float inv = record_inv_depth(q0, px, py);
-		const SampleColor<FP32> col = shade_record<FP32>(c, rec, px, py, inv);
+		const SampleColor<FP32> col = shade_record<FP32, TEX>(c, b, rec, px, py, inv);
 		if(col.contributes()) {
 			if(unsorted)
 				ps.acc.blend(col.rgba(), additive); // list order, no depth order
@@ -1338,7 +1465,7 @@ __device__ __forceinline__ void finish_pixels(const FrameConst &c, const FrameBu
 // One wave per block whose list fits the register sort (every low bin, most high blocks):
 // sorts the list by centroid depth in registers, then walks every pixel through it slice by
 // slice (each slice's shading records staged in LDS).
-template <bool FP32, int W>
+template <bool FP32, int W, bool TEX>
 __global__ __launch_bounds__(64) void block_raster_kernel(FrameConst c, FrameBufs b) {
 	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
 		return;
@@ -1416,7 +1543,7 @@ __global__ __launch_bounds__(64) void block_raster_kernel(FrameConst c, FrameBuf
 		if(opaque)
 			continue; // counted, not shaded
 		wave_sync_lds();
-		walk_pixel<FP32, W>(c, s_rec, bm, (float)px, (float)py, ps, additive);
+		walk_pixel<FP32, W, TEX>(c, b, s_rec, bm, (float)px, (float)py, ps, additive);
 		wave_sync_lds(); // every lane is done with this slice's records
 		opaque = threshold && all_below_alpha_threshold(ps.acc.tr);
 	}
@@ -1429,7 +1556,7 @@ __global__ __launch_bounds__(64) void block_raster_kernel(FrameConst c, FrameBuf
 // by sample count), walks it slice by slice with one ordering window per pixel, and leaves
 // a partial composite; wave 0 chains the partials front to back. A pixel is exact iff each
 // range's blended keys increase and every range starts behind the previous one's last key.
-template <bool FP32, int W>
+template <bool FP32, int W, bool TEX>
 __global__ __launch_bounds__(kBigWaves * 64) void block_raster_big_kernel(FrameConst c,
 																		   FrameBufs b) {
 	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
@@ -1484,7 +1611,7 @@ __global__ __launch_bounds__(kBigWaves * 64) void block_raster_big_kernel(FrameC
 			if(opaque)
 				continue; // counted, not shaded
 			wave_sync_lds();
-			walk_pixel<FP32, W>(c, s_rec[wave], bm, (float)px, (float)py, ps, additive);
+			walk_pixel<FP32, W, TEX>(c, b, s_rec[wave], bm, (float)px, (float)py, ps, additive);
 			wave_sync_lds(); // every lane is done with this slice's records
 			// the range's own transmittance bounds the pixel's (earlier ranges only lower it)
 			opaque = threshold && all_below_alpha_threshold(ps.acc.tr);
@@ -1574,7 +1701,7 @@ __device__ uint32_t gather_pixel_keys(const FrameBufs &b, const uint4 *recs, con
 
 // Blends n sorted keys front to back: 64 samples are shaded in parallel, then composited
 // in order.
-template <bool FP32>
+template <bool FP32, bool TEX>
 __device__ Accum blend_sorted(const FrameConst &c, const FrameBufs &b, const uint64_t *keys,
 							  uint32_t n, float px, float py, bool global_src) {
 	const int lane = lane_id();
@@ -1590,7 +1717,7 @@ __device__ Accum blend_sorted(const FrameConst &c, const FrameBufs &b, const uin
 															  __HIP_MEMORY_SCOPE_AGENT)
 										  : keys[i];
 			const uint32_t t = (uint32_t)k;
-			col = shade_sample<FP32>(c, b, t, px, py, sample_inv_depth(b, t, px, py));
+			col = shade_sample<FP32, TEX>(c, b, t, px, py, sample_inv_depth(b, t, px, py));
 		}
 		const uint32_t cnt = min(64u, n - base);
 		for(uint32_t j = 0; j < cnt; j++) {
@@ -1604,7 +1731,7 @@ __device__ Accum blend_sorted(const FrameConst &c, const FrameBufs &b, const uin
 
 // Exact path for one queued pixel: gathers its fragments from the block list, sorts the
 // keys (LDS bitonic up to kFallbackCap, else a global spill), shades and blends in order.
-template <bool FP32>
+template <bool FP32, bool TEX>
 __device__ void fallback_pixel(const FrameConst &c, const FrameBufs &b, uint4 item,
 							   uint64_t *s_keys) {
 	const int lane = lane_id();
@@ -1627,7 +1754,7 @@ __device__ void fallback_pixel(const FrameConst &c, const FrameBufs &b, uint4 it
 			s_keys[i] = ~0ull;
 		wave_sync_lds();
 		bitonic_lds_wave(s_keys, P);
-		acc = blend_sorted<FP32>(c, b, s_keys, n, fx, fy, false);
+		acc = blend_sorted<FP32, TEX>(c, b, s_keys, n, fx, fy, false);
 	} else {
 		// deep pixel: sort its keys in a global spill region
 		const uint32_t P = next_pow2(n);
@@ -1649,7 +1776,7 @@ __device__ void fallback_pixel(const FrameConst &c, const FrameBufs &b, uint4 it
 		__threadfence();
 		__builtin_amdgcn_wave_barrier();
 		bitonic_global_wave(g, P);
-		acc = blend_sorted<FP32>(c, b, g, n, fx, fy, true);
+		acc = blend_sorted<FP32, TEX>(c, b, g, n, fx, fy, true);
 	}
 	if(lane == 0) {
 		write_pixel(c, b, px, py, acc, n);
@@ -1664,7 +1791,7 @@ __device__ void fallback_pixel(const FrameConst &c, const FrameBufs &b, uint4 it
 }
 
 // Persistent over the per-pixel queue: wave w of the grid takes items w, w + waves, ...
-template <bool FP32>
+template <bool FP32, bool TEX>
 __global__ __launch_bounds__(kFallbackThreads) void fallback_kernel(FrameConst c, FrameBufs b) {
 	__shared__ uint64_t s_keys[kFallbackThreads / 64][kFallbackCap];
 	if(blockIdx.x == 0 && threadIdx.x < 64) {
@@ -1683,7 +1810,7 @@ __global__ __launch_bounds__(kFallbackThreads) void fallback_kernel(FrameConst c
 	const uint32_t waves = gridDim.x * (kFallbackThreads / 64);
 	const uint32_t w = blockIdx.x * (kFallbackThreads / 64) + (threadIdx.x >> 6);
 	for(uint32_t qi = w; qi < nq; qi += waves)
-		fallback_pixel<FP32>(c, b, b.fallback_queue[qi], s_keys[threadIdx.x >> 6]);
+		fallback_pixel<FP32, TEX>(c, b, b.fallback_queue[qi], s_keys[threadIdx.x >> 6]);
 }
 
 // ------------------------------------------------------------------ backward (adjoint)
@@ -1717,10 +1844,10 @@ struct BwdSample {
 
 // The fp32-sample shading of shade_record with the derivative of the lighting stage,
 // s = sat(L2S(S2L(x) f)) (lit_channel: ds/dx = fp above both linear segments, f below both).
-__device__ __forceinline__ bool shade_grad(const FrameConst &c, const uint4 *q, float px, float py,
-										   float inv_ray, BwdSample &o) {
+__device__ __forceinline__ bool shade_grad(const FrameConst &c, const FrameBufs &b, const uint4 *q,
+										   float px, float py, float inv_ray, BwdSample &o) {
 	ShadeState st;
-	if(!shade_prepare(c, q, px, py, inv_ray, st)) {
+	if(!shade_prepare<false>(c, b, q, px, py, inv_ray, st)) {
 		o.zero();
 		return false;
 	}
@@ -1958,7 +2085,7 @@ __global__ __launch_bounds__(kBigWaves * 64) void bwd_pass_a_kernel(FrameConst c
 			walk_bits(bm, [&](int e) {
 				const uint4 *rec = s_rec[wave] + e * kRecQuads;
 				const float inv = record_inv_depth(rec[0], fx, fy);
-				const SampleColor<true> col = shade_record<true>(c, rec, fx, fy, inv);
+				const SampleColor<true> col = shade_record<true, false>(c, b, rec, fx, fy, inv);
 				if(col.contributes()) {
 					BwdColor bc;
 					bc.r = col.col.x, bc.g = col.col.y, bc.b = col.col.z, bc.a = col.col.w;
@@ -2160,7 +2287,7 @@ __global__ __launch_bounds__(64) void bwd_pass_b_kernel(FrameConst c, FrameBufs 
 			const uint4 *rec = s_rec + (uint32_t)e * kRecQuads;
 			const float inv = record_inv_depth(rec[0], fx, fy);
 			BwdSample smp;
-			shade_grad(c, rec, fx, fy, inv, smp);
+			shade_grad(c, b, rec, fx, fy, inv, smp);
 			float out[kGradWords];
 			bwd_fragment(t, q, st.gr, st.gg, st.gb, smp, out);
 			if(live) {
@@ -2351,7 +2478,7 @@ __device__ void bwd_fallback_pixel(const FrameConst &c, const FrameBufs &b, cons
 		const uint32_t t = (uint32_t)key_at(i);
 		uint4 q[kRecQuads];
 		load_shading_record(b, t, q);
-		return shade_grad(c, q, fx, fy, record_inv_depth(q[0], fx, fy), smp);
+		return shade_grad(c, b, q, fx, fy, record_inv_depth(q[0], fx, fy), smp);
 	};
 	// pass A (serial over the sorted list, 64 samples shaded at a time)
 	BwdTotals tot;
@@ -2443,7 +2570,7 @@ __global__ __launch_bounds__(kFallbackThreads) void bwd_fallback_kernel(FrameCon
 		bwd_fallback_pixel(c, b, g, b.fallback_queue[qi], s_keys[threadIdx.x >> 6]);
 }
 
-template <bool FP32>
+template <bool FP32, bool TEX>
 static void launch_raster_t(const FrameConst &c, const FrameBufs &b, hipStream_t stream,
 							hipEvent_t *ev) {
 	hipLaunchKernelGGL(rowgen_low_kernel, dim3(c.bin_count), dim3(kRowgenThreads), 0, stream, c, b);
@@ -2457,19 +2584,20 @@ static void launch_raster_t(const FrameConst &c, const FrameBufs &b, hipStream_t
 	if(ev)
 		(void)hipEventRecord(ev[1], stream);
 	if(c.slice_words == 2) {
-		hipLaunchKernelGGL((block_raster_kernel<FP32, 2>), dim3(c.bin_count * 16), dim3(64), 0, stream,
-						   c, b);
-		hipLaunchKernelGGL((block_raster_big_kernel<FP32, 2>), dim3(c.bin_count * 16),
+		hipLaunchKernelGGL((block_raster_kernel<FP32, 2, TEX>), dim3(c.bin_count * 16), dim3(64), 0,
+						   stream, c, b);
+		hipLaunchKernelGGL((block_raster_big_kernel<FP32, 2, TEX>), dim3(c.bin_count * 16),
 						   dim3(kBigWaves * 64), 0, stream, c, b);
 	} else {
-		hipLaunchKernelGGL((block_raster_kernel<FP32, 1>), dim3(c.bin_count * 16), dim3(64), 0, stream,
-						   c, b);
-		hipLaunchKernelGGL((block_raster_big_kernel<FP32, 1>), dim3(c.bin_count * 16),
+		hipLaunchKernelGGL((block_raster_kernel<FP32, 1, TEX>), dim3(c.bin_count * 16), dim3(64), 0,
+						   stream, c, b);
+		hipLaunchKernelGGL((block_raster_big_kernel<FP32, 1, TEX>), dim3(c.bin_count * 16),
 						   dim3(kBigWaves * 64), 0, stream, c, b);
 	}
 	if(ev)
 		(void)hipEventRecord(ev[2], stream);
-	hipLaunchKernelGGL(fallback_kernel<FP32>, dim3(1024), dim3(kFallbackThreads), 0, stream, c, b);
+	hipLaunchKernelGGL((fallback_kernel<FP32, TEX>), dim3(1024), dim3(kFallbackThreads), 0, stream,
+					   c, b);
 	if(ev)
 		(void)hipEventRecord(ev[3], stream);
 }
@@ -2478,10 +2606,19 @@ static void launch_raster_t(const FrameConst &c, const FrameBufs &b, hipStream_t
 // after block rasterisation, after the fallback kernel.
 hipError_t launch_raster(const FrameConst &c, const FrameBufs &b, hipStream_t stream,
 						 hipEvent_t *events) {
-	if(c.opts & LUCID_OPT_FP32_SAMPLES)
-		launch_raster_t<true>(c, b, stream, events);
-	else
-		launch_raster_t<false>(c, b, stream, events);
+	// frames with albedo textures take kernels built with the texture path; the others
+	// carry none of its code (or registers)
+	const bool fp32 = (c.opts & LUCID_OPT_FP32_SAMPLES) != 0;
+	if(c.has_textures) {
+		if(fp32)
+			launch_raster_t<true, true>(c, b, stream, events);
+		else
+			launch_raster_t<false, true>(c, b, stream, events);
+	} else if(fp32) {
+		launch_raster_t<true, false>(c, b, stream, events);
+	} else {
+		launch_raster_t<false, false>(c, b, stream, events);
+	}
 	return hipGetLastError();
 }
 
@@ -2535,19 +2672,20 @@ hipError_t launch_raster_recording(const FrameConst &c, const FrameBufs &b, cons
 	hipLaunchKernelGGL(bwd_pass_a_kernel<true>, dim3(c.bin_count * 16), dim3(kBigWaves * 64), 0,
 					   stream, c, b, g);
 	if(c.slice_words == 2) {
-		hipLaunchKernelGGL((block_raster_kernel<true, 2>), dim3(c.bin_count * 16), dim3(64), 0, stream,
-						   c, b);
-		hipLaunchKernelGGL((block_raster_big_kernel<true, 2>), dim3(c.bin_count * 16),
+		hipLaunchKernelGGL((block_raster_kernel<true, 2, false>), dim3(c.bin_count * 16), dim3(64), 0,
+						   stream, c, b);
+		hipLaunchKernelGGL((block_raster_big_kernel<true, 2, false>), dim3(c.bin_count * 16),
 						   dim3(kBigWaves * 64), 0, stream, c, b);
 	} else {
-		hipLaunchKernelGGL((block_raster_kernel<true, 1>), dim3(c.bin_count * 16), dim3(64), 0, stream,
-						   c, b);
-		hipLaunchKernelGGL((block_raster_big_kernel<true, 1>), dim3(c.bin_count * 16),
+		hipLaunchKernelGGL((block_raster_kernel<true, 1, false>), dim3(c.bin_count * 16), dim3(64), 0,
+						   stream, c, b);
+		hipLaunchKernelGGL((block_raster_big_kernel<true, 1, false>), dim3(c.bin_count * 16),
 						   dim3(kBigWaves * 64), 0, stream, c, b);
 	}
 	if(ev)
 		(void)hipEventRecord(ev[2], stream);
-	hipLaunchKernelGGL(fallback_kernel<true>, dim3(1024), dim3(kFallbackThreads), 0, stream, c, b);
+	hipLaunchKernelGGL((fallback_kernel<true, false>), dim3(1024), dim3(kFallbackThreads), 0, stream,
+					   c, b);
 	if(ev)
 		(void)hipEventRecord(ev[3], stream);
 	return hipGetLastError();

@@ -250,6 +250,18 @@ __device__ int process_quad(const FrameConst &c, const FrameBufs &b, uint32_t q,
 #pragma unroll
 		for(int j = 0; j < kRecQuads; j++)
 			dst[j] = rec[j];
+		if(c.has_textures && (flags & LUCID_INST_HAS_ALBEDO_TEXTURE)) {
+			// texture coordinates of the triangle's corners, relative to corner 0
+			// (storeQuad, quad_setup.glsl:265-271)
+			const float *tc = b.tex_coords;
+			const uint32_t w0 = vq[0], w1 = vq[1 + s], w2 = vq[2 + s];
+			const float t0x = tc[2 * (size_t)w0], t0y = tc[2 * (size_t)w0 + 1];
+			const float d1x = tc[2 * (size_t)w1] - t0x, d1y = tc[2 * (size_t)w1 + 1] - t0y;
+			const float d2x = tc[2 * (size_t)w2] - t0x, d2y = tc[2 * (size_t)w2 + 1] - t0y;
+			uint4 *tt = b.tri_tex + 2 * (size_t)tri_rec(b, t);
+			tt[0] = make_uint4(f2bits(t0x), f2bits(t0y), f2bits(d1x), f2bits(d1y));
+			tt[1] = make_uint4(f2bits(d2x), f2bits(d2y), 0u, 0u);
+		}
 	}
 	return area <= 4u ? 0 : 1;
 }

@@ -50,6 +50,16 @@ class GpuScene:
                             scene.tri_alpha.astype(np.float32), np.float32)
         self.normals = up(None if scene.normals is None else
                           scene.normals.astype(np.uint32), np.int32)
+        self.tex_coords = up(None if scene.tex_coords is None else
+                             scene.tex_coords.astype(np.float32), np.float32)
+        # atlases: (device texels, width, height, levels) or None
+        self.textures = []
+        for t in (scene.textures or (None, None)):
+            self.textures.append(None if t is None else
+                                 (up(t.packed().astype(np.uint32), np.int32), t.width, t.height,
+                                  t.num_levels))
+        self.instance_uv_rects = (None if scene.instance_uv_rects is None else
+                                  np.ascontiguousarray(scene.instance_uv_rects, np.float32))
         self.instances = np.ascontiguousarray(scene.instances)
         self.instance_colors = (None if scene.instance_colors is None else
                                 np.ascontiguousarray(scene.instance_colors, np.uint32))
@@ -60,6 +70,9 @@ class GpuScene:
         tensors) replace the scene's own, as the differentiable path does."""
         def p(t):
             return None if t is None else t.data_ptr()
+
+        def tex(t):
+            return _lib.Texture(None, 0, 0, 0) if t is None else _lib.Texture(p(t[0]), *t[1:])
         return _lib.FrameInputs(
             self.instances.ctypes.data, len(self.instances),
             None if self.instance_colors is None else self.instance_colors.ctypes.data,
@@ -67,7 +80,10 @@ class GpuScene:
             p(self.quad_indices), p(self.positions), int(self.scene.num_verts),
             p(self.colors_rgba8),
             p(colors_rgb if colors_rgb is not None else self.colors_rgb),
-            p(tri_alpha if tri_alpha is not None else self.tri_alpha), p(self.normals))
+            p(tri_alpha if tri_alpha is not None else self.tri_alpha), p(self.normals),
+            p(self.tex_coords),
+            None if self.instance_uv_rects is None else self.instance_uv_rects.ctypes.data,
+            tex(self.textures[0]), tex(self.textures[1]))
 
 
 class LucidRenderer:

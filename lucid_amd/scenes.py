@@ -39,8 +39,12 @@ import numpy as np
 
 # definitions.glsl:84-94
 INST_HAS_VERTEX_COLORS = 0x001
+INST_HAS_VERTEX_TEX_COORDS = 0x002
 INST_HAS_VERTEX_NORMALS = 0x004
 INST_IS_OPAQUE = 0x008
+INST_TEX_OPAQUE = 0x010
+INST_HAS_UV_RECT = 0x020
+INST_HAS_ALBEDO_TEXTURE = 0x040
 INST_HAS_COLOR = 0x200
 
 MAX_INSTANCE_QUADS = 1024  # src/lucid_renderer.h:29
@@ -199,6 +203,67 @@ def make_config(
     return buf.tobytes()
 
 
+def unpack_rgba8(img: np.ndarray) -> np.ndarray:
+    """(..., ) uint32 RGBA8 -> (..., 4) uint8 channels."""
+    img = np.asarray(img, np.uint32)
+    return np.stack([(img >> (8 * k)) & 0xFF for k in range(4)], axis=-1).astype(np.uint8)
+
+
+def pack_channels(ch: np.ndarray) -> np.ndarray:
+    """(..., 4) uint8 channels -> (...) uint32 RGBA8."""
+    ch = np.asarray(ch, np.uint32)
+    return ch[..., 0] | (ch[..., 1] << 8) | (ch[..., 2] << 16) | (ch[..., 3] << 24)
+
+
+def box_mip(level: np.ndarray) -> np.ndarray:
+    """Next mip level of an RGBA8 image: the rounded mean of each 2 x 2 texel block (a size
+    of 1 stays 1; an odd size drops its last row or column, as max(1, size >> 1) does)."""
+    h, w = level.shape
+    nh, nw = max(1, h >> 1), max(1, w >> 1)
+    ch = unpack_rgba8(level).astype(np.float64)
+    ys = np.minimum(np.arange(2 * nh).reshape(nh, 2), h - 1)
+    xs = np.minimum(np.arange(2 * nw).reshape(nw, 2), w - 1)
+    acc = np.zeros((nh, nw, 4))
+    for dy in range(2):
+        for dx in range(2):
+            acc += ch[ys[:, dy]][:, xs[:, dx]]
+    return pack_channels(np.floor(acc / 4.0 + 0.5).astype(np.uint8))
+
+
+@dataclasses.dataclass
+class Texture:
+    """An RGBA8 atlas with its mip chain, as lucid_texture takes it: levels[l] is a
+    (max(1, H >> l), max(1, W >> l)) uint32 image, row 0 = v 0."""
+
+    levels: list
+
+    @property
+    def width(self) -> int:
+        return int(self.levels[0].shape[1])
+
+    @property
+    def height(self) -> int:
+        return int(self.levels[0].shape[0])
+
+    @property
+    def num_levels(self) -> int:
+        return len(self.levels)
+
+    def packed(self) -> np.ndarray:
+        return np.concatenate([np.ascontiguousarray(lv, np.uint32).reshape(-1)
+                               for lv in self.levels])
+
+    @staticmethod
+    def with_mips(level0: np.ndarray, num_levels: Optional[int] = None) -> "Texture":
+        """Full (or num_levels-deep) chain built with box_mip."""
+        lv = [np.ascontiguousarray(level0, np.uint32)]
+        top = max(level0.shape)
+        full = top.bit_length()
+        for _ in range((num_levels or full) - 1):
+            lv.append(box_mip(lv[-1]))
+        return Texture(lv)
+
+
 @dataclasses.dataclass
 class Scene:
     """Host-side scene arrays in the layouts the C ABI takes."""
@@ -216,6 +281,11 @@ class Scene:
     instance_colors: Optional[np.ndarray] = None  # (N,) uint32
     name: str = ""
     camera: Optional[Camera] = None  # the camera `config` was made from (None: the default)
+    # albedo textures: per-vertex coordinates, per-instance uv rectangles (min.xy, size.xy)
+    # and the (opaque, transparent) atlases
+    tex_coords: Optional[np.ndarray] = None  # (V, 2) float32
+    instance_uv_rects: Optional[np.ndarray] = None  # (N, 4) float32
+    textures: Optional[tuple] = None  # (Texture | None, Texture | None)
 
     @property
     def num_quads(self) -> int:
@@ -228,7 +298,7 @@ class Scene:
     def input_bytes(self) -> int:
         b = self.quad_indices.nbytes + self.positions.nbytes + self.instances.nbytes
         for arr in (self.colors_rgba8, self.colors_rgb, self.tri_alpha, self.normals,
-                    self.instance_colors):
+                    self.instance_colors, self.tex_coords, self.instance_uv_rects):
             if arr is not None:
                 b += arr.nbytes
         return int(b)
@@ -236,9 +306,13 @@ class Scene:
     def digest(self) -> str:
         h = hashlib.sha256()
         for arr in (self.positions, self.quad_indices, self.instances, self.colors_rgba8,
-                    self.colors_rgb, self.tri_alpha, self.normals, self.instance_colors):
+                    self.colors_rgb, self.tri_alpha, self.normals, self.instance_colors,
+                    self.tex_coords, self.instance_uv_rects):
             if arr is not None:
                 h.update(np.ascontiguousarray(arr).tobytes())
+        for t in (self.textures or ()):
+            if t is not None:
+                h.update(t.packed().tobytes())
         h.update(self.config)
         return h.hexdigest()
 
@@ -541,3 +615,116 @@ def block_stack(width: int, height: int, n: int, center_px=(12, 12), size_px: fl
     alpha = rng.uniform(0.02, 0.2, n)
     return soup_scene(width, height, ndc, depth, rgb, alpha, color_format=color_format,
                       name=f"stack_{n}")
+
+
+# ----------------------------------------------------------------------------- textures
+
+
+def checker_texture(width: int, height: int, seed: int = 0, alpha_range=(255, 255),
+                    num_levels: Optional[int] = None) -> Texture:
+    """A random-colour checkerboard of 4 x 4-texel cells with per-texel noise, and its box-
+    filtered mip chain; alpha uniform in alpha_range."""
+    rng = np.random.default_rng(seed)
+    cells = rng.integers(0, 256, ((height + 3) // 4, (width + 3) // 4, 3))
+    ys, xs = np.mgrid[0:height, 0:width]
+    rgb = cells[ys // 4, xs // 4] + rng.integers(-24, 25, (height, width, 3))
+    a = rng.integers(alpha_range[0], alpha_range[1] + 1, (height, width, 1))
+    ch = np.clip(np.concatenate([rgb, a], axis=-1), 0, 255).astype(np.uint8)
+    return Texture.with_mips(pack_channels(ch), num_levels)
+
+
+def textured_screen_plane(width: int, height: int, texture: Texture, *, uv_scale=(1.0, 1.0),
+                          uv_offset=(0.0, 0.0), opaque: bool = True, uv_rect=None,
+                          alpha: float = 1.0, depth: float = 4.0,
+                          color_format: str = "f32") -> Scene:
+    """One screen-covering fronto-parallel quad (slightly rotated, as planes_scene) whose
+    texture coordinate is affine in the screen: uv = uv_offset + uv_scale * (ndc + 1) / 2, so
+    that pixel (x, y) samples uv = uv_offset + uv_scale * ((x + 0.5) / W, (y + 0.5) / H).
+    White vertex colours with the given alpha; the texture is the opaque atlas (alpha read
+    as 1) or the transparent one; uv_rect (min.xy, size.xy) sets LUCID_INST_HAS_UV_RECT."""
+    cam = Camera(width, height)
+    ang = 0.013
+    rot = np.array([[np.cos(ang), -np.sin(ang)], [np.sin(ang), np.cos(ang)]])
+    ndc = np.array([[-1.2, -1.2], [1.2, -1.2], [1.2, 1.2], [-1.2, 1.2]]) @ rot.T
+    positions = cam.ndc_to_view(ndc, np.full(4, depth)).astype(np.float32)
+    uv = (np.asarray(uv_offset, np.float64)[None] +
+          np.asarray(uv_scale, np.float64)[None] * (ndc + 1.0) * 0.5).astype(np.float32)
+    flags = INST_HAS_VERTEX_COLORS | INST_HAS_VERTEX_TEX_COORDS | INST_HAS_ALBEDO_TEXTURE
+    flags |= INST_TEX_OPAQUE if opaque else 0
+    flags |= INST_HAS_UV_RECT if uv_rect is not None else 0
+    inst = split_instances(1, flags)
+    sc = Scene(width, height, positions, np.array([[0, 1, 2, 3]], np.uint32), inst,
+               make_config(cam, len(inst)), name="textured_plane")
+    rgb = np.ones((4, 3), np.float32)
+    if color_format == "rgba8":
+        sc.colors_rgba8 = pack_rgba8(rgb, np.full(4, alpha, np.float32))
+    else:
+        sc.colors_rgb = rgb
+        sc.tri_alpha = np.full(2, alpha, np.float32)
+    sc.tex_coords = uv
+    if uv_rect is not None:
+        sc.instance_uv_rects = np.asarray([uv_rect], np.float32)
+    sc.textures = (texture, None) if opaque else (None, texture)
+    return sc
+
+
+def textured_planes(width: int, height: int, n: int, seed: int = 11,
+                    color_format: str = "f32") -> Scene:
+    """n tilted, overlapping quads with random depth slopes, each its own instance: albedo
+    textures (opaque or transparent atlas, with or without a uv rectangle selecting one
+    quadrant of the atlas), texture coordinates repeated 0.3-6 times across the quad (both
+    magnified and minified levels, and anisotropic footprints), vertex colours and alpha, and
+    every third instance with an instance colour; a quarter of the instances untextured."""
+    rng = np.random.default_rng(seed)
+    cam = Camera(width, height)
+    ang = rng.uniform(0.01, 0.3, n)
+    cx, cy = rng.uniform(-0.6, 0.6, (2, n))
+    sx, sy = rng.uniform(0.3, 1.0, (2, n))
+    corners = np.array([[-1, -1], [1, -1], [1, 1], [-1, 1]], np.float64)
+    ndc = np.zeros((n, 4, 2))
+    depth = np.zeros((n, 4))
+    for i in range(n):
+        r = np.array([[np.cos(ang[i]), -np.sin(ang[i])], [np.sin(ang[i]), np.cos(ang[i])]])
+        ndc[i] = (corners * [sx[i], sy[i]]) @ r.T + [cx[i], cy[i]]
+        z0, s0, s1 = rng.uniform(3.0, 12.0), *rng.uniform(-2.0, 2.0, 2)
+        depth[i] = z0 + corners[:, 0] * s0 + corners[:, 1] * s1
+    pos = cam.ndc_to_view(ndc, depth).astype(np.float32).reshape(-1, 3)
+    base = np.arange(n, dtype=np.uint32) * 4
+    quads = np.stack([base, base + 1, base + 2, base + 3], axis=1).astype(np.uint32)
+    inst = np.zeros(n, INSTANCE_DTYPE)
+    inst["index_offset"] = np.arange(n) * 4
+    inst["num_quads"] = 1
+    rects = np.zeros((n, 4), np.float32)
+    icol = np.full(n, 0xFFFFFFFF, np.uint32)
+    uv = np.zeros((n, 4, 2), np.float32)
+    for i in range(n):
+        flags = INST_HAS_VERTEX_COLORS
+        if i % 4 != 3:
+            flags |= INST_HAS_VERTEX_TEX_COORDS | INST_HAS_ALBEDO_TEXTURE
+            flags |= INST_TEX_OPAQUE if i % 2 == 0 else 0
+            if i % 3 == 1:
+                flags |= INST_HAS_UV_RECT
+                qx, qy = rng.integers(0, 2, 2)
+                rects[i] = (0.5 * qx, 0.5 * qy, 0.5, 0.5)
+        if i % 3 == 0:
+            flags |= INST_HAS_COLOR
+            icol[i] = pack_rgba8(rng.uniform(0.3, 1.0, 3).astype(np.float32),
+                                 np.float32(rng.uniform(0.5, 1.0)))
+        inst["flags"][i] = flags
+        reps = rng.uniform(0.3, 6.0, 2)
+        off = rng.uniform(-1.0, 1.0, 2)
+        uv[i] = off + reps * (corners + 1.0) * 0.5
+    sc = Scene(width, height, pos, quads, inst, make_config(cam, n), name=f"textured_{n}")
+    rgb = rng.uniform(0.4, 1.0, (n, 4, 3)).astype(np.float32).reshape(-1, 3)
+    alpha = rng.uniform(0.3, 0.9, n).astype(np.float32)
+    if color_format == "rgba8":
+        sc.colors_rgba8 = pack_rgba8(rgb, np.repeat(alpha, 4))
+    else:
+        sc.colors_rgb = rgb
+        sc.tri_alpha = np.repeat(alpha, 2)
+    sc.instance_colors = icol
+    sc.instance_uv_rects = rects
+    sc.tex_coords = uv.reshape(-1, 2)
+    sc.textures = (checker_texture(128, 64, seed=seed + 1),
+                   checker_texture(64, 64, seed=seed + 2, alpha_range=(40, 255)))
+    return sc

@@ -12,6 +12,9 @@
  *   block keys   data/shaders/raster_low.glsl:108-150, raster_high.glsl:146-194,
  *                data/shaders/shared/raster.glsl:142-176
  *   shading      data/shaders/shared/shading.glsl:107-184, shared/funcs.glsl:100-164,261-271
+ *   textures     data/shaders/shared/shading.glsl:111-122,125-160 (textureGrad on the atlases with
+ *                the sampler of src/lucid_base.h:30, restated from the Vulkan sampling rules:
+ *                isotropic LOD, trilinear, repeat), data/shaders/quad_setup.glsl:265-271
  *   reduction    data/shaders/shared/shading.glsl:186-314 (REF3 mode)
  *
  * Arithmetic conventions (shared with the HIP kernels, fixed here because
@@ -183,6 +186,7 @@ typedef struct frame {
 	uint32_t *quad_color;  /* 4 u32 per quad */
 	uint32_t *quad_normal; /* 4 u32 per quad (INST_HAS_VERTEX_NORMALS) */
 	float *tri_color;     /* fp32 colour mode: 12 floats per tri (rgb0 a | rgb1 _ | rgb2 _) */
+	float *tri_tex;       /* textured instances: 6 floats per tri (tex0, tex1 - tex0, tex2 - tex0) */
 	/* bins */
 	uint32_t *counts; /* 10 * bin_count */
 	uint32_t *bin_quads, *bin_tris;
@@ -440,7 +444,18 @@ static int process_quad(frame *f, uint32_t quad_id, uint32_t inst_id, const uint
 		v3 t2 = v3_sub(vertex_load(f, vi[2 + second]), f->origin0);
 		store_tri(f, quad_id * 2 + second, flags_id, t0, t1, t2, y_aabb[second]);
 	}
-	/* storeQuad (:256-272): vertex colours / normals */
+	/* storeQuad (:256-272): texture coordinates relative to corner 0, vertex colours / normals */
+	if((flags & LUCID_INST_HAS_ALBEDO_TEXTURE) && f->in->tex_coords) {
+		const float *tc = f->in->tex_coords;
+		for(int second = 0; second < 2; second++) {
+			float *o = f->tri_tex + ((size_t)quad_id * 2 + second) * 6;
+			uint32_t w0 = vi[0], w1 = vi[1 + second], w2 = vi[2 + second];
+			float t0x = tc[2 * (size_t)w0], t0y = tc[2 * (size_t)w0 + 1];
+			o[0] = t0x, o[1] = t0y;
+			o[2] = tc[2 * (size_t)w1] - t0x, o[3] = tc[2 * (size_t)w1 + 1] - t0y;
+			o[4] = tc[2 * (size_t)w2] - t0x, o[5] = tc[2 * (size_t)w2 + 1] - t0y;
+		}
+	}
 	if((flags & LUCID_INST_HAS_VERTEX_NORMALS) && f->in->normals) {
 		uint32_t *nq = f->quad_normal + (size_t)quad_id * 4;
 		for(int k = 0; k < 4; k++)
@@ -636,6 +651,66 @@ typedef struct {
 	float light[3]; /* per-channel lighting factor (ambient + sun * light) */
 } sample_t;
 
+/* ---------------------------------------------------------------- textures
+ *
+ * textureGrad (shading.glsl:145-150) with linear magnification, minification and mip
+ * filtering and repeat addressing (src/lucid_base.h:30), from the Vulkan sampling rules:
+ * scale factors = coordinate derivatives x level-0 size, lambda = log2(max(|d/dx|, |d/dy|))
+ * (isotropic), clamped to [0, levels - 1], the two nearest levels blended by its fraction;
+ * bilinear at u = s * size - 0.5 with wrapped texel indices. Coordinates are reduced to
+ * [0, 1] first (repeat is periodic). */
+static void tex_bilinear(const oracle_texture *t, int level, float s, float v, float r[4]) {
+	int w = t->width >> level, h = t->height >> level;
+	w = w < 1 ? 1 : w;
+	h = h < 1 ? 1 : h;
+	size_t off = 0;
+	for(int l = 0; l < level; l++) {
+		int lw = t->width >> l, lh = t->height >> l;
+		off += (size_t)(lw < 1 ? 1 : lw) * (size_t)(lh < 1 ? 1 : lh);
+	}
+	const uint32_t *tx = t->texels + off;
+	float su = s - floorf(s), sv = v - floorf(v);
+	float u = su * (float)w - 0.5f, uv = sv * (float)h - 0.5f;
+	float fu = floorf(u), fv = floorf(uv);
+	float au = u - fu, av = uv - fv;
+	int x0 = (int)fu, y0 = (int)fv;
+	x0 = x0 < 0 ? w - 1 : x0;
+	y0 = y0 < 0 ? h - 1 : y0;
+	int x1 = x0 + 1 == w ? 0 : x0 + 1, y1 = y0 + 1 == h ? 0 : y0 + 1;
+	v4 c00 = decode_rgba8(tx[(size_t)y0 * w + x0]), c10 = decode_rgba8(tx[(size_t)y0 * w + x1]);
+	v4 c01 = decode_rgba8(tx[(size_t)y1 * w + x0]), c11 = decode_rgba8(tx[(size_t)y1 * w + x1]);
+	float bu = 1.0f - au, bv = 1.0f - av;
+	r[0] = (c00.x * bu + c10.x * au) * bv + (c01.x * bu + c11.x * au) * av;
+	r[1] = (c00.y * bu + c10.y * au) * bv + (c01.y * bu + c11.y * au) * av;
+	r[2] = (c00.z * bu + c10.z * au) * bv + (c01.z * bu + c11.z * au) * av;
+	r[3] = (c00.w * bu + c10.w * au) * bv + (c01.w * bu + c11.w * au) * av;
+}
+
+void oracle_texture_grad(const oracle_texture *t, float s, float v, float dsx, float dvx, float dsy,
+						 float dvy, float out[4]) {
+	float fw = (float)t->width, fh = (float)t->height;
+	float ux = dsx * fw, vx = dvx * fh, uy = dsy * fw, vy = dvy * fh;
+	float rho = fmx(sqrtf(ux * ux + vx * vx), sqrtf(uy * uy + vy * vy));
+	float lambda = log2f(rho);
+	if(!(lambda > 0.0f)) {
+		tex_bilinear(t, 0, s, v, out);
+		return;
+	}
+	float d = fmn(lambda, (float)(t->num_levels - 1));
+	float fl = floorf(d);
+	int lo = (int)fl;
+	float delta = d - fl;
+	tex_bilinear(t, lo, s, v, out);
+	if(delta > 0.0f) {
+		float h[4];
+		int hi = lo + 1 < t->num_levels - 1 ? lo + 1 : t->num_levels - 1;
+		tex_bilinear(t, hi, s, v, h);
+		float nd = 1.0f - delta;
+		for(int k = 0; k < 4; k++)
+			out[k] = out[k] * nd + h[k] * delta;
+	}
+}
+
 static void shade_sample(const frame *f, int px_i, int py_i, uint32_t t, sample_t *out) {
 	const lucid_config *cfg = f->in->config;
 	float px = (float)px_i, py = (float)py_i;
@@ -653,6 +728,17 @@ static void shade_sample(const frame *f, int px_i, int py_i, uint32_t t, sample_
 	float e0 = e0q[0] * px + (e0q[1] * py + e0q[2]);
 	float e1 = e1q[0] * px + (e1q[1] * py + e1q[2]);
 	float bary0 = e0 * ray_pos, bary1 = e1 * ray_pos;
+	/* shading.glsl:111-122: barycentrics one pixel right and one pixel down */
+	int textured = (flags & LUCID_INST_HAS_ALBEDO_TEXTURE) && f->in->tex_coords;
+	float bdx[2] = {0.0f, 0.0f}, bdy[2] = {0.0f, 0.0f};
+	if(textured) {
+		float ray_posx = 1.0f / (inv_ray_pos + de[0]);
+		float ray_posy = 1.0f / (inv_ray_pos + de[1]);
+		bdx[0] = (e0 + e0q[0]) * ray_posx - bary0;
+		bdx[1] = (e1 + e1q[0]) * ray_posx - bary1;
+		bdy[0] = (e0 + e0q[1]) * ray_posy - bary0;
+		bdy[1] = (e1 + e1q[1]) * ray_posy - bary1;
+	}
 	bary0 = bary0 - bp[0];
 	bary1 = bary1 - bp[1];
 	out->bary[0] = bary0, out->bary[1] = bary1;
@@ -662,6 +748,28 @@ static void shade_sample(const frame *f, int px_i, int py_i, uint32_t t, sample_
 	if((flags & LUCID_INST_HAS_COLOR) && f->in->instance_colors) {
 		v4 ic = decode_rgba8(f->in->instance_colors[inst]);
 		color[0] = ic.x, color[1] = ic.y, color[2] = ic.z, color[3] = ic.w;
+	}
+	if(textured) {
+		/* shading.glsl:131-159 */
+		const float *tt = f->tri_tex + (size_t)t * 6;
+		float s = bary0 * tt[2] + (bary1 * tt[4] + tt[0]);
+		float v = bary0 * tt[3] + (bary1 * tt[5] + tt[1]);
+		float dsx = bdx[0] * tt[2] + bdx[1] * tt[4], dvx = bdx[0] * tt[3] + bdx[1] * tt[5];
+		float dsy = bdy[0] * tt[2] + bdy[1] * tt[4], dvy = bdy[0] * tt[3] + bdy[1] * tt[5];
+		if((flags & LUCID_INST_HAS_UV_RECT) && f->in->instance_uv_rects) {
+			const float *rc = f->in->instance_uv_rects + (size_t)inst * 4;
+			s = rc[2] * (s - floorf(s)) + rc[0];
+			v = rc[3] * (v - floorf(v)) + rc[1];
+			dsx = dsx * rc[2], dsy = dsy * rc[2];
+			dvx = dvx * rc[3], dvy = dvy * rc[3];
+		}
+		int opaque = (flags & LUCID_INST_TEX_OPAQUE) != 0;
+		float tc[4];
+		oracle_texture_grad(&f->in->textures[opaque ? 0 : 1], s, v, dsx, dvx, dsy, dvy, tc);
+		if(opaque)
+			tc[3] = 1.0f;
+		for(int k = 0; k < 4; k++)
+			color[k] = color[k] * tc[k];
 	}
 	int fp32 = (f->in->opts & ORACLE_OPT_FP32_SAMPLES) != 0;
 	if(flags & LUCID_INST_HAS_VERTEX_COLORS) {
@@ -1263,12 +1371,14 @@ int oracle_render(const oracle_inputs *in, oracle_outputs *out) {
 	f->quad_color = out->quad_colors;
 	f->tri_color = (float *)calloc(T * 12 + 12, sizeof(float));
 	f->quad_normal = (uint32_t *)calloc(Q * 4 + 4, sizeof(uint32_t));
+	f->tri_tex = (float *)calloc(in->tex_coords ? T * 6 + 6 : 6, sizeof(float));
 	f->counts = out->info + LUCID_INFO_U32_SIZE;
 	f->bin_quads = out->bin_quads;
 	f->bin_tris = out->bin_tris;
-	if(!f->quad_inst || !f->scan || !f->depth || !f->bary || !f->tri_color || !f->quad_normal) {
+	if(!f->quad_inst || !f->scan || !f->depth || !f->bary || !f->tri_color || !f->quad_normal ||
+	   !f->tri_tex) {
 		free(prefix), free(f->quad_inst), free(f->scan), free(f->depth), free(f->bary),
-			free(f->tri_color), free(f->quad_normal);
+			free(f->tri_color), free(f->quad_normal), free(f->tri_tex);
 		return -2;
 	}
 	memset(out->info, 0, sizeof(uint32_t) * (LUCID_INFO_U32_SIZE + 10 * (size_t)f->bin_count));
@@ -1415,5 +1525,6 @@ done:
 	free(prefix);
 	free(f->quad_inst), free(f->scan), free(f->depth), free(f->bary), free(f->tri_color);
 	free(f->quad_normal);
+	free(f->tri_tex);
 	return rc;
 }

@@ -29,6 +29,13 @@ extern "C" {
 #define ORACLE_OPT_FP32_SAMPLES 0x1u /* no RGBA8 quantisation of shaded samples */
 #define ORACLE_OPT_ADDITIVE 0x2u     /* additive blending variant */
 
+/* RGBA8 atlas with its mip chain: level l (max(1, w >> l) x max(1, h >> l) texels, row-major)
+ * follows level l - 1. */
+typedef struct oracle_texture {
+	const uint32_t *texels; /* NULL = none */
+	int width, height, num_levels;
+} oracle_texture;
+
 typedef struct oracle_inputs {
 	int width, height;
 	const lucid_config *config;
@@ -44,6 +51,10 @@ typedef struct oracle_inputs {
 	const uint32_t *normals;      /* packed 10:10:10 per vertex, or NULL */
 	int mode;
 	uint32_t opts;
+	/* albedo textures (instances with LUCID_INST_HAS_ALBEDO_TEXTURE) */
+	const float *tex_coords;        /* float2 per vertex, or NULL */
+	const float *instance_uv_rects; /* float4 per instance (min.xy, size.xy), or NULL */
+	oracle_texture textures[2];     /* [0] opaque atlas, [1] transparent atlas */
 } oracle_inputs;
 
 /* One covered sample of the exact mode, listed per pixel in blend order (front to back,
@@ -87,6 +98,10 @@ uint32_t oracle_encode_rgba8(float r, float g, float b, float a);
 uint32_t oracle_encode_normal_uint(float x, float y, float z);
 int32_t oracle_f2i(float x);
 uint32_t oracle_f2u(float x);
+
+/* textureGrad of one atlas (see lucid_oracle.c), for unit tests: RGBA in out[4]. */
+void oracle_texture_grad(const oracle_texture *t, float s, float v, float dsx, float dvx, float dsy,
+						 float dvy, float out[4]);
 
 /* Number of worker threads used by oracle_render's raster stage (OpenMP); 1 if built without. */
 int oracle_num_threads(void);

@@ -21,6 +21,11 @@ MODE_EXACT, MODE_REF3, MODE_COUNTS = 0, 1, 2
 OPT_FP32_SAMPLES, OPT_ADDITIVE = 0x1, 0x2
 
 
+class _Texture(ctypes.Structure):
+    _fields_ = [("texels", ctypes.c_void_p), ("width", ctypes.c_int), ("height", ctypes.c_int),
+                ("num_levels", ctypes.c_int)]
+
+
 class _Inputs(ctypes.Structure):
     _fields_ = [
         ("width", ctypes.c_int), ("height", ctypes.c_int),
@@ -32,6 +37,8 @@ class _Inputs(ctypes.Structure):
         ("colors_rgba8", ctypes.c_void_p), ("colors_rgb", ctypes.c_void_p),
         ("tri_alpha", ctypes.c_void_p), ("normals", ctypes.c_void_p),
         ("mode", ctypes.c_int), ("opts", ctypes.c_uint32),
+        ("tex_coords", ctypes.c_void_p), ("instance_uv_rects", ctypes.c_void_p),
+        ("textures", _Texture * 2),
     ]
 
 
@@ -86,11 +93,32 @@ def lib():
         _lib.oracle_f2u.restype = ctypes.c_uint32
         _lib.oracle_f2u.argtypes = [ctypes.c_float]
         _lib.oracle_num_threads.restype = ctypes.c_int
+        _lib.oracle_texture_grad.restype = None
+        _lib.oracle_texture_grad.argtypes = [ctypes.POINTER(_Texture)] + [ctypes.c_float] * 6 + [
+            ctypes.c_void_p]
     return _lib
 
 
 def _ptr(a: Optional[np.ndarray]):
     return None if a is None else a.ctypes.data
+
+
+def _texture(tex, keep) -> _Texture:
+    """oracle_texture of a lucid_amd.scenes.Texture (or an empty one for None)."""
+    if tex is None:
+        return _Texture(None, 0, 0, 0)
+    packed = np.ascontiguousarray(tex.packed(), np.uint32)
+    keep.append(packed)
+    return _Texture(packed.ctypes.data, tex.width, tex.height, tex.num_levels)
+
+
+def texture_grad(tex, s, v, dsx, dvx, dsy, dvy) -> np.ndarray:
+    """The oracle's textureGrad of one atlas at one point (RGBA float32)."""
+    keep = []
+    t = _texture(tex, keep)
+    out = np.zeros(4, np.float32)
+    lib().oracle_texture_grad(ctypes.byref(t), s, v, dsx, dvx, dsy, dvy, out.ctypes.data)
+    return out
 
 
 def bin_count(width: int, height: int) -> int:
@@ -109,7 +137,7 @@ def render(scene, mode: int = MODE_EXACT, opts: int = 0, bin_capacity: Optional[
     nb = bin_count(w, h)
     q = scene.num_quads
     if bin_capacity is None:
-        bin_capacity = max(1024, 8 * q)
+        bin_capacity = max(1 << 16, 8 * q)
     keep = []
 
     def c(a, dtype):
@@ -130,6 +158,10 @@ def render(scene, mode: int = MODE_EXACT, opts: int = 0, bin_capacity: Optional[
         _ptr(c(scene.colors_rgba8, np.uint32)), _ptr(c(scene.colors_rgb, np.float32)),
         _ptr(c(scene.tri_alpha, np.float32)), _ptr(c(scene.normals, np.uint32)),
         mode, opts,
+        _ptr(c(getattr(scene, "tex_coords", None), np.float32)),
+        _ptr(c(getattr(scene, "instance_uv_rects", None), np.float32)),
+        (_Texture * 2)(*[_texture(t, keep) for t in (getattr(scene, "textures", None) or
+                                                       (None, None))]),
     )
     out_np = {
         "info": np.zeros(INFO_U32 + 10 * nb, np.uint32),
