@@ -192,6 +192,26 @@ int lucid_batch_render(lucid_batch *b, const lucid_frame_inputs *ins,
 					   const lucid_frame_outputs *outs, void *stream);
 int lucid_batch_finish(lucid_batch *b);
 
+/* One forward frame that allocates nothing: no renderer object, no hipMalloc/hipFree and no
+ * allocating library call. No counterpart in the reference; it is for callers that cannot
+ * allocate device memory (code under HIP graph capture set-up, tight serving loops, harnesses
+ * that forbid hipMalloc). Same kernels, inputs and outputs as lucid_render on a renderer of
+ * width x height with `opts` and `tuning` (NULL: the defaults 1, 256, 2048); results are
+ * identical to that renderer's. Every internal buffer lives in a fixed device array of the
+ * library (65 MiB, one copy per GPU the library runs on, held until the process ends), so the
+ * frame is limited to at most 256 bins of 32 x 32 pixels (e.g. 512 x 512), 65536 quads and
+ * 1024 instances (larger: LUCID_ERR_INVALID_ARG), no albedo textures (LUCID_ERR_UNSUPPORTED),
+ * and to that array's pools: high bins (1024+ triangles) may hold 131072 triangle
+ * references in all (a triangle counts once, a quad twice, per bin it touches; e.g. 20000
+ * clustered soup triangles on 256 x 256 take 62358), pre-sorted blocks 2 MiB and deep-pixel
+ * spills 1 MiB. A frame that would overflow a pool returns LUCID_ERR_CAPACITY (its outputs
+ * are incomplete; render it with lucid_render). Synchronous:
+ * the instance table is staged on `stream` and the call waits for the frame. All calls on one
+ * GPU share the array: calls that overlap on one GPU (other threads, other streams) race. */
+int lucid_render_noalloc(int width, int height, uint32_t opts, const lucid_tuning *tuning,
+						 const lucid_frame_inputs *in, const lucid_frame_outputs *out,
+						 void *stream);
+
 /* Waits for the last frame. Returns LUCID_ERR_CAPACITY if it overflowed an internal buffer
  * (the buffers have then been grown and the frame must be rendered again). */
 int lucid_finish(lucid_renderer *r);

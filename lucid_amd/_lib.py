@@ -43,7 +43,7 @@ EXPORTED = (
     "lucid_batch_create", "lucid_batch_destroy", "lucid_batch_render", "lucid_batch_finish",
     "lucid_generate_quads",
     "lucid_bin_count", "lucid_bin_size", "lucid_block_size", "lucid_max_visible_quads",
-    "lucid_max_scene_quads",
+    "lucid_max_scene_quads", "lucid_render_noalloc",
 )
 
 
@@ -161,6 +161,10 @@ def load(path: str = LIB_PATH):
                                        vp]
     lib.lucid_batch_finish.restype = i32
     lib.lucid_batch_finish.argtypes = [vp]
+    lib.lucid_render_noalloc.restype = i32
+    lib.lucid_render_noalloc.argtypes = [i32, i32, ctypes.c_uint32, ctypes.POINTER(Tuning),
+                                         ctypes.POINTER(FrameInputs), ctypes.POINTER(FrameOutputs),
+                                         vp]
     lib.lucid_generate_quads.restype = i32
     lib.lucid_generate_quads.argtypes = [vp, i64, vp, i64, ctypes.c_float, vp, ctypes.POINTER(i64)]
     for name in ("lucid_bin_count", "lucid_bin_size", "lucid_block_size"):

@@ -75,7 +75,7 @@ constexpr int kCtrlAllocWords = kLargeCursorBase + kLargeStripes * kStatStride;
 __device__ __forceinline__ uint32_t *large_cursor(uint32_t *ctrl, uint32_t stripe) {
 	return ctrl + kLargeCursorBase + (int)stripe * kStatStride;
 }
-__host__ __device__ __forceinline__ uint32_t large_stripe_cap(uint32_t num_quads) {
+__host__ __device__ constexpr uint32_t large_stripe_cap(uint32_t num_quads) {
 	const uint32_t wgs = (num_quads + kSetupThreads - 1) / kSetupThreads;
 	return (wgs + kLargeStripes - 1) / kLargeStripes * kSetupThreads;
 }

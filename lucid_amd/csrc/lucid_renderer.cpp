@@ -1210,6 +1210,224 @@ int lucid_batch_finish(lucid_batch *bt) {
 	return rc;
 }
 
+} // extern "C"
+
+// ------------------------------------------------------------------ allocation-free frames
+//
+// lucid_render_noalloc renders one frame without any allocation: every internal buffer of
+// the frame is carved from the file-scope device array `scratch` below, whose size is fixed
+// by the limits that follow. Each GPU the library runs on holds its own copy of the array
+// (allocated with the code object) until the process ends. Host code never passes the
+// array's name to a kernel or copy: it is a host-side stand-in, not a device address; the
+// device address comes from hipGetSymbolAddress, once per device.
+namespace {
+
+constexpr int kNaMaxBins = 256;                        // e.g. 512 x 512 or 1024 x 256 pixels
+constexpr int64_t kNaMaxQuads = (int64_t)1 << 16;
+constexpr int kNaMaxInstances = 1024;
+constexpr uint64_t kNaHighRecords = (uint64_t)1 << 19; // uint4 records of high bins
+constexpr uint64_t kNaListsExtra = (uint64_t)1 << 21;  // u32 list entries of high bins
+constexpr uint64_t kNaSorted = (uint64_t)1 << 19;      // u32 words of pre-sorted blocks
+constexpr uint64_t kNaSpill = (uint64_t)1 << 17;       // u64 keys of deep pixels
+constexpr uint32_t kNaFallback = (uint32_t)kNaMaxBins * 16 * 8; // pixels queued for the exact sort
+constexpr uint64_t kNaBinQuads = (uint64_t)kNaMaxQuads * 4;
+constexpr uint64_t kNaBinTris = (uint64_t)kNaMaxQuads * 4 + 65536;
+constexpr int kNaMaxDevices = 64;
+
+struct NaLayout {
+	size_t info, ctrl, instances, inst_prefix, inst_colors, quad_aabb, tri_scan, tri_depth,
+		tri_shade, large_list, wg_hist, bin_quads, bin_tris, low_records, high_records, fallback,
+		spill, blk_slot, blk_flags, blk_sorted, blk_lists, blk_count, band_count, sorted_pool,
+		total;
+};
+
+// Byte offsets of the frame's buffers in `scratch` (each 256-byte aligned), sized as
+// lucid_create and ensure_quad_capacity size a renderer's, for the limits above.
+constexpr NaLayout na_layout() {
+	NaLayout l{};
+	size_t o = 0;
+	const size_t nb = kNaMaxBins, q = (size_t)kNaMaxQuads, t = 2 * q;
+	size_t *fields[] = {&l.info,      &l.ctrl,         &l.instances,  &l.inst_prefix, &l.inst_colors,
+						&l.quad_aabb, &l.tri_scan,     &l.tri_depth,  &l.tri_shade,   &l.large_list,
+						&l.wg_hist,   &l.bin_quads,    &l.bin_tris,   &l.low_records, &l.high_records,
+						&l.fallback,  &l.spill,        &l.blk_slot,   &l.blk_flags,   &l.blk_sorted,
+						&l.blk_lists, &l.blk_count,    &l.band_count, &l.sorted_pool};
+	const size_t bytes[] = {
+		((size_t)LUCID_INFO_U32_SIZE + 10 * nb) * 4,
+		(size_t)kCtrlAllocWords * 4,
+		(size_t)kNaMaxInstances * sizeof(lucid_instance_data),
+		((size_t)kNaMaxInstances + 1) * 4,
+		(size_t)kNaMaxInstances * 4,
+		q * 4,
+		t * 32,
+		t * 16,
+		t * kRecStride * 16,
+		(size_t)kLargeStripes * large_stripe_cap((uint32_t)q) * 4,
+		(size_t)kNumBinWgs * 2 * nb * 4,
+		kNaBinQuads * 4,
+		kNaBinTris * 4,
+		nb * 4 * kLowRowCap * 16,
+		kNaHighRecords * 16,
+		(size_t)kNaFallback * 16,
+		kNaSpill * 8,
+		nb * 16 * 16,
+		nb * 16 * 4,
+		nb * 16 * 4,
+		(nb * 16 * kRegSortCap + kNaListsExtra) * 4,
+		nb * 16 * 4,
+		nb * 4 * 4,
+		kNaSorted * 4,
+	};
+	static_assert(sizeof(fields) / sizeof(fields[0]) == sizeof(bytes) / sizeof(bytes[0]),
+				  "one size per buffer");
+	for(size_t i = 0; i < sizeof(bytes) / sizeof(bytes[0]); i++) {
+		*fields[i] = o;
+		o = (o + bytes[i] + 255) & ~(size_t)255;
+	}
+	l.total = o;
+	return l;
+}
+
+constexpr NaLayout kNaLayout = na_layout();
+static_assert(kNaLayout.total <= ((size_t)72 << 20), "the allocation-free scratch stays modest");
+
+} // namespace
+
+// The frame scratch of lucid_render_noalloc (one copy per device).
+static __device__ __align__(256) uint8_t scratch[kNaLayout.total];
+
+namespace {
+
+// Device address of `scratch` per device, looked up on the first call on that device.
+uint8_t *g_scratch_ptr[kNaMaxDevices] = {};
+
+int noalloc_scratch(int device, uint8_t **out) {
+	if(device < 0 || device >= kNaMaxDevices)
+		return LUCID_ERR_INVALID_ARG;
+	uint8_t *p = __atomic_load_n(&g_scratch_ptr[device], __ATOMIC_ACQUIRE);
+	if(!p) {
+		void *addr = nullptr;
+		LUCID_CHECK(hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch)));
+		p = (uint8_t *)addr;
+		__atomic_store_n(&g_scratch_ptr[device], p, __ATOMIC_RELEASE);
+	}
+	*out = p;
+	return LUCID_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int lucid_render_noalloc(int width, int height, uint32_t opts, const lucid_tuning *tuning,
+						 const lucid_frame_inputs *in, const lucid_frame_outputs *out,
+						 void *stream_ptr) {
+	if(!in || !out || !in->config || width <= 0 || height <= 0 || width > kMaxWidth ||
+	   height > kMaxHeight)
+		return LUCID_ERR_INVALID_ARG;
+	const int nbx = (width + kBinSize - 1) / kBinSize, nby = (height + kBinSize - 1) / kBinSize;
+	if(nbx * nby > kNaMaxBins || in->num_instances > kNaMaxInstances)
+		return LUCID_ERR_INVALID_ARG;
+	lucid_tuning t = {1, 256, 2048};
+	if(tuning) {
+		t = *tuning;
+		if(t.slice_words < 1 || t.slice_words > 2 || t.presort_min < 64 || t.presort_min > 256 ||
+		   t.samples_per_wave < 256 || t.samples_per_wave > 65536)
+			return LUCID_ERR_INVALID_ARG;
+	}
+	int prefix[kNaMaxInstances + 1];
+	int64_t nq = 0;
+	int color_mode = kColorNone;
+	if(in->num_instances < 0)
+		return LUCID_ERR_INVALID_ARG;
+	bool textured = false;
+	int ve = validate_inputs(in, prefix, nq, color_mode, textured);
+	if(ve != LUCID_OK)
+		return ve;
+	if(nq > kNaMaxQuads)
+		return LUCID_ERR_INVALID_ARG;
+	if(textured)
+		return LUCID_ERR_UNSUPPORTED; // no texture-coordinate records in the fixed array
+	int device = 0;
+	LUCID_CHECK(hipGetDevice(&device));
+	uint8_t *base = nullptr;
+	int se = noalloc_scratch(device, &base);
+	if(se != LUCID_OK)
+		return se;
+	hipStream_t stream = (hipStream_t)stream_ptr;
+	const NaLayout &L = kNaLayout;
+
+	FrameConst c;
+	fill_const(width, height, opts & ~(uint32_t)LUCID_OPT_TIMERS, t, *in->config, nq,
+			   in->num_instances, color_mode, in->normals != nullptr, c);
+	c.keep_overflow = 0u;
+	FrameBufs b;
+	memset(&b, 0, sizeof(b));
+	b.instances = (const lucid_instance_data *)(base + L.instances);
+	b.inst_prefix = (const int *)(base + L.inst_prefix);
+	b.inst_colors = in->instance_colors ? (const uint32_t *)(base + L.inst_colors) : nullptr;
+	b.quad_indices = in->quad_indices;
+	b.positions = in->positions;
+	b.colors_rgba8 = in->colors_rgba8;
+	b.colors_rgb = in->colors_rgb;
+	b.tri_alpha = in->tri_alpha;
+	b.normals = in->normals;
+	b.info = (uint32_t *)(base + L.info);
+	b.ctrl = (uint32_t *)(base + L.ctrl);
+	b.tri_half = (uint32_t)kNaMaxQuads;
+	b.quad_aabb = (uint32_t *)(base + L.quad_aabb);
+	b.tri_scan = (uint4 *)(base + L.tri_scan);
+	b.tri_depth = (uint4 *)(base + L.tri_depth);
+	b.tri_shade = (uint4 *)(base + L.tri_shade);
+	b.large_list = (uint32_t *)(base + L.large_list);
+	b.wg_hist = (uint32_t *)(base + L.wg_hist);
+	b.bin_quads = (uint32_t *)(base + L.bin_quads);
+	b.bin_tris = (uint32_t *)(base + L.bin_tris);
+	b.bin_quads_cap = kNaBinQuads;
+	b.bin_tris_cap = kNaBinTris;
+	b.low_records = (uint4 *)(base + L.low_records);
+	b.high_records = (uint4 *)(base + L.high_records);
+	b.high_records_cap = kNaHighRecords;
+	b.fallback_queue = (uint4 *)(base + L.fallback);
+	b.fallback_cap = kNaFallback;
+	b.spill = (uint64_t *)(base + L.spill);
+	b.spill_cap = kNaSpill;
+	b.blk_slot = (uint4 *)(base + L.blk_slot);
+	b.blk_flags = (uint32_t *)(base + L.blk_flags);
+	b.blk_sorted = (uint32_t *)(base + L.blk_sorted);
+	b.blk_lists = (uint32_t *)(base + L.blk_lists);
+	b.blk_lists_cap = (uint64_t)kNaMaxBins * 16 * kRegSortCap + kNaListsExtra;
+	b.blk_count = (uint32_t *)(base + L.blk_count);
+	b.band_count = (uint32_t *)(base + L.band_count);
+	b.sorted_pool = (uint32_t *)(base + L.sorted_pool);
+	b.sorted_cap = kNaSorted;
+	b.image = out->image_rgba8;
+	b.image_f = out->image_rgba32f;
+	b.frag_counts = out->frag_counts;
+
+	(void)hipGetLastError(); // launches below report their own status, not an older call's
+	// the instance table, its quad prefix and the instance colours (uploadInstances,
+	// src/lucid_renderer.cpp:352-429), staged from host memory on the frame's stream
+	const size_t ninst = (size_t)in->num_instances;
+	if(ninst)
+		LUCID_CHECK(hipMemcpyAsync(base + L.instances, in->instances,
+								   ninst * sizeof(lucid_instance_data), hipMemcpyHostToDevice,
+								   stream));
+	LUCID_CHECK(hipMemcpyAsync(base + L.inst_prefix, prefix, (ninst + 1) * 4,
+							   hipMemcpyHostToDevice, stream));
+	if(ninst && in->instance_colors)
+		LUCID_CHECK(hipMemcpyAsync(base + L.inst_colors, in->instance_colors, ninst * 4,
+								   hipMemcpyHostToDevice, stream));
+	const uint32_t n_info = LUCID_INFO_U32_SIZE + 10u * (uint32_t)c.bin_count;
+	LUCID_CHECK(launch_setup(c, b, n_info, stream));
+	LUCID_CHECK(launch_binning(c, b, stream));
+	LUCID_CHECK(launch_raster(c, b, stream, nullptr));
+	uint32_t status[kCtrlWords];
+	LUCID_CHECK(hipMemcpyAsync(status, b.ctrl, sizeof(status), hipMemcpyDeviceToHost, stream));
+	LUCID_CHECK(hipStreamSynchronize(stream));
+	return status[kCtrlOverflow] ? LUCID_ERR_CAPACITY : LUCID_OK;
+}
+
 int lucid_finish(lucid_renderer *r) {
 	if(!r)
 		return LUCID_ERR_INVALID_ARG;
