@@ -156,6 +156,9 @@ struct FrameConst {
 	// are reported with this frame's status
 	uint32_t keep_overflow;
 	int has_textures; // some instance samples an albedo texture (setup stores texture coordinates)
+	// ambient 1 and no sun light: every lighting factor of the frame is exactly 1, and
+	// finalShading's sRGB round trip L2S(S2L(x) * 1) is the identity
+	int identity_light;
 	float vp[16]; // column-major view-projection matrix
 	float origin[3], dir0[3], dirx[3], diry[3], ray_dir0[3];
 	float ambient[3], sun_color[3], sun_dir[3];
@@ -288,6 +291,13 @@ __device__ __forceinline__ uint32_t encode_aabb28(uint32_t x0, uint32_t y0, uint
 __device__ __forceinline__ uint32_t encode_rgba8(float r, float g, float b, float a) {
 	return f2u(r * 255.0f) | (f2u(g * 255.0f) << 8) | (f2u(b * 255.0f) << 16) |
 		   (f2u(a * 255.0f) << 24);
+}
+
+// encode_rgba8 of colour channels already in [0, 1] (shaded samples are saturated): the
+// truncating conversion needs no clamp there; alpha keeps the saturating one
+__device__ __forceinline__ uint32_t encode_rgba8_unit_rgb(float r, float g, float b, float a) {
+	return (uint32_t)(r * 255.0f) | ((uint32_t)(g * 255.0f) << 8) |
+		   ((uint32_t)(b * 255.0f) << 16) | (f2u(a * 255.0f) << 24);
 }
 
 __device__ __forceinline__ float4 decode_rgba8(uint32_t c) {

@@ -302,6 +302,10 @@ void fill_const(int width, int height, uint32_t opts, const lucid_tuning &tuning
 	c.sun_color[0] = l.sun_color.x, c.sun_color[1] = l.sun_color.y, c.sun_color[2] = l.sun_color.z;
 	c.sun_dir[0] = l.sun_dir.x, c.sun_dir[1] = l.sun_dir.y, c.sun_dir[2] = l.sun_dir.z;
 	c.sun_power = l.sun_power, c.ambient_power = l.ambient_power;
+	bool ident = true;
+	for(int k = 0; k < 3; k++) // the lighting factor is ambient + sun * light (light_factor)
+		ident = ident && c.ambient[k] * c.ambient_power == 1.0f && c.sun_color[k] * c.sun_power == 0.0f;
+	c.identity_light = ident ? 1 : 0;
 	c.background[0] = cfg.background_color.x, c.background[1] = cfg.background_color.y;
 	c.background[2] = cfg.background_color.z, c.background[3] = cfg.background_color.w;
 }

@@ -325,6 +325,11 @@ __device__ __forceinline__ bool shade_prepare(const FrameConst &c, const FrameBu
 	st.color = color;
 	if(color.w == 0.0f)
 		return false;
+	if(c.identity_light) { // every lighting factor is 1 (no normal or light value is read)
+		st.f[0] = st.f[1] = st.f[2] = 1.0f;
+		st.fp[0] = st.fp[1] = st.fp[2] = 1.0f;
+		return true;
+	}
 	if(flags & LUCID_INST_HAS_VERTEX_NORMALS) {
 		V3 n0 = decode_normal(q[4].w), n1 = decode_normal(q[5].w), n2 = decode_normal(q6.y);
 		n1 = sub3(n1, n0);
@@ -360,13 +365,20 @@ __device__ __forceinline__ SampleColor<FP32> shade_record(const FrameConst &c, c
 		return out;
 	}
 	float rgb[3] = {st.color.x, st.color.y, st.color.z};
+	if(c.identity_light) {
+		// L2S(S2L(x) * 1) = x: the sRGB round trip of a unit lighting factor is skipped
 #pragma unroll
-	for(int k = 0; k < 3; k++)
-		rgb[k] = sat(lit_channel<false>(rgb[k], st.f[k], st.fp[k]).s);
+		for(int k = 0; k < 3; k++)
+			rgb[k] = sat(rgb[k]);
+	} else {
+#pragma unroll
+		for(int k = 0; k < 3; k++)
+			rgb[k] = sat(lit_channel<false>(rgb[k], st.f[k], st.fp[k]).s);
+	}
 	if constexpr(FP32)
 		out.col = make_float4(rgb[0], rgb[1], rgb[2], st.color.w);
 	else
-		out.enc = encode_rgba8(rgb[0], rgb[1], rgb[2], st.color.w);
+		out.enc = encode_rgba8_unit_rgb(rgb[0], rgb[1], rgb[2], st.color.w);
 	return out;
 }
 
@@ -1856,7 +1868,11 @@ __device__ __forceinline__ bool shade_grad(const FrameConst &c, const FrameBufs 
 	float sv[3], dv[3];
 #pragma unroll
 	for(int k = 0; k < 3; k++) {
-		const LitChannel l = lit_channel<true>(x[k], st.f[k], st.fp[k]);
+		LitChannel l;
+		if(c.identity_light) // s = x, ds/dx = 1 (the round trip of a unit factor)
+			l.s = x[k], l.deriv = 1.0f;
+		else
+			l = lit_channel<true>(x[k], st.f[k], st.fp[k]);
 		sv[k] = sat(l.s);
 		// clamp passes the gradient on [0, 1] (as torch.clamp does)
 		dv[k] = (l.s >= 0.0f && l.s <= 1.0f) ? l.deriv * ic[k] : 0.0f;
