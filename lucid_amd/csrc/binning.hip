@@ -96,14 +96,14 @@ __device__ __forceinline__ uint32_t large_at(const FrameBufs &b, const Slice &s,
 }
 
 // bin_counter.glsl: per-work-group histograms of small quads and large tris.
-__global__ __launch_bounds__(kBinThreads) void bin_count_kernel(FrameConst c, FrameBufs b) {
+__device__ __forceinline__ void bin_count_kernel_body(const FrameConst &c, const FrameBufs &b, WgPos wg) {
 	extern __shared__ uint32_t s_hist[]; // [2][bin_count]
 	const int NB = c.bin_count, NBX = c.bin_count_x;
 	uint32_t *hq = s_hist, *ht = s_hist + NB;
 	for(int i = threadIdx.x; i < 2 * NB; i += kBinThreads)
 		s_hist[i] = 0;
 	__syncthreads();
-	Slice sl = wg_slice(c, b, blockIdx.x);
+	Slice sl = wg_slice(c, b, wg.id);
 
 	// small quads: +1 over the bin rectangle (bin_counter.glsl:64-80)
 	for(uint32_t q = sl.q0 + threadIdx.x; q < sl.q1; q += kBinThreads) {
@@ -151,9 +151,20 @@ __global__ __launch_bounds__(kBinThreads) void bin_count_kernel(FrameConst c, Fr
 		}
 	}
 	__syncthreads();
-	uint32_t *dst = b.wg_hist + (size_t)blockIdx.x * 2 * NB;
+	uint32_t *dst = b.wg_hist + (size_t)wg.id * 2 * NB;
 	for(int i = threadIdx.x; i < 2 * NB; i += kBinThreads)
 		dst[i] = s_hist[i];
+}
+
+__global__ __launch_bounds__(kBinThreads) void bin_count_kernel(FrameConst c, FrameBufs b) {
+	bin_count_kernel_body(c, b, WgPos{blockIdx.x, gridDim.x});
+}
+__global__ __launch_bounds__(kBinThreads) void bin_count_kernel_batch(const FrameDesc *__restrict__ frames,
+		const uint32_t *__restrict__ pre, uint32_t num_frames) {
+	const StageBatch sb{frames, pre, num_frames};
+	const uint32_t f = batch_frame(sb, blockIdx.x);
+	const FrameDesc &d = sb.frames[f];
+	bin_count_kernel_body(d.c, d.b, WgPos{blockIdx.x - sb.pre[f], sb.pre[f + 1] - sb.pre[f]});
 }
 
 // Per bin: totals over work-groups and each work-group's exclusive offset inside the bin.
@@ -162,11 +173,10 @@ __global__ __launch_bounds__(kBinThreads) void bin_count_kernel(FrameConst c, Fr
 // a run of rows with independent loads, an LDS scan joins the parts, then each part writes
 // its exclusive prefixes.
 constexpr int kScanCols = 16, kScanParts = 16;
-__global__ __launch_bounds__(kScanCols * kScanParts) void bin_wg_scan_kernel(FrameConst c,
-																			  FrameBufs b) {
+__device__ __forceinline__ void bin_wg_scan_kernel_body(const FrameConst &c, const FrameBufs &b, WgPos wg) {
 	const int NB = c.bin_count, nw = c.num_bin_wgs;
 	const int col_l = threadIdx.x % kScanCols, part = threadIdx.x / kScanCols;
-	const int col = blockIdx.x * kScanCols + col_l; // 0 .. 2 NB: quads then tris
+	const int col = wg.id * kScanCols + col_l; // 0 .. 2 NB: quads then tris
 	const int rows = (nw + kScanParts - 1) / kScanParts; // <= 16: num_bin_wgs <= 256
 	const int w0 = part * rows, w1 = min(nw, w0 + rows);
 	__shared__ uint32_t s_part[kScanParts][kScanCols];
@@ -207,8 +217,19 @@ __global__ __launch_bounds__(kScanCols * kScanParts) void bin_wg_scan_kernel(Fra
 	}
 }
 
+__global__ __launch_bounds__(kScanCols * kScanParts) void bin_wg_scan_kernel(FrameConst c, FrameBufs b) {
+	bin_wg_scan_kernel_body(c, b, WgPos{blockIdx.x, gridDim.x});
+}
+__global__ __launch_bounds__(kScanCols * kScanParts) void bin_wg_scan_kernel_batch(const FrameDesc *__restrict__ frames,
+		const uint32_t *__restrict__ pre, uint32_t num_frames) {
+	const StageBatch sb{frames, pre, num_frames};
+	const uint32_t f = batch_frame(sb, blockIdx.x);
+	const FrameDesc &d = sb.frames[f];
+	bin_wg_scan_kernel_body(d.c, d.b, WgPos{blockIdx.x - sb.pre[f], sb.pre[f + 1] - sb.pre[f]});
+}
+
 // bin_categorizer.glsl: exclusive offsets over bins, bin levels and level lists.
-__global__ __launch_bounds__(1024) void bin_categorize_kernel(FrameConst c, FrameBufs b) {
+__device__ __forceinline__ void bin_categorize_kernel_body(const FrameConst &c, const FrameBufs &b, WgPos wg) {
 	const int NB = c.bin_count;
 	uint32_t *counts = b.info + LUCID_INFO_U32_SIZE;
 	lucid_info *info = (lucid_info *)b.info;
@@ -277,21 +298,32 @@ __global__ __launch_bounds__(1024) void bin_categorize_kernel(FrameConst c, Fram
 	}
 }
 
+__global__ __launch_bounds__(1024) void bin_categorize_kernel(FrameConst c, FrameBufs b) {
+	bin_categorize_kernel_body(c, b, WgPos{blockIdx.x, gridDim.x});
+}
+__global__ __launch_bounds__(1024) void bin_categorize_kernel_batch(const FrameDesc *__restrict__ frames,
+		const uint32_t *__restrict__ pre, uint32_t num_frames) {
+	const StageBatch sb{frames, pre, num_frames};
+	const uint32_t f = batch_frame(sb, blockIdx.x);
+	const FrameDesc &d = sb.frames[f];
+	bin_categorize_kernel_body(d.c, d.b, WgPos{blockIdx.x - sb.pre[f], sb.pre[f + 1] - sb.pre[f]});
+}
+
 // bin_dispatcher.glsl: replay the work-group's slice into its reserved ranges.
-__global__ __launch_bounds__(kBinThreads) void bin_dispatch_kernel(FrameConst c, FrameBufs b) {
+__device__ __forceinline__ void bin_dispatch_kernel_body(const FrameConst &c, const FrameBufs &b, WgPos wg) {
 	if(b.ctrl[kCtrlOverflow] & 1u)
 		return;
 	extern __shared__ uint32_t s_pos[]; // [2][bin_count]
 	const int NB = c.bin_count, NBX = c.bin_count_x;
 	const uint32_t *counts = b.info + LUCID_INFO_U32_SIZE;
-	const uint32_t *wgo = b.wg_hist + (size_t)blockIdx.x * 2 * NB;
+	const uint32_t *wgo = b.wg_hist + (size_t)wg.id * 2 * NB;
 	for(int i = threadIdx.x; i < NB; i += kBinThreads) {
 		s_pos[i] = counts[LUCID_COUNTS_QUAD_OFFSETS * NB + i] + wgo[i];
 		s_pos[NB + i] = counts[LUCID_COUNTS_TRI_OFFSETS * NB + i] + wgo[NB + i];
 	}
 	__syncthreads();
 	uint32_t *pq = s_pos, *pt = s_pos + NB;
-	Slice sl = wg_slice(c, b, blockIdx.x);
+	Slice sl = wg_slice(c, b, wg.id);
 	for(uint32_t q = sl.q0 + threadIdx.x; q < sl.q1; q += kBinThreads) {
 		uint32_t a = b.quad_aabb[q];
 		int bsx, bsy, bex, bey;
@@ -319,6 +351,29 @@ __global__ __launch_bounds__(kBinThreads) void bin_dispatch_kernel(FrameConst c,
 				b.bin_tris[atomicAdd(&pt[by * NBX + bx], 1u)] = t;
 		}
 	}
+}
+
+__global__ __launch_bounds__(kBinThreads) void bin_dispatch_kernel(FrameConst c, FrameBufs b) {
+	bin_dispatch_kernel_body(c, b, WgPos{blockIdx.x, gridDim.x});
+}
+__global__ __launch_bounds__(kBinThreads) void bin_dispatch_kernel_batch(const FrameDesc *__restrict__ frames,
+		const uint32_t *__restrict__ pre, uint32_t num_frames) {
+	const StageBatch sb{frames, pre, num_frames};
+	const uint32_t f = batch_frame(sb, blockIdx.x);
+	const FrameDesc &d = sb.frames[f];
+	bin_dispatch_kernel_body(d.c, d.b, WgPos{blockIdx.x - sb.pre[f], sb.pre[f + 1] - sb.pre[f]});
+}
+
+hipError_t launch_binning_batch(const BatchPlan &p, hipStream_t stream) {
+	hipLaunchKernelGGL(bin_count_kernel_batch, dim3(p.total[kBsBinCount]), dim3(kBinThreads),
+					   p.bin_lds, stream, p.frames, p.stage_pre(kBsBinCount), p.num_frames);
+	hipLaunchKernelGGL(bin_wg_scan_kernel_batch, dim3(p.total[kBsBinScan]),
+					   dim3(kScanCols * kScanParts), 0, stream, p.frames, p.stage_pre(kBsBinScan), p.num_frames);
+	hipLaunchKernelGGL(bin_categorize_kernel_batch, dim3(p.total[kBsBinCategorize]), dim3(1024), 0,
+					   stream, p.frames, p.stage_pre(kBsBinCategorize), p.num_frames);
+	hipLaunchKernelGGL(bin_dispatch_kernel_batch, dim3(p.total[kBsBinDispatch]), dim3(kBinThreads),
+					   p.bin_lds, stream, p.frames, p.stage_pre(kBsBinDispatch), p.num_frames);
+	return hipGetLastError();
 }
 
 hipError_t launch_binning(const FrameConst &c, const FrameBufs &b, hipStream_t stream) {

@@ -230,6 +230,75 @@ struct FrameBufs {
 	uint32_t *frag_counts;
 };
 
+// ------------------------------------------------------------------ batched launches
+//
+// A batch renders many frames (scenes of different sizes and resolutions) with one launch
+// per pipeline stage: the stage's grid is the concatenation of every frame's own grid, and a
+// work-group finds its frame and its position within that frame's grid (WgPos) from an
+// exclusive prefix of the per-frame work-group counts. Every stage kernel is written once,
+// as a body taking WgPos, with a single-frame wrapper (the frame's constants and buffers as
+// kernel arguments, WgPos = blockIdx.x / gridDim.x) and a batched wrapper.
+struct WgPos {
+	uint32_t id, count;
+};
+
+struct FrameDesc {
+	FrameConst c;
+	FrameBufs b;
+};
+
+enum BatchStage : int {
+	kBsClear,
+	kBsSetup,
+	kBsBinCount,
+	kBsBinScan,
+	kBsBinCategorize,
+	kBsBinDispatch,
+	kBsRowgenLow,
+	kBsHighPrep,
+	kBsRowgenHigh,
+	kBsHighFinalize,
+	kBsBlockRaster,
+	kBsBlockRasterBig,
+	kBsFallback,
+	kBsCount
+};
+
+// One stage of a batched launch: frame f owns work-groups [pre[f], pre[f + 1]).
+struct StageBatch {
+	const FrameDesc *frames;
+	const uint32_t *pre;
+	uint32_t num_frames;
+};
+
+// The frame of work-group wg: the last frame whose range starts at or before wg (frames
+// with no work-groups in this stage have empty ranges and are passed over).
+__device__ __forceinline__ uint32_t batch_frame(const StageBatch &sb, uint32_t wg) {
+	uint32_t lo = 0, hi = sb.num_frames - 1;
+	while(lo < hi) {
+		const uint32_t mid = (lo + hi + 1) >> 1;
+		if(sb.pre[mid] <= wg)
+			lo = mid;
+		else
+			hi = mid - 1;
+	}
+	return lo;
+}
+
+// A batched launch: the frames (device array), the per-stage prefixes (device,
+// kBsCount x (num_frames + 1)), and what selects the kernels (shared by every frame).
+struct BatchPlan {
+	const FrameDesc *frames;
+	const uint32_t *pre;
+	uint32_t num_frames;
+	uint32_t total[kBsCount]; // work-groups of each stage over all frames
+	size_t bin_lds;           // dynamic LDS of the bin kernels: the largest frame's
+	bool fp32, tex;
+	int slice_words;
+	uint32_t *status; // device, kCtrlWords per frame: each frame's control words after it
+	__host__ const uint32_t *stage_pre(int s) const { return pre + (size_t)s * (num_frames + 1); }
+};
+
 // Record index of triangle t (2 q + s) in the per-triangle setup arrays.
 __device__ __forceinline__ uint32_t tri_rec(const FrameBufs &b, uint32_t t) {
 	return (t >> 1) + (t & 1u) * b.tri_half;

@@ -24,6 +24,9 @@ namespace lucid {
 hipError_t launch_setup(const FrameConst &c, const FrameBufs &b, uint32_t n_info,
 						hipStream_t stream);
 hipError_t launch_binning(const FrameConst &c, const FrameBufs &b, hipStream_t stream);
+hipError_t launch_setup_batch(const BatchPlan &p, hipStream_t stream);
+hipError_t launch_binning_batch(const BatchPlan &p, hipStream_t stream);
+hipError_t launch_raster_batch(const BatchPlan &p, hipStream_t stream);
 hipError_t launch_raster(const FrameConst &c, const FrameBufs &b, hipStream_t stream,
 						 hipEvent_t *events);
 hipError_t launch_backward_prepare(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
@@ -1031,11 +1034,13 @@ int lucid_autotune(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_
 struct lucid_batch {
 	int device = 0;
 	std::vector<lucid_renderer *> rs;
-	std::vector<FrameConst> cs;
-	std::vector<FrameBufs> bs;
-	std::vector<hipStream_t> streams;
-	std::vector<hipEvent_t> join;
-	hipEvent_t fork = nullptr, done = nullptr;
+	std::vector<FrameDesc> frames;   // the frames of the recorded launch (host copy)
+	std::vector<uint32_t> pre;       // kBsCount x (n + 1) work-group prefixes
+	DevBuf frames_dev, pre_dev, status_dev;
+	uint32_t *status = nullptr;      // pinned: kCtrlWords per frame after the last launch
+	BatchPlan plan = {};
+	hipEvent_t done = nullptr;
+	hipStream_t capture_stream = nullptr;
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
 	bool pending = false;
@@ -1044,51 +1049,60 @@ struct lucid_batch {
 			(void)hipGraphExecDestroy(exec);
 		if(graph)
 			(void)hipGraphDestroy(graph);
-		for(hipEvent_t e : join)
-			(void)hipEventDestroy(e);
-		if(fork)
-			(void)hipEventDestroy(fork);
 		if(done)
 			(void)hipEventDestroy(done);
-		for(hipStream_t s : streams)
-			(void)hipStreamDestroy(s);
+		if(capture_stream)
+			(void)hipStreamDestroy(capture_stream);
+		if(status)
+			(void)hipHostFree(status);
 	}
 };
 
 namespace {
-constexpr int kBatchStreams = 8;
 
-// Records every frame of the batch into one graph: frames are spread round-robin over
-// kBatchStreams forked streams (default priority) so that small scenes run side by side.
+constexpr uint32_t kBatchFallbackWgs = 64; // persistent exact-fallback work-groups per frame
+constexpr int kScanColsHost = 16;         // columns per work-group of bin_wg_scan_kernel
+
+// Work-groups of each pipeline stage for one frame (the grids launch_setup, launch_binning and
+// launch_raster use for it alone; the exact fallback is persistent and takes fewer).
+void stage_wgs(const FrameConst &c, uint32_t w[kBsCount]) {
+	const uint32_t nb = (uint32_t)c.bin_count;
+	const uint32_t n_info = LUCID_INFO_U32_SIZE + 10u * nb;
+	const uint32_t n_clear = std::max(n_info, (uint32_t)kCtrlAllocWords);
+	w[kBsClear] = (n_clear + 255) / 256;
+	w[kBsSetup] = c.num_quads > 0 ? ((uint32_t)c.num_quads + kSetupThreads - 1) / kSetupThreads : 0;
+	w[kBsBinCount] = (uint32_t)c.num_bin_wgs;
+	w[kBsBinScan] = (2 * nb + kScanColsHost - 1) / kScanColsHost;
+	w[kBsBinCategorize] = 1;
+	w[kBsBinDispatch] = (uint32_t)c.num_bin_wgs;
+	w[kBsRowgenLow] = nb;
+	w[kBsHighPrep] = 1;
+	w[kBsRowgenHigh] = nb * kRowgenSplit;
+	w[kBsHighFinalize] = nb * 16;
+	w[kBsBlockRaster] = nb * 16;
+	w[kBsBlockRasterBig] = nb * 16;
+	w[kBsFallback] = kBatchFallbackWgs;
+}
+
+// Records the batch's launches into one graph: one launch per stage for all frames, then
+// the frames' control words to pinned host memory.
 int capture_batch(lucid_batch *bt) {
 	if(bt->exec)
 		(void)hipGraphExecDestroy(bt->exec), bt->exec = nullptr;
 	if(bt->graph)
 		(void)hipGraphDestroy(bt->graph), bt->graph = nullptr;
-	hipStream_t s0 = bt->streams[0];
-	LUCID_CHECK(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
-	hipError_t err = hipEventRecord(bt->fork, s0);
-	for(size_t k = 1; k < bt->streams.size() && err == hipSuccess; k++)
-		err = hipStreamWaitEvent(bt->streams[k], bt->fork, 0);
-	for(size_t i = 0; i < bt->rs.size() && err == hipSuccess; i++) {
-		lucid_renderer *r = bt->rs[i];
-		hipStream_t s = bt->streams[i % bt->streams.size()];
-		const uint32_t n_info = LUCID_INFO_U32_SIZE + 10u * (uint32_t)r->bin_count;
-		err = launch_setup(bt->cs[i], bt->bs[i], n_info, s);
-		if(err == hipSuccess)
-			err = launch_binning(bt->cs[i], bt->bs[i], s);
-		if(err == hipSuccess)
-			err = launch_raster(bt->cs[i], bt->bs[i], s, nullptr);
-		if(err == hipSuccess)
-			err = hipMemcpyAsync(r->status, r->ctrl.ptr, kCtrlWords * 4, hipMemcpyDeviceToHost, s);
-	}
-	for(size_t k = 1; k < bt->streams.size() && err == hipSuccess; k++) {
-		err = hipEventRecord(bt->join[k], bt->streams[k]);
-		if(err == hipSuccess)
-			err = hipStreamWaitEvent(s0, bt->join[k], 0);
-	}
+	hipStream_t s = bt->capture_stream;
+	const size_t n = bt->rs.size();
+	LUCID_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+	hipError_t err = launch_setup_batch(bt->plan, s);
+	if(err == hipSuccess)
+		err = launch_binning_batch(bt->plan, s);
+	if(err == hipSuccess)
+		err = launch_raster_batch(bt->plan, s);
+	if(err == hipSuccess)
+		err = hipMemcpyAsync(bt->status, bt->plan.status, n * kCtrlWords * 4, hipMemcpyDeviceToHost, s);
 	hipGraph_t g = nullptr;
-	hipError_t end = hipStreamEndCapture(s0, &g);
+	hipError_t end = hipStreamEndCapture(s, &g);
 	if(err != hipSuccess || end != hipSuccess) {
 		if(g)
 			(void)hipGraphDestroy(g);
@@ -1104,8 +1118,9 @@ int lucid_batch_create(lucid_renderer *const *renderers, int n, lucid_batch **ou
 	if(!renderers || n <= 0 || !out)
 		return LUCID_ERR_INVALID_ARG;
 	for(int i = 0; i < n; i++)
-		if(!renderers[i] || renderers[i]->device != renderers[0]->device || renderers[i]->timers)
-			return LUCID_ERR_INVALID_ARG; // one device; stage timers are per renderer only
+		if(!renderers[i] || renderers[i]->device != renderers[0]->device || renderers[i]->timers ||
+		   ((renderers[i]->opts ^ renderers[0]->opts) & LUCID_OPT_FP32_SAMPLES))
+			return LUCID_ERR_INVALID_ARG; // one device and sample format; timers are per renderer
 	for(int i = 0; i < n; i++)
 		for(int j = 0; j < i; j++)
 			if(renderers[i] == renderers[j])
@@ -1116,27 +1131,23 @@ int lucid_batch_create(lucid_renderer *const *renderers, int n, lucid_batch **ou
 	bt->device = renderers[0]->device;
 	bt->rs.assign(renderers, renderers + n);
 	int rc = LUCID_OK;
-	if(hipSetDevice(bt->device) != hipSuccess)
+	if(hipSetDevice(bt->device) != hipSuccess ||
+	   hipStreamCreateWithFlags(&bt->capture_stream, hipStreamNonBlocking) != hipSuccess ||
+	   hipEventCreateWithFlags(&bt->done, hipEventDisableTiming) != hipSuccess ||
+	   hipHostMalloc((void **)&bt->status, (size_t)n * kCtrlWords * 4, hipHostMallocDefault) != hipSuccess)
 		rc = LUCID_ERR_HIP;
-	const int ns = std::min(n, kBatchStreams);
-	for(int k = 0; k < ns && rc == LUCID_OK; k++) {
-		hipStream_t s = nullptr;
-		hipEvent_t e = nullptr;
-		if(hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
-		   hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
-			rc = LUCID_ERR_HIP;
-		if(s)
-			bt->streams.push_back(s);
-		if(e)
-			bt->join.push_back(e);
+	if(rc == LUCID_OK) {
+		rc = hip_status(bt->status_dev.reserve((size_t)n * kCtrlWords * 4));
+		if(rc == LUCID_OK)
+			rc = hip_status(bt->pre_dev.reserve((size_t)kBsCount * (n + 1) * 4));
+		if(rc == LUCID_OK)
+			rc = hip_status(bt->frames_dev.reserve((size_t)n * sizeof(FrameDesc)));
 	}
-	if(rc == LUCID_OK && (hipEventCreateWithFlags(&bt->fork, hipEventDisableTiming) != hipSuccess ||
-						  hipEventCreateWithFlags(&bt->done, hipEventDisableTiming) != hipSuccess))
-		rc = LUCID_ERR_HIP;
 	if(rc != LUCID_OK) {
 		delete bt;
 		return rc;
 	}
+	memset(bt->status, 0, (size_t)n * kCtrlWords * 4);
 	*out = bt;
 	return LUCID_OK;
 }
@@ -1162,8 +1173,8 @@ int lucid_batch_render(lucid_batch *bt, const lucid_frame_inputs *ins,
 			return f;
 	}
 	const size_t n = bt->rs.size();
-	std::vector<FrameConst> cs(n);
-	std::vector<FrameBufs> bs(n);
+	std::vector<FrameDesc> frames(n);
+	bool tex = false;
 	for(size_t i = 0; i < n; i++) {
 		lucid_renderer *r = bt->rs[i];
 		if(r->frame_pending) {
@@ -1172,20 +1183,55 @@ int lucid_batch_render(lucid_batch *bt, const lucid_frame_inputs *ins,
 				return f;
 		}
 		int64_t nq = 0;
-		int e = prepare_frame(r, &ins[i], &outs[i], stream, cs[i], bs[i], nq);
+		int e = prepare_frame(r, &ins[i], &outs[i], stream, frames[i].c, frames[i].b, nq);
 		if(e != LUCID_OK)
 			return e;
+		// every frame runs the block kernels of the first renderer's slice width (a tuning
+		// knob: results do not depend on it)
+		frames[i].c.slice_words = bt->rs[0]->tuning.slice_words;
+		tex = tex || frames[i].c.has_textures;
 		r->last_num_quads = nq;
 		r->last_stream = stream;
 		r->retained.valid = r->retained.pending = false;
 	}
-	// re-record the graph only when a frame's constants or buffers changed
-	const bool same = bt->exec && bt->cs.size() == n &&
-					  memcmp(bt->cs.data(), cs.data(), n * sizeof(FrameConst)) == 0 &&
-					  memcmp(bt->bs.data(), bs.data(), n * sizeof(FrameBufs)) == 0;
+	// per-stage work-group prefixes: frame i owns [pre[s][i], pre[s][i + 1]) of stage s
+	std::vector<uint32_t> pre((size_t)kBsCount * (n + 1), 0);
+	BatchPlan plan;
+	memset(&plan, 0, sizeof(plan)); // compared bytewise (padding included) below
+	size_t bin_lds = 0;
+	for(size_t i = 0; i < n; i++) {
+		uint32_t w[kBsCount];
+		stage_wgs(frames[i].c, w);
+		for(int st = 0; st < kBsCount; st++)
+			pre[(size_t)st * (n + 1) + i + 1] = pre[(size_t)st * (n + 1) + i] + w[st];
+		bin_lds = std::max(bin_lds, (size_t)2 * frames[i].c.bin_count * sizeof(uint32_t));
+	}
+	for(int st = 0; st < kBsCount; st++)
+		plan.total[st] = pre[(size_t)st * (n + 1) + n];
+	plan.frames = bt->frames_dev.as<FrameDesc>();
+	plan.pre = bt->pre_dev.as<uint32_t>();
+	plan.num_frames = (uint32_t)n;
+	plan.bin_lds = bin_lds;
+	plan.fp32 = (bt->rs[0]->opts & LUCID_OPT_FP32_SAMPLES) != 0;
+	plan.tex = tex;
+	plan.slice_words = bt->rs[0]->tuning.slice_words;
+	plan.status = bt->status_dev.as<uint32_t>();
+	// the launches read the frames' constants and buffers from device memory, so a frame
+	// whose inputs change only needs them uploaded; the graph is recorded again only when a
+	// grid, the LDS size or a kernel choice changes
+	if(frames.size() != bt->frames.size() ||
+	   memcmp(frames.data(), bt->frames.data(), n * sizeof(FrameDesc)) != 0) {
+		LUCID_CHECK(hipMemcpy(bt->frames_dev.ptr, frames.data(), n * sizeof(FrameDesc),
+							  hipMemcpyHostToDevice));
+		bt->frames = frames;
+	}
+	if(pre != bt->pre) {
+		LUCID_CHECK(hipMemcpy(bt->pre_dev.ptr, pre.data(), pre.size() * 4, hipMemcpyHostToDevice));
+		bt->pre = pre;
+	}
+	const bool same = bt->exec && memcmp(&plan, &bt->plan, sizeof(BatchPlan)) == 0;
+	bt->plan = plan;
 	if(!same) {
-		bt->cs = cs;
-		bt->bs = bs;
 		int e = capture_batch(bt);
 		if(e != LUCID_OK)
 			return e;
@@ -1205,7 +1251,9 @@ int lucid_batch_finish(lucid_batch *bt) {
 	LUCID_CHECK(hipEventSynchronize(bt->done));
 	bt->pending = false;
 	int rc = LUCID_OK;
-	for(lucid_renderer *r : bt->rs) {
+	for(size_t i = 0; i < bt->rs.size(); i++) {
+		lucid_renderer *r = bt->rs[i];
+		memcpy(r->status, bt->status + i * kCtrlWords, kCtrlWords * 4);
 		int g = grow_after_overflow(r);
 		memset(r->status, 0, kCtrlWords * 4);
 		if(g != LUCID_OK && rc == LUCID_OK)

@@ -801,10 +801,10 @@ __device__ __forceinline__ void enqueue_pixels(const FrameBufs &b, uint32_t slot
 
 // ------------------------------------------------------------------ rowgen_low
 
-__global__ __launch_bounds__(kRowgenThreads) void rowgen_low_kernel(FrameConst c, FrameBufs b) {
+__device__ __forceinline__ void rowgen_low_kernel_body(const FrameConst &c, const FrameBufs &b, WgPos wg) {
 	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
 		return;
-	const int bin = blockIdx.x;
+	const int bin = wg.id;
 	const BinInfo bi = load_bin(c, b, bin);
 	uint4 *slots = b.blk_slot + (size_t)bin * 16;
 	uint32_t *flags = b.blk_flags + (size_t)bin * 16;
@@ -879,11 +879,22 @@ __global__ __launch_bounds__(kRowgenThreads) void rowgen_low_kernel(FrameConst c
 	}
 }
 
+__global__ __launch_bounds__(kRowgenThreads) void rowgen_low_kernel(FrameConst c, FrameBufs b) {
+	rowgen_low_kernel_body(c, b, WgPos{blockIdx.x, gridDim.x});
+}
+__global__ __launch_bounds__(kRowgenThreads) void rowgen_low_kernel_batch(const FrameDesc *__restrict__ frames,
+		const uint32_t *__restrict__ pre, uint32_t num_frames) {
+	const StageBatch sb{frames, pre, num_frames};
+	const uint32_t f = batch_frame(sb, blockIdx.x);
+	const FrameDesc &d = sb.frames[f];
+	rowgen_low_kernel_body(d.c, d.b, WgPos{blockIdx.x - sb.pre[f], sb.pre[f + 1] - sb.pre[f]});
+}
+
 // ------------------------------------------------------------------ high bins
 
 // Allocates record and list space of every high bin (promotions included) with an exclusive
 // scan over the high list, and clears the counters rowgen_high accumulates into.
-__global__ __launch_bounds__(1024) void high_prep_kernel(FrameConst c, FrameBufs b) {
+__device__ __forceinline__ void high_prep_kernel_body(const FrameConst &c, const FrameBufs &b, WgPos wg) {
 	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
 		return;
 	const lucid_info *info = (const lucid_info *)b.info;
@@ -939,17 +950,28 @@ __global__ __launch_bounds__(1024) void high_prep_kernel(FrameConst c, FrameBufs
 	}
 }
 
+__global__ __launch_bounds__(1024) void high_prep_kernel(FrameConst c, FrameBufs b) {
+	high_prep_kernel_body(c, b, WgPos{blockIdx.x, gridDim.x});
+}
+__global__ __launch_bounds__(1024) void high_prep_kernel_batch(const FrameDesc *__restrict__ frames,
+		const uint32_t *__restrict__ pre, uint32_t num_frames) {
+	const StageBatch sb{frames, pre, num_frames};
+	const uint32_t f = batch_frame(sb, blockIdx.x);
+	const FrameDesc &d = sb.frames[f];
+	high_prep_kernel_body(d.c, d.b, WgPos{blockIdx.x - sb.pre[f], sb.pre[f + 1] - sb.pre[f]});
+}
+
 // Block-row records of a high bin with the 4-row rule (raster_high.glsl:54-90). Each lane
 // takes one triangle of the bin and scans its rows once, from the first row of its 4-row
 // group (the start row fixes the scanline arithmetic) to its last row, emitting one record
 // per 8-row band it covers; kRowgenSplit work-groups share a bin. Record and list
 // positions come from wave-aggregated atomics.
-__global__ __launch_bounds__(kRowgenThreads) void rowgen_high_kernel(FrameConst c, FrameBufs b) {
+__device__ __forceinline__ void rowgen_high_kernel_body(const FrameConst &c, const FrameBufs &b, WgPos wg) {
 	if(b.ctrl[kCtrlOverflow] & (kOvfBinLists | kOvfRecords | kOvfLists))
 		return;
 	const lucid_info *info = (const lucid_info *)b.info;
-	const uint32_t slot = blockIdx.x / kRowgenSplit;
-	const int split = blockIdx.x % kRowgenSplit;
+	const uint32_t slot = wg.id / kRowgenSplit;
+	const int split = wg.id % kRowgenSplit;
 	if(slot >= (uint32_t)info->bin_level_counts[LUCID_BIN_LEVEL_HIGH])
 		return;
 	const uint32_t *counts = b.info + LUCID_INFO_U32_SIZE;
@@ -1044,6 +1066,17 @@ __global__ __launch_bounds__(kRowgenThreads) void rowgen_high_kernel(FrameConst 
 	}
 }
 
+__global__ __launch_bounds__(kRowgenThreads) void rowgen_high_kernel(FrameConst c, FrameBufs b) {
+	rowgen_high_kernel_body(c, b, WgPos{blockIdx.x, gridDim.x});
+}
+__global__ __launch_bounds__(kRowgenThreads) void rowgen_high_kernel_batch(const FrameDesc *__restrict__ frames,
+		const uint32_t *__restrict__ pre, uint32_t num_frames) {
+	const StageBatch sb{frames, pre, num_frames};
+	const uint32_t f = batch_frame(sb, blockIdx.x);
+	const FrameDesc &d = sb.frames[f];
+	rowgen_high_kernel_body(d.c, d.b, WgPos{blockIdx.x - sb.pre[f], sb.pre[f + 1] - sb.pre[f]});
+}
+
 // Entries (triangles of one block in depth order) are handed to the rasteriser in slices
 // of up to kSliceEntries: entry lane + 64 k of the slice sits in register k of that lane.
 // (The block rasterisers take the slice width W = 1 or 2 words as a template parameter,
@@ -1065,12 +1098,12 @@ __device__ __forceinline__ uint32_t sorted_mask_offset(uint32_t m) {
 // pre-sorts long lists. A long list is sorted once by centroid depth (LDS bitonic by the
 // whole work-group) and split into `parts` pixel-row groups, each a compacted list of the
 // entries that cover it, so that several waves can rasterise a dense block.
-__global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst c, FrameBufs b) {
+__device__ __forceinline__ void high_finalize_kernel_body(const FrameConst &c, const FrameBufs &b, WgPos wg) {
 	if(b.ctrl[kCtrlOverflow] & (kOvfBinLists | kOvfRecords | kOvfLists))
 		return;
 	const lucid_info *info = (const lucid_info *)b.info;
-	const uint32_t hslot = blockIdx.x >> 4;
-	const int blk = blockIdx.x & 15, by = blk >> 2, bx = blk & 3;
+	const uint32_t hslot = wg.id >> 4;
+	const int blk = wg.id & 15, by = blk >> 2, bx = blk & 3;
 	if(hslot >= (uint32_t)info->bin_level_counts[LUCID_BIN_LEVEL_HIGH])
 		return;
 	const uint32_t *counts = b.info + LUCID_INFO_U32_SIZE;
@@ -1325,6 +1358,17 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst 
 	}
 }
 
+__global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst c, FrameBufs b) {
+	high_finalize_kernel_body(c, b, WgPos{blockIdx.x, gridDim.x});
+}
+__global__ __launch_bounds__(kSortThreads) void high_finalize_kernel_batch(const FrameDesc *__restrict__ frames,
+		const uint32_t *__restrict__ pre, uint32_t num_frames) {
+	const StageBatch sb{frames, pre, num_frames};
+	const uint32_t f = batch_frame(sb, blockIdx.x);
+	const FrameDesc &d = sb.frames[f];
+	high_finalize_kernel_body(d.c, d.b, WgPos{blockIdx.x - sb.pre[f], sb.pre[f + 1] - sb.pre[f]});
+}
+
 // ------------------------------------------------------------------ block_raster
 
 // 64 x 64 bit-matrix transpose across the wave: lane i holds row i (bit j = column j) and
@@ -1380,11 +1424,12 @@ __device__ __forceinline__ void stage_slice(const FrameConst &c, const FrameBufs
 	for(int k = 0; k < W; k++) {
 		const uint32_t i = k * 64 + lane;
 		if(i < n) {
-			uint4 q[kRecQuads];
-			load_shading_record(b, tri[k], q);
+			// quad by quad (no staging array: with a record pointer read from memory, as in a
+			// batched launch, an array between the load and the LDS store went to scratch)
+			const uint4 *src = b.tri_shade + (size_t)kRecStride * tri_rec(b, tri[k]);
 #pragma unroll
 			for(int j = 0; j < kRecQuads; j++)
-				s_rec[i * kRecQuads + j] = q[j];
+				s_rec[i * kRecQuads + j] = src[j];
 		}
 	}
 }
@@ -1478,12 +1523,12 @@ __device__ __forceinline__ void finish_pixels(const FrameConst &c, const FrameBu
 // sorts the list by centroid depth in registers, then walks every pixel through it slice by
 // slice (each slice's shading records staged in LDS).
 template <bool FP32, int W, bool TEX>
-__global__ __launch_bounds__(64) void block_raster_kernel(FrameConst c, FrameBufs b) {
+__device__ __forceinline__ void block_raster_kernel_body(const FrameConst &c, const FrameBufs &b, WgPos wg) {
 	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
 		return;
 	__shared__ uint4 s_rec[(64 * W) * kRecQuads];
 	const int lane = lane_id();
-	const uint32_t slot = blockIdx.x;
+	const uint32_t slot = wg.id;
 	const int bin = slot >> 4, blk = slot & 15, by = blk >> 2, bx = blk & 3;
 	const int bin_x = (bin % c.bin_count_x) * kBinSize, bin_y = (bin / c.bin_count_x) * kBinSize;
 	const int px0 = bin_x + bx * 8, py0 = bin_y + by * 8;
@@ -1563,21 +1608,33 @@ __global__ __launch_bounds__(64) void block_raster_kernel(FrameConst c, FrameBuf
 	finish_pixels(c, b, ps.acc, ps.w.bad, count, px, py, slot, m);
 }
 
+template <bool FP32, int W, bool TEX>
+__global__ __launch_bounds__(64) void block_raster_kernel(FrameConst c, FrameBufs b) {
+	block_raster_kernel_body<FP32, W, TEX>(c, b, WgPos{blockIdx.x, gridDim.x});
+}
+template <bool FP32, int W, bool TEX>
+__global__ __launch_bounds__(64) void block_raster_kernel_batch(const FrameDesc *__restrict__ frames,
+		const uint32_t *__restrict__ pre, uint32_t num_frames) {
+	const StageBatch sb{frames, pre, num_frames};
+	const uint32_t f = batch_frame(sb, blockIdx.x);
+	const FrameDesc &d = sb.frames[f];
+	block_raster_kernel_body<FP32, W, TEX>(d.c, d.b, WgPos{blockIdx.x - sb.pre[f], sb.pre[f + 1] - sb.pre[f]});
+}
+
 // Pre-sorted lists longer than one register sort: one work-group per block. Wave w takes
 // the contiguous depth range [start[w], start[w + 1]) of the sorted list (ranges balanced
 // by sample count), walks it slice by slice with one ordering window per pixel, and leaves
 // a partial composite; wave 0 chains the partials front to back. A pixel is exact iff each
 // range's blended keys increase and every range starts behind the previous one's last key.
 template <bool FP32, int W, bool TEX>
-__global__ __launch_bounds__(kBigWaves * 64) void block_raster_big_kernel(FrameConst c,
-																		   FrameBufs b) {
+__device__ __forceinline__ void block_raster_big_kernel_body(const FrameConst &c, const FrameBufs &b, WgPos wg) {
 	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
 		return;
 	const lucid_info *info = (const lucid_info *)b.info;
-	const uint32_t hslot = blockIdx.x >> 4;
+	const uint32_t hslot = wg.id >> 4;
 	if(hslot >= (uint32_t)info->bin_level_counts[LUCID_BIN_LEVEL_HIGH])
 		return;
-	const int blk = blockIdx.x & 15, by = blk >> 2, bx = blk & 3;
+	const int blk = wg.id & 15, by = blk >> 2, bx = blk & 3;
 	const int bin = (int)b.info[LUCID_INFO_U32_SIZE + LUCID_COUNTS_HIGH_BINS * c.bin_count + hslot];
 	const uint32_t slot = (uint32_t)bin * 16 + blk;
 	const uint32_t flags = b.blk_flags[slot];
@@ -1658,6 +1715,19 @@ __global__ __launch_bounds__(kBigWaves * 64) void block_raster_big_kernel(FrameC
 		}
 	}
 	finish_pixels(c, b, acc, bad, count, px, py, slot, m);
+}
+
+template <bool FP32, int W, bool TEX>
+__global__ __launch_bounds__(kBigWaves * 64) void block_raster_big_kernel(FrameConst c, FrameBufs b) {
+	block_raster_big_kernel_body<FP32, W, TEX>(c, b, WgPos{blockIdx.x, gridDim.x});
+}
+template <bool FP32, int W, bool TEX>
+__global__ __launch_bounds__(kBigWaves * 64) void block_raster_big_kernel_batch(const FrameDesc *__restrict__ frames,
+		const uint32_t *__restrict__ pre, uint32_t num_frames) {
+	const StageBatch sb{frames, pre, num_frames};
+	const uint32_t f = batch_frame(sb, blockIdx.x);
+	const FrameDesc &d = sb.frames[f];
+	block_raster_big_kernel_body<FP32, W, TEX>(d.c, d.b, WgPos{blockIdx.x - sb.pre[f], sb.pre[f + 1] - sb.pre[f]});
 }
 
 // ------------------------------------------------------------------ exact fallback
@@ -1804,9 +1874,9 @@ __device__ void fallback_pixel(const FrameConst &c, const FrameBufs &b, uint4 it
 
 // Persistent over the per-pixel queue: wave w of the grid takes items w, w + waves, ...
 template <bool FP32, bool TEX>
-__global__ __launch_bounds__(kFallbackThreads) void fallback_kernel(FrameConst c, FrameBufs b) {
+__device__ __forceinline__ void fallback_kernel_body(const FrameConst &c, const FrameBufs &b, WgPos wg) {
 	__shared__ uint64_t s_keys[kFallbackThreads / 64][kFallbackCap];
-	if(blockIdx.x == 0 && threadIdx.x < 64) {
+	if(wg.id == 0 && threadIdx.x < 64) {
 		// the block kernels' statistics stripes into LucidInfo (stats[0] fragments, [1]
 		// block-list entries); the block kernels have finished
 		static_assert(kStatStripes == 64, "one stripe per lane");
@@ -1819,10 +1889,23 @@ __global__ __launch_bounds__(kFallbackThreads) void fallback_kernel(FrameConst c
 		}
 	}
 	const uint32_t nq = min(b.ctrl[kCtrlFallbackCount], b.fallback_cap);
-	const uint32_t waves = gridDim.x * (kFallbackThreads / 64);
-	const uint32_t w = blockIdx.x * (kFallbackThreads / 64) + (threadIdx.x >> 6);
+	const uint32_t waves = wg.count * (kFallbackThreads / 64);
+	const uint32_t w = wg.id * (kFallbackThreads / 64) + (threadIdx.x >> 6);
 	for(uint32_t qi = w; qi < nq; qi += waves)
 		fallback_pixel<FP32, TEX>(c, b, b.fallback_queue[qi], s_keys[threadIdx.x >> 6]);
+}
+
+template <bool FP32, bool TEX>
+__global__ __launch_bounds__(kFallbackThreads) void fallback_kernel(FrameConst c, FrameBufs b) {
+	fallback_kernel_body<FP32, TEX>(c, b, WgPos{blockIdx.x, gridDim.x});
+}
+template <bool FP32, bool TEX>
+__global__ __launch_bounds__(kFallbackThreads) void fallback_kernel_batch(const FrameDesc *__restrict__ frames,
+		const uint32_t *__restrict__ pre, uint32_t num_frames) {
+	const StageBatch sb{frames, pre, num_frames};
+	const uint32_t f = batch_frame(sb, blockIdx.x);
+	const FrameDesc &d = sb.frames[f];
+	fallback_kernel_body<FP32, TEX>(d.c, d.b, WgPos{blockIdx.x - sb.pre[f], sb.pre[f + 1] - sb.pre[f]});
 }
 
 // ------------------------------------------------------------------ backward (adjoint)
@@ -2616,6 +2699,57 @@ static void launch_raster_t(const FrameConst &c, const FrameBufs &b, hipStream_t
 					   c, b);
 	if(ev)
 		(void)hipEventRecord(ev[3], stream);
+}
+
+template <bool FP32, int W, bool TEX>
+static void launch_raster_batch_t(const BatchPlan &p, hipStream_t stream) {
+	hipLaunchKernelGGL(rowgen_low_kernel_batch, dim3(p.total[kBsRowgenLow]), dim3(kRowgenThreads), 0,
+					   stream, p.frames, p.stage_pre(kBsRowgenLow), p.num_frames);
+	hipLaunchKernelGGL(high_prep_kernel_batch, dim3(p.total[kBsHighPrep]), dim3(1024), 0, stream,
+					   p.frames, p.stage_pre(kBsHighPrep), p.num_frames);
+	hipLaunchKernelGGL(rowgen_high_kernel_batch, dim3(p.total[kBsRowgenHigh]),
+					   dim3(kRowgenThreads), 0, stream, p.frames, p.stage_pre(kBsRowgenHigh), p.num_frames);
+	hipLaunchKernelGGL(high_finalize_kernel_batch, dim3(p.total[kBsHighFinalize]),
+					   dim3(kSortThreads), 0, stream, p.frames, p.stage_pre(kBsHighFinalize), p.num_frames);
+	hipLaunchKernelGGL((block_raster_kernel_batch<FP32, W, TEX>), dim3(p.total[kBsBlockRaster]),
+					   dim3(64), 0, stream, p.frames, p.stage_pre(kBsBlockRaster), p.num_frames);
+	hipLaunchKernelGGL((block_raster_big_kernel_batch<FP32, W, TEX>),
+					   dim3(p.total[kBsBlockRasterBig]), dim3(kBigWaves * 64), 0, stream,
+					   p.frames, p.stage_pre(kBsBlockRasterBig), p.num_frames);
+	hipLaunchKernelGGL((fallback_kernel_batch<FP32, TEX>), dim3(p.total[kBsFallback]),
+					   dim3(kFallbackThreads), 0, stream, p.frames, p.stage_pre(kBsFallback), p.num_frames);
+}
+
+// Each frame's control words, for the host's overflow check (one copy per batch).
+__global__ __launch_bounds__(64) void batch_status_kernel(const FrameDesc *frames, uint32_t n,
+														 uint32_t *status) {
+	const uint32_t f = blockIdx.x;
+	if(f < n && threadIdx.x < (uint32_t)kCtrlWords)
+		status[(size_t)f * kCtrlWords + threadIdx.x] = frames[f].b.ctrl[threadIdx.x];
+}
+
+// Row generation, sorting, block rasterisation and the exact fallback of every frame of a
+// batch: one launch per stage, kernels chosen by the batch's sample format, slice width and
+// texture use; then the frames' control words into p.status.
+hipError_t launch_raster_batch(const BatchPlan &p, hipStream_t stream) {
+	if(p.slice_words == 2) {
+		if(p.fp32)
+			p.tex ? launch_raster_batch_t<true, 2, true>(p, stream)
+				  : launch_raster_batch_t<true, 2, false>(p, stream);
+		else
+			p.tex ? launch_raster_batch_t<false, 2, true>(p, stream)
+				  : launch_raster_batch_t<false, 2, false>(p, stream);
+	} else {
+		if(p.fp32)
+			p.tex ? launch_raster_batch_t<true, 1, true>(p, stream)
+				  : launch_raster_batch_t<true, 1, false>(p, stream);
+		else
+			p.tex ? launch_raster_batch_t<false, 1, true>(p, stream)
+				  : launch_raster_batch_t<false, 1, false>(p, stream);
+	}
+	hipLaunchKernelGGL(batch_status_kernel, dim3(p.num_frames), dim3(64), 0, stream, p.frames,
+					   p.num_frames, p.status);
+	return hipGetLastError();
 }
 
 // events (optional, 4): after rowgen_low, after the high-bin row generation and sorting,

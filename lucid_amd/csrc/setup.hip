@@ -266,13 +266,13 @@ __device__ int process_quad(const FrameConst &c, const FrameBufs &b, uint32_t q,
 	return area <= 4u ? 0 : 1;
 }
 
-__global__ __launch_bounds__(kSetupThreads) void setup_kernel(FrameConst c, FrameBufs b) {
+__device__ __forceinline__ void setup_kernel_body(const FrameConst &c, const FrameBufs &b, WgPos wg) {
 	__shared__ uint32_t s_counts[6]; // small, large, rejected x4
 	if(threadIdx.x < 6)
 		s_counts[threadIdx.x] = 0;
 	__syncthreads();
 
-	uint32_t q = blockIdx.x * (uint32_t)kSetupThreads + threadIdx.x;
+	uint32_t q = wg.id * (uint32_t)kSetupThreads + threadIdx.x;
 	int kind = -1;
 	if(q < (uint32_t)c.num_quads) {
 		int lo = 0, hi = c.num_instances - 1;
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(kSetupThreads) void setup_kernel(FrameConst c, Fram
 	}
 	// wave-aggregated counters and this work-group's large-quad sub-list
 	uint64_t large = ballot(kind == 1);
-	const uint32_t stripe = blockIdx.x % kLargeStripes;
+	const uint32_t stripe = wg.id % kLargeStripes;
 	uint32_t base = 0;
 	if(lane_id() == 0 && large)
 		base = atomicAdd(large_cursor(b.ctrl, stripe), (uint32_t)__popcll(large));
@@ -320,11 +320,22 @@ __global__ __launch_bounds__(kSetupThreads) void setup_kernel(FrameConst c, Fram
 	}
 }
 
+__global__ __launch_bounds__(kSetupThreads) void setup_kernel(FrameConst c, FrameBufs b) {
+	setup_kernel_body(c, b, WgPos{blockIdx.x, gridDim.x});
+}
+__global__ __launch_bounds__(kSetupThreads) void setup_kernel_batch(const FrameDesc *__restrict__ frames,
+		const uint32_t *__restrict__ pre, uint32_t num_frames) {
+	const StageBatch sb{frames, pre, num_frames};
+	const uint32_t f = batch_frame(sb, blockIdx.x);
+	const FrameDesc &d = sb.frames[f];
+	setup_kernel_body(d.c, d.b, WgPos{blockIdx.x - sb.pre[f], sb.pre[f + 1] - sb.pre[f]});
+}
+
 // Per-frame reset of LucidInfo + per-bin counters and the control words
 // (src/lucid_renderer.cpp:437 clears the same region with a fill).
-__global__ __launch_bounds__(256) void clear_kernel(uint32_t *info, uint32_t n_info, uint32_t *ctrl,
-													uint32_t num_quads, uint32_t keep_overflow) {
-	uint32_t i = blockIdx.x * 256u + threadIdx.x;
+__device__ __forceinline__ void clear_body(uint32_t *info, uint32_t n_info, uint32_t *ctrl,
+										   uint32_t num_quads, uint32_t keep_overflow, uint32_t wg) {
+	uint32_t i = wg * 256u + threadIdx.x;
 	if(i < n_info)
 		info[i] = i == 0 ? num_quads : 0u;
 	// control words and statistics stripes; the overflow bits of an unfinished earlier
@@ -336,6 +347,29 @@ __global__ __launch_bounds__(256) void clear_kernel(uint32_t *info, uint32_t n_i
 	} else if(i < (uint32_t)kCtrlAllocWords && i != (uint32_t)kCtrlOverflowCarry) {
 		ctrl[i] = 0u;
 	}
+}
+
+__global__ __launch_bounds__(256) void clear_kernel(uint32_t *info, uint32_t n_info, uint32_t *ctrl,
+													uint32_t num_quads, uint32_t keep_overflow) {
+	clear_body(info, n_info, ctrl, num_quads, keep_overflow, blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void clear_kernel_batch(const FrameDesc *__restrict__ frames,
+		const uint32_t *__restrict__ pre, uint32_t num_frames) {
+	const StageBatch sb{frames, pre, num_frames};
+	const uint32_t f = batch_frame(sb, blockIdx.x);
+	const FrameDesc &d = sb.frames[f];
+	clear_body(d.b.info, LUCID_INFO_U32_SIZE + 10u * (uint32_t)d.c.bin_count, d.b.ctrl,
+			   (uint32_t)d.c.num_quads, d.c.keep_overflow, blockIdx.x - sb.pre[f]);
+}
+
+hipError_t launch_setup_batch(const BatchPlan &p, hipStream_t stream) {
+	hipLaunchKernelGGL(clear_kernel_batch, dim3(p.total[kBsClear]), dim3(256), 0, stream,
+					   p.frames, p.stage_pre(kBsClear), p.num_frames);
+	if(p.total[kBsSetup])
+		hipLaunchKernelGGL(setup_kernel_batch, dim3(p.total[kBsSetup]), dim3(kSetupThreads), 0,
+						   stream, p.frames, p.stage_pre(kBsSetup), p.num_frames);
+	return hipGetLastError();
 }
 
 hipError_t launch_setup(const FrameConst &c, const FrameBufs &b, uint32_t n_info,
