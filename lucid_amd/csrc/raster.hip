@@ -778,6 +778,44 @@ __device__ __forceinline__ BinInfo load_bin(const FrameConst &c, const FrameBufs
 	return bi;
 }
 
+// The live triangles of a bin's list, handed to a wave 64 at a time. The list holds quads
+// (two triangle ids each) and large triangles; a culled triangle (every second one of a
+// soup quad) would leave its lane idle through the whole row scan, so the wave gathers the
+// live ones of its chunks (first, first + stride, ...) in a 128-entry LDS queue and calls
+// fn(t) with a full wave of them (the last call may be partial: t = ~0 on idle lanes). fn is
+// called by the whole wave.
+template <class F>
+__device__ __forceinline__ void for_live_tris(const FrameBufs &b, const BinInfo &bi, uint32_t first,
+											  uint32_t stride, uint32_t *q, F &&fn) {
+	const int lane = lane_id();
+	uint32_t pending = 0; // queued triangles (< 64 between calls)
+	for(uint32_t base = first;;) {
+		if(base < bi.nlevel) {
+			const uint32_t i = base + lane;
+			const uint32_t t = i < bi.nlevel ? bin_tri_at(b, i, bi.qc, bi.qo, bi.to) : 0xffffffffu;
+			const uint64_t live = ballot(t != 0xffffffffu);
+			if(t != 0xffffffffu)
+				q[pending + (uint32_t)__popcll(live & ((1ull << lane) - 1ull))] = t;
+			pending += (uint32_t)__popcll(live);
+			base += stride;
+		}
+		const bool last = base >= bi.nlevel;
+		if(pending >= 64 || (last && pending > 0)) {
+			wave_sync_lds();
+			const uint32_t t = (uint32_t)lane < pending ? q[lane] : 0xffffffffu;
+			const uint32_t rem = pending > 64 ? pending - 64 : 0;
+			const uint32_t carry = (uint32_t)lane < rem ? q[64 + lane] : 0u;
+			wave_sync_lds();
+			if((uint32_t)lane < rem)
+				q[lane] = carry;
+			pending = rem;
+			fn(t);
+		}
+		if(last && pending == 0)
+			break;
+	}
+}
+
 // Exact-fallback queue: one uint4 item per pixel, {block slot, px, py, whole}. Called by
 // a whole wave; lanes with `mine` set queue their pixel (px, py).
 __device__ __forceinline__ void enqueue_pixels(const FrameBufs &b, uint32_t slot, bool mine, int px,
@@ -826,11 +864,11 @@ __device__ __forceinline__ void rowgen_low_kernel_body(const FrameConst &c, cons
 
 	uint4 *recs = b.low_records + (size_t)bin * 4 * kLowRowCap;
 	uint32_t *lists = b.blk_lists + (size_t)bin * 16 * kRegSortCap;
-	// block-row records, 8-row rule (raster_low.glsl:39-79)
-	for(uint32_t i = threadIdx.x; i < bi.nlevel; i += kRowgenThreads) {
-		uint32_t t = bin_tri_at(b, i, bi.qc, bi.qo, bi.to);
+	// block-row records, 8-row rule (raster_low.glsl:39-79), one live triangle per lane
+	__shared__ uint32_t s_live[kRowgenThreads / 64][128];
+	for_live_tris(b, bi, threadIdx.x & ~63u, kRowgenThreads, s_live[threadIdx.x >> 6], [&](uint32_t t) {
 		if(t == 0xffffffffu)
-			continue;
+			return;
 		uint4 s0 = b.tri_scan[2 * (size_t)tri_rec(b, t)], s1 = b.tri_scan[2 * (size_t)tri_rec(b, t) + 1];
 		int min_by = min(max((int)(s0.w & 0xffffu) - bi.bin_y, 0), kBinMask) >> 3;
 		int max_by = min(max((int)(s0.w >> 16) - bi.bin_y, 0), kBinMask) >> 3;
@@ -854,7 +892,7 @@ __device__ __forceinline__ void rowgen_low_kernel_body(const FrameConst &c, cons
 						lists[(by * 4 + bx) * kRegSortCap + k] = idx;
 				}
 		}
-	}
+	});
 	__syncthreads();
 	bool promote = false;
 	for(int k = 0; k < 16; k++)
@@ -983,9 +1021,8 @@ __device__ __forceinline__ void rowgen_high_kernel_body(const FrameConst &c, con
 	const int lane = lane_id();
 	const uint32_t stride = kRowgenThreads * kRowgenSplit;
 	const uint32_t wave_base = split * kRowgenThreads + (threadIdx.x & ~63u);
-	for(uint32_t base = wave_base; base < bi.nlevel; base += stride) {
-		const uint32_t i = base + lane;
-		const uint32_t t = i < bi.nlevel ? bin_tri_at(b, i, bi.qc, bi.qo, bi.to) : 0xffffffffu;
+	__shared__ uint32_t s_live[kRowgenThreads / 64][128];
+	for_live_tris(b, bi, wave_base, stride, s_live[threadIdx.x >> 6], [&](uint32_t t) {
 		int row = 0, row_lo = 0, row_hi = -1;
 		RowScan rs;
 		if(t != 0xffffffffu) {
@@ -1063,7 +1100,7 @@ __device__ __forceinline__ void rowgen_high_kernel_body(const FrameConst &c, con
 								(uint32_t)__popcll(sel[bx] & ((1ull << lane) - 1ull))] = idx;
 			}
 		}
-	}
+	});
 }
 
 __global__ __launch_bounds__(kRowgenThreads) void rowgen_high_kernel(FrameConst c, FrameBufs b) {
