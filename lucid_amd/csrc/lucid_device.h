@@ -287,13 +287,28 @@ __device__ __forceinline__ uint32_t batch_frame(const StageBatch &sb, uint32_t w
 
 // A batched launch: the frames (device array), the per-stage prefixes (device,
 // kBsCount x (num_frames + 1)), and what selects the kernels (shared by every frame).
+// Shading modes of the block rasterisers: general shading records, the same with albedo
+// textures, or flat colour planes (RGBA8 samples, unit lighting, vertex colours absent or
+// fp32 with a per-triangle alpha, no textures: a sample's colour is then an affine function
+// of the pixel divided by the inverse depth).
+enum ShadeMode : int { kShadeGeneral = 0, kShadeTextured = 1, kShadeFlat = 2 };
+
+inline int frame_shade_mode(const FrameConst &c) {
+	if(c.has_textures)
+		return kShadeTextured;
+	const bool fp32 = (c.opts & LUCID_OPT_FP32_SAMPLES) != 0;
+	const bool flat_colors = c.color_mode == kColorNone || c.color_mode == kColorF32;
+	return !fp32 && c.identity_light && flat_colors ? kShadeFlat : kShadeGeneral;
+}
+
 struct BatchPlan {
 	const FrameDesc *frames;
 	const uint32_t *pre;
 	uint32_t num_frames;
 	uint32_t total[kBsCount]; // work-groups of each stage over all frames
 	size_t bin_lds;           // dynamic LDS of the bin kernels: the largest frame's
-	bool fp32, tex;
+	bool fp32;
+	int mode; // ShadeMode of the block kernels: textured if a frame is, flat if every frame is
 	int slice_words;
 	uint32_t *status; // device, kCtrlWords per frame: each frame's control words after it
 	__host__ const uint32_t *stage_pre(int s) const { return pre + (size_t)s * (num_frames + 1); }

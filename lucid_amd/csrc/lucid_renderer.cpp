@@ -1174,7 +1174,7 @@ int lucid_batch_render(lucid_batch *bt, const lucid_frame_inputs *ins,
 	}
 	const size_t n = bt->rs.size();
 	std::vector<FrameDesc> frames(n);
-	bool tex = false;
+	bool tex = false, flat = true;
 	for(size_t i = 0; i < n; i++) {
 		lucid_renderer *r = bt->rs[i];
 		if(r->frame_pending) {
@@ -1190,6 +1190,7 @@ int lucid_batch_render(lucid_batch *bt, const lucid_frame_inputs *ins,
 		// knob: results do not depend on it)
 		frames[i].c.slice_words = bt->rs[0]->tuning.slice_words;
 		tex = tex || frames[i].c.has_textures;
+		flat = flat && frame_shade_mode(frames[i].c) == kShadeFlat;
 		r->last_num_quads = nq;
 		r->last_stream = stream;
 		r->retained.valid = r->retained.pending = false;
@@ -1213,7 +1214,7 @@ int lucid_batch_render(lucid_batch *bt, const lucid_frame_inputs *ins,
 	plan.num_frames = (uint32_t)n;
 	plan.bin_lds = bin_lds;
 	plan.fp32 = (bt->rs[0]->opts & LUCID_OPT_FP32_SAMPLES) != 0;
-	plan.tex = tex;
+	plan.mode = tex ? kShadeTextured : flat ? kShadeFlat : kShadeGeneral;
 	plan.slice_words = bt->rs[0]->tuning.slice_words;
 	plan.status = bt->status_dev.as<uint32_t>();
 	// the launches read the frames' constants and buffers from device memory, so a frame

@@ -1451,8 +1451,77 @@ __device__ __forceinline__ void transpose_masks(const uint64_t (&msk)[W], int nw
 		bm[k] = k < nwords ? transpose64(msk[k]) : 0ull;
 }
 
+// ------------------------------------------------------------------ flat colour records
+//
+// Shading modes of the block rasterisers (ShadeMode): the general record (shade_record),
+// the same with albedo textures, and flat colour planes for frames with RGBA8 samples, unit
+// lighting (identity_light), vertex colours absent or fp32 with a per-triangle alpha, and
+// no textures. There a sample's colour is an affine function of the barycentrics, so per
+// channel (with d1 = c1 - c0, d2 = c2 - c0 and the barycentric planes e0, e1 with offsets
+// P0, P1 of shading.glsl:116-120)
+//   c0 + b0 d1 + b1 d2 = K + ray_pos (Lx px + Ly py + Lz),
+//   L = e0 d1 + e1 d2,  K = c0 - P0 d1 - P1 d2,
+// which the staging computes once per (block, triangle) and the pixel walk evaluates with
+// three fused multiply-adds per channel instead of the barycentric interpolation. The value
+// is the same affine function (rounding differs by a few ulps, inside the 1/255 colour bar);
+// the depth key keeps its unfused evaluation. The instance colour is folded into the planes.
+constexpr int kFlatQuads = 5; // f0 depth plane | triangle id; f1-f3 r, g, b planes; f4 alpha
+constexpr int rec_quads(int mode) { return mode == kShadeFlat ? kFlatQuads : kRecQuads; }
+
+__device__ __forceinline__ void make_flat_record(const uint4 *src, uint4 *dst) {
+	const uint4 q0 = src[0], q1 = src[1], q2 = src[2], q3 = src[3], q4 = src[4], q5 = src[5],
+				q6 = src[6];
+	const uint32_t flags = q0.w & 0xffffu;
+	float c0[3] = {1.0f, 1.0f, 1.0f}, c1[3] = {1.0f, 1.0f, 1.0f}, c2[3] = {1.0f, 1.0f, 1.0f};
+	float a = 1.0f;
+	if(flags & LUCID_INST_HAS_VERTEX_COLORS) {
+		c0[0] = bits2f(q3.x), c0[1] = bits2f(q3.y), c0[2] = bits2f(q3.z), a = bits2f(q3.w);
+		c1[0] = bits2f(q4.x), c1[1] = bits2f(q4.y), c1[2] = bits2f(q4.z);
+		c2[0] = bits2f(q5.x), c2[1] = bits2f(q5.y), c2[2] = bits2f(q5.z);
+	}
+	if(flags & LUCID_INST_HAS_COLOR) {
+		const float4 ic = decode_rgba8(q6.x);
+		const float icc[3] = {ic.x, ic.y, ic.z};
+#pragma unroll
+		for(int k = 0; k < 3; k++)
+			c0[k] = icc[k] * c0[k], c1[k] = icc[k] * c1[k], c2[k] = icc[k] * c2[k];
+		a = ic.w * a;
+	}
+	const float e0[3] = {bits2f(q1.x), bits2f(q1.y), bits2f(q1.z)}, p0 = bits2f(q1.w);
+	const float e1[3] = {bits2f(q2.x), bits2f(q2.y), bits2f(q2.z)}, p1 = bits2f(q2.w);
+	dst[0] = make_uint4(q0.x, q0.y, q0.z, q6.z);
+#pragma unroll
+	for(int k = 0; k < 3; k++) {
+		const float d1 = c1[k] - c0[k], d2 = c2[k] - c0[k];
+		const float lx = fmaf(e0[0], d1, e1[0] * d2), ly = fmaf(e0[1], d1, e1[1] * d2);
+		const float lz = fmaf(e0[2], d1, e1[2] * d2);
+		const float kk = fmaf(-p0, d1, fmaf(-p1, d2, c0[k]));
+		dst[1 + k] = make_uint4(f2bits(lx), f2bits(ly), f2bits(lz), f2bits(kk));
+	}
+	// RGBA8 alpha byte; a zero alpha contributes nothing (shading.glsl:167-168)
+	dst[4] = make_uint4(f2u(a * 255.0f) << 24, a == 0.0f ? 1u : 0u, 0u, 0u);
+}
+
+// RGBA8 sample of a flat record at pixel (px, py); inv_ray = record_inv_depth(f[0], ...).
+__device__ __forceinline__ SampleColor<false> shade_flat(const uint4 *f, float px, float py,
+														 float inv_ray) {
+	const float rp = __builtin_amdgcn_rcpf(inv_ray);
+	float rgb[3];
+#pragma unroll
+	for(int k = 0; k < 3; k++) {
+		const uint4 pl = f[1 + k];
+		const float l = fmaf(bits2f(pl.x), px, fmaf(bits2f(pl.y), py, bits2f(pl.z)));
+		rgb[k] = sat(fmaf(rp, l, bits2f(pl.w)));
+	}
+	const uint4 f4 = f[4];
+	SampleColor<false> out;
+	out.enc = f4.y ? 0u : ((uint32_t)(rgb[0] * 255.0f) | ((uint32_t)(rgb[1] * 255.0f) << 8) |
+						   ((uint32_t)(rgb[2] * 255.0f) << 16) | f4.x);
+	return out;
+}
+
 // Stages the shading records of a slice in LDS: lane l writes entries l + 64 k.
-template <int W>
+template <int W, int MODE = kShadeGeneral>
 __device__ __forceinline__ void stage_slice(const FrameConst &c, const FrameBufs &b,
 											const uint32_t (&tri)[W], uint32_t n,
 											uint4 *s_rec) {
@@ -1464,9 +1533,13 @@ __device__ __forceinline__ void stage_slice(const FrameConst &c, const FrameBufs
 			// quad by quad (no staging array: with a record pointer read from memory, as in a
 			// batched launch, an array between the load and the LDS store went to scratch)
 			const uint4 *src = b.tri_shade + (size_t)kRecStride * tri_rec(b, tri[k]);
+			if constexpr(MODE == kShadeFlat) {
+				make_flat_record(src, s_rec + i * kFlatQuads);
+			} else {
 #pragma unroll
-			for(int j = 0; j < kRecQuads; j++)
-				s_rec[i * kRecQuads + j] = src[j];
+				for(int j = 0; j < kRecQuads; j++)
+					s_rec[i * kRecQuads + j] = src[j];
+			}
 		}
 	}
 }
@@ -1511,11 +1584,28 @@ __device__ __forceinline__ void walk_bits(uint64_t (&bm)[W], Visit &&visit) {
 	}
 }
 
-template <bool FP32, int W, bool TEX>
+template <bool FP32, int W, int MODE>
 __device__ __forceinline__ void walk_pixel(const FrameConst &c, const FrameBufs &b,
 										   const uint4 *s_rec, uint64_t (&bm)[W], float px,
 										   float py, PixelState<FP32> &ps, bool additive) {
 	const bool unsorted = (c.opts & LUCID_OPT_UNSORTED) != 0;
+	if constexpr(MODE == kShadeFlat) {
+		static_assert(!FP32, "flat records carry RGBA8 samples");
+		walk_bits(bm, [&](int e) {
+			const uint4 *rec = s_rec + e * kFlatQuads;
+			const uint4 f0 = rec[0];
+			const float inv = record_inv_depth(f0, px, py);
+			const SampleColor<false> col = shade_flat(rec, px, py, inv);
+			if(col.contributes()) {
+				if(unsorted)
+					ps.acc.blend(col.rgba(), additive);
+				else
+					ps.insert(pixel_key(inv, f0.w), col, additive);
+			}
+		});
+		return;
+	}
+	constexpr bool TEX = MODE == kShadeTextured;
 	walk_bits(bm, [&](int e) {
 		const uint4 *rec = s_rec + e * kRecQuads;
 		const uint4 q0 = rec[0];
@@ -1559,11 +1649,11 @@ __device__ __forceinline__ void finish_pixels(const FrameConst &c, const FrameBu
 // One wave per block whose list fits the register sort (every low bin, most high blocks):
 // sorts the list by centroid depth in registers, then walks every pixel through it slice by
 // slice (each slice's shading records staged in LDS).
-template <bool FP32, int W, bool TEX>
+template <bool FP32, int W, int MODE>
 __device__ __forceinline__ void block_raster_kernel_body(const FrameConst &c, const FrameBufs &b, WgPos wg) {
 	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
 		return;
-	__shared__ uint4 s_rec[(64 * W) * kRecQuads];
+	__shared__ uint4 s_rec[(64 * W) * rec_quads(MODE)];
 	const int lane = lane_id();
 	const uint32_t slot = wg.id;
 	const int bin = slot >> 4, blk = slot & 15, by = blk >> 2, bx = blk & 3;
@@ -1630,14 +1720,14 @@ __device__ __forceinline__ void block_raster_kernel_body(const FrameConst &c, co
 		}
 		const uint32_t n = min(m - s0, (uint32_t)(64 * W));
 		if(!opaque)
-			stage_slice(c, b, tri, n, s_rec); // loads in flight during the transpose
+			stage_slice<W, MODE>(c, b, tri, n, s_rec); // loads in flight during the transpose
 		uint64_t bm[W];
 		transpose_masks(msk, (int)((n + 63) / 64), bm);
 		count += bitmap_count(bm);
 		if(opaque)
 			continue; // counted, not shaded
 		wave_sync_lds();
-		walk_pixel<FP32, W, TEX>(c, b, s_rec, bm, (float)px, (float)py, ps, additive);
+		walk_pixel<FP32, W, MODE>(c, b, s_rec, bm, (float)px, (float)py, ps, additive);
 		wave_sync_lds(); // every lane is done with this slice's records
 		opaque = threshold && all_below_alpha_threshold(ps.acc.tr);
 	}
@@ -1645,17 +1735,17 @@ __device__ __forceinline__ void block_raster_kernel_body(const FrameConst &c, co
 	finish_pixels(c, b, ps.acc, ps.w.bad, count, px, py, slot, m);
 }
 
-template <bool FP32, int W, bool TEX>
+template <bool FP32, int W, int MODE>
 __global__ __launch_bounds__(64) void block_raster_kernel(FrameConst c, FrameBufs b) {
-	block_raster_kernel_body<FP32, W, TEX>(c, b, WgPos{blockIdx.x, gridDim.x});
+	block_raster_kernel_body<FP32, W, MODE>(c, b, WgPos{blockIdx.x, gridDim.x});
 }
-template <bool FP32, int W, bool TEX>
+template <bool FP32, int W, int MODE>
 __global__ __launch_bounds__(64) void block_raster_kernel_batch(const FrameDesc *__restrict__ frames,
 		const uint32_t *__restrict__ pre, uint32_t num_frames) {
 	const StageBatch sb{frames, pre, num_frames};
 	const uint32_t f = batch_frame(sb, blockIdx.x);
 	const FrameDesc &d = sb.frames[f];
-	block_raster_kernel_body<FP32, W, TEX>(d.c, d.b, WgPos{blockIdx.x - sb.pre[f], sb.pre[f + 1] - sb.pre[f]});
+	block_raster_kernel_body<FP32, W, MODE>(d.c, d.b, WgPos{blockIdx.x - sb.pre[f], sb.pre[f + 1] - sb.pre[f]});
 }
 
 // Pre-sorted lists longer than one register sort: one work-group per block. Wave w takes
@@ -1663,7 +1753,7 @@ __global__ __launch_bounds__(64) void block_raster_kernel_batch(const FrameDesc 
 // by sample count), walks it slice by slice with one ordering window per pixel, and leaves
 // a partial composite; wave 0 chains the partials front to back. A pixel is exact iff each
 // range's blended keys increase and every range starts behind the previous one's last key.
-template <bool FP32, int W, bool TEX>
+template <bool FP32, int W, int MODE>
 __device__ __forceinline__ void block_raster_big_kernel_body(const FrameConst &c, const FrameBufs &b, WgPos wg) {
 	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
 		return;
@@ -1677,7 +1767,7 @@ __device__ __forceinline__ void block_raster_big_kernel_body(const FrameConst &c
 	const uint32_t flags = b.blk_flags[slot];
 	if(!(flags & kSlotPresorted) || (flags & kSlotPromoted) || b.ctrl[kCtrlRecorded])
 		return; // (a recording pass A renders the block)
-	__shared__ uint4 s_rec[kBigWaves][(64 * W) * kRecQuads];
+	__shared__ uint4 s_rec[kBigWaves][(64 * W) * rec_quads(MODE)];
 	__shared__ float4 s_part[kBigWaves][64];
 	__shared__ uint64_t s_first[kBigWaves][64], s_last[kBigWaves][64];
 	__shared__ uint32_t s_cnt[kBigWaves][64];
@@ -1710,14 +1800,14 @@ __device__ __forceinline__ void block_raster_big_kernel_body(const FrameConst &c
 				t[k] = i < n && !opaque ? tri[s0 + i] : 0u;
 			}
 			if(!opaque)
-				stage_slice(c, b, t, n, s_rec[wave]); // loads in flight during the transpose
+				stage_slice<W, MODE>(c, b, t, n, s_rec[wave]); // loads in flight during the transpose
 			uint64_t bm[W];
 			transpose_masks(msk, (int)((n + 63) / 64), bm);
 			count += bitmap_count(bm);
 			if(opaque)
 				continue; // counted, not shaded
 			wave_sync_lds();
-			walk_pixel<FP32, W, TEX>(c, b, s_rec[wave], bm, (float)px, (float)py, ps, additive);
+			walk_pixel<FP32, W, MODE>(c, b, s_rec[wave], bm, (float)px, (float)py, ps, additive);
 			wave_sync_lds(); // every lane is done with this slice's records
 			// the range's own transmittance bounds the pixel's (earlier ranges only lower it)
 			opaque = threshold && all_below_alpha_threshold(ps.acc.tr);
@@ -1754,17 +1844,17 @@ __device__ __forceinline__ void block_raster_big_kernel_body(const FrameConst &c
 	finish_pixels(c, b, acc, bad, count, px, py, slot, m);
 }
 
-template <bool FP32, int W, bool TEX>
+template <bool FP32, int W, int MODE>
 __global__ __launch_bounds__(kBigWaves * 64) void block_raster_big_kernel(FrameConst c, FrameBufs b) {
-	block_raster_big_kernel_body<FP32, W, TEX>(c, b, WgPos{blockIdx.x, gridDim.x});
+	block_raster_big_kernel_body<FP32, W, MODE>(c, b, WgPos{blockIdx.x, gridDim.x});
 }
-template <bool FP32, int W, bool TEX>
+template <bool FP32, int W, int MODE>
 __global__ __launch_bounds__(kBigWaves * 64) void block_raster_big_kernel_batch(const FrameDesc *__restrict__ frames,
 		const uint32_t *__restrict__ pre, uint32_t num_frames) {
 	const StageBatch sb{frames, pre, num_frames};
 	const uint32_t f = batch_frame(sb, blockIdx.x);
 	const FrameDesc &d = sb.frames[f];
-	block_raster_big_kernel_body<FP32, W, TEX>(d.c, d.b, WgPos{blockIdx.x - sb.pre[f], sb.pre[f + 1] - sb.pre[f]});
+	block_raster_big_kernel_body<FP32, W, MODE>(d.c, d.b, WgPos{blockIdx.x - sb.pre[f], sb.pre[f + 1] - sb.pre[f]});
 }
 
 // ------------------------------------------------------------------ exact fallback
@@ -2706,9 +2796,10 @@ __global__ __launch_bounds__(kFallbackThreads) void bwd_fallback_kernel(FrameCon
 		bwd_fallback_pixel(c, b, g, b.fallback_queue[qi], s_keys[threadIdx.x >> 6]);
 }
 
-template <bool FP32, bool TEX>
+template <bool FP32, int MODE>
 static void launch_raster_t(const FrameConst &c, const FrameBufs &b, hipStream_t stream,
 							hipEvent_t *ev) {
+	constexpr bool TEX = MODE == kShadeTextured;
 	hipLaunchKernelGGL(rowgen_low_kernel, dim3(c.bin_count), dim3(kRowgenThreads), 0, stream, c, b);
 	if(ev)
 		(void)hipEventRecord(ev[0], stream);
@@ -2720,14 +2811,14 @@ static void launch_raster_t(const FrameConst &c, const FrameBufs &b, hipStream_t
 	if(ev)
 		(void)hipEventRecord(ev[1], stream);
 	if(c.slice_words == 2) {
-		hipLaunchKernelGGL((block_raster_kernel<FP32, 2, TEX>), dim3(c.bin_count * 16), dim3(64), 0,
+		hipLaunchKernelGGL((block_raster_kernel<FP32, 2, MODE>), dim3(c.bin_count * 16), dim3(64), 0,
 						   stream, c, b);
-		hipLaunchKernelGGL((block_raster_big_kernel<FP32, 2, TEX>), dim3(c.bin_count * 16),
+		hipLaunchKernelGGL((block_raster_big_kernel<FP32, 2, MODE>), dim3(c.bin_count * 16),
 						   dim3(kBigWaves * 64), 0, stream, c, b);
 	} else {
-		hipLaunchKernelGGL((block_raster_kernel<FP32, 1, TEX>), dim3(c.bin_count * 16), dim3(64), 0,
+		hipLaunchKernelGGL((block_raster_kernel<FP32, 1, MODE>), dim3(c.bin_count * 16), dim3(64), 0,
 						   stream, c, b);
-		hipLaunchKernelGGL((block_raster_big_kernel<FP32, 1, TEX>), dim3(c.bin_count * 16),
+		hipLaunchKernelGGL((block_raster_big_kernel<FP32, 1, MODE>), dim3(c.bin_count * 16),
 						   dim3(kBigWaves * 64), 0, stream, c, b);
 	}
 	if(ev)
@@ -2738,8 +2829,9 @@ static void launch_raster_t(const FrameConst &c, const FrameBufs &b, hipStream_t
 		(void)hipEventRecord(ev[3], stream);
 }
 
-template <bool FP32, int W, bool TEX>
+template <bool FP32, int W, int MODE>
 static void launch_raster_batch_t(const BatchPlan &p, hipStream_t stream) {
+	constexpr bool TEX = MODE == kShadeTextured;
 	hipLaunchKernelGGL(rowgen_low_kernel_batch, dim3(p.total[kBsRowgenLow]), dim3(kRowgenThreads), 0,
 					   stream, p.frames, p.stage_pre(kBsRowgenLow), p.num_frames);
 	hipLaunchKernelGGL(high_prep_kernel_batch, dim3(p.total[kBsHighPrep]), dim3(1024), 0, stream,
@@ -2748,9 +2840,9 @@ static void launch_raster_batch_t(const BatchPlan &p, hipStream_t stream) {
 					   dim3(kRowgenThreads), 0, stream, p.frames, p.stage_pre(kBsRowgenHigh), p.num_frames);
 	hipLaunchKernelGGL(high_finalize_kernel_batch, dim3(p.total[kBsHighFinalize]),
 					   dim3(kSortThreads), 0, stream, p.frames, p.stage_pre(kBsHighFinalize), p.num_frames);
-	hipLaunchKernelGGL((block_raster_kernel_batch<FP32, W, TEX>), dim3(p.total[kBsBlockRaster]),
+	hipLaunchKernelGGL((block_raster_kernel_batch<FP32, W, MODE>), dim3(p.total[kBsBlockRaster]),
 					   dim3(64), 0, stream, p.frames, p.stage_pre(kBsBlockRaster), p.num_frames);
-	hipLaunchKernelGGL((block_raster_big_kernel_batch<FP32, W, TEX>),
+	hipLaunchKernelGGL((block_raster_big_kernel_batch<FP32, W, MODE>),
 					   dim3(p.total[kBsBlockRasterBig]), dim3(kBigWaves * 64), 0, stream,
 					   p.frames, p.stage_pre(kBsBlockRasterBig), p.num_frames);
 	hipLaunchKernelGGL((fallback_kernel_batch<FP32, TEX>), dim3(p.total[kBsFallback]),
@@ -2767,23 +2859,25 @@ __global__ __launch_bounds__(64) void batch_status_kernel(const FrameDesc *frame
 
 // Row generation, sorting, block rasterisation and the exact fallback of every frame of a
 // batch: one launch per stage, kernels chosen by the batch's sample format, slice width and
-// texture use; then the frames' control words into p.status.
+// shading mode; then the frames' control words into p.status.
+template <int W>
+static void launch_raster_batch_w(const BatchPlan &p, hipStream_t stream) {
+	if(p.fp32)
+		p.mode == kShadeTextured ? launch_raster_batch_t<true, W, kShadeTextured>(p, stream)
+								 : launch_raster_batch_t<true, W, kShadeGeneral>(p, stream);
+	else if(p.mode == kShadeTextured)
+		launch_raster_batch_t<false, W, kShadeTextured>(p, stream);
+	else if(p.mode == kShadeFlat)
+		launch_raster_batch_t<false, W, kShadeFlat>(p, stream);
+	else
+		launch_raster_batch_t<false, W, kShadeGeneral>(p, stream);
+}
+
 hipError_t launch_raster_batch(const BatchPlan &p, hipStream_t stream) {
-	if(p.slice_words == 2) {
-		if(p.fp32)
-			p.tex ? launch_raster_batch_t<true, 2, true>(p, stream)
-				  : launch_raster_batch_t<true, 2, false>(p, stream);
-		else
-			p.tex ? launch_raster_batch_t<false, 2, true>(p, stream)
-				  : launch_raster_batch_t<false, 2, false>(p, stream);
-	} else {
-		if(p.fp32)
-			p.tex ? launch_raster_batch_t<true, 1, true>(p, stream)
-				  : launch_raster_batch_t<true, 1, false>(p, stream);
-		else
-			p.tex ? launch_raster_batch_t<false, 1, true>(p, stream)
-				  : launch_raster_batch_t<false, 1, false>(p, stream);
-	}
+	if(p.slice_words == 2)
+		launch_raster_batch_w<2>(p, stream);
+	else
+		launch_raster_batch_w<1>(p, stream);
 	hipLaunchKernelGGL(batch_status_kernel, dim3(p.num_frames), dim3(64), 0, stream, p.frames,
 					   p.num_frames, p.status);
 	return hipGetLastError();
@@ -2793,19 +2887,19 @@ hipError_t launch_raster_batch(const BatchPlan &p, hipStream_t stream) {
 // after block rasterisation, after the fallback kernel.
 hipError_t launch_raster(const FrameConst &c, const FrameBufs &b, hipStream_t stream,
 						 hipEvent_t *events) {
-	// frames with albedo textures take kernels built with the texture path; the others
-	// carry none of its code (or registers)
+	// frames with albedo textures take kernels built with the texture path (the others carry
+	// none of its code or registers); flat-colour frames take the colour-plane records
 	const bool fp32 = (c.opts & LUCID_OPT_FP32_SAMPLES) != 0;
-	if(c.has_textures) {
-		if(fp32)
-			launch_raster_t<true, true>(c, b, stream, events);
-		else
-			launch_raster_t<false, true>(c, b, stream, events);
-	} else if(fp32) {
-		launch_raster_t<true, false>(c, b, stream, events);
-	} else {
-		launch_raster_t<false, false>(c, b, stream, events);
-	}
+	const int mode = frame_shade_mode(c);
+	if(fp32)
+		mode == kShadeTextured ? launch_raster_t<true, kShadeTextured>(c, b, stream, events)
+							   : launch_raster_t<true, kShadeGeneral>(c, b, stream, events);
+	else if(mode == kShadeTextured)
+		launch_raster_t<false, kShadeTextured>(c, b, stream, events);
+	else if(mode == kShadeFlat)
+		launch_raster_t<false, kShadeFlat>(c, b, stream, events);
+	else
+		launch_raster_t<false, kShadeGeneral>(c, b, stream, events);
 	return hipGetLastError();
 }
 
