@@ -45,6 +45,8 @@ enum CtrlWord : int {
 	kCtrlBwdGradMax = 14,   // max |dL/dpixel| (float bits) over the gradient image
 	kCtrlTopoChanged = 15,  // backward: the corner -> vertex map differs from the cached one
 	kCtrlWords = 16,
+	// device-only words (not copied to the host) up to kStatStripeBase:
+	kCtrlBigSorts = 16, // blocks queued in big_queue for the long-list sort pass
 };
 
 // Frame statistics summed per block (fragments, list entries) go to kStatStripes counters
@@ -214,6 +216,7 @@ struct FrameBufs {
 	uint4 *blk_slot;     // bin_count * 16: list offset, list length, record offset, record count
 	uint32_t *blk_flags; // bin_count * 16
 	uint32_t *blk_sorted; // bin_count * 16: offset of the pre-sorted arrays
+	uint32_t *big_queue;  // bin_count * 16: blocks left to high_finalize's long-list pass
 	uint32_t *blk_lists;  // low bins: fixed bin_count * 16 * kRegSortCap; then high bump region
 	uint64_t blk_lists_cap;
 	uint32_t *blk_count;  // bin_count * 16: list lengths of high bins (atomic counters)

@@ -369,7 +369,7 @@ int lucid_create(const lucid_create_info *ci, lucid_renderer **out) {
 	chk(r->spill.reserve(r->spill_cap * 8));
 	chk(r->blk_slot.reserve((size_t)NB * 16 * 16));
 	chk(r->blk_flags.reserve((size_t)NB * 16 * 4));
-	chk(r->blk_sorted.reserve((size_t)NB * 16 * 4));
+	chk(r->blk_sorted.reserve((size_t)NB * 16 * 4 * 2)); // then the long-list queue
 	chk(r->blk_count.reserve((size_t)NB * 16 * 4));
 	chk(r->band_count.reserve((size_t)NB * 4 * 4));
 	r->blk_lists_cap = (uint64_t)NB * 16 * kRegSortCap + (1u << 22);
@@ -625,6 +625,7 @@ int prepare_frame(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_f
 	b.blk_slot = r->blk_slot.as<uint4>();
 	b.blk_flags = r->blk_flags.as<uint32_t>();
 	b.blk_sorted = r->blk_sorted.as<uint32_t>();
+	b.big_queue = b.blk_sorted + (size_t)r->bin_count * 16;
 	b.blk_lists = r->blk_lists.as<uint32_t>();
 	b.blk_lists_cap = r->blk_lists_cap;
 	b.blk_count = r->blk_count.as<uint32_t>();
@@ -1325,7 +1326,7 @@ constexpr NaLayout na_layout() {
 		kNaSpill * 8,
 		nb * 16 * 16,
 		nb * 16 * 4,
-		nb * 16 * 4,
+		nb * 16 * 4 * 2, // blk_sorted, then the long-list queue
 		(nb * 16 * kRegSortCap + kNaListsExtra) * 4,
 		nb * 16 * 4,
 		nb * 4 * 4,
@@ -1448,6 +1449,7 @@ int lucid_render_noalloc(int width, int height, uint32_t opts, const lucid_tunin
 	b.blk_slot = (uint4 *)(base + L.blk_slot);
 	b.blk_flags = (uint32_t *)(base + L.blk_flags);
 	b.blk_sorted = (uint32_t *)(base + L.blk_sorted);
+	b.big_queue = b.blk_sorted + (size_t)kNaMaxBins * 16;
 	b.blk_lists = (uint32_t *)(base + L.blk_lists);
 	b.blk_lists_cap = (uint64_t)kNaMaxBins * 16 * kRegSortCap + kNaListsExtra;
 	b.blk_count = (uint32_t *)(base + L.blk_count);

@@ -940,6 +940,8 @@ __device__ __forceinline__ void high_prep_kernel_body(const FrameConst &c, const
 	const uint32_t *high = b.info + LUCID_INFO_U32_SIZE + LUCID_COUNTS_HIGH_BINS * c.bin_count;
 	__shared__ uint32_t s_wave[16];
 	__shared__ uint32_t s_carry;
+	__shared__ uint32_t s_ex[1024], s_n[1024]; // per bin of the chunk: earlier tris, its tris
+	__shared__ uint32_t s_bad[1024 / 32];      // per bin of the chunk: allocation failed
 	if(threadIdx.x == 0)
 		s_carry = 0;
 	__syncthreads();
@@ -963,27 +965,41 @@ __device__ __forceinline__ void high_prep_kernel_body(const FrameConst &c, const
 			s_carry = run;
 		}
 		__syncthreads();
+		bool bad = false;
 		if(i < nh) {
 			uint64_t ex = (uint64_t)s_wave[wave] + inc - n; // tris of earlier high bins
 			uint64_t rec = 4 * ex, lst = list0 + 16 * ex;
 			bool ok = rec + 4ull * n <= b.high_records_cap && lst + 16ull * n <= b.blk_lists_cap;
 			if(!ok)
 				atomicOr(&b.ctrl[kCtrlOverflow], kOvfRecords | kOvfLists);
+			bad = !ok; // (ex then need not fit 32 bits: the bin's slots are not used)
 			if(i == nh - 1) { // sizes the host grows the pools to after an overflow
 				b.ctrl[kCtrlRecordCursor] = (uint32_t)min(rec + 4ull * n, 0xffffffffull);
 				b.ctrl[kCtrlListCursor] = (uint32_t)min(16ull * (ex + n), 0xffffffffull);
 			}
-			for(int blk = 0; blk < 16; blk++) {
-				int band = blk >> 2, bx = blk & 3;
-				b.blk_slot[(size_t)bin * 16 + blk] =
-					make_uint4((uint32_t)(lst + (uint64_t)(band * 4 + bx) * n), 0u,
-							   (uint32_t)(rec + (uint64_t)band * n), 0u);
-				b.blk_count[(size_t)bin * 16 + blk] = 0;
-				b.blk_flags[(size_t)bin * 16 + blk] = ok ? 0u : kSlotPromoted;
-			}
-			for(int band = 0; band < 4; band++)
-				b.band_count[(size_t)bin * 4 + band] = 0;
+			s_ex[threadIdx.x] = (uint32_t)ex;
+			s_n[threadIdx.x] = n;
 		}
+		if((threadIdx.x & 31) == 0)
+			s_bad[threadIdx.x >> 5] = 0;
+		__syncthreads();
+		if(bad)
+			atomicOr(&s_bad[threadIdx.x >> 5], 1u << (threadIdx.x & 31));
+		__syncthreads();
+		// the chunk's block slots, 16 consecutive ones per bin, one per thread and step
+		const uint32_t nb = min(nh - base, 1024u);
+		for(uint32_t p = threadIdx.x; p < nb * 16; p += 1024) {
+			const uint32_t k = p >> 4, blk = p & 15, band = blk >> 2, bx = blk & 3;
+			const uint32_t bin = high[base + k], n = s_n[k];
+			const uint64_t ex = s_ex[k], lst = list0 + 16 * ex;
+			b.blk_slot[(size_t)bin * 16 + blk] =
+				make_uint4((uint32_t)(lst + (uint64_t)(band * 4 + bx) * n), 0u,
+						   (uint32_t)(4 * ex + (uint64_t)band * n), 0u);
+			b.blk_count[(size_t)bin * 16 + blk] = 0;
+			b.blk_flags[(size_t)bin * 16 + blk] = (s_bad[k >> 5] >> (k & 31)) & 1u ? kSlotPromoted : 0u;
+		}
+		for(uint32_t p = threadIdx.x; p < nb * 4; p += 1024)
+			b.band_count[(size_t)high[base + (p >> 2)] * 4 + (p & 3)] = 0;
 		__syncthreads();
 	}
 }
@@ -1135,6 +1151,14 @@ __device__ __forceinline__ uint32_t sorted_mask_offset(uint32_t m) {
 // pre-sorts long lists. A long list is sorted once by centroid depth (LDS bitonic by the
 // whole work-group) and split into `parts` pixel-row groups, each a compacted list of the
 // entries that cover it, so that several waves can rasterise a dense block.
+// Two passes share the body: CAP = kMidSortCap does the slot bookkeeping of every block,
+// sorts the lists of up to kMidSortCap entries (8 KB of keys in LDS, so twice as many
+// work-groups per CU as with the full capacity) and queues the longer ones in big_queue;
+// CAP = kBigSortCap then takes the queued blocks (a small persistent grid: the queue is
+// usually empty). Both sort with the same keys, so the output does not depend on the pass.
+constexpr uint32_t kMidSortCap = 2048;
+
+template <uint32_t CAP>
 __device__ __forceinline__ void high_finalize_kernel_body(const FrameConst &c, const FrameBufs &b, WgPos wg) {
 	if(b.ctrl[kCtrlOverflow] & (kOvfBinLists | kOvfRecords | kOvfLists))
 		return;
@@ -1149,15 +1173,23 @@ __device__ __forceinline__ void high_finalize_kernel_body(const FrameConst &c, c
 	if(b.blk_flags[slot] & kSlotPromoted)
 		return; // allocation failed: the frame reports an overflow
 	const uint32_t m = b.blk_count[slot];
-	const uint32_t nrec = b.band_count[(size_t)bin * 4 + by];
 	const uint4 sl = b.blk_slot[slot];
 	const int bin_x = (bin % c.bin_count_x) * kBinSize, bin_y = (bin / c.bin_count_x) * kBinSize;
 	const int px0 = bin_x + bx * 8, py0 = bin_y + by * 8;
-	if(threadIdx.x == 0)
-		b.blk_slot[slot] = make_uint4(sl.x, m, sl.z, nrec);
-	if(m <= c.presort_min) { // sorted in registers by the block rasteriser (<= kRegSortCap)
+	if constexpr(CAP == kMidSortCap) {
+		const uint32_t nrec = b.band_count[(size_t)bin * 4 + by];
 		if(threadIdx.x == 0)
+			b.blk_slot[slot] = make_uint4(sl.x, m, sl.z, nrec);
+	}
+	if(m <= c.presort_min) { // sorted in registers by the block rasteriser (<= kRegSortCap)
+		if(CAP == kMidSortCap && threadIdx.x == 0)
 			b.blk_flags[slot] = LUCID_BIN_LEVEL_HIGH;
+		return;
+	}
+	static_assert(CAP == kMidSortCap || CAP == (uint32_t)kBigSortCap, "two sort passes");
+	if(CAP == kMidSortCap && m > kMidSortCap) {
+		if(threadIdx.x == 0) // the long-list pass takes it
+			b.big_queue[atomicAdd(&b.ctrl[kCtrlBigSorts], 1u)] = wg.id;
 		return;
 	}
 	if(m > (uint32_t)kBigSortCap) {
@@ -1171,7 +1203,7 @@ __device__ __forceinline__ void high_finalize_kernel_body(const FrameConst &c, c
 		}
 		return;
 	}
-	__shared__ uint32_t s_key[kBigSortCap];
+	__shared__ uint32_t s_key[CAP];
 	__shared__ uint32_t s_lo[kSortThreads / 64], s_hi[kSortThreads / 64];
 	__shared__ uint32_t s_wsum[2][kSortThreads / 64];
 	__shared__ uint32_t s_tdk[2][kSortThreads]; // full depth keys of tied entries, per round
@@ -1396,14 +1428,37 @@ __device__ __forceinline__ void high_finalize_kernel_body(const FrameConst &c, c
 }
 
 __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst c, FrameBufs b) {
-	high_finalize_kernel_body(c, b, WgPos{blockIdx.x, gridDim.x});
+	high_finalize_kernel_body<kMidSortCap>(c, b, WgPos{blockIdx.x, gridDim.x});
+}
+
+// The long-list pass over the blocks queued by the first: work-group w of `count` takes
+// queue entries w, w + count, ... (the queue is complete: the first pass has finished).
+__device__ __forceinline__ void high_finalize_long_body(const FrameConst &c, const FrameBufs &b,
+														 WgPos wg) {
+	if(b.ctrl[kCtrlOverflow] & (kOvfBinLists | kOvfRecords | kOvfLists))
+		return;
+	const uint32_t n = b.ctrl[kCtrlBigSorts];
+	for(uint32_t q = wg.id; q < n; q += wg.count) {
+		high_finalize_kernel_body<(uint32_t)kBigSortCap>(c, b, WgPos{b.big_queue[q], 0u});
+		__syncthreads(); // the next block reuses the work-group's LDS
+	}
+}
+__global__ __launch_bounds__(kSortThreads) void high_finalize_long_kernel(FrameConst c, FrameBufs b) {
+	high_finalize_long_body(c, b, WgPos{blockIdx.x, gridDim.x});
 }
 __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel_batch(const FrameDesc *__restrict__ frames,
 		const uint32_t *__restrict__ pre, uint32_t num_frames) {
 	const StageBatch sb{frames, pre, num_frames};
 	const uint32_t f = batch_frame(sb, blockIdx.x);
 	const FrameDesc &d = sb.frames[f];
-	high_finalize_kernel_body(d.c, d.b, WgPos{blockIdx.x - sb.pre[f], sb.pre[f + 1] - sb.pre[f]});
+	high_finalize_kernel_body<kMidSortCap>(d.c, d.b, WgPos{blockIdx.x - sb.pre[f], sb.pre[f + 1] - sb.pre[f]});
+}
+__global__ __launch_bounds__(kSortThreads) void high_finalize_long_kernel_batch(const FrameDesc *__restrict__ frames,
+		const uint32_t *__restrict__ pre, uint32_t num_frames) {
+	const StageBatch sb{frames, pre, num_frames};
+	const uint32_t f = batch_frame(sb, blockIdx.x);
+	const FrameDesc &d = sb.frames[f];
+	high_finalize_long_body(d.c, d.b, WgPos{blockIdx.x - sb.pre[f], sb.pre[f + 1] - sb.pre[f]});
 }
 
 // ------------------------------------------------------------------ block_raster
@@ -2796,6 +2851,14 @@ __global__ __launch_bounds__(kFallbackThreads) void bwd_fallback_kernel(FrameCon
 		bwd_fallback_pixel(c, b, g, b.fallback_queue[qi], s_keys[threadIdx.x >> 6]);
 }
 
+// Block bookkeeping and the pre-sort of long lists (up to kMidSortCap entries), then the
+// queued lists longer than that (one work-group per CU).
+static void launch_high_finalize(const FrameConst &c, const FrameBufs &b, hipStream_t stream) {
+	hipLaunchKernelGGL(high_finalize_kernel, dim3(c.bin_count * 16), dim3(kSortThreads), 0, stream,
+					   c, b);
+	hipLaunchKernelGGL(high_finalize_long_kernel, dim3(256), dim3(kSortThreads), 0, stream, c, b);
+}
+
 template <bool FP32, int MODE>
 static void launch_raster_t(const FrameConst &c, const FrameBufs &b, hipStream_t stream,
 							hipEvent_t *ev) {
@@ -2806,8 +2869,7 @@ static void launch_raster_t(const FrameConst &c, const FrameBufs &b, hipStream_t
 	hipLaunchKernelGGL(high_prep_kernel, dim3(1), dim3(1024), 0, stream, c, b);
 	hipLaunchKernelGGL(rowgen_high_kernel, dim3(c.bin_count * kRowgenSplit), dim3(kRowgenThreads), 0,
 					   stream, c, b);
-	hipLaunchKernelGGL(high_finalize_kernel, dim3(c.bin_count * 16), dim3(kSortThreads), 0, stream,
-					   c, b);
+	launch_high_finalize(c, b, stream);
 	if(ev)
 		(void)hipEventRecord(ev[1], stream);
 	if(c.slice_words == 2) {
@@ -2839,6 +2901,8 @@ static void launch_raster_batch_t(const BatchPlan &p, hipStream_t stream) {
 	hipLaunchKernelGGL(rowgen_high_kernel_batch, dim3(p.total[kBsRowgenHigh]),
 					   dim3(kRowgenThreads), 0, stream, p.frames, p.stage_pre(kBsRowgenHigh), p.num_frames);
 	hipLaunchKernelGGL(high_finalize_kernel_batch, dim3(p.total[kBsHighFinalize]),
+					   dim3(kSortThreads), 0, stream, p.frames, p.stage_pre(kBsHighFinalize), p.num_frames);
+	hipLaunchKernelGGL(high_finalize_long_kernel_batch, dim3(p.total[kBsHighFinalize]),
 					   dim3(kSortThreads), 0, stream, p.frames, p.stage_pre(kBsHighFinalize), p.num_frames);
 	hipLaunchKernelGGL((block_raster_kernel_batch<FP32, W, MODE>), dim3(p.total[kBsBlockRaster]),
 					   dim3(64), 0, stream, p.frames, p.stage_pre(kBsBlockRaster), p.num_frames);
@@ -2911,8 +2975,7 @@ hipError_t launch_backward_prepare(const FrameConst &c, const FrameBufs &b, cons
 	hipLaunchKernelGGL(high_prep_kernel, dim3(1), dim3(1024), 0, stream, c, b);
 	hipLaunchKernelGGL(rowgen_high_kernel, dim3(c.bin_count * kRowgenSplit), dim3(kRowgenThreads), 0,
 					   stream, c, b);
-	hipLaunchKernelGGL(high_finalize_kernel, dim3(c.bin_count * 16), dim3(kSortThreads), 0, stream,
-					   c, b);
+	launch_high_finalize(c, b, stream);
 	hipLaunchKernelGGL(grad_row_scan_kernel<false>, dim3(1), dim3(1024), 0, stream, c, b, g);
 	return hipGetLastError();
 }
@@ -2944,8 +3007,7 @@ hipError_t launch_raster_recording(const FrameConst &c, const FrameBufs &b, cons
 	hipLaunchKernelGGL(high_prep_kernel, dim3(1), dim3(1024), 0, stream, c, b);
 	hipLaunchKernelGGL(rowgen_high_kernel, dim3(c.bin_count * kRowgenSplit), dim3(kRowgenThreads), 0,
 					   stream, c, b);
-	hipLaunchKernelGGL(high_finalize_kernel, dim3(c.bin_count * 16), dim3(kSortThreads), 0, stream,
-					   c, b);
+	launch_high_finalize(c, b, stream);
 	hipLaunchKernelGGL(grad_row_scan_kernel<false>, dim3(1), dim3(1024), 0, stream, c, b, g);
 	hipLaunchKernelGGL(record_gate_kernel, dim3(1), dim3(64), 0, stream, b, g);
 	if(ev)

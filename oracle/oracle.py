@@ -137,7 +137,17 @@ def render(scene, mode: int = MODE_EXACT, opts: int = 0, bin_capacity: Optional[
     nb = bin_count(w, h)
     q = scene.num_quads
     if bin_capacity is None:
-        bin_capacity = max(1 << 16, 8 * q)
+        # bin lists hold a quad once per bin it touches: start from 8 per quad and grow
+        # (the oracle reports -3 when its lists overflow)
+        cap = max(1 << 16, 8 * q)
+        while True:
+            try:
+                return render(scene, mode, opts, cap, want_records, want_fragments,
+                              fragment_rect)
+            except RuntimeError as e:
+                if str(e) != "oracle_render failed: -3" or cap >= 2 * q * nb:
+                    raise
+                cap = min(cap * 4, 2 * q * nb)
     keep = []
 
     def c(a, dtype):

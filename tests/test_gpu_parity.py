@@ -166,6 +166,20 @@ def test_block_over_lds_capacity(oracle_mod):
     _check(sc, gpu, ref, oracle_mod)
 
 
+def test_long_lists_second_sort_pass(oracle_mod):
+    """Blocks of 2048-8192 entries: high_finalize's first pass queues them and its long-list
+    pass (a fixed grid of 256 work-groups) pre-sorts them. 289 such blocks (136 x 136 px),
+    so work-groups take more than one queued block."""
+    rng = np.random.default_rng(17)
+    depths = rng.permutation(np.linspace(2.0, 200.0, 2100))
+    layers = [(float(d), tuple(rng.uniform(0, 1, 3)), float(rng.uniform(0.005, 0.05)))
+              for d in depths]
+    sc = scenes.planes_scene(136, 136, layers)
+    r, gpu, ref = _render_both(sc, oracle_mod)
+    _check(sc, gpu, ref, oracle_mod)
+    assert int(ref["frag_counts"].min()) >= 2100
+
+
 def test_block_over_sort_capacity(oracle_mod):
     # more than 8192 triangles in one 8x8 block: every pixel takes the exact per-pixel path,
     # and pixels deeper than the fallback's LDS capacity spill their keys to global memory
@@ -244,6 +258,40 @@ def test_normals_instance_colours_and_sun_lighting(oracle_mod):
     funcs.glsl:261-271) against the oracle: bins and counts bit-exact, colours within 1/255,
     in RGBA8 and fp32 samples."""
     sc = _lit_scene()
+    for opts in (0, _lib.OPT_FP32_SAMPLES):
+        r, gpu, ref = _render_both(sc, oracle_mod, opts=opts)
+        _check(sc, gpu, ref, oracle_mod)
+
+
+def _flat_colour_scene(seed=44):
+    """Frames that take the block rasterisers' flat colour planes (RGBA8 samples, unit
+    lighting, fp32 vertex colours with a per-triangle alpha), with every input those planes
+    fold in: instances with vertex and instance colours, instance colours only (a constant
+    plane), vertex colours only, translucent instance colours, and triangles of alpha 0."""
+    sc = scenes.config0(num_tris=5000, width=128, height=96, seed=seed)
+    rng = np.random.default_rng(seed)
+    assert len(sc.instances) == 5
+    flags = [scenes.INST_HAS_VERTEX_COLORS | scenes.INST_HAS_COLOR, scenes.INST_HAS_COLOR,
+             scenes.INST_HAS_VERTEX_COLORS, scenes.INST_HAS_VERTEX_COLORS | scenes.INST_HAS_COLOR,
+             scenes.INST_HAS_VERTEX_COLORS]
+    sc.instances["flags"] = np.array(flags, np.uint32)
+    c = rng.integers(0, 256, size=(5, 4)).astype(np.uint32)
+    c[:, 3] = [255, 160, 255, 90, 255]
+    sc.instance_colors = (c[:, 0] | (c[:, 1] << 8) | (c[:, 2] << 16) | (c[:, 3] << 24)).astype(
+        np.uint32)
+    ta = sc.tri_alpha.copy()
+    ta[0::2][rng.random(sc.num_quads) < 0.1] = 0.0
+    sc.tri_alpha = ta
+    sc.name = f"flat_{sc.name}"
+    return sc
+
+
+def test_flat_colour_planes(oracle_mod):
+    """The flat colour-plane records (colour = K + L(px, py) / inverse depth per channel,
+    instance colour folded in) against the oracle's barycentric shading: bins and counts
+    bit-exact, colours within 1/255; the same scene with fp32 samples (general records)
+    as well."""
+    sc = _flat_colour_scene()
     for opts in (0, _lib.OPT_FP32_SAMPLES):
         r, gpu, ref = _render_both(sc, oracle_mod, opts=opts)
         _check(sc, gpu, ref, oracle_mod)
