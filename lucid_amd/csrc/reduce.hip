@@ -3,12 +3,11 @@
 // sort (atomics only count and place row indices), and each triangle's group is summed in
 // 64-bit fixed point, whose sums do not depend on the order of the rows. The per-vertex
 // sums then run over the triangle corners grouped by vertex (a stable rocPRIM radix sort of
-// the corner -> vertex map, kept while the mesh does not change). No gradient value goes
-// through a global atomic, and the result does not depend on scheduling.
+// the corner -> vertex map, run only when the mesh changes and kept otherwise). No gradient
+// value goes through a global atomic, and the result does not depend on scheduling.
 #include <hip/hip_runtime.h>
 
 #include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_scan.hpp>
 
 #include "lucid_device.h"
 
@@ -165,6 +164,128 @@ __global__ void alpha_grad_kernel(const float *tri_grad, uint32_t ntri, float *g
 		grad_alpha[t] = tri_grad[(size_t)t * kGradWords + 9];
 }
 
+// Exclusive prefix sum of u32 counts in three launches: tile totals, their exclusive scan
+// (one work-group), then each tile's scan plus its offset. A tile is kScanTile consecutive
+// counts, kScanItems per thread in thread order.
+constexpr int kScanThreads = 256, kScanItems = 16, kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ uint32_t scan_load(const uint32_t *in, size_t n, size_t i) {
+	return i < n ? in[i] : 0u;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_tile_sums_kernel(const uint32_t *in, size_t n,
+																		uint32_t *tile_sums) {
+	__shared__ uint32_t s_wave[kScanThreads / 64];
+	const size_t base = (size_t)blockIdx.x * kScanTile;
+	uint32_t sum = 0;
+#pragma unroll
+	for(int k = 0; k < kScanItems; k++) // coalesced: item k of the tile's k-th row
+		sum += scan_load(in, n, base + (size_t)k * kScanThreads + threadIdx.x);
+	sum = __shfl(wave_inclusive_add(sum), 63, 64);
+	if(lane_id() == 0)
+		s_wave[threadIdx.x >> 6] = sum;
+	__syncthreads();
+	if(threadIdx.x == 0) {
+		uint32_t t = 0;
+		for(int w = 0; w < kScanThreads / 64; w++)
+			t += s_wave[w];
+		tile_sums[blockIdx.x] = t;
+	}
+}
+
+// In place: tile_sums[i] becomes the sum of the tiles before i.
+__global__ __launch_bounds__(1024) void scan_tile_offsets_kernel(uint32_t *tile_sums, uint32_t ntiles) {
+	__shared__ uint32_t s_wave[16];
+	__shared__ uint32_t s_carry;
+	if(threadIdx.x == 0)
+		s_carry = 0;
+	__syncthreads();
+	for(uint32_t base = 0; base < ntiles; base += 1024) {
+		const uint32_t i = base + threadIdx.x;
+		const uint32_t v = i < ntiles ? tile_sums[i] : 0u;
+		const uint32_t inc = wave_inclusive_add(v);
+		if(lane_id() == 63)
+			s_wave[threadIdx.x >> 6] = inc;
+		__syncthreads();
+		uint32_t before = s_carry, total = 0;
+		for(int w = 0; w < 16; w++) {
+			const uint32_t t = s_wave[w];
+			before += w < (int)(threadIdx.x >> 6) ? t : 0u;
+			total += t;
+		}
+		if(i < ntiles)
+			tile_sums[i] = before + inc - v;
+		__syncthreads(); // s_wave and s_carry are read before they change
+		if(threadIdx.x == 0)
+			s_carry += total;
+		__syncthreads();
+	}
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const uint32_t *in, size_t n,
+																	const uint32_t *tile_offsets,
+																	uint32_t *out) {
+	__shared__ uint32_t s_tile[kScanTile];
+	__shared__ uint32_t s_wave[kScanThreads / 64];
+	const size_t base = (size_t)blockIdx.x * kScanTile;
+#pragma unroll
+	for(int k = 0; k < kScanItems; k++) { // coalesced into LDS, then read in thread order
+		const int j = k * kScanThreads + (int)threadIdx.x;
+		s_tile[j] = scan_load(in, n, base + (size_t)j);
+	}
+	__syncthreads();
+	uint32_t v[kScanItems], sum = 0;
+#pragma unroll
+	for(int k = 0; k < kScanItems; k++) {
+		// rotated reads: thread t reads its items starting at item t % 16 (no bank conflicts)
+		const int kk = (k + (int)threadIdx.x) & (kScanItems - 1);
+		v[kk] = s_tile[threadIdx.x * kScanItems + kk];
+	}
+#pragma unroll
+	for(int k = 0; k < kScanItems; k++)
+		sum += v[k];
+	const uint32_t inc = wave_inclusive_add(sum);
+	if(lane_id() == 63)
+		s_wave[threadIdx.x >> 6] = inc;
+	__syncthreads();
+	uint32_t run = tile_offsets[blockIdx.x] + inc - sum;
+	for(int w = 0; w < (int)(threadIdx.x >> 6); w++)
+		run += s_wave[w];
+#pragma unroll
+	for(int k = 0; k < kScanItems; k++) {
+		const uint32_t x = v[k];
+		v[k] = run;
+		run += x;
+	}
+	__syncthreads(); // every thread has read its items
+#pragma unroll
+	for(int k = 0; k < kScanItems; k++) {
+		const int kk = (k + (int)threadIdx.x) & (kScanItems - 1);
+		s_tile[threadIdx.x * kScanItems + kk] = v[kk];
+	}
+	__syncthreads();
+#pragma unroll
+	for(int k = 0; k < kScanItems; k++) {
+		const int j = k * kScanThreads + (int)threadIdx.x;
+		if(base + (size_t)j < n)
+			out[base + (size_t)j] = s_tile[j];
+	}
+}
+
+inline uint32_t scan_tiles(size_t n) { return (uint32_t)((n + kScanTile - 1) / kScanTile); }
+
+// out[i] = in[0] + ... + in[i - 1] for i < n; `temp` holds scan_tiles(n) words.
+hipError_t exclusive_sum(const uint32_t *in, uint32_t *out, size_t n, uint32_t *temp,
+						 hipStream_t stream) {
+	const uint32_t nt = scan_tiles(n);
+	if(!nt)
+		return hipSuccess;
+	hipLaunchKernelGGL(scan_tile_sums_kernel, dim3(nt), dim3(kScanThreads), 0, stream, in, n, temp);
+	hipLaunchKernelGGL(scan_tile_offsets_kernel, dim3(1), dim3(1024), 0, stream, temp, nt);
+	hipLaunchKernelGGL(scan_apply_kernel, dim3(nt), dim3(kScanThreads), 0, stream, in, n, temp, out);
+	return hipGetLastError();
+}
+
 int key_bits(uint32_t max_key) {
 	int bits = 1;
 	while(bits < 32 && (max_key >> bits) != 0)
@@ -179,13 +300,11 @@ inline dim3 grid_for(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 // Temporary storage of the reduction: one stable key/value radix sort of `npairs` pairs or
 // one exclusive scan of `nscan` counts.
 size_t reduce_temp_bytes(uint32_t npairs, uint32_t nscan) {
-	size_t sort = 0, scan = 0;
+	size_t sort = 0;
 	(void)rocprim::radix_sort_pairs((void *)nullptr, sort, (const uint32_t *)nullptr,
 									(uint32_t *)nullptr, (const uint32_t *)nullptr,
 									(uint32_t *)nullptr, npairs, 0, 32);
-	(void)rocprim::exclusive_scan((void *)nullptr, scan, (const uint32_t *)nullptr,
-								  (uint32_t *)nullptr, 0u, (size_t)nscan,
-								  rocprim::plus<uint32_t>());
+	const size_t scan = (size_t)scan_tiles(nscan) * 4;
 	return sort > scan ? sort : scan;
 }
 
@@ -211,9 +330,9 @@ hipError_t reduce_group_rows(const FrameConst &c, const FrameBufs &b, const BwdB
 	if(g.rows_cap)
 		hipLaunchKernelGGL(row_hist_kernel, grid_for(g.rows_cap), dim3(256), 0, stream, b.ctrl, g,
 						   ntri, r.counts);
-	size_t bytes = r.temp_bytes;
-	hipError_t e = rocprim::exclusive_scan(r.temp, bytes, (const uint32_t *)r.counts, r.offsets,
-										   0u, (size_t)ntri + 1, rocprim::plus<uint32_t>(), stream);
+	if(r.temp_bytes < (size_t)scan_tiles((size_t)ntri + 1) * 4)
+		return hipErrorInvalidValue;
+	hipError_t e = exclusive_sum(r.counts, r.offsets, (size_t)ntri + 1, (uint32_t *)r.temp, stream);
 	if(e != hipSuccess)
 		return e;
 	if(g.rows_cap)
