@@ -291,7 +291,7 @@ def main():
     tuning = r.get_tuning()
     if not args.no_autotune:
         print("[bench] autotune", file=sys.stderr, flush=True)
-        tuning = r.autotune(g, out, iters=3)
+        tuning = r.autotune(g, out, iters=8)
     print("[bench] warmup", file=sys.stderr, flush=True)
     for _ in range(max(1, args.warmup)):
         r.render(g, out)

@@ -75,6 +75,11 @@ typedef struct lucid_tuning {
 	int slice_words;      /* entries staged in LDS per step: 64 * slice_words (1 or 2) */
 	int presort_min;      /* high-bin blocks with more entries are pre-sorted (64..256) */
 	int samples_per_wave; /* target samples per wave of a pre-sorted block (256..65536) */
+	int window;           /* per-pixel ordering window of the block rasterisers in frames of
+						   * RGBA8 samples with unit lighting and flat colours: 1 (one slot;
+						   * two in pre-sorted blocks) for lists whose block order is nearly
+						   * every pixel's depth order, 4 for more disorder. A pixel the window
+						   * cannot order takes the exact per-pixel sort either way. */
 } lucid_tuning;
 
 typedef struct lucid_create_info {
@@ -171,7 +176,7 @@ int lucid_backward(lucid_renderer *r, const lucid_frame_inputs *in, const float 
 int lucid_backward_ex(lucid_renderer *r, const lucid_frame_inputs *in, const float *grad_image,
 					  float *grad_colors_rgb, float *grad_tri_alpha, uint32_t flags, void *stream);
 
-/* Tuning of a renderer (defaults 1, 256, 2048). No counterpart in the reference. */
+/* Tuning of a renderer (defaults 1, 128, 1024, 1). No counterpart in the reference. */
 int lucid_set_tuning(lucid_renderer *r, const lucid_tuning *t);
 int lucid_get_tuning(const lucid_renderer *r, lucid_tuning *t);
 /* Times every tuning candidate on this frame (`iters` timed frames each, HIP events on
@@ -196,7 +201,7 @@ int lucid_batch_finish(lucid_batch *b);
  * allocating library call. No counterpart in the reference; it is for callers that cannot
  * allocate device memory (code under HIP graph capture set-up, tight serving loops, harnesses
  * that forbid hipMalloc). Same kernels, inputs and outputs as lucid_render on a renderer of
- * width x height with `opts` and `tuning` (NULL: the defaults 1, 256, 2048); results are
+ * width x height with `opts` and `tuning` (NULL: the defaults 1, 128, 1024, 1); results are
  * identical to that renderer's. Every internal buffer lives in a fixed device array of the
  * library (65 MiB, one copy per GPU the library runs on, held until the process ends), so the
  * frame is limited to at most 256 bins of 32 x 32 pixels (e.g. 512 x 512), 65536 quads and

@@ -56,11 +56,11 @@ class CreateInfo(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("slice_words", ctypes.c_int), ("presort_min", ctypes.c_int),
-                ("samples_per_wave", ctypes.c_int)]
+                ("samples_per_wave", ctypes.c_int), ("window", ctypes.c_int)]
 
     def as_dict(self):
         return {"slice_words": self.slice_words, "presort_min": self.presort_min,
-                "samples_per_wave": self.samples_per_wave}
+                "samples_per_wave": self.samples_per_wave, "window": self.window}
 
 
 class Texture(ctypes.Structure):

@@ -151,6 +151,7 @@ struct FrameConst {
 	uint32_t presort_min;      // high-bin blocks with more entries are pre-sorted (<= kRegSortCap)
 	uint32_t samples_per_wave; // target samples per wave of a pre-sorted block
 	int slice_words;           // block rasteriser slice width: 1 or 2 words of 64 entries
+	int window;                // ordering window of flat-colour frames (tuning): 1 or 4
 	int color_mode;
 	int has_normals;
 	// 1 when an earlier frame of the renderer may still be running: the clear kernel then
@@ -294,14 +295,17 @@ __device__ __forceinline__ uint32_t batch_frame(const StageBatch &sb, uint32_t w
 // textures, or flat colour planes (RGBA8 samples, unit lighting, vertex colours absent or
 // fp32 with a per-triangle alpha, no textures: a sample's colour is then an affine function
 // of the pixel divided by the inverse depth).
-enum ShadeMode : int { kShadeGeneral = 0, kShadeTextured = 1, kShadeFlat = 2 };
+// kShadeFlatNarrow: flat records with the narrow ordering windows (tuning window 1).
+enum ShadeMode : int { kShadeGeneral = 0, kShadeTextured = 1, kShadeFlat = 2, kShadeFlatNarrow = 3 };
 
 inline int frame_shade_mode(const FrameConst &c) {
 	if(c.has_textures)
 		return kShadeTextured;
 	const bool fp32 = (c.opts & LUCID_OPT_FP32_SAMPLES) != 0;
 	const bool flat_colors = c.color_mode == kColorNone || c.color_mode == kColorF32;
-	return !fp32 && c.identity_light && flat_colors ? kShadeFlat : kShadeGeneral;
+	if(fp32 || !c.identity_light || !flat_colors)
+		return kShadeGeneral;
+	return c.window == 1 ? kShadeFlatNarrow : kShadeFlat;
 }
 
 struct BatchPlan {

@@ -97,7 +97,7 @@ struct lucid_renderer {
 	int device = 0;
 	int width = 0, height = 0;
 	uint32_t opts = 0;
-	lucid_tuning tuning = {1, 256, 2048};
+	lucid_tuning tuning = {1, 128, 1024, 1};
 	int bin_count_x = 0, bin_count_y = 0, bin_count = 0;
 
 	DevBuf info, ctrl;
@@ -266,6 +266,12 @@ int grow_after_overflow(lucid_renderer *r) {
 	return LUCID_ERR_CAPACITY;
 }
 
+bool tuning_valid(const lucid_tuning &t) {
+	return (t.slice_words == 1 || t.slice_words == 2) && t.presort_min >= 64 &&
+		   t.presort_min <= kRegSortCap && t.samples_per_wave >= 256 && t.samples_per_wave <= 65536 &&
+		   (t.window == 1 || t.window == 4);
+}
+
 // Per-frame constants of a width x height frame (bin grid, options, tuning, LucidConfig).
 void fill_const(int width, int height, uint32_t opts, const lucid_tuning &tuning,
 				const lucid_config &cfg, int64_t nq, int ninst, int color_mode, bool has_normals,
@@ -283,6 +289,7 @@ void fill_const(int width, int height, uint32_t opts, const lucid_tuning &tuning
 	c.presort_min = (uint32_t)tuning.presort_min;
 	c.samples_per_wave = (uint32_t)tuning.samples_per_wave;
 	c.slice_words = tuning.slice_words;
+	c.window = tuning.window;
 	c.color_mode = color_mode;
 	c.has_normals = has_normals ? 1 : 0;
 	for(int col = 0; col < 4; col++) {
@@ -948,8 +955,7 @@ int lucid_backward_ex(lucid_renderer *r, const lucid_frame_inputs *in, const flo
 }
 
 int lucid_set_tuning(lucid_renderer *r, const lucid_tuning *t) {
-	if(!r || !t || (t->slice_words != 1 && t->slice_words != 2) || t->presort_min < 64 ||
-	   t->presort_min > kRegSortCap || t->samples_per_wave < 256 || t->samples_per_wave > 65536)
+	if(!r || !t || !tuning_valid(*t))
 		return LUCID_ERR_INVALID_ARG;
 	r->tuning = *t;
 	return LUCID_OK;
@@ -986,42 +992,44 @@ int lucid_autotune(lucid_renderer *r, const lucid_frame_inputs *in, const lucid_
 	const int slice_words[] = {1, 2};
 	const int presort[] = {64, 128, 256};
 	const int spw[] = {1024, 2048, 4096};
+	const int windows[] = {1, 4};
 	lucid_tuning best = saved;
 	float best_ms = 1e30f;
 	int rc = LUCID_OK;
-	for(int sw : slice_words)
-		for(int pm : presort)
-			for(int sp : spw) {
-				r->tuning = {sw, pm, sp};
-				int e = LUCID_OK, f = LUCID_OK;
-				// A candidate that overflows a buffer (e.g. a larger pre-sorted pool) grows it
-				// on its untimed first frame and is timed again with the grown buffers.
-				for(int attempt = 0; attempt < 3; attempt++) {
-					e = lucid_render(r, in, out, stream); // untimed: settles this candidate's sizes
-					f = e == LUCID_OK ? lucid_finish(r) : LUCID_OK;
-					if(e != LUCID_OK || f != LUCID_ERR_CAPACITY)
+	for(int win : windows)
+		for(int sw : slice_words)
+			for(int pm : presort)
+				for(int sp : spw) {
+					r->tuning = {sw, pm, sp, win};
+					int e = LUCID_OK, f = LUCID_OK;
+					// A candidate that overflows a buffer (e.g. a larger pre-sorted pool) grows it
+					// on its untimed first frame and is timed again with the grown buffers.
+					for(int attempt = 0; attempt < 3; attempt++) {
+						e = lucid_render(r, in, out, stream); // untimed: settles this candidate's sizes
+						f = e == LUCID_OK ? lucid_finish(r) : LUCID_OK;
+						if(e != LUCID_OK || f != LUCID_ERR_CAPACITY)
+							break;
+					}
+					if(e == LUCID_OK && f == LUCID_OK)
+						e = hip_status(hipEventRecord(e0, stream));
+					for(int k = 0; k < iters && e == LUCID_OK && f == LUCID_OK; k++)
+						e = lucid_render(r, in, out, stream);
+					if(e == LUCID_OK && f == LUCID_OK)
+						e = hip_status(hipEventRecord(e1, stream));
+					int f2 = lucid_finish(r);
+					if(f == LUCID_OK)
+						f = f2;
+					if(e != LUCID_OK || f != LUCID_OK) {
+						rc = e != LUCID_OK ? e : f;
+						if(rc == LUCID_ERR_CAPACITY)
+							continue; // still overflowing after growth: skipped, the others still run
 						break;
+					}
+					float ms = 0.0f;
+					if(hipEventSynchronize(e1) == hipSuccess &&
+					   hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < best_ms)
+						best_ms = ms, best = r->tuning;
 				}
-				if(e == LUCID_OK && f == LUCID_OK)
-					e = hip_status(hipEventRecord(e0, stream));
-				for(int k = 0; k < iters && e == LUCID_OK && f == LUCID_OK; k++)
-					e = lucid_render(r, in, out, stream);
-				if(e == LUCID_OK && f == LUCID_OK)
-					e = hip_status(hipEventRecord(e1, stream));
-				int f2 = lucid_finish(r);
-				if(f == LUCID_OK)
-					f = f2;
-				if(e != LUCID_OK || f != LUCID_OK) {
-					rc = e != LUCID_OK ? e : f;
-					if(rc == LUCID_ERR_CAPACITY)
-						continue; // still overflowing after growth: skipped, the others still run
-					break;
-				}
-				float ms = 0.0f;
-				if(hipEventSynchronize(e1) == hipSuccess &&
-				   hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < best_ms)
-					best_ms = ms, best = r->tuning;
-			}
 	(void)hipEventDestroy(e0);
 	(void)hipEventDestroy(e1);
 	r->tuning = best_ms < 1e30f ? best : saved;
@@ -1191,7 +1199,7 @@ int lucid_batch_render(lucid_batch *bt, const lucid_frame_inputs *ins,
 		// knob: results do not depend on it)
 		frames[i].c.slice_words = bt->rs[0]->tuning.slice_words;
 		tex = tex || frames[i].c.has_textures;
-		flat = flat && frame_shade_mode(frames[i].c) == kShadeFlat;
+		flat = flat && frame_shade_mode(frames[i].c) >= kShadeFlat; // flat records, either window
 		r->last_num_quads = nq;
 		r->last_stream = stream;
 		r->retained.valid = r->retained.pending = false;
@@ -1215,7 +1223,9 @@ int lucid_batch_render(lucid_batch *bt, const lucid_frame_inputs *ins,
 	plan.num_frames = (uint32_t)n;
 	plan.bin_lds = bin_lds;
 	plan.fp32 = (bt->rs[0]->opts & LUCID_OPT_FP32_SAMPLES) != 0;
-	plan.mode = tex ? kShadeTextured : flat ? kShadeFlat : kShadeGeneral;
+	plan.mode = tex ? kShadeTextured
+					: flat ? (bt->rs[0]->tuning.window == 1 ? kShadeFlatNarrow : kShadeFlat)
+						   : kShadeGeneral;
 	plan.slice_words = bt->rs[0]->tuning.slice_words;
 	plan.status = bt->status_dev.as<uint32_t>();
 	// the launches read the frames' constants and buffers from device memory, so a frame
@@ -1382,11 +1392,10 @@ int lucid_render_noalloc(int width, int height, uint32_t opts, const lucid_tunin
 	const int nbx = (width + kBinSize - 1) / kBinSize, nby = (height + kBinSize - 1) / kBinSize;
 	if(nbx * nby > kNaMaxBins || in->num_instances > kNaMaxInstances)
 		return LUCID_ERR_INVALID_ARG;
-	lucid_tuning t = {1, 256, 2048};
+	lucid_tuning t = {1, 128, 1024, 1};
 	if(tuning) {
 		t = *tuning;
-		if(t.slice_words < 1 || t.slice_words > 2 || t.presort_min < 64 || t.presort_min > 256 ||
-		   t.samples_per_wave < 256 || t.samples_per_wave > 65536)
+		if(!tuning_valid(t))
 			return LUCID_ERR_INVALID_ARG;
 	}
 	int prefix[kNaMaxInstances + 1];

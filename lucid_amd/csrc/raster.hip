@@ -31,6 +31,9 @@ constexpr int kRowgenThreads = 256;
 constexpr int kSortThreads = 256;
 constexpr int kFallbackThreads = 256;
 constexpr int kWindow = 4;
+// Narrow ordering windows (tuning window 1): one slot in the register-sorted blocks, two in
+// the pre-sorted ones (whose 19-bit depth ranks tie more often).
+constexpr int kNarrowWindow = 1, kNarrowBigWindow = 2;
 constexpr int kFallbackCap = 512; // fragments per pixel sorted in LDS by the fallback kernel
 
 // ------------------------------------------------------------------ shading
@@ -491,10 +494,73 @@ struct OrderWindow {
 	}
 };
 
-// Forward per-pixel state: the ordering window feeding the front-to-back blend.
-template <bool FP32>
+// Smaller ordering windows (N = 1 or 2 pending samples), same contract as OrderWindow: for
+// lists whose block order is already (nearly) each pixel's depth order, where the window
+// only has to catch neighbours in the wrong order; a pixel it cannot put in order is still
+// detected (bad) and sorted exactly.
+template <class P, int N>
+struct SmallWindow {
+	static_assert(N == 1 || N == 2, "one or two slots");
+	uint64_t k0, k1;
+	P c0, c1;
+	uint32_t n;
+	uint64_t first, last;
+	bool bad;
+
+	__device__ __forceinline__ void init() {
+		k0 = k1 = kEmptyKey;
+		c0.zero(), c1.zero();
+		n = 0, first = kEmptyKey, last = 0, bad = false;
+	}
+	template <class F> __device__ __forceinline__ void emit(uint64_t k, const P &col, F &&f) {
+		bad |= k <= last;
+		first = first == kEmptyKey ? k : first;
+		last = k;
+		f(col);
+	}
+	template <class F> __device__ __forceinline__ void insert(uint64_t key, const P &col, F &&f) {
+		if(n < (uint32_t)N) {
+			k0 = n == 0 ? key : k0, c0 = P::sel(n == 0, col, c0);
+			if constexpr(N == 2)
+				k1 = n == 1 ? key : k1, c1 = P::sel(n == 1, col, c1);
+			n++;
+			return;
+		}
+		if constexpr(N == 1) {
+			const bool take_new = key < k0;
+			emit(take_new ? key : k0, P::sel(take_new, col, c0), f);
+			k0 = take_new ? k0 : key, c0 = P::sel(take_new, c0, col);
+		} else {
+			const bool a = k1 < k0;
+			const uint64_t km = a ? k1 : k0;
+			const P cm = P::sel(a, c1, c0);
+			const bool take_new = key < km;
+			emit(take_new ? key : km, P::sel(take_new, col, cm), f);
+			const bool s0 = !take_new && !a, s1 = !take_new && a;
+			k0 = s0 ? key : k0, c0 = P::sel(s0, col, c0);
+			k1 = s1 ? key : k1, c1 = P::sel(s1, col, c1);
+		}
+	}
+	template <class F> __device__ __forceinline__ void flush(F &&f) {
+		if constexpr(N == 1) {
+			if(n)
+				emit(k0, c0, f);
+		} else {
+			for(uint32_t r = 0; r < n; r++) {
+				const bool a = k1 < k0;
+				emit(a ? k1 : k0, P::sel(a, c1, c0), f);
+				k0 = a ? k0 : kEmptyKey;
+				k1 = a ? kEmptyKey : k1;
+			}
+		}
+		n = 0;
+	}
+};
+
+// Forward per-pixel state: the ordering window (N slots) feeding the front-to-back blend.
+template <bool FP32, int N = kWindow>
 struct PixelState {
-	OrderWindow<SampleColor<FP32>> w;
+	std::conditional_t<N == kWindow, OrderWindow<SampleColor<FP32>>, SmallWindow<SampleColor<FP32>, N>> w;
 	Accum acc;
 
 	__device__ __forceinline__ void init() {
@@ -1521,7 +1587,8 @@ __device__ __forceinline__ void transpose_masks(const uint64_t (&msk)[W], int nw
 // is the same affine function (rounding differs by a few ulps, inside the 1/255 colour bar);
 // the depth key keeps its unfused evaluation. The instance colour is folded into the planes.
 constexpr int kFlatQuads = 5; // f0 depth plane | triangle id; f1-f3 r, g, b planes; f4 alpha
-constexpr int rec_quads(int mode) { return mode == kShadeFlat ? kFlatQuads : kRecQuads; }
+constexpr bool is_flat(int mode) { return mode == kShadeFlat || mode == kShadeFlatNarrow; }
+constexpr int rec_quads(int mode) { return is_flat(mode) ? kFlatQuads : kRecQuads; }
 
 __device__ __forceinline__ void make_flat_record(const uint4 *src, uint4 *dst) {
 	const uint4 q0 = src[0], q1 = src[1], q2 = src[2], q3 = src[3], q4 = src[4], q5 = src[5],
@@ -1588,7 +1655,7 @@ __device__ __forceinline__ void stage_slice(const FrameConst &c, const FrameBufs
 			// quad by quad (no staging array: with a record pointer read from memory, as in a
 			// batched launch, an array between the load and the LDS store went to scratch)
 			const uint4 *src = b.tri_shade + (size_t)kRecStride * tri_rec(b, tri[k]);
-			if constexpr(MODE == kShadeFlat) {
+			if constexpr(is_flat(MODE)) {
 				make_flat_record(src, s_rec + i * kFlatQuads);
 			} else {
 #pragma unroll
@@ -1639,12 +1706,12 @@ __device__ __forceinline__ void walk_bits(uint64_t (&bm)[W], Visit &&visit) {
 	}
 }
 
-template <bool FP32, int W, int MODE>
+template <bool FP32, int W, int MODE, class PS>
 __device__ __forceinline__ void walk_pixel(const FrameConst &c, const FrameBufs &b,
 										   const uint4 *s_rec, uint64_t (&bm)[W], float px,
-										   float py, PixelState<FP32> &ps, bool additive) {
+										   float py, PS &ps, bool additive) {
 	const bool unsorted = (c.opts & LUCID_OPT_UNSORTED) != 0;
-	if constexpr(MODE == kShadeFlat) {
+	if constexpr(is_flat(MODE)) {
 		static_assert(!FP32, "flat records carry RGBA8 samples");
 		walk_bits(bm, [&](int e) {
 			const uint4 *rec = s_rec + e * kFlatQuads;
@@ -1756,7 +1823,7 @@ __device__ __forceinline__ void block_raster_kernel_body(const FrameConst &c, co
 	}
 	const bool additive = (c.opts & (1u << 5)) != 0;
 	const bool threshold = alpha_threshold_on(c);
-	PixelState<FP32> ps;
+	PixelState<FP32, MODE == kShadeFlatNarrow ? kNarrowWindow : kWindow> ps;
 	ps.init();
 	uint32_t count = 0;
 	bool opaque = false; // every pixel of the block is nearly opaque (alpha threshold)
@@ -1839,7 +1906,7 @@ __device__ __forceinline__ void block_raster_big_kernel_body(const FrameConst &c
 	const bool additive = (c.opts & (1u << 5)) != 0;
 	if((uint32_t)wave < nw) {
 		const uint32_t e0 = hdr[1 + wave], e1 = hdr[2 + wave];
-		PixelState<FP32> ps;
+		PixelState<FP32, MODE == kShadeFlatNarrow ? kNarrowBigWindow : kWindow> ps;
 		ps.init();
 		uint32_t count = 0;
 		const bool threshold = alpha_threshold_on(c);
@@ -2933,6 +3000,8 @@ static void launch_raster_batch_w(const BatchPlan &p, hipStream_t stream) {
 		launch_raster_batch_t<false, W, kShadeTextured>(p, stream);
 	else if(p.mode == kShadeFlat)
 		launch_raster_batch_t<false, W, kShadeFlat>(p, stream);
+	else if(p.mode == kShadeFlatNarrow)
+		launch_raster_batch_t<false, W, kShadeFlatNarrow>(p, stream);
 	else
 		launch_raster_batch_t<false, W, kShadeGeneral>(p, stream);
 }
@@ -2962,6 +3031,8 @@ hipError_t launch_raster(const FrameConst &c, const FrameBufs &b, hipStream_t st
 		launch_raster_t<false, kShadeTextured>(c, b, stream, events);
 	else if(mode == kShadeFlat)
 		launch_raster_t<false, kShadeFlat>(c, b, stream, events);
+	else if(mode == kShadeFlatNarrow)
+		launch_raster_t<false, kShadeFlatNarrow>(c, b, stream, events);
 	else
 		launch_raster_t<false, kShadeGeneral>(c, b, stream, events);
 	return hipGetLastError();

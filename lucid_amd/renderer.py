@@ -130,9 +130,9 @@ class LucidRenderer:
         _lib.check(self.lib.lucid_get_tuning(self.handle, ctypes.byref(t)), "lucid_get_tuning")
         return t.as_dict()
 
-    def set_tuning(self, slice_words: int = 1, presort_min: int = 256,
-                   samples_per_wave: int = 2048) -> None:
-        t = _lib.Tuning(slice_words, presort_min, samples_per_wave)
+    def set_tuning(self, slice_words: int = 1, presort_min: int = 128,
+                   samples_per_wave: int = 1024, window: int = 1) -> None:
+        t = _lib.Tuning(slice_words, presort_min, samples_per_wave, window)
         _lib.check(self.lib.lucid_set_tuning(self.handle, ctypes.byref(t)), "lucid_set_tuning")
 
     def autotune(self, gscene: "GpuScene", outputs: Optional[dict] = None, iters: int = 3,

@@ -17,7 +17,8 @@ from test_gpu_parity import _check, _render_both
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("tuning", [(1, 64, 1024), (2, 128, 4096), (2, 256, 2048)])
+@pytest.mark.parametrize("tuning", [(1, 64, 1024, 1), (2, 128, 4096, 4), (2, 256, 2048, 1),
+                                    (1, 128, 1024, 4)])
 def test_tuning_keeps_parity(oracle_mod, tuning):
     from lucid_amd.renderer import GpuScene, LucidRenderer
     sc = scenes.config1(num_tris=60_000, width=256, height=192, seed=71)
@@ -31,7 +32,8 @@ def test_tuning_keeps_parity(oracle_mod, tuning):
     gpu["bin_quads"], gpu["bin_tris"] = r.read_bin_lists()
     ref = oracle_mod.render(sc)
     _check(sc, gpu, ref, oracle_mod)
-    assert r.get_tuning() == dict(zip(("slice_words", "presort_min", "samples_per_wave"), tuning))
+    assert r.get_tuning() == dict(zip(("slice_words", "presort_min", "samples_per_wave", "window"),
+                                      tuning))
 
 
 def test_autotune_picks_a_candidate_and_keeps_parity(oracle_mod):

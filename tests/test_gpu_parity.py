@@ -19,11 +19,13 @@ pytestmark = pytest.mark.gpu
 INFO_U32 = 1152
 
 
-def _render_both(sc, oracle_mod, opts=0):
+def _render_both(sc, oracle_mod, opts=0, tuning=None):
     from lucid_amd.renderer import GpuScene, LucidRenderer
-    print(f"\n[case] {sc.name} {sc.width}x{sc.height} quads={sc.num_quads} opts={opts:#x}",
-          flush=True)
+    print(f"\n[case] {sc.name} {sc.width}x{sc.height} quads={sc.num_quads} opts={opts:#x} "
+          f"tuning={tuning}", flush=True)
     r = LucidRenderer((sc.width, sc.height), opts=opts)
+    if tuning:
+        r.set_tuning(**tuning)
     out = r.render(GpuScene(sc), r.alloc_outputs(rgba8=True, rgba32f=True, counts=True))
     gpu = {
         "image": out["image"].cpu().numpy(),
@@ -189,10 +191,16 @@ def test_block_over_sort_capacity(oracle_mod):
     assert gpu["stats"]["oversized_blocks"] >= 1
 
 
-def test_ordering_violations_fallback(oracle_mod):
+@pytest.mark.parametrize("window", [1, 4])
+def test_ordering_violations_fallback(oracle_mod, window):
+    """Steep intersecting planes: the block order is not the pixels' depth order. The wide
+    window (4 slots) puts this scene's pixels in order; the narrow one leaves pixels to the
+    exact per-pixel sort. Both match the oracle."""
     sc = scenes.tilted_planes(128, 96, 24)
-    r, gpu, ref = _render_both(sc, oracle_mod)
+    r, gpu, ref = _render_both(sc, oracle_mod, tuning={"window": window})
     _check(sc, gpu, ref, oracle_mod)
+    if window == 1:
+        assert gpu["stats"]["fallback_pixels"] > 0
 
 
 def test_deep_pixels_spill(oracle_mod):

@@ -70,6 +70,6 @@ def test_noalloc_rejects_sizes_beyond_its_limits():
     # 1024 x 1024 is 1024 bins (limit 256)
     assert lib.lucid_render_noalloc(1024, 1024, 0, None, ctypes.byref(inp), ctypes.byref(o),
                                     st) == _lib.LUCID_ERR_INVALID_ARG
-    bad = _lib.Tuning(3, 256, 2048)
-    assert lib.lucid_render_noalloc(64, 64, 0, ctypes.byref(bad), ctypes.byref(inp),
-                                    ctypes.byref(o), st) == _lib.LUCID_ERR_INVALID_ARG
+    for bad in (_lib.Tuning(3, 256, 2048, 1), _lib.Tuning(1, 256, 2048, 3)):
+        assert lib.lucid_render_noalloc(64, 64, 0, ctypes.byref(bad), ctypes.byref(inp),
+                                        ctypes.byref(o), st) == _lib.LUCID_ERR_INVALID_ARG
