@@ -26,7 +26,13 @@ sys.path.insert(0, ROOT)
 METRIC = "ms/frame and fragments blended/s @1080p 1M tris; fwd+bwd vs plain fwd ratio"
 HBM_PEAK_GBS = 8000.0
 DEFAULT_TRIS = 1_000_000
-RASTER_TRAFFIC_BYTES = 626.6e6  # block_raster + block_raster_big, one C1 frame (rocprofv3)
+# Measured on this workload at the default tuning with rocprofv3 (profiles/c1_forward_kernels.md):
+# HBM bytes of the block-raster stage (block_raster + block_raster_big) per frame, from
+# FETCH_SIZE and WRITE_SIZE passes scaled by a 256 MiB copy, and the VALU issue of the same
+# kernels (SQ_INSTS_VALU over kernel time x 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per
+# wave64 instruction). No counters are read in-process.
+RASTER_TRAFFIC_BYTES = 705.1e6
+RASTER_VALU_BUSY = 0.92
 
 
 def parse():
@@ -403,6 +409,8 @@ def main():
             # workload with rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a 256 MiB copy
             # (profiles/c1_forward_kernels.md); not re-measured here (no counters in-process)
             "traffic": RASTER_TRAFFIC_BYTES if args.tris == DEFAULT_TRIS else None,
+            # the bound that holds: the stage issues VALU instructions most of its time
+            "valu_busy": RASTER_VALU_BUSY if args.tris == DEFAULT_TRIS else None,
             "traffic_source": "rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated",
             "algorithmic_bytes": dom_bytes,
             "frame_algorithmic_bytes": ab["frame"],

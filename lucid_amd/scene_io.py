@@ -12,9 +12,11 @@ uploadInstances (src/lucid_renderer.cpp:352-429) builds them from Scene::draws
 (src/scene.cpp:483-512), with the scene-opacity factor of LucidApp::drawScene
 (src/lucid_app.cpp:646-650).
 
-Left out: textures and texture atlases (maps are parsed and kept, never sampled), vertex
-merging (off by default in the reference's InputScene) and the vertex-cache triangle
-reordering (src/tri_optimizer.cpp), which only permutes triangles within a mesh.
+Vertex merging (Scene::mergeVertices, off by default in the reference's InputScene) is
+`merge_vertices`, opt-in as there. Left out: decoding the MTL texture maps (their names
+are parsed and kept; no image decoder is built, so OBJ scenes render untextured, while the
+renderer samples textures a caller provides, `scenes.Texture`) and the vertex-cache
+triangle reordering (src/tri_optimizer.cpp), which only permutes triangles within a mesh.
 """
 
 from __future__ import annotations
