@@ -2360,7 +2360,8 @@ __device__ __forceinline__ bool bwd_fits(const FrameBufs &b, const BwdBufs &g) {
 // counts, statistics, exact-fallback queue) and, instead of the start states (which need
 // dL/dpixel), the BwdRecord of each (range, pixel) and the pixel's clamp/exact flags;
 // bwd_start_kernel turns them into start states when the backward runs.
-template <bool RECORD>
+// WIN: slots of the ordering window (4, or 2 with the narrow-window tuning).
+template <bool RECORD, int WIN>
 __global__ __launch_bounds__(kBigWaves * 64) void bwd_pass_a_kernel(FrameConst c, FrameBufs b,
 																	 BwdBufs g) {
 	if(b.ctrl[kCtrlOverflow] & kOvfBinLists)
@@ -2408,7 +2409,7 @@ __global__ __launch_bounds__(kBigWaves * 64) void bwd_pass_a_kernel(FrameConst c
 	const uint32_t row0 = g.row_base[slot];
 
 	if((uint32_t)wave < ent.nw) {
-		OrderWindow<BwdColor> w;
+		std::conditional_t<WIN == kWindow, OrderWindow<BwdColor>, SmallWindow<BwdColor, WIN>> w;
 		w.init();
 		BwdTotals part;
 		part.init();
@@ -3083,17 +3084,21 @@ hipError_t launch_raster_recording(const FrameConst &c, const FrameBufs &b, cons
 	hipLaunchKernelGGL(record_gate_kernel, dim3(1), dim3(64), 0, stream, b, g);
 	if(ev)
 		(void)hipEventRecord(ev[1], stream);
-	hipLaunchKernelGGL(bwd_pass_a_kernel<true>, dim3(c.bin_count * 16), dim3(kBigWaves * 64), 0,
-					   stream, c, b, g);
+	if(c.window == 1)
+		hipLaunchKernelGGL((bwd_pass_a_kernel<true, 2>), dim3(c.bin_count * 16), dim3(kBigWaves * 64),
+						   0, stream, c, b, g);
+	else
+		hipLaunchKernelGGL((bwd_pass_a_kernel<true, kWindow>), dim3(c.bin_count * 16),
+						   dim3(kBigWaves * 64), 0, stream, c, b, g);
 	if(c.slice_words == 2) {
-		hipLaunchKernelGGL((block_raster_kernel<true, 2, false>), dim3(c.bin_count * 16), dim3(64), 0,
+		hipLaunchKernelGGL((block_raster_kernel<true, 2, kShadeGeneral>), dim3(c.bin_count * 16), dim3(64), 0,
 						   stream, c, b);
-		hipLaunchKernelGGL((block_raster_big_kernel<true, 2, false>), dim3(c.bin_count * 16),
+		hipLaunchKernelGGL((block_raster_big_kernel<true, 2, kShadeGeneral>), dim3(c.bin_count * 16),
 						   dim3(kBigWaves * 64), 0, stream, c, b);
 	} else {
-		hipLaunchKernelGGL((block_raster_kernel<true, 1, false>), dim3(c.bin_count * 16), dim3(64), 0,
+		hipLaunchKernelGGL((block_raster_kernel<true, 1, kShadeGeneral>), dim3(c.bin_count * 16), dim3(64), 0,
 						   stream, c, b);
-		hipLaunchKernelGGL((block_raster_big_kernel<true, 1, false>), dim3(c.bin_count * 16),
+		hipLaunchKernelGGL((block_raster_big_kernel<true, 1, kShadeGeneral>), dim3(c.bin_count * 16),
 						   dim3(kBigWaves * 64), 0, stream, c, b);
 	}
 	if(ev)
@@ -3146,8 +3151,12 @@ hipError_t launch_backward_reuse(const FrameConst &c, const FrameBufs &b, const 
 // rows follow the block rows). After these every row's triangle is known.
 hipError_t launch_backward_order(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
 								 hipStream_t stream) {
-	hipLaunchKernelGGL(bwd_pass_a_kernel<false>, dim3(c.bin_count * 16), dim3(kBigWaves * 64), 0,
-					   stream, c, b, g);
+	if(c.window == 1) // narrow-window tuning: two slots
+		hipLaunchKernelGGL((bwd_pass_a_kernel<false, 2>), dim3(c.bin_count * 16),
+						   dim3(kBigWaves * 64), 0, stream, c, b, g);
+	else
+		hipLaunchKernelGGL((bwd_pass_a_kernel<false, kWindow>), dim3(c.bin_count * 16),
+						   dim3(kBigWaves * 64), 0, stream, c, b, g);
 	hipLaunchKernelGGL(bwd_fallback_kernel, dim3(1024), dim3(kFallbackThreads), 0, stream, c, b, g);
 	return hipGetLastError();
 }
