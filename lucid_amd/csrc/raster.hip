@@ -501,6 +501,11 @@ struct OrderWindow {
 template <class P, int N>
 struct SmallWindow {
 	static_assert(N == 1 || N == 2, "one or two slots");
+	// N = 1 holds a placeholder from the start (key 0, below every sample key; a zero
+	// payload), which the first sample pushes out: no fill count is kept, and the emit
+	// callback must treat the zero payload as nothing (the forward blend does). The keys of
+	// one pixel are distinct (one sample per triangle), so `<` there finds what `<=` does.
+	// After flush an N = 1 window takes no more samples.
 	uint64_t k0, k1;
 	P c0, c1;
 	uint32_t n;
@@ -508,29 +513,34 @@ struct SmallWindow {
 	bool bad;
 
 	__device__ __forceinline__ void init() {
-		k0 = k1 = kEmptyKey;
+		k0 = N == 1 ? 0ull : kEmptyKey;
+		k1 = kEmptyKey;
 		c0.zero(), c1.zero();
 		n = 0, first = kEmptyKey, last = 0, bad = false;
 	}
 	template <class F> __device__ __forceinline__ void emit(uint64_t k, const P &col, F &&f) {
-		bad |= k <= last;
-		first = first == kEmptyKey ? k : first;
+		if constexpr(N == 1) {
+			bad |= k < last;
+			first = first == kEmptyKey && k != 0ull ? k : first;
+		} else {
+			bad |= k <= last;
+			first = first == kEmptyKey ? k : first;
+		}
 		last = k;
 		f(col);
 	}
 	template <class F> __device__ __forceinline__ void insert(uint64_t key, const P &col, F &&f) {
-		if(n < (uint32_t)N) {
-			k0 = n == 0 ? key : k0, c0 = P::sel(n == 0, col, c0);
-			if constexpr(N == 2)
-				k1 = n == 1 ? key : k1, c1 = P::sel(n == 1, col, c1);
-			n++;
-			return;
-		}
 		if constexpr(N == 1) {
 			const bool take_new = key < k0;
 			emit(take_new ? key : k0, P::sel(take_new, col, c0), f);
 			k0 = take_new ? k0 : key, c0 = P::sel(take_new, c0, col);
 		} else {
+			if(n < (uint32_t)N) {
+				k0 = n == 0 ? key : k0, c0 = P::sel(n == 0, col, c0);
+				k1 = n == 1 ? key : k1, c1 = P::sel(n == 1, col, c1);
+				n++;
+				return;
+			}
 			const bool a = k1 < k0;
 			const uint64_t km = a ? k1 : k0;
 			const P cm = P::sel(a, c1, c0);
@@ -543,8 +553,7 @@ struct SmallWindow {
 	}
 	template <class F> __device__ __forceinline__ void flush(F &&f) {
 		if constexpr(N == 1) {
-			if(n)
-				emit(k0, c0, f);
+			emit(k0, c0, f); // the last sample, or the placeholder if there was none
 		} else {
 			for(uint32_t r = 0; r < n; r++) {
 				const bool a = k1 < k0;
@@ -552,8 +561,8 @@ struct SmallWindow {
 				k0 = a ? k0 : kEmptyKey;
 				k1 = a ? kEmptyKey : k1;
 			}
+			n = 0;
 		}
-		n = 0;
 	}
 };
 
@@ -1612,16 +1621,19 @@ __device__ __forceinline__ void make_flat_record(const uint4 *src, uint4 *dst) {
 	const float e0[3] = {bits2f(q1.x), bits2f(q1.y), bits2f(q1.z)}, p0 = bits2f(q1.w);
 	const float e1[3] = {bits2f(q2.x), bits2f(q2.y), bits2f(q2.z)}, p1 = bits2f(q2.w);
 	dst[0] = make_uint4(q0.x, q0.y, q0.z, q6.z);
+	// a zero alpha contributes nothing (shading.glsl:167-168): zero planes and a zero alpha
+	// byte encode such a sample as 0, the RGBA8 word that contributes nothing
+	const bool none = a == 0.0f;
 #pragma unroll
 	for(int k = 0; k < 3; k++) {
 		const float d1 = c1[k] - c0[k], d2 = c2[k] - c0[k];
 		const float lx = fmaf(e0[0], d1, e1[0] * d2), ly = fmaf(e0[1], d1, e1[1] * d2);
 		const float lz = fmaf(e0[2], d1, e1[2] * d2);
 		const float kk = fmaf(-p0, d1, fmaf(-p1, d2, c0[k]));
-		dst[1 + k] = make_uint4(f2bits(lx), f2bits(ly), f2bits(lz), f2bits(kk));
+		dst[1 + k] = none ? make_uint4(0u, 0u, 0u, 0u)
+						  : make_uint4(f2bits(lx), f2bits(ly), f2bits(lz), f2bits(kk));
 	}
-	// RGBA8 alpha byte; a zero alpha contributes nothing (shading.glsl:167-168)
-	dst[4] = make_uint4(f2u(a * 255.0f) << 24, a == 0.0f ? 1u : 0u, 0u, 0u);
+	dst[4] = make_uint4(f2u(a * 255.0f) << 24, 0u, 0u, 0u); // RGBA8 alpha byte
 }
 
 // RGBA8 sample of a flat record at pixel (px, py); inv_ray = record_inv_depth(f[0], ...).
@@ -1635,10 +1647,9 @@ __device__ __forceinline__ SampleColor<false> shade_flat(const uint4 *f, float p
 		const float l = fmaf(bits2f(pl.x), px, fmaf(bits2f(pl.y), py, bits2f(pl.z)));
 		rgb[k] = sat(fmaf(rp, l, bits2f(pl.w)));
 	}
-	const uint4 f4 = f[4];
 	SampleColor<false> out;
-	out.enc = f4.y ? 0u : ((uint32_t)(rgb[0] * 255.0f) | ((uint32_t)(rgb[1] * 255.0f) << 8) |
-						   ((uint32_t)(rgb[2] * 255.0f) << 16) | f4.x);
+	out.enc = (uint32_t)(rgb[0] * 255.0f) | ((uint32_t)(rgb[1] * 255.0f) << 8) |
+			  ((uint32_t)(rgb[2] * 255.0f) << 16) | f[4].x;
 	return out;
 }
 
