@@ -498,14 +498,17 @@ struct OrderWindow {
 // lists whose block order is already (nearly) each pixel's depth order, where the window
 // only has to catch neighbours in the wrong order; a pixel it cannot put in order is still
 // detected (bad) and sorted exactly.
-template <class P, int N>
+template <class P, int N, bool PH = (N == 1)>
 struct SmallWindow {
 	static_assert(N == 1 || N == 2, "one or two slots");
-	// N = 1 holds a placeholder from the start (key 0, below every sample key; a zero
-	// payload), which the first sample pushes out: no fill count is kept, and the emit
-	// callback must treat the zero payload as nothing (the forward blend does). The keys of
-	// one pixel are distinct (one sample per triangle), so `<` there finds what `<=` does.
-	// After flush an N = 1 window takes no more samples.
+	static_assert(N == 2 || PH, "the one-slot window keeps a placeholder");
+	// PH: the slots hold placeholders from the start (key 0, below every sample key; a zero
+	// payload), which the first samples push out, so no fill count is kept; the emit
+	// callback must treat the zero payload as nothing (the forward blend does; the
+	// backward's order recording does not, so it keeps the count). Placeholders always
+	// leave before any sample (they are the smallest keys), and the keys of one pixel are
+	// distinct (one sample per triangle), so `<` finds what `<=` does. After flush such a
+	// window takes no more samples.
 	uint64_t k0, k1;
 	P c0, c1;
 	uint32_t n;
@@ -513,13 +516,13 @@ struct SmallWindow {
 	bool bad;
 
 	__device__ __forceinline__ void init() {
-		k0 = N == 1 ? 0ull : kEmptyKey;
-		k1 = kEmptyKey;
+		k0 = PH ? 0ull : kEmptyKey;
+		k1 = PH ? 0ull : kEmptyKey;
 		c0.zero(), c1.zero();
 		n = 0, first = kEmptyKey, last = 0, bad = false;
 	}
 	template <class F> __device__ __forceinline__ void emit(uint64_t k, const P &col, F &&f) {
-		if constexpr(N == 1) {
+		if constexpr(PH) {
 			bad |= k < last;
 			first = first == kEmptyKey && k != 0ull ? k : first;
 		} else {
@@ -535,11 +538,13 @@ struct SmallWindow {
 			emit(take_new ? key : k0, P::sel(take_new, col, c0), f);
 			k0 = take_new ? k0 : key, c0 = P::sel(take_new, c0, col);
 		} else {
-			if(n < (uint32_t)N) {
-				k0 = n == 0 ? key : k0, c0 = P::sel(n == 0, col, c0);
-				k1 = n == 1 ? key : k1, c1 = P::sel(n == 1, col, c1);
-				n++;
-				return;
+			if constexpr(!PH) {
+				if(n < (uint32_t)N) {
+					k0 = n == 0 ? key : k0, c0 = P::sel(n == 0, col, c0);
+					k1 = n == 1 ? key : k1, c1 = P::sel(n == 1, col, c1);
+					n++;
+					return;
+				}
 			}
 			const bool a = k1 < k0;
 			const uint64_t km = a ? k1 : k0;
@@ -555,7 +560,7 @@ struct SmallWindow {
 		if constexpr(N == 1) {
 			emit(k0, c0, f); // the last sample, or the placeholder if there was none
 		} else {
-			for(uint32_t r = 0; r < n; r++) {
+			for(uint32_t r = 0; r < (PH ? 2u : n); r++) { // (placeholders leave first)
 				const bool a = k1 < k0;
 				emit(a ? k1 : k0, P::sel(a, c1, c0), f);
 				k0 = a ? k0 : kEmptyKey;
@@ -569,7 +574,7 @@ struct SmallWindow {
 // Forward per-pixel state: the ordering window (N slots) feeding the front-to-back blend.
 template <bool FP32, int N = kWindow>
 struct PixelState {
-	std::conditional_t<N == kWindow, OrderWindow<SampleColor<FP32>>, SmallWindow<SampleColor<FP32>, N>> w;
+	std::conditional_t<N == kWindow, OrderWindow<SampleColor<FP32>>, SmallWindow<SampleColor<FP32>, N, true>> w;
 	Accum acc;
 
 	__device__ __forceinline__ void init() {
@@ -1717,11 +1722,14 @@ __device__ __forceinline__ void walk_bits(uint64_t (&bm)[W], Visit &&visit) {
 	}
 }
 
-template <bool FP32, int W, int MODE, class PS>
-__device__ __forceinline__ void walk_pixel(const FrameConst &c, const FrameBufs &b,
-										   const uint4 *s_rec, uint64_t (&bm)[W], float px,
-										   float py, PS &ps, bool additive) {
-	const bool unsorted = (c.opts & LUCID_OPT_UNSORTED) != 0;
+// OVER: the common case compiled apart (sorted, front-to-back "over"), so that its loop
+// carries no test of the blend options; otherwise they are read per sample.
+template <bool FP32, int W, int MODE, bool OVER, class PS>
+__device__ __forceinline__ void walk_pixel_t(const FrameConst &c, const FrameBufs &b,
+											 const uint4 *s_rec, uint64_t (&bm)[W], float px,
+											 float py, PS &ps, bool additive_opt) {
+	const bool unsorted = !OVER && (c.opts & LUCID_OPT_UNSORTED) != 0;
+	const bool additive = !OVER && additive_opt;
 	if constexpr(is_flat(MODE)) {
 		static_assert(!FP32, "flat records carry RGBA8 samples");
 		walk_bits(bm, [&](int e) {
@@ -1729,12 +1737,12 @@ __device__ __forceinline__ void walk_pixel(const FrameConst &c, const FrameBufs 
 			const uint4 f0 = rec[0];
 			const float inv = record_inv_depth(f0, px, py);
 			const SampleColor<false> col = shade_flat(rec, px, py, inv);
-			if(col.contributes()) {
-				if(unsorted)
-					ps.acc.blend(col.rgba(), additive);
-				else
-					ps.insert(pixel_key(inv, f0.w), col, additive);
-			}
+			// a sample that contributes nothing (word 0) blends as nothing, so it is not
+			// tested for: it takes its place in the window like any other
+			if(unsorted)
+				ps.acc.blend(col.rgba(), additive);
+			else
+				ps.insert(pixel_key(inv, f0.w), col, additive);
 		});
 		return;
 	}
@@ -1751,6 +1759,16 @@ __device__ __forceinline__ void walk_pixel(const FrameConst &c, const FrameBufs 
 				ps.insert(pixel_key(inv, rec[6].z), col, additive);
 		}
 	});
+}
+
+template <bool FP32, int W, int MODE, class PS>
+__device__ __forceinline__ void walk_pixel(const FrameConst &c, const FrameBufs &b,
+										   const uint4 *s_rec, uint64_t (&bm)[W], float px,
+										   float py, PS &ps, bool additive) {
+	if(!additive && !(c.opts & LUCID_OPT_UNSORTED))
+		walk_pixel_t<FP32, W, MODE, true>(c, b, s_rec, bm, px, py, ps, false);
+	else
+		walk_pixel_t<FP32, W, MODE, false>(c, b, s_rec, bm, px, py, ps, additive);
 }
 
 template <int W>
@@ -2420,7 +2438,7 @@ __global__ __launch_bounds__(kBigWaves * 64) void bwd_pass_a_kernel(FrameConst c
 	const uint32_t row0 = g.row_base[slot];
 
 	if((uint32_t)wave < ent.nw) {
-		std::conditional_t<WIN == kWindow, OrderWindow<BwdColor>, SmallWindow<BwdColor, WIN>> w;
+		std::conditional_t<WIN == kWindow, OrderWindow<BwdColor>, SmallWindow<BwdColor, WIN, false>> w;
 		w.init();
 		BwdTotals part;
 		part.init();
