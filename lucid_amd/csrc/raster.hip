@@ -1918,10 +1918,17 @@ __device__ __forceinline__ void block_raster_big_kernel_body(const FrameConst &c
 	const uint32_t flags = b.blk_flags[slot];
 	if(!(flags & kSlotPresorted) || (flags & kSlotPromoted) || b.ctrl[kCtrlRecorded])
 		return; // (a recording pass A renders the block)
+	// Each wave's slice staging area holds, once its walk is done, what the chaining of the
+	// ranges reads: 20 KB of LDS per work-group (flat records) instead of 29, so that more
+	// work-groups fit on a CU.
+	struct RangeEnd {
+		float4 part[64]; // the range's partial composite
+		uint64_t first[64], last[64];
+		uint32_t cnt[64]; // fragments | 0x80000000 when the range's order is not exact
+	};
 	__shared__ uint4 s_rec[kBigWaves][(64 * W) * rec_quads(MODE)];
-	__shared__ float4 s_part[kBigWaves][64];
-	__shared__ uint64_t s_first[kBigWaves][64], s_last[kBigWaves][64];
-	__shared__ uint32_t s_cnt[kBigWaves][64];
+	static_assert(sizeof(RangeEnd) <= sizeof(s_rec[0]), "a range's end state fits its staging");
+	auto range_end = [&](int w) -> RangeEnd & { return *reinterpret_cast<RangeEnd *>(&s_rec[w][0]); };
 	const int wave = threadIdx.x >> 6, lane = lane_id();
 	const uint4 sl = b.blk_slot[slot];
 	const uint32_t m = sl.y;
@@ -1964,10 +1971,12 @@ __device__ __forceinline__ void block_raster_big_kernel_body(const FrameConst &c
 			opaque = threshold && all_below_alpha_threshold(ps.acc.tr);
 		}
 		ps.flush(additive);
-		s_part[wave][lane] = make_float4(ps.acc.cr, ps.acc.cg, ps.acc.cb, ps.acc.tr);
-		s_first[wave][lane] = ps.w.first;
-		s_last[wave][lane] = ps.w.last;
-		s_cnt[wave][lane] = count | (ps.w.bad ? 0x80000000u : 0u);
+		wave_sync_lds(); // every lane is done with the staged records (also after an early end)
+		RangeEnd &re = range_end(wave);
+		re.part[lane] = make_float4(ps.acc.cr, ps.acc.cg, ps.acc.cb, ps.acc.tr);
+		re.first[lane] = ps.w.first;
+		re.last[lane] = ps.w.last;
+		re.cnt[lane] = count | (ps.w.bad ? 0x80000000u : 0u);
 	}
 	__syncthreads();
 	if(wave != 0)
@@ -1978,14 +1987,15 @@ __device__ __forceinline__ void block_raster_big_kernel_body(const FrameConst &c
 	bool bad = false;
 	uint32_t count = 0;
 	for(uint32_t w = 0; w < nw; w++) {
-		const uint32_t cw = s_cnt[w][lane];
+		const RangeEnd &re = range_end((int)w);
+		const uint32_t cw = re.cnt[lane];
 		count += cw & 0x7fffffffu;
 		bad |= (cw >> 31) != 0;
-		const uint64_t f = s_first[w][lane];
+		const uint64_t f = re.first[lane];
 		if(f != kEmptyKey) {
 			bad |= f <= last;
-			last = s_last[w][lane];
-			const float4 p = s_part[w][lane];
+			last = re.last[lane];
+			const float4 p = re.part[lane];
 			acc.cr = acc.cr + acc.tr * p.x;
 			acc.cg = acc.cg + acc.tr * p.y;
 			acc.cb = acc.cb + acc.tr * p.z;
