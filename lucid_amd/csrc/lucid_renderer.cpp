@@ -944,7 +944,7 @@ int lucid_backward_ex(lucid_renderer *r, const lucid_frame_inputs *in, const flo
 	if(changed && !regroup && grad_colors_rgb) {
 		// same sizes, different mesh: the vertex sums above used the old grouping
 		LUCID_CHECK(reduce_vertices(rb, (uint32_t)ntri, (uint32_t)nverts, true, grad_colors_rgb,
-									grad_tri_alpha, stream));
+									nullptr, stream)); // (the alpha gradients stand)
 		LUCID_CHECK(hipStreamSynchronize(stream));
 		grouped = true;
 	}

@@ -59,10 +59,11 @@ __global__ void row_scatter_kernel(const uint32_t *ctrl, BwdBufs g, uint32_t ntr
 
 // One thread per triangle: the sum of its group of rows in fixed point at the frame's scale
 // 2^(k - 6) (a row is below 2^50 units there), lowered by one bit per doubling of the group
-// beyond 4096 rows so that the sum stays below 2^62.
+// beyond 4096 rows so that the sum stays below 2^62. The alpha component also goes to
+// grad_alpha (when given).
 __global__ void tri_group_sum_kernel(FrameConst c, const uint32_t *ctrl, const uint32_t *offsets,
 									 uint32_t ntri, const uint32_t *grouped, const float *rows,
-									 float *tri_grad) {
+									 float *tri_grad, float *grad_alpha) {
 	const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
 	if(t >= ntri)
 		return;
@@ -94,9 +95,14 @@ __global__ void tri_group_sum_kernel(FrameConst c, const uint32_t *ctrl, const u
 		for(int w = 0; w < kGradWords; w++)
 			acc[w] += to_fixed(r0[w], scale);
 	}
+	float out[kGradWords];
 #pragma unroll
-	for(int w = 0; w < kGradWords; w++)
-		tri_grad[(size_t)t * kGradWords + w] = (float)ldexp((double)(long long)acc[w], -k);
+	for(int w = 0; w < kGradWords; w++) {
+		out[w] = (float)ldexp((double)(long long)acc[w], -k);
+		tri_grad[(size_t)t * kGradWords + w] = out[w];
+	}
+	if(grad_alpha)
+		grad_alpha[t] = out[9];
 }
 
 // Vertex of each triangle corner: triangle 2 q + s of quad q uses quad vertices
@@ -349,8 +355,9 @@ hipError_t reduce_sum(const FrameConst &c, const FrameBufs &b, const BwdBufs &g,
 					  float *grad_colors, float *grad_alpha, hipStream_t stream) {
 	if(ntri)
 		hipLaunchKernelGGL(tri_group_sum_kernel, grid_for(ntri), dim3(256), 0, stream, c, b.ctrl,
-						   r.offsets, ntri, r.grouped, g.rows, r.tri_grad);
-	return reduce_vertices(r, ntri, nverts, regroup, grad_colors, grad_alpha, stream);
+						   r.offsets, ntri, r.grouped, g.rows, r.tri_grad, grad_alpha);
+	// (the alpha gradients are written above)
+	return reduce_vertices(r, ntri, nverts, regroup, grad_colors, nullptr, stream);
 }
 
 // Per-vertex colour sums (and the alpha copy) from r.tri_grad.
