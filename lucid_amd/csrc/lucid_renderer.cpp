@@ -895,23 +895,26 @@ int lucid_backward_ex(lucid_renderer *r, const lucid_frame_inputs *in, const flo
 		LUCID_CHECK(launch_backward_prepare(c, b, g, stream));
 	}
 	// The helper stream takes the gradient scale and the corner check beside pass A, then the
-	// row grouping beside pass B (which needs the scale).
+	// row grouping beside pass B (which needs the scale). Pass A is submitted first: the GPU
+	// would otherwise wait for the host to submit the helper stream's work before it.
 	LUCID_CHECK(hipEventRecord(r->aux_fork, stream));
-	LUCID_CHECK(hipStreamWaitEvent(r->aux, r->aux_fork, 0));
-	LUCID_CHECK(launch_grad_max(c, g, b.ctrl, r->aux));
-	LUCID_CHECK(reduce_check_corners(c, b, (uint32_t)ntri, rb.vkeys,
-									 r->ctrl.as<uint32_t>() + kCtrlTopoChanged, r->aux));
-	LUCID_CHECK(hipEventRecord(r->aux_scale, r->aux));
 	if(recorded)
 		LUCID_CHECK(launch_backward_recorded_order(c, b, g, stream));
 	else
 		LUCID_CHECK(launch_backward_order(c, b, g, stream));
 	LUCID_CHECK(hipEventRecord(r->aux_order, stream));
+	LUCID_CHECK(hipStreamWaitEvent(r->aux, r->aux_fork, 0));
+	LUCID_CHECK(launch_grad_max(c, g, b.ctrl, r->aux));
+	LUCID_CHECK(reduce_check_corners(c, b, (uint32_t)ntri, rb.vkeys,
+									 r->ctrl.as<uint32_t>() + kCtrlTopoChanged, r->aux));
+	LUCID_CHECK(hipEventRecord(r->aux_scale, r->aux));
+	// pass B is submitted before the grouping beside it: submitted the other way round, the
+	// grouping's large grids took the CUs first and pass B started only after them
+	LUCID_CHECK(hipStreamWaitEvent(stream, r->aux_scale, 0));
+	LUCID_CHECK(launch_backward_rows(c, b, g, stream));
 	LUCID_CHECK(hipStreamWaitEvent(r->aux, r->aux_order, 0));
 	LUCID_CHECK(reduce_group_rows(c, b, g, rb, (uint32_t)ntri, r->aux));
 	LUCID_CHECK(hipEventRecord(r->aux_join, r->aux));
-	LUCID_CHECK(hipStreamWaitEvent(stream, r->aux_scale, 0));
-	LUCID_CHECK(launch_backward_rows(c, b, g, stream));
 	LUCID_CHECK(hipStreamWaitEvent(stream, r->aux_join, 0));
 	LUCID_CHECK(reduce_sum(c, b, g, rb, (uint32_t)ntri, (uint32_t)nverts, regroup,
 						   grad_colors_rgb, grad_tri_alpha, stream));
