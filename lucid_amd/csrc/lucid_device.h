@@ -262,6 +262,7 @@ enum BatchStage : int {
 	kBsHighPrep,
 	kBsRowgenHigh,
 	kBsHighFinalize,
+	kBsHighFinalizeLong, // persistent: a few work-groups per frame over its queue
 	kBsBlockRaster,
 	kBsBlockRasterBig,
 	kBsFallback,

@@ -1073,6 +1073,7 @@ struct lucid_batch {
 namespace {
 
 constexpr uint32_t kBatchFallbackWgs = 64; // persistent exact-fallback work-groups per frame
+constexpr uint32_t kBatchLongSortWgs = 16; // persistent long-list sort work-groups per frame
 constexpr int kScanColsHost = 16;         // columns per work-group of bin_wg_scan_kernel
 
 // Work-groups of each pipeline stage for one frame (the grids launch_setup, launch_binning and
@@ -1091,6 +1092,7 @@ void stage_wgs(const FrameConst &c, uint32_t w[kBsCount]) {
 	w[kBsHighPrep] = 1;
 	w[kBsRowgenHigh] = nb * kRowgenSplit;
 	w[kBsHighFinalize] = nb * 16;
+	w[kBsHighFinalizeLong] = kBatchLongSortWgs;
 	w[kBsBlockRaster] = nb * 16;
 	w[kBsBlockRasterBig] = nb * 16;
 	w[kBsFallback] = kBatchFallbackWgs;

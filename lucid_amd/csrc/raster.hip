@@ -3009,8 +3009,9 @@ static void launch_raster_batch_t(const BatchPlan &p, hipStream_t stream) {
 					   dim3(kRowgenThreads), 0, stream, p.frames, p.stage_pre(kBsRowgenHigh), p.num_frames);
 	hipLaunchKernelGGL(high_finalize_kernel_batch, dim3(p.total[kBsHighFinalize]),
 					   dim3(kSortThreads), 0, stream, p.frames, p.stage_pre(kBsHighFinalize), p.num_frames);
-	hipLaunchKernelGGL(high_finalize_long_kernel_batch, dim3(p.total[kBsHighFinalize]),
-					   dim3(kSortThreads), 0, stream, p.frames, p.stage_pre(kBsHighFinalize), p.num_frames);
+	hipLaunchKernelGGL(high_finalize_long_kernel_batch, dim3(p.total[kBsHighFinalizeLong]),
+					   dim3(kSortThreads), 0, stream, p.frames, p.stage_pre(kBsHighFinalizeLong),
+					   p.num_frames);
 	hipLaunchKernelGGL((block_raster_kernel_batch<FP32, W, MODE>), dim3(p.total[kBsBlockRaster]),
 					   dim3(64), 0, stream, p.frames, p.stage_pre(kBsBlockRaster), p.num_frames);
 	hipLaunchKernelGGL((block_raster_big_kernel_batch<FP32, W, MODE>),
