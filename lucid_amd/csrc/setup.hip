@@ -266,25 +266,58 @@ __device__ int process_quad(const FrameConst &c, const FrameBufs &b, uint32_t q,
 	return area <= 4u ? 0 : 1;
 }
 
+// Instance of quad q: the last instance whose first quad (inst_prefix) is at most q
+// (instances without quads share their successor's prefix and are skipped).
+__device__ __forceinline__ int find_instance(const FrameConst &c, const FrameBufs &b, uint32_t q) {
+	int lo = 0, hi = c.num_instances - 1;
+	while(lo < hi) {
+		int mid = (lo + hi + 1) >> 1;
+		if((uint32_t)b.inst_prefix[mid] <= q)
+			lo = mid;
+		else
+			hi = mid - 1;
+	}
+	return lo;
+}
+
 __device__ __forceinline__ void setup_kernel_body(const FrameConst &c, const FrameBufs &b, WgPos wg) {
 	__shared__ uint32_t s_counts[6]; // small, large, rejected x4
+	__shared__ int s_inst[2];         // instances of the work-group's first and last quads
+	__shared__ uint32_t s_prefix[kSetupThreads]; // their prefixes, when they span few instances
 	if(threadIdx.x < 6)
 		s_counts[threadIdx.x] = 0;
+	const uint32_t q0 = wg.id * (uint32_t)kSetupThreads;
+	const uint32_t nq = (uint32_t)c.num_quads;
+	// the two searches over the whole instance table run once per work-group (one lane of
+	// each of two waves); each quad then searches the few instances of its work-group in LDS
+	if(q0 < nq && (threadIdx.x == 0 || threadIdx.x == 64))
+		s_inst[threadIdx.x >> 6] = find_instance(c, b, threadIdx.x == 0 ? q0 : min(q0 + kSetupThreads - 1, nq - 1));
+	__syncthreads();
+	const int i0 = q0 < nq ? s_inst[0] : 0, span = q0 < nq ? s_inst[1] - i0 + 1 : 0;
+	const bool local = span <= kSetupThreads;
+	if(local && (int)threadIdx.x < span)
+		s_prefix[threadIdx.x] = (uint32_t)b.inst_prefix[i0 + threadIdx.x];
 	__syncthreads();
 
-	uint32_t q = wg.id * (uint32_t)kSetupThreads + threadIdx.x;
+	uint32_t q = q0 + threadIdx.x;
 	int kind = -1;
-	if(q < (uint32_t)c.num_quads) {
-		int lo = 0, hi = c.num_instances - 1;
-		while(lo < hi) {
-			int mid = (lo + hi + 1) >> 1;
-			if((uint32_t)b.inst_prefix[mid] <= q)
-				lo = mid;
-			else
-				hi = mid - 1;
+	if(q < nq) {
+		int lo;
+		if(local) {
+			int l = 0, h = span - 1;
+			while(l < h) {
+				const int mid = (l + h + 1) >> 1;
+				if(s_prefix[mid] <= q)
+					l = mid;
+				else
+					h = mid - 1;
+			}
+			lo = i0 + l;
+		} else {
+			lo = find_instance(c, b, q);
 		}
 		lucid_instance_data ins = b.instances[lo];
-		uint32_t k = q - (uint32_t)b.inst_prefix[lo];
+		uint32_t k = q - (local ? s_prefix[lo - i0] : (uint32_t)b.inst_prefix[lo]);
 		const uint32_t *ix = b.quad_indices + (size_t)ins.index_offset + (size_t)k * 4;
 		uint32_t vi[4];
 #pragma unroll
