@@ -1129,9 +1129,15 @@ __device__ __forceinline__ void rowgen_high_kernel_body(const FrameConst &c, con
 			row_scan_init(rs, s0, s1, (float)bi.bin_x, (float)(bi.bin_y + row_lo));
 			row = row_lo;
 		}
+		// the four bands' records first, then one reservation for all of them (one atomic
+		// round trip per wave instead of one per band), then the records and list entries
+		uint64_t lo_b[4];
+		uint32_t hi_b[4], bxm_b[4];
+#pragma unroll
 		for(int by = 0; by < 4; by++) {
 			const int band0 = by * 8;
 			const bool in_band = row_hi >= band0 && row_lo <= band0 + 7;
+			lo_b[by] = 0, hi_b[by] = 0, bxm_b[by] = 0;
 			if(!ballot(in_band))
 				continue;
 			// rows 0-5 of the band in lo (10 bits each), rows 6-7 in hi; rows the triangle
@@ -1162,38 +1168,46 @@ __device__ __forceinline__ void rowgen_high_kernel_body(const FrameConst &c, con
 				else
 					hi |= (uint32_t)kBinMask << (10 * (k - 6));
 			}
-			const uint64_t emit = ballot(bxm != 0);
-			if(!emit)
-				continue;
-			// the band's record counter and its four block-list counters: lanes 0-4 reserve
-			// all five ranges with one atomic instruction (one round trip, not five)
-			uint64_t sel[4];
+			lo_b[by] = lo, hi_b[by] = hi, bxm_b[by] = bxm;
+		}
+		// per band: its record counter, then its four block-list counters; lane 5 by + k
+		// reserves counter k of band by
+		uint64_t emit[4], sel[4][4];
+		uint32_t add = 0;
+		uint32_t *ctr = band_n;
 #pragma unroll
-			for(int bx = 0; bx < 4; bx++)
-				sel[bx] = ballot((bxm >> bx) & 1u);
-			uint32_t add = 0;
-			uint32_t *ctr = band_n + by;
-#pragma unroll
-			for(int bx = 0; bx < 4; bx++)
-				if(lane == 1 + bx)
-					add = (uint32_t)__popcll(sel[bx]), ctr = blk_n + by * 4 + bx;
-			if(lane == 0)
-				add = (uint32_t)__popcll(emit);
-			uint32_t got = 0;
-			if(lane < 5 && add)
-				got = atomicAdd(ctr, add);
-			const uint32_t rbase = __shfl(got, 0, 64);
-			const uint32_t idx = rbase + (uint32_t)__popcll(emit & ((1ull << lane) - 1ull));
-			if(bxm)
-				b.high_records[slot16[by * 4].z + idx] =
-					make_uint4(t, (uint32_t)lo & 0x3fffffffu, (uint32_t)(lo >> 30) & 0x3fffffffu,
-							   hi | (bxm << 20));
+		for(int by = 0; by < 4; by++) {
+			emit[by] = ballot(bxm_b[by] != 0);
+			if(lane == 5 * by)
+				add = (uint32_t)__popcll(emit[by]), ctr = band_n + by;
 #pragma unroll
 			for(int bx = 0; bx < 4; bx++) {
-				const uint32_t lb = __shfl(got, 1 + bx, 64);
+				sel[by][bx] = ballot((bxm_b[by] >> bx) & 1u);
+				if(lane == 5 * by + 1 + bx)
+					add = (uint32_t)__popcll(sel[by][bx]), ctr = blk_n + by * 4 + bx;
+			}
+		}
+		uint32_t got = 0;
+		if(lane < 20 && add)
+			got = atomicAdd(ctr, add);
+		const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+		for(int by = 0; by < 4; by++) {
+			if(!emit[by])
+				continue;
+			const uint32_t bxm = bxm_b[by];
+			const uint32_t rbase = __shfl(got, 5 * by, 64);
+			const uint32_t idx = rbase + (uint32_t)__popcll(emit[by] & below);
+			if(bxm)
+				b.high_records[slot16[by * 4].z + idx] =
+					make_uint4(t, (uint32_t)lo_b[by] & 0x3fffffffu,
+							   (uint32_t)(lo_b[by] >> 30) & 0x3fffffffu, hi_b[by] | (bxm << 20));
+#pragma unroll
+			for(int bx = 0; bx < 4; bx++) {
+				const uint32_t lb = __shfl(got, 5 * by + 1 + bx, 64);
 				if((bxm >> bx) & 1u)
 					b.blk_lists[slot16[by * 4 + bx].x + lb +
-								(uint32_t)__popcll(sel[bx] & ((1ull << lane) - 1ull))] = idx;
+								(uint32_t)__popcll(sel[by][bx] & below)] = idx;
 			}
 		}
 	});
