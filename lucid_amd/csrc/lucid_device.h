@@ -246,6 +246,21 @@ struct WgPos {
 	uint32_t id, count;
 };
 
+// Work-groups go to the 8 XCDs (each with its own L2) in turn: work-group w runs on XCD
+// w % 8. bin_run_id renumbers a grid so that each run of 16 consecutive ids (the blocks of
+// one bin, which gather the same triangles' records) runs on one XCD, one after another,
+// while the runs still go to the XCDs in turn (so dense bins stay spread over the chip).
+// Ids from the last whole round of 8 runs on keep their place; a grid whose per-frame
+// ranges are multiples of 16 keeps every run inside one frame.
+__device__ __forceinline__ uint32_t bin_run_id(uint32_t w, uint32_t count) {
+	constexpr uint32_t kXcds = 8, kRun = 16;
+	const uint32_t whole = count / (kXcds * kRun) * (kXcds * kRun);
+	if(w >= whole)
+		return w;
+	const uint32_t x = w % kXcds, k = w / kXcds;
+	return ((k / kRun) * kXcds + x) * kRun + k % kRun;
+}
+
 struct FrameDesc {
 	FrameConst c;
 	FrameBufs b;

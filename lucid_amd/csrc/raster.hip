@@ -1522,7 +1522,7 @@ __device__ __forceinline__ void high_finalize_kernel_body(const FrameConst &c, c
 }
 
 __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel(FrameConst c, FrameBufs b) {
-	high_finalize_kernel_body<kMidSortCap>(c, b, WgPos{blockIdx.x, gridDim.x});
+	high_finalize_kernel_body<kMidSortCap>(c, b, WgPos{bin_run_id(blockIdx.x, gridDim.x), gridDim.x});
 }
 
 // The long-list pass over the blocks queued by the first: work-group w of `count` takes
@@ -1543,9 +1543,10 @@ __global__ __launch_bounds__(kSortThreads) void high_finalize_long_kernel(FrameC
 __global__ __launch_bounds__(kSortThreads) void high_finalize_kernel_batch(const FrameDesc *__restrict__ frames,
 		const uint32_t *__restrict__ pre, uint32_t num_frames) {
 	const StageBatch sb{frames, pre, num_frames};
-	const uint32_t f = batch_frame(sb, blockIdx.x);
+	const uint32_t wg = bin_run_id(blockIdx.x, gridDim.x);
+	const uint32_t f = batch_frame(sb, wg);
 	const FrameDesc &d = sb.frames[f];
-	high_finalize_kernel_body<kMidSortCap>(d.c, d.b, WgPos{blockIdx.x - sb.pre[f], sb.pre[f + 1] - sb.pre[f]});
+	high_finalize_kernel_body<kMidSortCap>(d.c, d.b, WgPos{wg - sb.pre[f], sb.pre[f + 1] - sb.pre[f]});
 }
 __global__ __launch_bounds__(kSortThreads) void high_finalize_long_kernel_batch(const FrameDesc *__restrict__ frames,
 		const uint32_t *__restrict__ pre, uint32_t num_frames) {
@@ -1902,15 +1903,16 @@ __device__ __forceinline__ void block_raster_kernel_body(const FrameConst &c, co
 
 template <bool FP32, int W, int MODE>
 __global__ __launch_bounds__(64) void block_raster_kernel(FrameConst c, FrameBufs b) {
-	block_raster_kernel_body<FP32, W, MODE>(c, b, WgPos{blockIdx.x, gridDim.x});
+	block_raster_kernel_body<FP32, W, MODE>(c, b, WgPos{bin_run_id(blockIdx.x, gridDim.x), gridDim.x});
 }
 template <bool FP32, int W, int MODE>
 __global__ __launch_bounds__(64) void block_raster_kernel_batch(const FrameDesc *__restrict__ frames,
 		const uint32_t *__restrict__ pre, uint32_t num_frames) {
 	const StageBatch sb{frames, pre, num_frames};
-	const uint32_t f = batch_frame(sb, blockIdx.x);
+	const uint32_t wg = bin_run_id(blockIdx.x, gridDim.x);
+	const uint32_t f = batch_frame(sb, wg);
 	const FrameDesc &d = sb.frames[f];
-	block_raster_kernel_body<FP32, W, MODE>(d.c, d.b, WgPos{blockIdx.x - sb.pre[f], sb.pre[f + 1] - sb.pre[f]});
+	block_raster_kernel_body<FP32, W, MODE>(d.c, d.b, WgPos{wg - sb.pre[f], sb.pre[f + 1] - sb.pre[f]});
 }
 
 // Pre-sorted lists longer than one register sort: one work-group per block. Wave w takes
@@ -2021,15 +2023,16 @@ __device__ __forceinline__ void block_raster_big_kernel_body(const FrameConst &c
 
 template <bool FP32, int W, int MODE>
 __global__ __launch_bounds__(kBigWaves * 64) void block_raster_big_kernel(FrameConst c, FrameBufs b) {
-	block_raster_big_kernel_body<FP32, W, MODE>(c, b, WgPos{blockIdx.x, gridDim.x});
+	block_raster_big_kernel_body<FP32, W, MODE>(c, b, WgPos{bin_run_id(blockIdx.x, gridDim.x), gridDim.x});
 }
 template <bool FP32, int W, int MODE>
 __global__ __launch_bounds__(kBigWaves * 64) void block_raster_big_kernel_batch(const FrameDesc *__restrict__ frames,
 		const uint32_t *__restrict__ pre, uint32_t num_frames) {
 	const StageBatch sb{frames, pre, num_frames};
-	const uint32_t f = batch_frame(sb, blockIdx.x);
+	const uint32_t wg = bin_run_id(blockIdx.x, gridDim.x);
+	const uint32_t f = batch_frame(sb, wg);
 	const FrameDesc &d = sb.frames[f];
-	block_raster_big_kernel_body<FP32, W, MODE>(d.c, d.b, WgPos{blockIdx.x - sb.pre[f], sb.pre[f + 1] - sb.pre[f]});
+	block_raster_big_kernel_body<FP32, W, MODE>(d.c, d.b, WgPos{wg - sb.pre[f], sb.pre[f + 1] - sb.pre[f]});
 }
 
 // ------------------------------------------------------------------ exact fallback
