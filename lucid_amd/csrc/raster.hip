@@ -1097,9 +1097,14 @@ __global__ __launch_bounds__(1024) void high_prep_kernel_batch(const FrameDesc *
 
 // Block-row records of a high bin with the 4-row rule (raster_high.glsl:54-90). Each lane
 // takes one triangle of the bin and scans its rows once, from the first row of its 4-row
-// group (the start row fixes the scanline arithmetic) to its last row, emitting one record
-// per 8-row band it covers; kRowgenSplit work-groups share a bin. Record and list
-// positions come from wave-aggregated atomics.
+// group (the start row fixes the scanline arithmetic) to its last row, then emits one record
+// per 8-row band it covers; kRowgenSplit work-groups share a bin. A wave's row loop runs as
+// long as its tallest triangle, so each work-group first sorts its triangles by height (a
+// counting sort over the 8 heights in 4-row groups, up to kRowgenRound list chunks at a
+// time) and its waves scan them 64 at a time in that order. Record and list positions come
+// from wave-aggregated atomics.
+constexpr int kRowgenRound = 4; // list chunks of kRowgenThreads entries per sorting round
+
 __device__ __forceinline__ void rowgen_high_kernel_body(const FrameConst &c, const FrameBufs &b, WgPos wg) {
 	if(b.ctrl[kCtrlOverflow] & (kOvfBinLists | kOvfRecords | kOvfLists))
 		return;
@@ -1114,64 +1119,62 @@ __device__ __forceinline__ void rowgen_high_kernel_body(const FrameConst &c, con
 	const uint4 *slot16 = b.blk_slot + (size_t)bin * 16;
 	uint32_t *band_n = b.band_count + (size_t)bin * 4;
 	uint32_t *blk_n = b.blk_count + (size_t)bin * 16;
-	const int lane = lane_id();
+	const int lane = lane_id(), wave = threadIdx.x >> 6;
 	const uint32_t stride = kRowgenThreads * kRowgenSplit;
-	const uint32_t wave_base = split * kRowgenThreads + (threadIdx.x & ~63u);
-	__shared__ uint32_t s_live[kRowgenThreads / 64][128];
-	for_live_tris(b, bi, wave_base, stride, s_live[threadIdx.x >> 6], [&](uint32_t t) {
-		int row = 0, row_lo = 0, row_hi = -1;
+	constexpr int kHeights = kBinSize / 4;
+	__shared__ uint32_t s_sorted[kRowgenRound * kRowgenThreads];
+	__shared__ uint32_t s_cnt[kHeights], s_off[kHeights], s_n;
+	// the row spans of each lane's triangle: rows[r][lane] of the wave
+	__shared__ uint16_t s_rows[kRowgenThreads / 64][kBinSize][64];
+	uint16_t *rows = &s_rows[wave][0][lane];
+	// first row group and height in row groups of triangle t in this bin
+	auto groups = [&](uint32_t yr, int &g0, int &g1) {
+		g0 = min(max((int)(yr & 0xffffu) - bi.bin_y, 0), kBinMask) >> 2;
+		g1 = min(max((int)(yr >> 16) - bi.bin_y, 0), kBinMask) >> 2;
+	};
+	auto scan_tri = [&](uint32_t t) { // called by the whole wave; t = ~0 on idle lanes
+		int row_lo = 0, row_hi = -1;
 		RowScan rs;
 		if(t != 0xffffffffu) {
 			const uint4 s0 = b.tri_scan[2 * (size_t)tri_rec(b, t)], s1 = b.tri_scan[2 * (size_t)tri_rec(b, t) + 1];
-			const int min_g = min(max((int)(s0.w & 0xffffu) - bi.bin_y, 0), kBinMask) >> 2;
-			const int max_g = min(max((int)(s0.w >> 16) - bi.bin_y, 0), kBinMask) >> 2;
-			row_lo = min_g * 4, row_hi = max_g * 4 + 3;
+			int g0, g1;
+			groups(s0.w, g0, g1);
+			row_lo = g0 * 4, row_hi = g1 * 4 + 3;
 			row_scan_init(rs, s0, s1, (float)bi.bin_x, (float)(bi.bin_y + row_lo));
-			row = row_lo;
 		}
-		// the four bands' records first, then one reservation for all of them (one atomic
-		// round trip per wave instead of one per band), then the records and list entries
+		uint32_t bxm_all = 0; // block-column masks of the four bands, 4 bits each
+		for(int r = row_lo; r <= row_hi; r++) {
+			const uint32_t v = row_scan_step(rs);
+			rows[r * 64] = (uint16_t)v;
+			bxm_all |= span_bx_mask(v) << (4 * (r >> 3));
+		}
+		// each band's record: rows 0-5 in lo (10 bits each), rows 6-7 in hi; rows the
+		// triangle does not reach stay empty (31, 0)
 		uint64_t lo_b[4];
 		uint32_t hi_b[4], bxm_b[4];
 #pragma unroll
 		for(int by = 0; by < 4; by++) {
-			const int band0 = by * 8;
-			const bool in_band = row_hi >= band0 && row_lo <= band0 + 7;
-			lo_b[by] = 0, hi_b[by] = 0, bxm_b[by] = 0;
-			if(!ballot(in_band))
+			const uint32_t bxm = (bxm_all >> (4 * by)) & 15u;
+			lo_b[by] = 0, hi_b[by] = 0, bxm_b[by] = bxm;
+			if(!ballot(bxm != 0))
 				continue;
-			// rows 0-5 of the band in lo (10 bits each), rows 6-7 in hi; rows the triangle
-			// does not reach stay empty (31, 0)
-			uint64_t lo = 0;
-			uint32_t hi = 0, bxm = 0;
-			int r0 = 8, r1 = -1;
-			if(in_band) {
-				// rows before band0 were scanned with the previous band
-				const int last = min(row_hi, band0 + 7);
-				r0 = row - band0, r1 = last - band0;
-				for(; row <= last; row++) {
-					const uint32_t v = row_scan_step(rs);
-					const int r = row - band0;
-					bxm |= span_bx_mask(v);
-					if(r < 6)
-						lo |= (uint64_t)v << (10 * r);
-					else
-						hi |= v << (10 * (r - 6));
-				}
-			}
+			if(bxm) {
+				uint64_t lo = 0;
+				uint32_t hi = 0;
 #pragma unroll
-			for(int k = 0; k < 8; k++) {
-				if(k >= r0 && k <= r1)
-					continue;
-				if(k < 6)
-					lo |= (uint64_t)kBinMask << (10 * k);
-				else
-					hi |= (uint32_t)kBinMask << (10 * (k - 6));
+				for(int k = 0; k < 8; k++) {
+					const int r = by * 8 + k;
+					const uint32_t v = r >= row_lo && r <= row_hi ? (uint32_t)rows[r * 64] : (uint32_t)kBinMask;
+					if(k < 6)
+						lo |= (uint64_t)v << (10 * k);
+					else
+						hi |= v << (10 * (k - 6));
+				}
+				lo_b[by] = lo, hi_b[by] = hi;
 			}
-			lo_b[by] = lo, hi_b[by] = hi, bxm_b[by] = bxm;
 		}
 		// per band: its record counter, then its four block-list counters; lane 5 by + k
-		// reserves counter k of band by
+		// reserves counter k of band by (one atomic instruction for all of them)
 		uint64_t emit[4], sel[4][4];
 		uint32_t add = 0;
 		uint32_t *ctr = band_n;
@@ -1210,7 +1213,48 @@ __device__ __forceinline__ void rowgen_high_kernel_body(const FrameConst &c, con
 								(uint32_t)__popcll(sel[by][bx] & below)] = idx;
 			}
 		}
-	});
+	};
+	// the work-group's list positions: chunks split, split + kRowgenSplit, ... of
+	// kRowgenThreads entries; culled triangles (every second one of a soup quad) are dropped
+	const uint32_t round_len = stride * kRowgenRound;
+	for(uint32_t base = split * kRowgenThreads; base < bi.nlevel; base += round_len) {
+		if(threadIdx.x < kHeights)
+			s_cnt[threadIdx.x] = 0;
+		__syncthreads();
+		uint32_t tk[kRowgenRound];
+		int hk[kRowgenRound];
+#pragma unroll
+		for(int k = 0; k < kRowgenRound; k++) {
+			const uint32_t i = base + k * stride + threadIdx.x;
+			const uint32_t t = i < bi.nlevel ? bin_tri_at(b, i, bi.qc, bi.qo, bi.to) : 0xffffffffu;
+			int h = -1;
+			if(t != 0xffffffffu) {
+				int g0, g1;
+				groups(b.tri_scan[2 * (size_t)tri_rec(b, t)].w, g0, g1);
+				h = g1 > g0 ? g1 - g0 : 0;
+				atomicAdd(&s_cnt[h], 1u);
+			}
+			tk[k] = t, hk[k] = h;
+		}
+		__syncthreads();
+		if(threadIdx.x == 0) {
+			uint32_t run = 0;
+			for(int h = 0; h < kHeights; h++)
+				s_off[h] = run, run += s_cnt[h];
+			s_n = run;
+		}
+		__syncthreads();
+#pragma unroll
+		for(int k = 0; k < kRowgenRound; k++)
+			if(hk[k] >= 0)
+				s_sorted[atomicAdd(&s_off[hk[k]], 1u)] = tk[k];
+		__syncthreads();
+		const uint32_t n = s_n;
+		for(uint32_t q = wave * 64; q < n; q += kRowgenThreads)
+			scan_tri(q + lane < n ? s_sorted[q + lane] : 0xffffffffu);
+		if(base + round_len < bi.nlevel)
+			__syncthreads(); // the next round reuses the sorted list
+	}
 }
 
 __global__ __launch_bounds__(kRowgenThreads) void rowgen_high_kernel(FrameConst c, FrameBufs b) {
