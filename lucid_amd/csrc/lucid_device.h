@@ -371,6 +371,11 @@ __device__ __forceinline__ uint32_t ord_f32(float f) {
 	uint32_t u = f2bits(f);
 	return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
+// ~ord_f32(f) (reversed order) as one select: u for negative f, else u ^ 0x7fffffff.
+__device__ __forceinline__ uint32_t inv_ord_f32(float f) {
+	const uint32_t u = f2bits(f);
+	return (int32_t)u < 0 ? u : u ^ 0x7fffffffu;
+}
 
 // ------------------------------------------------------------------ small vectors
 

@@ -396,7 +396,7 @@ __device__ __forceinline__ SampleColor<FP32> shade_sample(const FrameConst &c, c
 }
 
 __device__ __forceinline__ uint64_t pixel_key(float inv_ray, uint32_t t) {
-	return ((uint64_t)(~ord_f32(inv_ray)) << 32) | t;
+	return ((uint64_t)inv_ord_f32(inv_ray) << 32) | t;
 }
 
 // ------------------------------------------------------------------ per-pixel state
@@ -417,6 +417,20 @@ struct Accum {
 			tr = tr * (1.0f - col.w);
 		}
 	}
+	// The same for an RGBA8 sample (bytes c and a): the byte scales fold into one weight per
+	// sample, so each channel takes one fused multiply-add, cr += c (a tr / 255) with
+	// a = byte / 255 (the sums differ from blend()'s by rounding only; the transmittance
+	// step is blend()'s, operation for operation).
+	__device__ __forceinline__ void blend(SampleColor<false> s, bool additive) {
+		const float a = (float)(s.enc >> 24) * (1.0f / 255.0f);
+		const float w = (additive ? a : a * tr) * (1.0f / 255.0f);
+		cr = fmaf((float)(s.enc & 0xffu), w, cr);
+		cg = fmaf((float)((s.enc >> 8) & 0xffu), w, cg);
+		cb = fmaf((float)((s.enc >> 16) & 0xffu), w, cb);
+		if(!additive)
+			tr = tr * (1.0f - a);
+	}
+	__device__ __forceinline__ void blend(SampleColor<true> s, bool additive) { blend(s.col, additive); }
 };
 
 // Sample order key: nearer (larger inverse depth) first, equal depths by triangle id.
@@ -513,26 +527,30 @@ struct SmallWindow {
 	P c0, c1;
 	uint32_t n;
 	uint64_t first, last;
-	bool bad;
+	uint32_t bad; // out-of-order emits (a count: one add with carry per sample)
 
 	__device__ __forceinline__ void init() {
 		k0 = PH ? 0ull : kEmptyKey;
 		k1 = PH ? 0ull : kEmptyKey;
 		c0.zero(), c1.zero();
-		n = 0, first = kEmptyKey, last = 0, bad = false;
+		n = 0, first = kEmptyKey, last = 0, bad = 0;
 	}
 	template <class F> __device__ __forceinline__ void emit(uint64_t k, const P &col, F &&f) {
 		if constexpr(PH) {
-			bad |= k < last;
-			first = first == kEmptyKey && k != 0ull ? k : first;
+			bad += k < last ? 1u : 0u; // (first is kept by insert)
 		} else {
-			bad |= k <= last;
+			bad += k <= last ? 1u : 0u;
 			first = first == kEmptyKey ? k : first;
 		}
 		last = k;
 		f(col);
 	}
 	template <class F> __device__ __forceinline__ void insert(uint64_t key, const P &col, F &&f) {
+		// with placeholders, first is the smallest key inserted: while the emitted keys
+		// increase (an exact pixel) that is the first sample emitted, and a pixel whose keys
+		// do not increase is resolved by the exact fallback, whatever first holds
+		if constexpr(PH)
+			first = key < first ? key : first;
 		if constexpr(N == 1) {
 			const bool take_new = key < k0;
 			emit(take_new ? key : k0, P::sel(take_new, col, c0), f);
@@ -582,10 +600,10 @@ struct PixelState {
 		acc.init();
 	}
 	__device__ __forceinline__ void insert(uint64_t key, SampleColor<FP32> col, bool additive) {
-		w.insert(key, col, [&](const SampleColor<FP32> &s) { acc.blend(s.rgba(), additive); });
+		w.insert(key, col, [&](const SampleColor<FP32> &s) { acc.blend(s, additive); });
 	}
 	__device__ __forceinline__ void flush(bool additive) {
-		w.flush([&](const SampleColor<FP32> &s) { acc.blend(s.rgba(), additive); });
+		w.flush([&](const SampleColor<FP32> &s) { acc.blend(s, additive); });
 	}
 };
 
@@ -1799,7 +1817,7 @@ __device__ __forceinline__ void walk_pixel_t(const FrameConst &c, const FrameBuf
 			// a sample that contributes nothing (word 0) blends as nothing, so it is not
 			// tested for: it takes its place in the window like any other
 			if(unsorted)
-				ps.acc.blend(col.rgba(), additive);
+				ps.acc.blend(col, additive);
 			else
 				ps.insert(pixel_key(inv, f0.w), col, additive);
 		});
@@ -1813,7 +1831,7 @@ __device__ __forceinline__ void walk_pixel_t(const FrameConst &c, const FrameBuf
 		const SampleColor<FP32> col = shade_record<FP32, TEX>(c, b, rec, px, py, inv);
 		if(col.contributes()) {
 			if(unsorted)
-				ps.acc.blend(col.rgba(), additive); // list order, no depth order
+				ps.acc.blend(col, additive); // list order, no depth order
 			else
 				ps.insert(pixel_key(inv, rec[6].z), col, additive);
 		}
@@ -2154,7 +2172,7 @@ __device__ Accum blend_sorted(const FrameConst &c, const FrameBufs &b, const uin
 		for(uint32_t j = 0; j < cnt; j++) {
 			const SampleColor<FP32> c2 = col.shfl((int)j);
 			if(c2.contributes())
-				acc.blend(c2.rgba(), additive);
+				acc.blend(c2, additive);
 		}
 	}
 	return acc;
