@@ -1365,6 +1365,7 @@ __device__ __forceinline__ void high_finalize_kernel_body(const FrameConst &c, c
 	__shared__ uint32_t s_tdk[2][kSortThreads]; // full depth keys of tied entries, per round
 	__shared__ uint32_t s_bound[kBigWaves + 1];
 	__shared__ uint32_t s_base, s_parts, s_total;
+	__shared__ uint8_t s_smp[CAP]; // samples (pixel-mask bits) of each entry, in list order
 	static_assert(kBigSortCap <= (1 << 13), "list positions fit 13 bits of the key");
 	static_assert(kSortThreads == 256, "four waves share the chunks");
 	const uint32_t *list = b.blk_lists + sl.x;
@@ -1417,6 +1418,7 @@ __device__ __forceinline__ void high_finalize_kernel_body(const FrameConst &c, c
 				const uint64_t msk = record_block_mask(rec, bx);
 				scratch_tri[j] = rec.x;
 				scratch_msk[j] = msk;
+				s_smp[j] = (uint8_t)__popcll(msk);
 				mine += (uint32_t)__popcll(msk);
 				if(unsorted) {
 					s_key[j] = j; // list order: rank 0, the position alone
@@ -1508,7 +1510,55 @@ __device__ __forceinline__ void high_finalize_kernel_body(const FrameConst &c, c
 	// recompute their full depth key and take their place among the run.
 	const bool exact_runs = (c.opts & LUCID_OPT_FP32_SAMPLES) != 0 && !unsorted;
 	uint32_t carry = 0, found = 0;
-	for(uint32_t r = 0; r < rounds; r++) {
+	if(!exact_runs) {
+		// Without the exact runs, the range starts need only the sample counts, which pass 1
+		// left in LDS: each thread takes a contiguous run of sorted positions, one block scan
+		// of the runs' totals gives the samples before each entry, and the output is a copy
+		// with no barrier between its loads.
+		const uint32_t per = (m + kSortThreads - 1) / kSortThreads;
+		const uint32_t i0 = min(m, threadIdx.x * per), i1 = min(m, i0 + per);
+		uint32_t sum = 0;
+		for(uint32_t i = i0; i < i1; i++)
+			sum += s_smp[s_key[i] & 0x1fffu];
+		const uint32_t inc = wave_inclusive_add(sum);
+		if(lane_s == 63)
+			s_wsum[0][wave_s] = inc;
+		__syncthreads();
+		uint32_t run = inc - sum;
+#pragma unroll
+		for(int w = 0; w < kSortThreads / 64; w++)
+			run += w < wave_s ? s_wsum[0][w] : 0u;
+		for(uint32_t i = i0; i < i1; i++) {
+			const uint32_t key = s_key[i];
+			const bool cut = unsorted || i == 0 || (key >> 13) != (s_key[i - 1] >> 13);
+			for(uint32_t w = 1; w < nw; w++) {
+				const uint32_t target = (uint32_t)(((uint64_t)w * total) / nw);
+				if(!((found >> w) & 1u) && run >= target && cut) {
+					atomicMin(&s_bound[w], i);
+					found |= 1u << w;
+				}
+			}
+			run += s_smp[key & 0x1fffu];
+		}
+		for(uint32_t i = threadIdx.x; i < m; i += 4 * kSortThreads) {
+			uint32_t tk[4];
+			uint64_t mk[4];
+#pragma unroll
+			for(int k = 0; k < 4; k++) {
+				const uint32_t ik = i + k * kSortThreads;
+				const uint32_t pos = ik < m ? s_key[ik] & 0x1fffu : 0u;
+				tk[k] = ik < m ? scratch_tri[pos] : 0u;
+				mk[k] = ik < m ? scratch_msk[pos] : 0ull;
+			}
+#pragma unroll
+			for(int k = 0; k < 4; k++) {
+				const uint32_t ik = i + k * kSortThreads;
+				if(ik < m)
+					tri_out[ik] = tk[k], mask_out[ik] = mk[k];
+			}
+		}
+	}
+	for(uint32_t r = 0; r < rounds && exact_runs; r++) {
 		const uint32_t i = r * kSortThreads + threadIdx.x;
 		const uint32_t r0 = r * kSortThreads, r1 = min(m, r0 + kSortThreads);
 		uint32_t tri = 0, key = 0, dk = 0;
