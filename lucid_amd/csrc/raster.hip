@@ -31,9 +31,9 @@ constexpr int kRowgenThreads = 256;
 constexpr int kSortThreads = 256;
 constexpr int kFallbackThreads = 256;
 constexpr int kWindow = 4;
-// Narrow ordering windows (tuning window 1): one slot in the register-sorted blocks, two in
-// the pre-sorted ones (whose 19-bit depth ranks tie more often).
-constexpr int kNarrowWindow = 1, kNarrowBigWindow = 2;
+// Narrow ordering windows (tuning window 1): one slot in the register-sorted blocks and in
+// the pre-sorted ones (whose depth ranks keep 32 - log2 of the padded list length bits).
+constexpr int kNarrowWindow = 1, kNarrowBigWindow = 1;
 constexpr int kFallbackCap = 512; // fragments per pixel sorted in LDS by the fallback kernel
 
 // ------------------------------------------------------------------ shading
@@ -775,14 +775,13 @@ __device__ __forceinline__ float block_inv_depth(const FrameBufs &b, uint4 rec, 
 // first): the key minus the block's smallest, shifted right just enough that the block's key
 // range fits BITS bits. Monotone in the key, and float keys keep relative precision (about
 // BITS - log2(binades spanned) mantissa bits), so ties are rare and are resolved per pixel.
-template <int BITS>
 struct DepthGrid {
 	uint32_t lo;
 	int shift;
-	__device__ __forceinline__ DepthGrid(uint32_t lo_, uint32_t hi_) : lo(lo_) {
+	__device__ __forceinline__ DepthGrid(uint32_t lo_, uint32_t hi_, int bits) : lo(lo_) {
 		const uint32_t span = hi_ - lo_;
 		const int need = span ? 32 - __clz(span) : 0;
-		shift = need > BITS ? need - BITS : 0;
+		shift = need > bits ? need - bits : 0;
 	}
 	__device__ __forceinline__ uint32_t rank(uint32_t key) const { return (key - lo) >> shift; }
 };
@@ -820,7 +819,7 @@ __device__ __forceinline__ void sort_block_positions(const FrameBufs &b, const u
 			hi = max(hi, dk[k]);
 		}
 	}
-	const DepthGrid<24> grid(wave_min_u32(lo), wave_max_u32(hi));
+	const DepthGrid grid(wave_min_u32(lo), wave_max_u32(hi), 24);
 	// a real key never equals the padding: position 255 exists only when nothing is padded
 	auto key = [&](int k) -> uint32_t {
 		const uint32_t j = k * 64 + lane;
@@ -1373,6 +1372,10 @@ __device__ __forceinline__ void high_finalize_kernel_body(const FrameConst &c, c
 	const int wave_s = threadIdx.x >> 6, lane_s = lane_id();
 	const uint32_t P = max(256u, next_pow2(m));
 	const uint32_t nchunks = P / 256;
+	// a key is (depth rank << pb | list position), with pb = log2(P) position bits, so
+	// shorter lists keep more rank bits (24 at P = 256, 19 at 8192)
+	const int pb = 31 - __clz(P);
+	const uint32_t pmask = P - 1u;
 	const bool unsorted = (c.opts & LUCID_OPT_UNSORTED) != 0; // the unsorted baseline
 	// The block's pool space (allocated first): the sorted entries (header, triangle ids,
 	// masks) and a scratch copy of the same in list order, which pass 1 fills so that the
@@ -1404,7 +1407,7 @@ __device__ __forceinline__ void high_finalize_kernel_body(const FrameConst &c, c
 	uint32_t *hdr = b.sorted_pool + s_base;
 	uint32_t *scratch_tri = hdr + sorted_words;
 	uint64_t *scratch_msk = (uint64_t *)(hdr + scratch_mask);
-	// Keys are (19-bit depth rank << 13 | list position). Pass 1 gathers the records once:
+	// Keys are (depth rank << pb | list position). Pass 1 gathers the records once:
 	// depth keys (in s_key), the scratch copy, the total of the sample counts and the
 	// block's key range.
 	uint32_t lo = ~0u, hi = 0u;
@@ -1449,17 +1452,17 @@ __device__ __forceinline__ void high_finalize_kernel_body(const FrameConst &c, c
 		for(int w = 0; w <= kBigWaves; w++)
 			s_bound[w] = w == 0 ? 0u : m;
 	}
-	const DepthGrid<19> grid(lo, hi);
+	const DepthGrid grid(lo, hi, 32 - pb);
 	// Bitonic sort: levels up to 256 in registers (one 256-key chunk per wave at a time; each
 	// lane reads back only what it wrote in pass 1), then per level the strides >= 256 in LDS
-	// and the rest in registers. A real key never equals the padding ~0u: position 8191
+	// and the rest in registers. A real key never equals the padding ~0u: position P - 1
 	// exists only when nothing is padded.
 	for(uint32_t ch = wave_s; ch < nchunks && !unsorted; ch += kSortThreads / 64) {
 		uint32_t v[4];
 #pragma unroll
 		for(int e = 0; e < 4; e++) {
 			const uint32_t j = ch * 256 + e * 64 + lane_s;
-			v[e] = j < m ? (grid.rank(s_key[j]) << 13) | j : ~0u;
+			v[e] = j < m ? (grid.rank(s_key[j]) << pb) | j : ~0u;
 		}
 		bitonic_regs_from<4>(v, ch * 256);
 #pragma unroll
@@ -1493,7 +1496,7 @@ __device__ __forceinline__ void high_finalize_kernel_body(const FrameConst &c, c
 		}
 		__syncthreads();
 	}
-	// s_key[i] & 0x1fff is the list position of the i-th entry in depth order. Entries are
+	// s_key[i] & pmask is the list position of the i-th entry in depth order. Entries are
 	// visited in rounds of kSortThreads consecutive positions (coalesced LDS and output).
 	const uint32_t rounds = (m + kSortThreads - 1) / kSortThreads;
 	const uint32_t nw = s_parts, total = s_total;
@@ -1519,7 +1522,7 @@ __device__ __forceinline__ void high_finalize_kernel_body(const FrameConst &c, c
 		const uint32_t i0 = min(m, threadIdx.x * per), i1 = min(m, i0 + per);
 		uint32_t sum = 0;
 		for(uint32_t i = i0; i < i1; i++)
-			sum += s_smp[s_key[i] & 0x1fffu];
+			sum += s_smp[s_key[i] & pmask];
 		const uint32_t inc = wave_inclusive_add(sum);
 		if(lane_s == 63)
 			s_wsum[0][wave_s] = inc;
@@ -1530,7 +1533,7 @@ __device__ __forceinline__ void high_finalize_kernel_body(const FrameConst &c, c
 			run += w < wave_s ? s_wsum[0][w] : 0u;
 		for(uint32_t i = i0; i < i1; i++) {
 			const uint32_t key = s_key[i];
-			const bool cut = unsorted || i == 0 || (key >> 13) != (s_key[i - 1] >> 13);
+			const bool cut = unsorted || i == 0 || (key >> pb) != (s_key[i - 1] >> pb);
 			for(uint32_t w = 1; w < nw; w++) {
 				const uint32_t target = (uint32_t)(((uint64_t)w * total) / nw);
 				if(!((found >> w) & 1u) && run >= target && cut) {
@@ -1538,7 +1541,7 @@ __device__ __forceinline__ void high_finalize_kernel_body(const FrameConst &c, c
 					found |= 1u << w;
 				}
 			}
-			run += s_smp[key & 0x1fffu];
+			run += s_smp[key & pmask];
 		}
 		for(uint32_t i = threadIdx.x; i < m; i += 4 * kSortThreads) {
 			uint32_t tk[4];
@@ -1546,7 +1549,7 @@ __device__ __forceinline__ void high_finalize_kernel_body(const FrameConst &c, c
 #pragma unroll
 			for(int k = 0; k < 4; k++) {
 				const uint32_t ik = i + k * kSortThreads;
-				const uint32_t pos = ik < m ? s_key[ik] & 0x1fffu : 0u;
+				const uint32_t pos = ik < m ? s_key[ik] & pmask : 0u;
 				tk[k] = ik < m ? scratch_tri[pos] : 0u;
 				mk[k] = ik < m ? scratch_msk[pos] : 0ull;
 			}
@@ -1566,12 +1569,12 @@ __device__ __forceinline__ void high_finalize_kernel_body(const FrameConst &c, c
 		bool tie = false;
 		if(i < m) {
 			key = s_key[i];
-			tri = scratch_tri[key & 0x1fffu];
-			msk = scratch_msk[key & 0x1fffu];
-			tie = exact_runs && ((i > 0 && (s_key[i - 1] >> 13) == (key >> 13)) ||
-								 (i + 1 < m && (s_key[i + 1] >> 13) == (key >> 13)));
+			tri = scratch_tri[key & pmask];
+			msk = scratch_msk[key & pmask];
+			tie = exact_runs && ((i > 0 && (s_key[i - 1] >> pb) == (key >> pb)) ||
+								 (i + 1 < m && (s_key[i + 1] >> pb) == (key >> pb)));
 			if(tie)
-				dk = ~ord_f32(block_inv_depth(b, recs[list[key & 0x1fffu]], bx, px0, py0));
+				dk = ~ord_f32(block_inv_depth(b, recs[list[key & pmask]], bx, px0, py0));
 		}
 		s_tdk[r & 1][threadIdx.x] = dk;
 		const uint32_t smp = (uint32_t)__popcll(msk);
@@ -1589,7 +1592,7 @@ __device__ __forceinline__ void high_finalize_kernel_body(const FrameConst &c, c
 		carry += round_tot;
 		if(i < m) {
 			const uint32_t run = before + inc - smp; // samples of the entries before i
-			const bool cut = unsorted || i == 0 || (s_key[i] >> 13) != (s_key[i - 1] >> 13);
+			const bool cut = unsorted || i == 0 || (s_key[i] >> pb) != (s_key[i - 1] >> pb);
 			for(uint32_t w = 1; w < nw; w++) {
 				const uint32_t target = (uint32_t)(((uint64_t)w * total) / nw);
 				if(!((found >> w) & 1u) && run >= target && cut) {
@@ -1599,21 +1602,21 @@ __device__ __forceinline__ void high_finalize_kernel_body(const FrameConst &c, c
 			}
 			uint32_t out = i;
 			if(tie) {
-				const uint32_t rk = key >> 13;
+				const uint32_t rk = key >> pb;
 				uint32_t gs = i, ge = i;
-				while(gs > r0 && i - gs < kTieRun && (s_key[gs - 1] >> 13) == rk)
+				while(gs > r0 && i - gs < kTieRun && (s_key[gs - 1] >> pb) == rk)
 					gs--;
-				while(ge + 1 < r1 && ge - i < kTieRun && (s_key[ge + 1] >> 13) == rk)
+				while(ge + 1 < r1 && ge - i < kTieRun && (s_key[ge + 1] >> pb) == rk)
 					ge++;
 				// the whole run is [gs, ge]: it starts and ends within this round
-				const bool inside = (gs == 0 || (s_key[gs - 1] >> 13) != rk) &&
-									(ge + 1 >= m || (s_key[ge + 1] >> 13) != rk) && ge - gs < kTieRun;
+				const bool inside = (gs == 0 || (s_key[gs - 1] >> pb) != rk) &&
+									(ge + 1 >= m || (s_key[ge + 1] >> pb) != rk) && ge - gs < kTieRun;
 				if(inside) {
 					uint32_t ahead = 0;
 					for(uint32_t k = gs; k <= ge; k++) {
 						const uint32_t dk_k = s_tdk[r & 1][k - r0];
-						const uint32_t j_k = s_key[k] & 0x1fffu;
-						ahead += (dk_k < dk || (dk_k == dk && j_k < (key & 0x1fffu))) ? 1u : 0u;
+						const uint32_t j_k = s_key[k] & pmask;
+						ahead += (dk_k < dk || (dk_k == dk && j_k < (key & pmask))) ? 1u : 0u;
 					}
 					out = gs + ahead;
 				}
