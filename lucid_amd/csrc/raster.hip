@@ -961,11 +961,48 @@ __device__ __forceinline__ void rowgen_low_kernel_body(const FrameConst &c, cons
 
 	uint4 *recs = b.low_records + (size_t)bin * 4 * kLowRowCap;
 	uint32_t *lists = b.blk_lists + (size_t)bin * 16 * kRegSortCap;
+	// The bin's live triangles (its list has fewer than kLowLevelLimit entries: one position
+	// per thread and chunk) are counting-sorted by their number of 8-row bands, so that the
+	// waves' band loops run about as long as each of their triangles needs.
+	static_assert(kLowLevelLimit <= 4 * kRowgenThreads, "a low bin's list in one round");
+	__shared__ uint32_t s_sorted[kLowLevelLimit];
+	__shared__ uint32_t s_nb[4], s_nb_off[4], s_nlive;
+	if(threadIdx.x < 4)
+		s_nb[threadIdx.x] = 0;
+	__syncthreads();
+	uint32_t tk[4];
+	int hk[4];
+#pragma unroll
+	for(int k = 0; k < 4; k++) {
+		const uint32_t i = k * kRowgenThreads + threadIdx.x;
+		const uint32_t t = i < bi.nlevel ? bin_tri_at(b, i, bi.qc, bi.qo, bi.to) : 0xffffffffu;
+		int h = -1;
+		if(t != 0xffffffffu) {
+			const uint32_t yr = b.tri_scan[2 * (size_t)tri_rec(b, t)].w;
+			const int lo_by = min(max((int)(yr & 0xffffu) - bi.bin_y, 0), kBinMask) >> 3;
+			const int hi_by = min(max((int)(yr >> 16) - bi.bin_y, 0), kBinMask) >> 3;
+			h = hi_by > lo_by ? hi_by - lo_by : 0;
+			atomicAdd(&s_nb[h], 1u);
+		}
+		tk[k] = t, hk[k] = h;
+	}
+	__syncthreads();
+	if(threadIdx.x == 0) {
+		uint32_t run = 0;
+		for(int h = 0; h < 4; h++)
+			s_nb_off[h] = run, run += s_nb[h];
+		s_nlive = run;
+	}
+	__syncthreads();
+#pragma unroll
+	for(int k = 0; k < 4; k++)
+		if(hk[k] >= 0)
+			s_sorted[atomicAdd(&s_nb_off[hk[k]], 1u)] = tk[k];
+	__syncthreads();
+	const uint32_t nlive = s_nlive;
 	// block-row records, 8-row rule (raster_low.glsl:39-79), one live triangle per lane
-	__shared__ uint32_t s_live[kRowgenThreads / 64][128];
-	for_live_tris(b, bi, threadIdx.x & ~63u, kRowgenThreads, s_live[threadIdx.x >> 6], [&](uint32_t t) {
-		if(t == 0xffffffffu)
-			return;
+	for(uint32_t q = threadIdx.x; q < nlive; q += kRowgenThreads) {
+		const uint32_t t = s_sorted[q];
 		uint4 s0 = b.tri_scan[2 * (size_t)tri_rec(b, t)], s1 = b.tri_scan[2 * (size_t)tri_rec(b, t) + 1];
 		int min_by = min(max((int)(s0.w & 0xffffu) - bi.bin_y, 0), kBinMask) >> 3;
 		int max_by = min(max((int)(s0.w >> 16) - bi.bin_y, 0), kBinMask) >> 3;
@@ -989,7 +1026,7 @@ __device__ __forceinline__ void rowgen_low_kernel_body(const FrameConst &c, cons
 						lists[(by * 4 + bx) * kRegSortCap + k] = idx;
 				}
 		}
-	});
+	}
 	__syncthreads();
 	bool promote = false;
 	for(int k = 0; k < 16; k++)
