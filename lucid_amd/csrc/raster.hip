@@ -875,44 +875,6 @@ __device__ __forceinline__ BinInfo load_bin(const FrameConst &c, const FrameBufs
 	return bi;
 }
 
-// The live triangles of a bin's list, handed to a wave 64 at a time. The list holds quads
-// (two triangle ids each) and large triangles; a culled triangle (every second one of a
-// soup quad) would leave its lane idle through the whole row scan, so the wave gathers the
-// live ones of its chunks (first, first + stride, ...) in a 128-entry LDS queue and calls
-// fn(t) with a full wave of them (the last call may be partial: t = ~0 on idle lanes). fn is
-// called by the whole wave.
-template <class F>
-__device__ __forceinline__ void for_live_tris(const FrameBufs &b, const BinInfo &bi, uint32_t first,
-											  uint32_t stride, uint32_t *q, F &&fn) {
-	const int lane = lane_id();
-	uint32_t pending = 0; // queued triangles (< 64 between calls)
-	for(uint32_t base = first;;) {
-		if(base < bi.nlevel) {
-			const uint32_t i = base + lane;
-			const uint32_t t = i < bi.nlevel ? bin_tri_at(b, i, bi.qc, bi.qo, bi.to) : 0xffffffffu;
-			const uint64_t live = ballot(t != 0xffffffffu);
-			if(t != 0xffffffffu)
-				q[pending + (uint32_t)__popcll(live & ((1ull << lane) - 1ull))] = t;
-			pending += (uint32_t)__popcll(live);
-			base += stride;
-		}
-		const bool last = base >= bi.nlevel;
-		if(pending >= 64 || (last && pending > 0)) {
-			wave_sync_lds();
-			const uint32_t t = (uint32_t)lane < pending ? q[lane] : 0xffffffffu;
-			const uint32_t rem = pending > 64 ? pending - 64 : 0;
-			const uint32_t carry = (uint32_t)lane < rem ? q[64 + lane] : 0u;
-			wave_sync_lds();
-			if((uint32_t)lane < rem)
-				q[lane] = carry;
-			pending = rem;
-			fn(t);
-		}
-		if(last && pending == 0)
-			break;
-	}
-}
-
 // Exact-fallback queue: one uint4 item per pixel, {block slot, px, py, whole}. Called by
 // a whole wave; lanes with `mine` set queue their pixel (px, py).
 __device__ __forceinline__ void enqueue_pixels(const FrameBufs &b, uint32_t slot, bool mine, int px,
