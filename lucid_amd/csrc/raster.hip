@@ -748,27 +748,24 @@ __device__ __forceinline__ uint4 make_record(uint32_t t, const uint32_t sp[8], u
 					  sp[6] | (sp[7] << 10) | (bxm << 20));
 }
 
-// Inverse depth of a record inside block column bx at the centroid of its covered pixels:
-// the block-level sort key (ties and centroid error are resolved exactly per pixel later).
-__device__ __forceinline__ float block_inv_depth(const FrameBufs &b, uint4 rec, int bx, int px0,
-												 int py0) {
-	int n = 0, sx = 0, sy = 0;
-	int x0 = bx * 8;
-#pragma unroll
-	for(int r = 0; r < 8; r++) {
-		uint32_t sp = record_span(rec, r);
-		int a = (int)(sp & 31u) - x0, e = (int)((sp >> 5) & 31u) - x0;
-		a = a < 0 ? 0 : a;
-		e = e > 7 ? 7 : e;
-		int cnt = e >= a ? e - a + 1 : 0;
-		n += cnt;
-		sx += (a + e) * cnt;
-		sy += r * cnt;
-	}
-	float inv_n = 1.0f / (float)(n > 0 ? n : 1);
-	float cx = (float)px0 + (float)sx * 0.5f * inv_n;
-	float cy = (float)py0 + (float)sy * inv_n;
-	return sample_inv_depth(b, rec.x, cx, cy);
+// Inverse depth of triangle t inside a block at the centroid of its covered pixels, from
+// its pixel mask m (bit 8 r + c: row r, column c of the block whose corner pixel is
+// (px0, py0)): the block-level sort key (ties and centroid error are resolved exactly per
+// pixel later). The column and row sums are integers, taken bit by bit with masked
+// popcounts: the sum of the columns is sum_k 2^k popcount(m & pixels whose column has bit
+// k), likewise for the rows.
+__device__ __forceinline__ float mask_inv_depth(const FrameBufs &b, uint32_t t, uint64_t m,
+												int px0, int py0) {
+	auto pc = [](uint64_t x) { return (int)__popcll(x); };
+	const int n = pc(m);
+	const int sc = pc(m & 0xAAAAAAAAAAAAAAAAull) + 2 * pc(m & 0xCCCCCCCCCCCCCCCCull) +
+				   4 * pc(m & 0xF0F0F0F0F0F0F0F0ull);
+	const int sr = pc(m & 0xFF00FF00FF00FF00ull) + 2 * pc(m & 0xFFFF0000FFFF0000ull) +
+				   4 * pc(m & 0xFFFFFFFF00000000ull);
+	const float inv_n = 1.0f / (float)(n > 0 ? n : 1);
+	const float cx = (float)px0 + (float)(2 * sc) * 0.5f * inv_n;
+	const float cy = (float)py0 + (float)sr * inv_n;
+	return sample_inv_depth(b, t, cx, cy);
 }
 
 // Depth rank of a block entry from its depth key ~ord(inverse depth) (ascending = nearest
@@ -814,7 +811,8 @@ __device__ __forceinline__ void sort_block_positions(const FrameBufs &b, const u
 		const uint32_t j = k * 64 + lane;
 		dk[k] = 0u;
 		if(j < m) {
-			dk[k] = ~ord_f32(block_inv_depth(b, recs[list[j]], bx, px0, py0));
+			const uint4 rec = recs[list[j]];
+			dk[k] = inv_ord_f32(mask_inv_depth(b, rec.x, record_block_mask(rec, bx), px0, py0));
 			lo = min(lo, dk[k]);
 			hi = max(hi, dk[k]);
 		}
@@ -1425,7 +1423,7 @@ __device__ __forceinline__ void high_finalize_kernel_body(const FrameConst &c, c
 				if(unsorted) {
 					s_key[j] = j; // list order: rank 0, the position alone
 				} else {
-					const uint32_t dk = ~ord_f32(block_inv_depth(b, rec, bx, px0, py0));
+					const uint32_t dk = inv_ord_f32(mask_inv_depth(b, rec.x, msk, px0, py0));
 					s_key[j] = dk;
 					lo = min(lo, dk);
 					hi = max(hi, dk);
@@ -1573,7 +1571,7 @@ __device__ __forceinline__ void high_finalize_kernel_body(const FrameConst &c, c
 			tie = exact_runs && ((i > 0 && (s_key[i - 1] >> pb) == (key >> pb)) ||
 								 (i + 1 < m && (s_key[i + 1] >> pb) == (key >> pb)));
 			if(tie)
-				dk = ~ord_f32(block_inv_depth(b, recs[list[key & pmask]], bx, px0, py0));
+				dk = inv_ord_f32(mask_inv_depth(b, tri, msk, px0, py0));
 		}
 		s_tdk[r & 1][threadIdx.x] = dk;
 		const uint32_t smp = (uint32_t)__popcll(msk);
