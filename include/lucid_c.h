@@ -77,8 +77,8 @@ typedef struct lucid_tuning {
 	int samples_per_wave; /* target samples per wave of a pre-sorted block (256..65536) */
 	int window;           /* per-pixel ordering window of the block rasterisers in frames of
 						   * RGBA8 samples with unit lighting and flat colours, and of the
-						   * backward's first pass: 1 (one slot; two in pre-sorted blocks and
-						   * in the backward) for lists whose block order is nearly every
+						   * backward's first pass: 1 (one slot; two in the backward) for
+						   * lists whose block order is nearly every
 						   * pixel's depth order, 4 for more disorder. A pixel the window
 						   * cannot order takes the exact per-pixel sort either way. */
 } lucid_tuning;
