@@ -12,6 +12,7 @@
 namespace lucid {
 
 constexpr int kBinThreads = 512;
+constexpr int kBinPrefetch = 8; // quad AABBs a thread loads before it counts or places them
 
 // loadScanlineParamsBin + scanlineStep (scanline.glsl:28-52, bin_counter.glsl:53-62)
 struct BinScan {
@@ -105,15 +106,24 @@ __device__ __forceinline__ void bin_count_kernel_body(const FrameConst &c, const
 	__syncthreads();
 	Slice sl = wg_slice(c, b, wg.id);
 
-	// small quads: +1 over the bin rectangle (bin_counter.glsl:64-80)
-	for(uint32_t q = sl.q0 + threadIdx.x; q < sl.q1; q += kBinThreads) {
-		uint32_t a = b.quad_aabb[q];
-		int bsx, bsy, bex, bey;
-		if(a == kInvalidAabb || !small_quad(a, bsx, bsy, bex, bey))
-			continue;
-		for(int by = bsy; by <= bey; by++)
-			for(int bx = bsx; bx <= bex; bx++)
-				atomicAdd(&hq[by * NBX + bx], 1u);
+	// small quads: +1 over the bin rectangle (bin_counter.glsl:64-80); a thread loads
+	// kBinPrefetch AABBs at a time, so that their loads are in flight together
+	for(uint32_t q0 = sl.q0; q0 < sl.q1; q0 += kBinThreads * kBinPrefetch) {
+		uint32_t av[kBinPrefetch];
+#pragma unroll
+		for(int k = 0; k < kBinPrefetch; k++) {
+			const uint32_t q = q0 + k * kBinThreads + threadIdx.x;
+			av[k] = q < sl.q1 ? b.quad_aabb[q] : kInvalidAabb;
+		}
+#pragma unroll
+		for(int k = 0; k < kBinPrefetch; k++) {
+			int bsx, bsy, bex, bey;
+			if(av[k] == kInvalidAabb || !small_quad(av[k], bsx, bsy, bex, bey))
+				continue;
+			for(int by = bsy; by <= bey; by++)
+				for(int bx = bsx; bx <= bex; bx++)
+					atomicAdd(&hq[by * NBX + bx], 1u);
+		}
 	}
 	// large tris: per bin row +1/-1 marks (bin_counter.glsl:112-134)
 	for(uint32_t i = sl.l0 * 2 + threadIdx.x; i < sl.l1 * 2; i += kBinThreads) {
@@ -324,15 +334,24 @@ __device__ __forceinline__ void bin_dispatch_kernel_body(const FrameConst &c, co
 	__syncthreads();
 	uint32_t *pq = s_pos, *pt = s_pos + NB;
 	Slice sl = wg_slice(c, b, wg.id);
-	for(uint32_t q = sl.q0 + threadIdx.x; q < sl.q1; q += kBinThreads) {
-		uint32_t a = b.quad_aabb[q];
-		int bsx, bsy, bex, bey;
-		if(a == kInvalidAabb || !small_quad(a, bsx, bsy, bex, bey))
-			continue;
-		uint32_t entry = q | (a & 0xf0000000u);
-		for(int by = bsy; by <= bey; by++)
-			for(int bx = bsx; bx <= bex; bx++)
-				b.bin_quads[atomicAdd(&pq[by * NBX + bx], 1u)] = entry;
+	for(uint32_t q0 = sl.q0; q0 < sl.q1; q0 += kBinThreads * kBinPrefetch) {
+		uint32_t av[kBinPrefetch];
+#pragma unroll
+		for(int k = 0; k < kBinPrefetch; k++) {
+			const uint32_t q = q0 + k * kBinThreads + threadIdx.x;
+			av[k] = q < sl.q1 ? b.quad_aabb[q] : kInvalidAabb;
+		}
+#pragma unroll
+		for(int k = 0; k < kBinPrefetch; k++) {
+			const uint32_t q = q0 + k * kBinThreads + threadIdx.x;
+			int bsx, bsy, bex, bey;
+			if(av[k] == kInvalidAabb || !small_quad(av[k], bsx, bsy, bex, bey))
+				continue;
+			const uint32_t entry = q | (av[k] & 0xf0000000u);
+			for(int by = bsy; by <= bey; by++)
+				for(int bx = bsx; bx <= bex; bx++)
+					b.bin_quads[atomicAdd(&pq[by * NBX + bx], 1u)] = entry;
+		}
 	}
 	for(uint32_t i = sl.l0 * 2 + threadIdx.x; i < sl.l1 * 2; i += kBinThreads) {
 		uint32_t q = large_at(b, sl, i >> 1), s = i & 1u;
