@@ -31,7 +31,7 @@ DEFAULT_TRIS = 1_000_000
 # FETCH_SIZE and WRITE_SIZE passes scaled by a 256 MiB copy, and the VALU issue of the same
 # kernels (SQ_INSTS_VALU over kernel time x 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per
 # wave64 instruction). No counters are read in-process.
-RASTER_TRAFFIC_BYTES = 373.3e6
+RASTER_TRAFFIC_BYTES = 373.5e6
 RASTER_VALU_BUSY = 1.02  # above 1: faster than one wave64 instruction per 4 cycles
 
 
