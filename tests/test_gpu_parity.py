@@ -1,7 +1,6 @@
 """GPU parity: the HIP path (through the C ABI) against the C oracle.
 
-Bar: bin counts, per-bin lists (as multisets) and per-pixel fragment
-counts bit-exact; blended image within 1/255 per channel (exact per-pixel order, ties by
+Bar: bin counts, per-bin lists (as multisets) and per-pixel fragment counts bit-exact; blended image within 1/255 per channel (exact per-pixel order, ties by
 triangle id). Scenes cover the reference's edge cases: empty frame, culled/between-sample
 quads, low bins promoted to high (> 256 tris in an 8x8 block), high bins (>= 1024 tris),
 blocks too large for LDS, per-pixel ordering violations (exact fallback) and pixels deeper
