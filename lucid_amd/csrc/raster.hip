@@ -799,9 +799,14 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 
 // Depth order of a block list of m <= 256 entries, sorted by one wave in registers: sorted
 // entry 64 k + lane is list position pos[k]. Keys are (24-bit depth rank << 8 | position).
+// With s_msk / s_tri (LDS, kRegSortCap entries each, or null), each entry's pixel mask and
+// triangle id are also left there at its list position, for the caller to read in sorted
+// order (after a wave_sync_lds) instead of gathering the records again.
 __device__ __forceinline__ void sort_block_positions(const FrameBufs &b, const uint4 *recs,
 													 const uint32_t *list, uint32_t m, int bx,
-													 int px0, int py0, uint32_t (&pos)[4]) {
+													 int px0, int py0, uint32_t (&pos)[4],
+													 uint64_t *s_msk = nullptr,
+													 uint32_t *s_tri = nullptr) {
 	static_assert(kRegSortCap <= 256, "list positions fit 8 bits of the key");
 	const int lane = lane_id();
 	uint32_t dk[4];
@@ -812,7 +817,12 @@ __device__ __forceinline__ void sort_block_positions(const FrameBufs &b, const u
 		dk[k] = 0u;
 		if(j < m) {
 			const uint4 rec = recs[list[j]];
-			dk[k] = inv_ord_f32(mask_inv_depth(b, rec.x, record_block_mask(rec, bx), px0, py0));
+			const uint64_t msk = record_block_mask(rec, bx);
+			if(s_msk) {
+				s_msk[j] = msk;
+				s_tri[j] = rec.x;
+			}
+			dk[k] = inv_ord_f32(mask_inv_depth(b, rec.x, msk, px0, py0));
 			lo = min(lo, dk[k]);
 			hi = max(hi, dk[k]);
 		}
@@ -1958,24 +1968,37 @@ __device__ __forceinline__ void block_raster_kernel_body(const FrameConst &c, co
 		return; // the recording pass A renders this block
 	const uint4 *recs = (level == LUCID_BIN_LEVEL_LOW ? b.low_records : b.high_records) + sl.z;
 	const uint32_t *list = b.blk_lists + sl.x;
-	uint32_t pos[4];
-	if(c.opts & LUCID_OPT_UNSORTED) {
-#pragma unroll
-		for(int k = 0; k < 4; k++)
-			pos[k] = k * 64 + lane; // list order
-	} else {
-		sort_block_positions(b, recs, list, m, bx, px0, py0, pos);
-	}
 	uint64_t msk_all[4];
 	uint32_t tri_all[4];
+	if(c.opts & LUCID_OPT_UNSORTED) { // list order
 #pragma unroll
-	for(int j = 0; j < 4; j++) {
-		msk_all[j] = 0, tri_all[j] = 0;
-		if(j * 64 + lane < (int)m) {
-			uint4 rec = recs[list[pos[j]]];
-			msk_all[j] = record_block_mask(rec, bx);
-			tri_all[j] = rec.x;
+		for(int j = 0; j < 4; j++) {
+			msk_all[j] = 0, tri_all[j] = 0;
+			if(j * 64 + lane < (int)m) {
+				const uint4 rec = recs[list[j * 64 + lane]];
+				msk_all[j] = record_block_mask(rec, bx);
+				tri_all[j] = rec.x;
+			}
 		}
+	} else {
+		// the sort leaves the masks and triangle ids by list position in the (not yet used)
+		// slice staging area; the lanes read them back in depth order
+		static_assert(sizeof(s_rec) >= kRegSortCap * (sizeof(uint64_t) + sizeof(uint32_t)),
+					  "the sort's masks and ids fit the staging area");
+		uint64_t *s_msk = reinterpret_cast<uint64_t *>(s_rec);
+		uint32_t *s_tri = reinterpret_cast<uint32_t *>(s_msk + kRegSortCap);
+		uint32_t pos[4];
+		sort_block_positions(b, recs, list, m, bx, px0, py0, pos, s_msk, s_tri);
+		wave_sync_lds();
+#pragma unroll
+		for(int j = 0; j < 4; j++) {
+			msk_all[j] = 0, tri_all[j] = 0;
+			if(j * 64 + lane < (int)m) {
+				msk_all[j] = s_msk[pos[j]];
+				tri_all[j] = s_tri[pos[j]];
+			}
+		}
+		wave_sync_lds(); // read before the first slice is staged over them
 	}
 	const bool additive = (c.opts & (1u << 5)) != 0;
 	const bool threshold = alpha_threshold_on(c);
